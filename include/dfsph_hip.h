@@ -1,0 +1,213 @@
+/*
+ * dfsph_hip.h -- C ABI of the MI355X (gfx950) DFSPH time-step library
+ * (libdfsph_hip.so).
+ *
+ * One entry point per function of the reference's GPU DFSPH path; each
+ * declaration cites the reference interface it replaces (paths relative to
+ * the reference repository root).  The reference passes SPH::DFSPHCData& to
+ * C++ free functions that exit() on error; here a dfsph_handle owns all
+ * device memory and every call returns a status code.
+ *
+ * Conventions
+ *   - every function returns DFSPH_OK (0) or a DFSPH_ERR_* code; the text of
+ *     the last error of the calling thread is in dfsph_last_error();
+ *   - plain pointers and sizes only; host arrays are float32 / uint32, xyz
+ *     arrays are interleaved [x0 y0 z0 x1 ...];
+ *   - one handle per host thread; work is queued on one HIP stream per
+ *     handle and the calls that return host values synchronise it;
+ *   - particles are kept in cell-sorted order on the device and reordered by
+ *     every sorting neighbor search; DFSPH_FIELD_ID gives each slot's
+ *     persistent particle id (the id passed to dfsph_upload_fluid).
+ */
+#ifndef DFSPH_HIP_H
+#define DFSPH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DFSPH_OK 0
+#define DFSPH_ERR_HIP 1            /* a HIP runtime call failed             */
+#define DFSPH_ERR_ARG 2            /* invalid argument                      */
+#define DFSPH_ERR_CAPACITY 3       /* more particles than the capacity      */
+#define DFSPH_ERR_NEIGHBOR_OVERFLOW 4 /* a particle has more than max_neighbors */
+#define DFSPH_ERR_STATE 5          /* call out of order (e.g. no boundary)  */
+#define DFSPH_ERR_NO_MODEL 6       /* learned warm start without weights    */
+
+/* warm_start_mode */
+#define DFSPH_WARMSTART_REFERENCE 0   /* kappa / kappaV rescaled from the last step */
+#define DFSPH_WARMSTART_LEARNED 1     /* kappa / kappaV predicted by the MLP          */
+#define DFSPH_WARMSTART_NONE 2        /* no warm start (cold Jacobi start)             */
+
+typedef struct dfsph_params {
+    double particle_radius;   /* r; kernel support 4r (DFSPH_c_arrays_structure.cpp:857-869) */
+    double density0;          /* rest density rho0                                           */
+    double viscosity;         /* XSPH coefficient (DFSPH_CUDA.cpp:98)                         */
+    double gravity[3];        /* (DFSPH_c_arrays_structure.cpp:778)                          */
+    double dt;                /* initial time step h                                         */
+    uint32_t max_iter_v;      /* divergence solver iteration cap                              */
+    uint32_t max_iter;        /* density solver iteration cap                                 */
+    uint32_t min_iter_v;      /* 3 in the reference (DFSPH_core_cuda.cu:828)                  */
+    uint32_t min_iter;        /* 2 in the reference (DFSPH_core_cuda.cu:1378)                 */
+    double max_error_v;       /* percent, divergence                                         */
+    double max_error;         /* percent, density                                            */
+    int32_t use_cfl;          /* 1: h_future from dfsph_cfl inside dfsph_step                 */
+    double cfl_factor;        /* h = cfl_factor * 0.4 * 2r / max|v + a h|                     */
+    double dt_min, dt_max;
+    int32_t enable_divergence; /* m_enableDivergenceSolver (DFSPH_CUDA.h:64)                  */
+    int32_t check_positions_mode; /* checkParticlesPositions: -1 off, 0 count, 2 remove       */
+    uint32_t max_neighbors;   /* per-particle list capacity, fluid + boundary (<= 255)       */
+    uint32_t fluid_capacity;  /* 0: 1.3 x the first upload (DFSPH_c_arrays_structure.cpp:170-172) */
+    int32_t warm_start_mode;  /* DFSPH_WARMSTART_*                                            */
+    uint32_t check_interval;  /* solver iterations queued per convergence readback (0: 4)     */
+} dfsph_params;
+
+typedef struct dfsph_step_stats {
+    uint32_t iters_v, iters;  /* divergence / density Jacobi iterations */
+    double err_v, err;        /* final average source: divergence (rho/s), density (rho) */
+    double dt;                /* h used for the position update        */
+    double time;              /* simulated time after the step         */
+    uint32_t n_fluid;         /* active fluid particles after the step */
+    uint32_t removed;         /* particles removed by the position check */
+    uint32_t max_neighbors_seen;
+    uint32_t overflow;        /* particles whose list was truncated    */
+} dfsph_step_stats;
+
+typedef struct dfsph_info {
+    uint32_t n_fluid, n_boundary, fluid_capacity, max_neighbors;
+    uint32_t grid_dim[3];
+    double grid_origin[3];
+    double cell_size;
+    double h, h_future, h_past, time;
+    double bbox_min[3], bbox_max[3];
+    uint32_t max_neighbors_seen;   /* largest count of the last neighbor search */
+    uint32_t overflow;             /* particles whose list was truncated        */
+    uint32_t escaped;              /* outside the box after the last position update */
+    uint32_t pad;
+} dfsph_info;
+
+/* fields for dfsph_download / dfsph_upload_field */
+#define DFSPH_FIELD_POS 0          /* float xyz (3n)   */
+#define DFSPH_FIELD_VEL 1          /* float xyz (3n)   */
+#define DFSPH_FIELD_MASS 2         /* float            */
+#define DFSPH_FIELD_DENSITY 3      /* float            */
+#define DFSPH_FIELD_FACTOR 4       /* float (alpha)    */
+#define DFSPH_FIELD_DENSITY_ADV 5  /* float            */
+#define DFSPH_FIELD_KAPPA 6        /* float            */
+#define DFSPH_FIELD_KAPPAV 7       /* float            */
+#define DFSPH_FIELD_ACC 8          /* float xyz (3n)   */
+#define DFSPH_FIELD_ID 9           /* uint32           */
+#define DFSPH_FIELD_NBR_FLUID 10   /* uint32 fluid neighbor count    */
+#define DFSPH_FIELD_NBR_BOUNDARY 11 /* uint32 boundary neighbor count */
+#define DFSPH_FIELD_BOUNDARY_POS 12 /* float xyz (3 n_boundary), cell-sorted */
+#define DFSPH_FIELD_BOUNDARY_PSI 13 /* float (n_boundary), cell-sorted */
+
+void dfsph_default_params(dfsph_params* p);
+const char* dfsph_last_error(void);
+
+/* DFSPHCData::DFSPHCData(FluidModel*) + DFSPHCUDA::DFSPHCUDA
+ * (DFSPH_c_arrays_structure.cpp:852-944, DFSPH_CUDA.cpp:38-75) */
+int dfsph_create(const dfsph_params* p, void** handle);
+/* DFSPHCData::~DFSPHCData (DFSPH_c_arrays_structure.cpp:946-973) */
+int dfsph_destroy(void* handle);
+
+/* Static boundary set: allocate_UnifiedParticleSet_cuda + load_UnifiedParticleSet_cuda
+ * (DFSPH_memory_management_cuda.cu:502-564, :566-650), the one-time boundary cell table
+ * (DFSPH_c_arrays_structure.cpp:1016) and initGridOffset (:1050-1098).
+ * psi == NULL: computed on the device as FluidModel::computeBoundaryPsi
+ * (FluidModel.cpp:233-258). */
+int dfsph_upload_boundary(void* handle, const float* xyz, const float* psi, uint32_t n);
+
+/* Fluid set: UnifiedParticleSet::reset<FluidModel> + first cell sort
+ * (DFSPH_c_arrays_structure.cpp:1035-1037).  mass == NULL: 0.8 (2r)^3 rho0
+ * (FluidModel.cpp:94-108); vel == NULL: zero; id == NULL: 0..n-1.
+ * Resets kappa, kappaV and the time-step history. */
+int dfsph_upload_fluid(void* handle, const float* xyz, const float* vel, const float* mass,
+                       const uint32_t* id, uint32_t n);
+
+/* cuda_neighborsSearch (DFSPH_core_cuda.h:65, DFSPH_core_cuda.cu:2651-2770):
+ * cell hash, deterministic counting sort, reorder, neighbor lists.  Queued
+ * without a host round trip; a list overflow is reported by dfsph_get_info. */
+int dfsph_neighbor_search(void* handle, int sort);
+
+/* The density + factor part of cuda_divergence_warmstart_init (DFSPH_core_cuda.h:24,
+ * DFSPH_core_cuda.cu:220-376) without the warm start, for steps that skip the
+ * divergence solver (the reference then keeps stale values, DFSPH_CUDA.cpp:261-270). */
+int dfsph_compute_density_factor(void* handle);
+
+/* cuda_divergenceSolve (DFSPH_core_cuda.h:31, DFSPH_core_cuda.cu:785-861); also
+ * computes density and factor (the fused warm-start init, :220-376). */
+int dfsph_divergence_solve(void* handle, uint32_t max_iter, double max_error,
+                           uint32_t* iters, double* err);
+
+/* cuda_externalForces (DFSPH_core_cuda.h:55, DFSPH_core_cuda.cu:1613-1627): a = g + XSPH */
+int dfsph_external_forces(void* handle);
+
+/* cuda_CFL (DFSPH_core_cuda.h:77, DFSPH_core_cuda.cu:3321-3399); returns the new h
+ * and applies it with DFSPHCData::updateTimeStep. */
+int dfsph_cfl(void* handle, double min_dt, double cfl_factor, double max_dt, double* dt_out);
+
+/* DFSPHCData::updateTimeStep (DFSPH_c_arrays_structure.cpp:1100-1106) */
+int dfsph_update_time_step(void* handle, double h_future);
+
+/* cuda_update_vel (DFSPH_core_cuda.h:75, DFSPH_core_cuda.cu:3047-3070) */
+int dfsph_update_vel(void* handle);
+
+/* cuda_pressureSolve (DFSPH_core_cuda.h:48, DFSPH_core_cuda.cu:1338-1419) */
+int dfsph_pressure_solve(void* handle, uint32_t max_iter, double max_error,
+                         uint32_t* iters, double* err);
+
+/* cuda_update_pos (DFSPH_core_cuda.h:76, DFSPH_core_cuda.cu:3127-3256) */
+int dfsph_update_pos(void* handle);
+
+/* DFSPHCData::onSimulationStepEnd (DFSPH_c_arrays_structure.cpp:1108-1118) and the
+ * TimeManager time advance of DFSPHCUDA::step (DFSPH_CUDA.cpp:441-452) */
+int dfsph_step_end(void* handle);
+
+/* Restore the time-step history when a run resumes from a snapshot (the
+ * reference reloads only dt from general_data.txt, DFSPH_CUDA.cpp:1887-1979):
+ * h, h_past and the simulated time; h_future = h. */
+int dfsph_set_time(void* handle, double h, double h_past, double time);
+
+/* DFSPHCData::checkParticlesPositions -> check_particles_positions_cuda
+ * (DFSPH_c_arrays_structure.h:486-492, SPH_other_systems_cuda.cu:1283-1330):
+ * mode 0 counts, mode 2 removes the particles outside the boundary bounding box
+ * (order of the others kept). */
+int dfsph_check_particles_positions(void* handle, int mode, uint32_t* count);
+
+/* remove_tagged_particles (SPH_other_systems_cuda.cu:1606-1641): remove the
+ * particles whose slot has keep[slot] == 0 (device order), order kept. */
+int dfsph_remove_particles(void* handle, const uint8_t* keep, uint32_t* n_out);
+
+/* add_particles_cuda (DFSPH_memory_management_cuda.cu:1036-1062): append
+ * particles after the active ones (kappa = kappaV = 0). */
+int dfsph_add_particles(void* handle, const float* xyz, const float* vel, const float* mass,
+                        const uint32_t* id, uint32_t n);
+
+/* DFSPHCUDA::step (DFSPH_CUDA.cpp:82-794): the sequence above in the
+ * reference order, with the library's kernel fusions. */
+int dfsph_step(void* handle, dfsph_step_stats* stats);
+
+/* read_UnifiedParticleSet_cuda (DFSPH_memory_management_cuda.cu:652-674):
+ * copy a field of the active particles (device order) to host memory.
+ * count must equal the number of elements of the field. */
+int dfsph_download(void* handle, int field, void* dst, uint32_t count);
+/* overwrite a fluid field (device order): POS, VEL, MASS, KAPPA, KAPPAV */
+int dfsph_upload_field(void* handle, int field, const void* src, uint32_t count);
+
+/* neighbor lists of the last search, device order, row-major [i*max + k]:
+ * fluid neighbor slots first, then boundary slots (DFSPH_macro_cuda.h:208-287) */
+int dfsph_get_neighbors(void* handle, uint32_t* n_fluid, uint32_t* n_boundary,
+                        uint32_t* list, uint32_t max);
+
+int dfsph_get_info(void* handle, dfsph_info* info);
+int dfsph_synchronize(void* handle);
+
+/* ---- learned warm start (no reference counterpart; see dfsph_mlp.h) ---- */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

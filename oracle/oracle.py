@@ -1,0 +1,268 @@
+"""ctypes loader of the CPU oracle (liboracle.so, built by oracle/Makefile).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  Never used by the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+MODE_GPU = 0
+MODE_CPU = 1
+F_POS, F_VEL, F_MASS, F_DENSITY, F_FACTOR, F_DENSITY_ADV, F_KAPPA, F_KAPPAV, F_ACC, F_ID = range(10)
+_VEC = {F_POS, F_VEL, F_ACC}
+
+MLP_SIZES = [(64, 12), (64,), (64, 64), (64,), (64, 64), (64,), (2, 64), (2,)]
+MLP_NPARAM = sum(int(np.prod(s)) for s in MLP_SIZES)
+
+
+class OrParams(C.Structure):
+    _fields_ = [
+        ("particle_radius", C.c_double), ("density0", C.c_double), ("viscosity", C.c_double),
+        ("gravity", C.c_double * 3), ("dt", C.c_double), ("mode", C.c_int), ("kernel_table", C.c_int),
+        ("fp32_predicate", C.c_int), ("enable_divergence", C.c_int),
+        ("max_iter_v", C.c_uint32), ("max_iter", C.c_uint32), ("min_iter_v", C.c_uint32), ("min_iter", C.c_uint32),
+        ("max_error_v", C.c_double), ("max_error", C.c_double), ("use_cfl", C.c_int),
+        ("cfl_factor", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double), ("cfl_seed", C.c_double),
+        ("remove_escaped", C.c_int), ("max_neighbors", C.c_uint32),
+    ]
+
+
+class OrStats(C.Structure):
+    _fields_ = [("iters_v", C.c_uint32), ("iters", C.c_uint32), ("err_v", C.c_double), ("err", C.c_double),
+                ("dt_used", C.c_double), ("removed", C.c_uint32), ("n_fluid", C.c_uint32),
+                ("max_nbr", C.c_uint32), ("overflow", C.c_uint32)]
+
+
+_lib = None
+_DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
+_UP = C.POINTER(C.c_uint32)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        sig = {
+            "or_default_params": (None, [C.POINTER(OrParams)]),
+            "or_create": (C.c_void_p, [C.POINTER(OrParams)]),
+            "or_destroy": (None, [C.c_void_p]),
+            "or_set_fluid": (C.c_int, [C.c_void_p, C.c_uint32, _DP, _DP, _DP, _UP]),
+            "or_set_boundary": (C.c_int, [C.c_void_p, C.c_uint32, _DP, _DP]),
+            "or_get_boundary_psi": (C.c_int, [C.c_void_p, _DP]),
+            "or_num_fluid": (C.c_uint32, [C.c_void_p]),
+            "or_get": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+            "or_set": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+            "or_get_ids": (C.c_int, [C.c_void_p, _UP]),
+            "or_get_time": (None, [C.c_void_p, _DP, _DP, _DP, _DP]),
+            "or_set_time_step": (None, [C.c_void_p, C.c_double]),
+            "or_neighbor_search": (C.c_int, [C.c_void_p]),
+            "or_divergence_solve": (C.c_uint32, [C.c_void_p, _DP]),
+            "or_external_forces": (None, [C.c_void_p]),
+            "or_cfl": (C.c_double, [C.c_void_p]),
+            "or_update_time_step": (None, [C.c_void_p, C.c_double]),
+            "or_update_vel": (None, [C.c_void_p]),
+            "or_pressure_solve": (C.c_uint32, [C.c_void_p, _DP]),
+            "or_update_pos": (None, [C.c_void_p]),
+            "or_step_end": (None, [C.c_void_p]),
+            "or_check_positions": (C.c_uint32, [C.c_void_p, C.c_int]),
+            "or_step": (C.c_int, [C.c_void_p, C.POINTER(OrStats)]),
+            "or_get_neighbors": (C.c_int, [C.c_void_p, _UP, _UP, _UP]),
+            "or_neighbors_fp32": (C.c_uint32, [C.c_uint32, _FP, C.c_uint32, _FP, C.c_float, C.c_int, C.c_uint32,
+                                               _UP, _UP, _UP]),
+            "or_kernel_W": (C.c_double, [C.c_double, C.c_double, C.c_int]),
+            "or_kernel_gradW": (None, [_DP, C.c_double, C.c_int, _DP]),
+            "or_mlp_forward": (None, [C.c_uint32, _DP, _DP, _DP]),
+            "or_mlp_loss_grad": (C.c_double, [C.c_uint32, _DP, _DP, _DP, _DP]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def params(**kw) -> OrParams:
+    p = OrParams()
+    lib().or_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "gravity":
+            p.gravity = (C.c_double * 3)(*v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Oracle:
+    """FP64 CPU restatement of the DFSPH step (see dfsph_oracle.h)."""
+
+    def __init__(self, p: OrParams | None = None, **kw):
+        self.L = lib()
+        self.p = p if p is not None else params(**kw)
+        self.s = self.L.or_create(C.byref(self.p))
+
+    def __del__(self):
+        try:
+            if self.s:
+                self.L.or_destroy(self.s)
+                self.s = None
+        except Exception:
+            pass
+
+    def set_boundary(self, xyz, psi=None):
+        x = _d(xyz).reshape(-1)
+        p = None if psi is None else _d(psi)
+        self.L.or_set_boundary(self.s, x.size // 3, x.ctypes.data_as(_DP), None if p is None else p.ctypes.data_as(_DP))
+
+    def boundary_psi(self, n):
+        out = np.zeros(n)
+        self.L.or_get_boundary_psi(self.s, out.ctypes.data_as(_DP))
+        return out
+
+    def set_fluid(self, xyz, vel=None, mass=None, ids=None):
+        x = _d(xyz).reshape(-1)
+        v = None if vel is None else _d(vel).reshape(-1)
+        m = None if mass is None else _d(mass)
+        i = None if ids is None else np.ascontiguousarray(ids, np.uint32)
+        self.L.or_set_fluid(self.s, x.size // 3, x.ctypes.data_as(_DP), None if v is None else v.ctypes.data_as(_DP),
+                            None if m is None else m.ctypes.data_as(_DP), None if i is None else i.ctypes.data_as(_UP))
+
+    @property
+    def n(self):
+        return self.L.or_num_fluid(self.s)
+
+    def get(self, field):
+        n = self.n
+        out = np.zeros(3 * n if field in _VEC else n)
+        self.L.or_get(self.s, field, out.ctypes.data_as(_DP))
+        return out.reshape(-1, 3) if field in _VEC else out
+
+    def set(self, field, values):
+        a = _d(values).reshape(-1)
+        self.L.or_set(self.s, field, a.ctypes.data_as(_DP))
+
+    def ids(self):
+        out = np.zeros(self.n, np.uint32)
+        self.L.or_get_ids(self.s, out.ctypes.data_as(_UP))
+        return out
+
+    def time(self):
+        v = [C.c_double() for _ in range(4)]
+        self.L.or_get_time(self.s, *[C.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def state(self):
+        return {"id": self.ids(), "pos": self.get(F_POS), "vel": self.get(F_VEL), "mass": self.get(F_MASS),
+                "density": self.get(F_DENSITY), "factor": self.get(F_FACTOR),
+                "density_adv": self.get(F_DENSITY_ADV), "kappa": self.get(F_KAPPA), "kappaV": self.get(F_KAPPAV),
+                "acc": self.get(F_ACC)}
+
+    def neighbor_search(self):
+        return self.L.or_neighbor_search(self.s)
+
+    def neighbors(self):
+        n = self.n
+        mx = self.p.max_neighbors
+        nf = np.zeros(n, np.uint32)
+        nb = np.zeros(n, np.uint32)
+        L = np.zeros((n, mx), np.uint32)
+        self.L.or_get_neighbors(self.s, nf.ctypes.data_as(_UP), nb.ctypes.data_as(_UP), L.ctypes.data_as(_UP))
+        return nf, nb, L
+
+    def divergence_solve(self):
+        e = C.c_double()
+        it = self.L.or_divergence_solve(self.s, C.byref(e))
+        return it, e.value
+
+    def external_forces(self):
+        self.L.or_external_forces(self.s)
+
+    def cfl(self):
+        return self.L.or_cfl(self.s)
+
+    def update_time_step(self, hf):
+        self.L.or_update_time_step(self.s, hf)
+
+    def update_vel(self):
+        self.L.or_update_vel(self.s)
+
+    def pressure_solve(self):
+        e = C.c_double()
+        it = self.L.or_pressure_solve(self.s, C.byref(e))
+        return it, e.value
+
+    def update_pos(self):
+        self.L.or_update_pos(self.s)
+
+    def step_end(self):
+        self.L.or_step_end(self.s)
+
+    def check_positions(self, remove):
+        return self.L.or_check_positions(self.s, 1 if remove else 0)
+
+    def step(self):
+        st = OrStats()
+        self.L.or_step(self.s, C.byref(st))
+        return st
+
+
+def neighbors_fp32(xf, xb, hk, brute=False, max_nbr=90):
+    """FP32-predicate neighbor lists (fluid first, then boundary), row-major."""
+    xf = np.ascontiguousarray(xf, np.float32).reshape(-1)
+    xb = np.ascontiguousarray(xb, np.float32).reshape(-1)
+    nf_ = xf.size // 3
+    nf = np.zeros(nf_, np.uint32)
+    nb = np.zeros(nf_, np.uint32)
+    L = np.zeros((nf_, max_nbr), np.uint32)
+    mx = lib().or_neighbors_fp32(nf_, xf.ctypes.data_as(_FP), xb.size // 3, xb.ctypes.data_as(_FP), C.c_float(hk),
+                                 1 if brute else 0, max_nbr, nf.ctypes.data_as(_UP), nb.ctypes.data_as(_UP),
+                                 L.ctypes.data_as(_UP))
+    return nf, nb, L, mx
+
+
+def kernel_W(r, hk, table=False):
+    return lib().or_kernel_W(r, hk, 1 if table else 0)
+
+
+def kernel_gradW(r3, hk, table=False):
+    a = _d(r3)
+    out = np.zeros(3)
+    lib().or_kernel_gradW(a.ctypes.data_as(_DP), hk, 1 if table else 0, out.ctypes.data_as(_DP))
+    return out
+
+
+def mlp_forward(params_flat, x):
+    p = _d(params_flat)
+    x = _d(x).reshape(-1, 12)
+    y = np.zeros((x.shape[0], 2))
+    lib().or_mlp_forward(x.shape[0], p.ctypes.data_as(_DP), x.ctypes.data_as(_DP), y.ctypes.data_as(_DP))
+    return y
+
+
+def mlp_loss_grad(params_flat, x, t):
+    p = _d(params_flat)
+    x = _d(x).reshape(-1, 12)
+    t = _d(t).reshape(-1, 2)
+    g = np.zeros(MLP_NPARAM)
+    loss = lib().or_mlp_loss_grad(x.shape[0], p.ctypes.data_as(_DP), x.ctypes.data_as(_DP), t.ctypes.data_as(_DP),
+                                  g.ctypes.data_as(_DP))
+    return loss, g

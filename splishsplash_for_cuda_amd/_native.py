@@ -1,0 +1,458 @@
+"""ctypes binding of libdfsph_hip.so (include/dfsph_hip.h) and libdfsph_host.so
+(include/dfsph_host.h).
+
+The libraries are built in-tree by ``splishsplash_for_cuda_amd.build``.  There
+is no fallback: if a library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB = os.path.join(HERE, "libdfsph_hip.so")
+HOST_LIB = os.path.join(HERE, "libdfsph_host.so")
+
+DFSPH_OK = 0
+DFSPH_ERR_HIP = 1
+DFSPH_ERR_ARG = 2
+DFSPH_ERR_CAPACITY = 3
+DFSPH_ERR_NEIGHBOR_OVERFLOW = 4
+DFSPH_ERR_STATE = 5
+DFSPH_ERR_NO_MODEL = 6
+
+WARMSTART_REFERENCE = 0
+WARMSTART_LEARNED = 1
+WARMSTART_NONE = 2
+
+FIELD_POS = 0
+FIELD_VEL = 1
+FIELD_MASS = 2
+FIELD_DENSITY = 3
+FIELD_FACTOR = 4
+FIELD_DENSITY_ADV = 5
+FIELD_KAPPA = 6
+FIELD_KAPPAV = 7
+FIELD_ACC = 8
+FIELD_ID = 9
+FIELD_NBR_FLUID = 10
+FIELD_NBR_BOUNDARY = 11
+FIELD_BOUNDARY_POS = 12
+FIELD_BOUNDARY_PSI = 13
+_VECTOR_FIELDS = {FIELD_POS, FIELD_VEL, FIELD_ACC, FIELD_BOUNDARY_POS}
+_UINT_FIELDS = {FIELD_ID, FIELD_NBR_FLUID, FIELD_NBR_BOUNDARY}
+
+
+class DFSPHParams(C.Structure):
+    _fields_ = [
+        ("particle_radius", C.c_double),
+        ("density0", C.c_double),
+        ("viscosity", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("dt", C.c_double),
+        ("max_iter_v", C.c_uint32),
+        ("max_iter", C.c_uint32),
+        ("min_iter_v", C.c_uint32),
+        ("min_iter", C.c_uint32),
+        ("max_error_v", C.c_double),
+        ("max_error", C.c_double),
+        ("use_cfl", C.c_int32),
+        ("cfl_factor", C.c_double),
+        ("dt_min", C.c_double),
+        ("dt_max", C.c_double),
+        ("enable_divergence", C.c_int32),
+        ("check_positions_mode", C.c_int32),
+        ("max_neighbors", C.c_uint32),
+        ("fluid_capacity", C.c_uint32),
+        ("warm_start_mode", C.c_int32),
+        ("check_interval", C.c_uint32),
+    ]
+
+
+class DFSPHStepStats(C.Structure):
+    _fields_ = [
+        ("iters_v", C.c_uint32),
+        ("iters", C.c_uint32),
+        ("err_v", C.c_double),
+        ("err", C.c_double),
+        ("dt", C.c_double),
+        ("time", C.c_double),
+        ("n_fluid", C.c_uint32),
+        ("removed", C.c_uint32),
+        ("max_neighbors_seen", C.c_uint32),
+        ("overflow", C.c_uint32),
+    ]
+
+
+class DFSPHInfo(C.Structure):
+    _fields_ = [
+        ("n_fluid", C.c_uint32),
+        ("n_boundary", C.c_uint32),
+        ("fluid_capacity", C.c_uint32),
+        ("max_neighbors", C.c_uint32),
+        ("grid_dim", C.c_uint32 * 3),
+        ("grid_origin", C.c_double * 3),
+        ("cell_size", C.c_double),
+        ("h", C.c_double),
+        ("h_future", C.c_double),
+        ("h_past", C.c_double),
+        ("time", C.c_double),
+        ("bbox_min", C.c_double * 3),
+        ("bbox_max", C.c_double * 3),
+        ("max_neighbors_seen", C.c_uint32),
+        ("overflow", C.c_uint32),
+        ("escaped", C.c_uint32),
+        ("pad", C.c_uint32),
+    ]
+
+
+class DFSPHCudaStats(C.Structure):
+    _fields_ = [
+        ("iters_v", C.c_uint32),
+        ("iters", C.c_uint32),
+        ("err_v", C.c_double),
+        ("err", C.c_double),
+        ("h", C.c_double),
+        ("time", C.c_double),
+        ("n_fluid", C.c_uint32),
+        ("removed", C.c_int32),
+    ]
+
+
+# every function declared in include/dfsph_hip.h, with its ctypes signature
+_P = C.c_void_p
+_F32P = C.POINTER(C.c_float)
+_U32P = C.POINTER(C.c_uint32)
+_U8P = C.POINTER(C.c_uint8)
+_D = C.c_double
+_U32 = C.c_uint32
+_I = C.c_int
+
+HIP_SIGNATURES = {
+    "dfsph_default_params": (None, [C.POINTER(DFSPHParams)]),
+    "dfsph_last_error": (C.c_char_p, []),
+    "dfsph_create": (_I, [C.POINTER(DFSPHParams), C.POINTER(C.c_void_p)]),
+    "dfsph_destroy": (_I, [_P]),
+    "dfsph_upload_boundary": (_I, [_P, _F32P, _F32P, _U32]),
+    "dfsph_upload_fluid": (_I, [_P, _F32P, _F32P, _F32P, _U32P, _U32]),
+    "dfsph_neighbor_search": (_I, [_P, _I]),
+    "dfsph_compute_density_factor": (_I, [_P]),
+    "dfsph_divergence_solve": (_I, [_P, _U32, _D, _U32P, C.POINTER(_D)]),
+    "dfsph_external_forces": (_I, [_P]),
+    "dfsph_cfl": (_I, [_P, _D, _D, _D, C.POINTER(_D)]),
+    "dfsph_update_time_step": (_I, [_P, _D]),
+    "dfsph_update_vel": (_I, [_P]),
+    "dfsph_pressure_solve": (_I, [_P, _U32, _D, _U32P, C.POINTER(_D)]),
+    "dfsph_update_pos": (_I, [_P]),
+    "dfsph_step_end": (_I, [_P]),
+    "dfsph_set_time": (_I, [_P, _D, _D, _D]),
+    "dfsph_check_particles_positions": (_I, [_P, _I, _U32P]),
+    "dfsph_remove_particles": (_I, [_P, _U8P, _U32P]),
+    "dfsph_add_particles": (_I, [_P, _F32P, _F32P, _F32P, _U32P, _U32]),
+    "dfsph_step": (_I, [_P, C.POINTER(DFSPHStepStats)]),
+    "dfsph_download": (_I, [_P, _I, _P, _U32]),
+    "dfsph_upload_field": (_I, [_P, _I, _P, _U32]),
+    "dfsph_get_neighbors": (_I, [_P, _U32P, _U32P, _U32P, _U32]),
+    "dfsph_get_info": (_I, [_P, C.POINTER(DFSPHInfo)]),
+    "dfsph_synchronize": (_I, [_P]),
+}
+
+HOST_SIGNATURES = {
+    "dfsph_cuda_new": (C.c_void_p, [C.POINTER(DFSPHParams), _F32P, _U32, _F32P, _F32P, _U32, C.c_char_p, _U32]),
+    "dfsph_cuda_delete": (None, [_P]),
+    "dfsph_cuda_step": (_I, [_P, C.POINTER(DFSPHCudaStats)]),
+    "dfsph_cuda_reset": (_I, [_P]),
+    "dfsph_cuda_set_divergence": (None, [_P, _I]),
+    "dfsph_cuda_handle": (C.c_void_p, [_P]),
+    "dfsph_cuda_save": (_I, [_P, C.c_char_p]),
+    "dfsph_cuda_load": (_I, [_P, C.c_char_p]),
+    "dfsph_cuda_last_error": (C.c_char_p, []),
+}
+
+_hip = None
+_host = None
+
+
+def _bind(lib, sigs):
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def load_hip() -> C.CDLL:
+    """Load libdfsph_hip.so (raises if it has not been built)."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB):
+            raise RuntimeError("libdfsph_hip.so is missing: run `python -m splishsplash_for_cuda_amd.build`")
+        lib = C.CDLL(HIP_LIB)
+        _bind(lib, HIP_SIGNATURES)
+        _hip = lib
+    return _hip
+
+
+def load_host() -> C.CDLL:
+    global _host
+    if _host is None:
+        load_hip()
+        if not os.path.exists(HOST_LIB):
+            raise RuntimeError("libdfsph_host.so is missing: run `python -m splishsplash_for_cuda_amd.build`")
+        lib = C.CDLL(HOST_LIB)
+        _bind(lib, HOST_SIGNATURES)
+        _host = lib
+    return _host
+
+
+class DFSPHError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("DFSPH error %d: %s" % (code, msg))
+        self.code = code
+
+
+def _check(rc: int, what: str = ""):
+    if rc != DFSPH_OK:
+        msg = load_hip().dfsph_last_error().decode(errors="replace")
+        raise DFSPHError(rc, (what + ": " if what else "") + msg)
+
+
+def _f32(a):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a
+
+
+def _ptr(a, ct):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+def default_params(**kw) -> DFSPHParams:
+    p = DFSPHParams()
+    load_hip().dfsph_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "gravity":
+            p.gravity = (C.c_double * 3)(*v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Solver:
+    """One DFSPH simulation on the GPU (a libdfsph_hip.so handle).
+
+    Methods map one to one onto the C ABI, which mirrors the reference's
+    cuda_* functions (DFSPH_core_cuda.h:21-83)."""
+
+    def __init__(self, params: DFSPHParams | None = None, **kw):
+        self.lib = load_hip()
+        self.params = params if params is not None else default_params(**kw)
+        h = C.c_void_p()
+        _check(self.lib.dfsph_create(C.byref(self.params), C.byref(h)), "dfsph_create")
+        self.h = h
+        self._owns = True
+
+    def close(self):
+        if getattr(self, "h", None) and getattr(self, "_owns", False):
+            self.lib.dfsph_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- data in ----
+    def upload_boundary(self, xyz, psi=None):
+        x = _f32(xyz).reshape(-1)
+        p = _f32(psi)
+        _check(self.lib.dfsph_upload_boundary(self.h, _ptr(x, C.c_float), _ptr(p, C.c_float), x.size // 3),
+               "dfsph_upload_boundary")
+
+    def upload_fluid(self, xyz, vel=None, mass=None, ids=None):
+        x = _f32(xyz).reshape(-1)
+        v = None if vel is None else _f32(vel).reshape(-1)
+        m = _f32(mass)
+        i = None if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
+        _check(self.lib.dfsph_upload_fluid(self.h, _ptr(x, C.c_float), _ptr(v, C.c_float), _ptr(m, C.c_float),
+                                           _ptr(i, C.c_uint32), x.size // 3), "dfsph_upload_fluid")
+
+    def add_particles(self, xyz, vel=None, mass=None, ids=None):
+        x = _f32(xyz).reshape(-1)
+        v = None if vel is None else _f32(vel).reshape(-1)
+        m = _f32(mass)
+        i = None if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
+        _check(self.lib.dfsph_add_particles(self.h, _ptr(x, C.c_float), _ptr(v, C.c_float), _ptr(m, C.c_float),
+                                            _ptr(i, C.c_uint32), x.size // 3), "dfsph_add_particles")
+
+    def upload_field(self, field, values):
+        a = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
+        _check(self.lib.dfsph_upload_field(self.h, field, a.ctypes.data_as(C.c_void_p), a.size), "dfsph_upload_field")
+
+    # ---- the step's phases ----
+    def neighbor_search(self, sort=True):
+        _check(self.lib.dfsph_neighbor_search(self.h, 1 if sort else 0), "dfsph_neighbor_search")
+
+    def compute_density_factor(self):
+        _check(self.lib.dfsph_compute_density_factor(self.h), "dfsph_compute_density_factor")
+
+    def divergence_solve(self, max_iter=None, max_error=None):
+        it = C.c_uint32()
+        err = C.c_double()
+        _check(self.lib.dfsph_divergence_solve(self.h, self.params.max_iter_v if max_iter is None else max_iter,
+                                               self.params.max_error_v if max_error is None else max_error,
+                                               C.byref(it), C.byref(err)), "dfsph_divergence_solve")
+        return it.value, err.value
+
+    def external_forces(self):
+        _check(self.lib.dfsph_external_forces(self.h), "dfsph_external_forces")
+
+    def cfl(self, min_dt=None, cfl_factor=None, max_dt=None):
+        out = C.c_double()
+        p = self.params
+        _check(self.lib.dfsph_cfl(self.h, p.dt_min if min_dt is None else min_dt,
+                                  p.cfl_factor if cfl_factor is None else cfl_factor,
+                                  p.dt_max if max_dt is None else max_dt, C.byref(out)), "dfsph_cfl")
+        return out.value
+
+    def update_time_step(self, h_future):
+        _check(self.lib.dfsph_update_time_step(self.h, h_future), "dfsph_update_time_step")
+
+    def update_vel(self):
+        _check(self.lib.dfsph_update_vel(self.h), "dfsph_update_vel")
+
+    def pressure_solve(self, max_iter=None, max_error=None):
+        it = C.c_uint32()
+        err = C.c_double()
+        _check(self.lib.dfsph_pressure_solve(self.h, self.params.max_iter if max_iter is None else max_iter,
+                                             self.params.max_error if max_error is None else max_error,
+                                             C.byref(it), C.byref(err)), "dfsph_pressure_solve")
+        return it.value, err.value
+
+    def update_pos(self):
+        _check(self.lib.dfsph_update_pos(self.h), "dfsph_update_pos")
+
+    def step_end(self):
+        _check(self.lib.dfsph_step_end(self.h), "dfsph_step_end")
+
+    def set_time(self, h, h_past, time):
+        _check(self.lib.dfsph_set_time(self.h, h, h_past, time), "dfsph_set_time")
+
+    def check_particles_positions(self, mode=0):
+        c = C.c_uint32()
+        _check(self.lib.dfsph_check_particles_positions(self.h, mode, C.byref(c)), "dfsph_check_particles_positions")
+        return c.value
+
+    def remove_particles(self, keep):
+        k = np.ascontiguousarray(keep, dtype=np.uint8)
+        n = C.c_uint32()
+        _check(self.lib.dfsph_remove_particles(self.h, _ptr(k, C.c_uint8), C.byref(n)), "dfsph_remove_particles")
+        return n.value
+
+    def step(self, raise_on_overflow=True):
+        st = DFSPHStepStats()
+        rc = self.lib.dfsph_step(self.h, C.byref(st))
+        if rc != DFSPH_OK and (raise_on_overflow or rc != DFSPH_ERR_NEIGHBOR_OVERFLOW):
+            _check(rc, "dfsph_step")
+        return st
+
+    def synchronize(self):
+        _check(self.lib.dfsph_synchronize(self.h), "dfsph_synchronize")
+
+    # ---- data out ----
+    def info(self) -> DFSPHInfo:
+        inf = DFSPHInfo()
+        _check(self.lib.dfsph_get_info(self.h, C.byref(inf)), "dfsph_get_info")
+        return inf
+
+    @property
+    def n(self) -> int:
+        return self.info().n_fluid
+
+    def download(self, field):
+        inf = self.info()
+        n = inf.n_boundary if field in (FIELD_BOUNDARY_POS, FIELD_BOUNDARY_PSI) else inf.n_fluid
+        cnt = 3 * n if field in _VECTOR_FIELDS else n
+        out = np.empty(cnt, dtype=np.uint32 if field in _UINT_FIELDS else np.float32)
+        _check(self.lib.dfsph_download(self.h, field, out.ctypes.data_as(C.c_void_p), cnt), "dfsph_download")
+        return out.reshape(-1, 3) if field in _VECTOR_FIELDS else out
+
+    def neighbors(self):
+        """(n_fluid, n_boundary, list[n, max]) in device order."""
+        inf = self.info()
+        n, mx = inf.n_fluid, inf.max_neighbors
+        nf = np.zeros(n, np.uint32)
+        nb = np.zeros(n, np.uint32)
+        L = np.zeros((n, mx), np.uint32)
+        _check(self.lib.dfsph_get_neighbors(self.h, _ptr(nf, C.c_uint32), _ptr(nb, C.c_uint32),
+                                            _ptr(L, C.c_uint32), mx), "dfsph_get_neighbors")
+        return nf, nb, L
+
+    def state_by_id(self):
+        """Fluid fields reordered by particle id (dict of numpy arrays)."""
+        ids = self.download(FIELD_ID)
+        order = np.argsort(ids, kind="stable")
+        out = {"id": ids[order]}
+        for name, f in (("pos", FIELD_POS), ("vel", FIELD_VEL), ("mass", FIELD_MASS), ("density", FIELD_DENSITY),
+                        ("factor", FIELD_FACTOR), ("density_adv", FIELD_DENSITY_ADV), ("kappa", FIELD_KAPPA),
+                        ("kappaV", FIELD_KAPPAV), ("acc", FIELD_ACC)):
+            out[name] = self.download(f)[order]
+        return out
+
+
+class HostSolver:
+    """SPH::DFSPHCUDA (the C++ host class) through libdfsph_host.so."""
+
+    def __init__(self, params: DFSPHParams, boundary_xyz, fluid_xyz, fluid_vel=None):
+        self.lib = load_host()
+        self.params = params
+        b = _f32(boundary_xyz).reshape(-1)
+        f = _f32(fluid_xyz).reshape(-1)
+        v = None if fluid_vel is None else _f32(fluid_vel).reshape(-1)
+        err = C.create_string_buffer(512)
+        s = self.lib.dfsph_cuda_new(C.byref(params), _ptr(b, C.c_float), b.size // 3, _ptr(f, C.c_float),
+                                    _ptr(v, C.c_float), f.size // 3, err, 512)
+        if not s:
+            raise DFSPHError(DFSPH_ERR_STATE, err.value.decode(errors="replace"))
+        self.s = s
+        # a Solver view over the handle owned by the C++ object (not owning)
+        self.view = Solver.__new__(Solver)
+        self.view.lib = load_hip()
+        self.view.params = params
+        self.view.h = C.c_void_p(self.lib.dfsph_cuda_handle(s))
+        self.view._owns = False
+
+    def step(self) -> DFSPHCudaStats:
+        st = DFSPHCudaStats()
+        rc = self.lib.dfsph_cuda_step(self.s, C.byref(st))
+        if rc != DFSPH_OK:
+            raise DFSPHError(rc, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
+        return st
+
+    def save(self, path):
+        rc = self.lib.dfsph_cuda_save(self.s, path.encode())
+        if rc != DFSPH_OK:
+            raise DFSPHError(rc, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
+
+    def load(self, path):
+        rc = self.lib.dfsph_cuda_load(self.s, path.encode())
+        if rc != DFSPH_OK:
+            raise DFSPHError(rc, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
+
+    def set_divergence(self, enable: bool):
+        self.lib.dfsph_cuda_set_divergence(self.s, 1 if enable else 0)
+
+    def close(self):
+        if getattr(self, "s", None):
+            self.view.h = None
+            self.lib.dfsph_cuda_delete(self.s)
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,331 @@
+// DFSPH_CUDA.cpp -- C++ host classes over the C ABI of libdfsph_hip.so.
+// Built with g++ (no HIP header): every GPU action goes through dfsph_hip.h.
+#include "DFSPH_CUDA.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+
+#include "dfsph_host.h"
+
+namespace SPH {
+
+namespace {
+
+void check(int rc, const char* what)
+{
+    if (rc != DFSPH_OK) throw DFSPHError(rc, std::string(what) + ": " + dfsph_last_error());
+}
+
+constexpr uint32_t kSnapMagic = 0x48505344u;   // "DSPH"
+constexpr uint32_t kSnapVersion = 1;
+
+}  // namespace
+
+// ---------------------------------------------------------------- DFSPHCData
+
+DFSPHCData::DFSPHCData(const dfsph_params& p, const ParticleArrays& boundary, const ParticleArrays& fluid)
+    : params(p)
+{
+    check(dfsph_create(&params, &m_handle), "dfsph_create");
+    try {
+        check(dfsph_upload_boundary(m_handle, boundary.pos.data(), boundary.mass.empty() ? nullptr : boundary.mass.data(),
+                                    boundary.size()),
+              "dfsph_upload_boundary");
+        reset(fluid);
+    } catch (...) {
+        dfsph_destroy(m_handle);
+        m_handle = nullptr;
+        throw;
+    }
+}
+
+DFSPHCData::~DFSPHCData()
+{
+    if (m_handle) dfsph_destroy(m_handle);
+}
+
+void DFSPHCData::reset(const ParticleArrays& fluid)
+{
+    const uint32_t n = fluid.size();
+    check(dfsph_upload_fluid(m_handle, fluid.pos.data(), fluid.vel.empty() ? nullptr : fluid.vel.data(),
+                             fluid.mass.empty() ? nullptr : fluid.mass.data(), fluid.id.empty() ? nullptr : fluid.id.data(),
+                             n),
+          "dfsph_upload_fluid");
+    h = h_future = h_past = params.dt;
+    time = 0.0;
+}
+
+void DFSPHCData::updateTimeStep(double h_fut)
+{
+    check(dfsph_update_time_step(m_handle, h_fut), "dfsph_update_time_step");
+    h_future = h_fut;
+}
+
+void DFSPHCData::onSimulationStepEnd()
+{
+    check(dfsph_step_end(m_handle), "dfsph_step_end");
+    h_past = h;
+    h = h_future;
+    time += h;
+}
+
+int DFSPHCData::checkParticlesPositions(int mode, bool report)
+{
+    uint32_t count = 0;
+    check(dfsph_check_particles_positions(m_handle, mode == 2 ? 2 : 0, &count), "dfsph_check_particles_positions");
+    if (report && count)
+        std::fprintf(stderr, "checkParticlesPositions: %u particles outside the bounding box%s\n", count,
+                     mode == 2 ? " (removed)" : "");
+    return (int)count;
+}
+
+void DFSPHCData::zeroFluidVelocities()
+{
+    const uint32_t n = (uint32_t)getFluidParticlesCount();
+    std::vector<float> z(3 * (size_t)n, 0.0f);
+    check(dfsph_upload_field(m_handle, DFSPH_FIELD_VEL, z.data(), 3 * n), "dfsph_upload_field");
+}
+
+int DFSPHCData::getFluidParticlesCount() const
+{
+    dfsph_info info;
+    check(dfsph_get_info(m_handle, &info), "dfsph_get_info");
+    return (int)info.n_fluid;
+}
+
+int DFSPHCData::getFluidParticlesCountMax() const
+{
+    dfsph_info info;
+    check(dfsph_get_info(m_handle, &info), "dfsph_get_info");
+    return (int)info.fluid_capacity;
+}
+
+std::vector<float> DFSPHCData::readField(int field) const
+{
+    const uint32_t n = (uint32_t)getFluidParticlesCount();
+    const bool vec = field == DFSPH_FIELD_POS || field == DFSPH_FIELD_VEL || field == DFSPH_FIELD_ACC;
+    std::vector<float> out((vec ? 3 : 1) * (size_t)n);
+    check(dfsph_download(m_handle, field, out.data(), (uint32_t)out.size()), "dfsph_download");
+    return out;
+}
+
+ParticleArrays DFSPHCData::readFluid() const
+{
+    const uint32_t n = (uint32_t)getFluidParticlesCount();
+    std::vector<float> x = readField(DFSPH_FIELD_POS), v = readField(DFSPH_FIELD_VEL), m = readField(DFSPH_FIELD_MASS);
+    std::vector<uint32_t> id(n);
+    check(dfsph_download(m_handle, DFSPH_FIELD_ID, id.data(), n), "dfsph_download");
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return id[a] < id[b]; });
+    ParticleArrays out;
+    out.pos.resize(3 * (size_t)n);
+    out.vel.resize(3 * (size_t)n);
+    out.mass.resize(n);
+    out.id.resize(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t i = order[k];
+        for (int a = 0; a < 3; ++a) {
+            out.pos[3 * k + a] = x[3 * i + a];
+            out.vel[3 * k + a] = v[3 * i + a];
+        }
+        out.mass[k] = m[i];
+        out.id[k] = id[i];
+    }
+    return out;
+}
+
+// ------------------------------------------------------------------ DFSPHCUDA
+
+DFSPHCUDA::DFSPHCUDA(const dfsph_params& p, const ParticleArrays& boundary, const ParticleArrays& fluid)
+    : m_maxError(p.max_error), m_maxIterations(p.max_iter), m_maxErrorV(p.max_error_v),
+      m_maxIterationsV(p.max_iter_v), m_data(p, boundary, fluid), m_initialFluid(fluid),
+      m_enableDivergenceSolver(p.enable_divergence != 0)
+{
+}
+
+DFSPHCUDA::~DFSPHCUDA() = default;
+
+// The sequence of DFSPHCUDA::step (DFSPH_CUDA.cpp:197-456): neighbor search
+// with sort, divergence solve, external forces, (CFL,) time-step update,
+// velocity update, density solve, position update, step end, removal of the
+// particles that left the bounding box.
+void DFSPHCUDA::step()
+{
+    void* h = m_data.handle();
+    check(dfsph_neighbor_search(h, 1), "dfsph_neighbor_search");
+    if (m_enableDivergenceSolver) {
+        check(dfsph_divergence_solve(h, m_maxIterationsV, m_maxErrorV, &m_iterationsV, &m_divergenceError),
+              "dfsph_divergence_solve");
+    } else {
+        m_iterationsV = 0;
+        m_divergenceError = 0.0;
+        check(dfsph_compute_density_factor(h), "dfsph_compute_density_factor");
+    }
+    check(dfsph_external_forces(h), "dfsph_external_forces");
+    if (m_data.params.use_cfl) {
+        double hf = 0.0;
+        check(dfsph_cfl(h, m_data.params.dt_min, m_data.params.cfl_factor, m_data.params.dt_max, &hf), "dfsph_cfl");
+        m_data.h_future = hf;
+    } else {
+        m_data.updateTimeStep(m_data.h);
+    }
+    check(dfsph_update_vel(h), "dfsph_update_vel");
+    check(dfsph_pressure_solve(h, m_maxIterations, m_maxError, &m_iterations, &m_densityError), "dfsph_pressure_solve");
+    check(dfsph_update_pos(h), "dfsph_update_pos");
+    m_data.onSimulationStepEnd();
+    m_removedLastStep = 0;
+    if (m_data.params.check_positions_mode >= 0)
+        m_removedLastStep = m_data.checkParticlesPositions(m_data.params.check_positions_mode, false);
+    count_steps++;
+}
+
+void DFSPHCUDA::reset()
+{
+    m_iterations = 0;
+    m_iterationsV = 0;
+    count_steps = 0;
+    m_data.reset(m_initialFluid);
+}
+
+void DFSPHCUDA::handleSimulationSave(const std::string& path) const
+{
+    void* h = m_data.handle();
+    const uint32_t n = (uint32_t)m_data.getFluidParticlesCount();
+    std::vector<float> x = m_data.readField(DFSPH_FIELD_POS), v = m_data.readField(DFSPH_FIELD_VEL),
+                       m = m_data.readField(DFSPH_FIELD_MASS), k = m_data.readField(DFSPH_FIELD_KAPPA),
+                       kv = m_data.readField(DFSPH_FIELD_KAPPAV);
+    std::vector<uint32_t> id(n);
+    check(dfsph_download(h, DFSPH_FIELD_ID, id.data(), n), "dfsph_download");
+    dfsph_info info;
+    check(dfsph_get_info(h, &info), "dfsph_get_info");
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw DFSPHError(DFSPH_ERR_ARG, "cannot open " + path);
+    const uint32_t hdr[4] = {kSnapMagic, kSnapVersion, n, (uint32_t)count_steps};
+    const double tm[3] = {info.h, info.h_past, info.time};
+    f.write(reinterpret_cast<const char*>(hdr), sizeof hdr);
+    f.write(reinterpret_cast<const char*>(tm), sizeof tm);
+    f.write(reinterpret_cast<const char*>(x.data()), sizeof(float) * x.size());
+    f.write(reinterpret_cast<const char*>(v.data()), sizeof(float) * v.size());
+    f.write(reinterpret_cast<const char*>(m.data()), sizeof(float) * m.size());
+    f.write(reinterpret_cast<const char*>(k.data()), sizeof(float) * k.size());
+    f.write(reinterpret_cast<const char*>(kv.data()), sizeof(float) * kv.size());
+    f.write(reinterpret_cast<const char*>(id.data()), sizeof(uint32_t) * id.size());
+    if (!f) throw DFSPHError(DFSPH_ERR_ARG, "write failed: " + path);
+}
+
+void DFSPHCUDA::handleSimulationLoad(const std::string& path)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw DFSPHError(DFSPH_ERR_ARG, "cannot open " + path);
+    uint32_t hdr[4];
+    double tm[3];
+    f.read(reinterpret_cast<char*>(hdr), sizeof hdr);
+    f.read(reinterpret_cast<char*>(tm), sizeof tm);
+    if (!f || hdr[0] != kSnapMagic || hdr[1] != kSnapVersion) throw DFSPHError(DFSPH_ERR_ARG, "not a snapshot: " + path);
+    const uint32_t n = hdr[2];
+    ParticleArrays p;
+    p.pos.resize(3 * (size_t)n);
+    p.vel.resize(3 * (size_t)n);
+    p.mass.resize(n);
+    p.id.resize(n);
+    std::vector<float> k(n), kv(n);
+    f.read(reinterpret_cast<char*>(p.pos.data()), sizeof(float) * p.pos.size());
+    f.read(reinterpret_cast<char*>(p.vel.data()), sizeof(float) * p.vel.size());
+    f.read(reinterpret_cast<char*>(p.mass.data()), sizeof(float) * p.mass.size());
+    f.read(reinterpret_cast<char*>(k.data()), sizeof(float) * n);
+    f.read(reinterpret_cast<char*>(kv.data()), sizeof(float) * n);
+    f.read(reinterpret_cast<char*>(p.id.data()), sizeof(uint32_t) * n);
+    if (!f) throw DFSPHError(DFSPH_ERR_ARG, "truncated snapshot: " + path);
+    m_data.reset(p);
+    void* h = m_data.handle();
+    check(dfsph_upload_field(h, DFSPH_FIELD_KAPPA, k.data(), n), "dfsph_upload_field");
+    check(dfsph_upload_field(h, DFSPH_FIELD_KAPPAV, kv.data(), n), "dfsph_upload_field");
+    check(dfsph_set_time(h, tm[0], tm[1], tm[2]), "dfsph_set_time");
+    m_data.h = m_data.h_future = tm[0];
+    m_data.h_past = tm[1];
+    m_data.time = tm[2];
+    count_steps = (int)hdr[3];
+}
+
+}  // namespace SPH
+
+// ---------------------------------------------------------------- C entry points
+
+namespace {
+thread_local std::string g_host_err;
+
+SPH::DFSPHCUDA* S(void* s) { return static_cast<SPH::DFSPHCUDA*>(s); }
+
+template <class F>
+int guarded(F f)
+{
+    try {
+        f();
+        return DFSPH_OK;
+    } catch (const SPH::DFSPHError& e) {
+        g_host_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_host_err = e.what();
+        return DFSPH_ERR_STATE;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+void* dfsph_cuda_new(const dfsph_params* p, const float* bxyz, uint32_t nb, const float* fxyz, const float* fvel,
+                     uint32_t nf, char* err, uint32_t err_len)
+{
+    try {
+        SPH::ParticleArrays b, f;
+        b.pos.assign(bxyz, bxyz + 3 * (size_t)nb);
+        f.pos.assign(fxyz, fxyz + 3 * (size_t)nf);
+        if (fvel) f.vel.assign(fvel, fvel + 3 * (size_t)nf);
+        return new SPH::DFSPHCUDA(*p, b, f);
+    } catch (const std::exception& e) {
+        g_host_err = e.what();
+        if (err && err_len) {
+            std::strncpy(err, e.what(), err_len - 1);
+            err[err_len - 1] = 0;
+        }
+        return nullptr;
+    }
+}
+
+void dfsph_cuda_delete(void* s) { delete S(s); }
+
+int dfsph_cuda_step(void* s, dfsph_cuda_stats* st)
+{
+    return guarded([&] {
+        SPH::DFSPHCUDA* d = S(s);
+        d->step();
+        if (st) {
+            st->iters_v = d->getIterationsV();
+            st->iters = d->getIterations();
+            st->err_v = d->getDivergenceError();
+            st->err = d->getDensityError();
+            st->h = d->data().h_past;
+            st->time = d->data().time;
+            st->n_fluid = (uint32_t)d->data().getFluidParticlesCount();
+            st->removed = d->getRemovedLastStep();
+        }
+    });
+}
+
+int dfsph_cuda_reset(void* s) { return guarded([&] { S(s)->reset(); }); }
+
+void dfsph_cuda_set_divergence(void* s, int enable) { S(s)->setEnableDivergenceSolver(enable != 0); }
+
+void* dfsph_cuda_handle(void* s) { return S(s)->data().handle(); }
+
+int dfsph_cuda_save(void* s, const char* path) { return guarded([&] { S(s)->handleSimulationSave(path); }); }
+
+int dfsph_cuda_load(void* s, const char* path) { return guarded([&] { S(s)->handleSimulationLoad(path); }); }
+
+const char* dfsph_cuda_last_error(void) { return g_host_err.c_str(); }
+
+}  // extern "C"

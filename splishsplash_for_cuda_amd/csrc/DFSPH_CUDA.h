@@ -1,0 +1,126 @@
+// DFSPH_CUDA.h -- C++ host classes of the MI355X DFSPH solver.
+//
+// Keeps the host API of the reference's GPU path: SPH::DFSPHCData (time-step
+// state and particle sets, DFSPH_c_arrays_structure.h:297-496) and
+// SPH::DFSPHCUDA with step() / reset() and the iteration and tolerance
+// members of TimeStep (DFSPH_CUDA.h:26-154).  The classes own a handle of
+// libdfsph_hip.so and call the GPU only through its C ABI
+// (include/dfsph_hip.h); this file and DFSPH_CUDA.cpp include no HIP
+// header and build with a plain C++ compiler.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dfsph_hip.h"
+
+namespace SPH {
+
+// Error raised when a C ABI call fails (the reference exits the process,
+// DFSPH_macro_cuda.h:21-29).
+class DFSPHError : public std::runtime_error {
+public:
+    DFSPHError(int code, const std::string& what) : std::runtime_error(what), code(code) {}
+    int code;
+};
+
+// Host-side particle arrays in the interleaved xyz layout of the C ABI.
+struct ParticleArrays {
+    std::vector<float> pos;     // 3n
+    std::vector<float> vel;     // 3n (may be empty: zero)
+    std::vector<float> mass;    // n  (may be empty: 0.8 (2r)^3 rho0)
+    std::vector<uint32_t> id;   // n  (may be empty: 0..n-1)
+    uint32_t size() const { return (uint32_t)(pos.size() / 3); }
+};
+
+class DFSPHCData {
+public:
+    // DFSPHCData(FluidModel*) (DFSPH_c_arrays_structure.cpp:852-944): the
+    // boundary set is uploaded first (it fixes the grid, initGridOffset
+    // :1050-1098), then the fluid.
+    DFSPHCData(const dfsph_params& params, const ParticleArrays& boundary, const ParticleArrays& fluid);
+    ~DFSPHCData();
+    DFSPHCData(const DFSPHCData&) = delete;
+    DFSPHCData& operator=(const DFSPHCData&) = delete;
+
+    // DFSPHCData::reset (:975-1048): reload the fluid, reset the step history.
+    void reset(const ParticleArrays& fluid);
+    // DFSPHCData::updateTimeStep / onSimulationStepEnd (:1100-1118)
+    void updateTimeStep(double h_fut);
+    void onSimulationStepEnd();
+    // DFSPHCData::checkParticlesPositions (DFSPH_c_arrays_structure.h:486-492):
+    // 0 report, 2 remove; returns the number of particles outside the box.
+    int checkParticlesPositions(int mode = 0, bool report = true);
+    // DFSPHCData::zeroFluidVelocities
+    void zeroFluidVelocities();
+
+    int getFluidParticlesCount() const;
+    int getFluidParticlesCountMax() const;
+    double get_current_timestep() const { return h; }
+    double getSimulationTime() const { return time; }
+
+    // read_UnifiedParticleSet_cuda: fluid state reordered by particle id
+    ParticleArrays readFluid() const;
+    std::vector<float> readField(int field) const;   // device order
+
+    void* handle() const { return m_handle; }
+
+    // mirrors of the device time-step state
+    double h = 0.0, h_future = 0.0, h_past = 0.0, time = 0.0;
+    dfsph_params params;
+
+private:
+    void* m_handle = nullptr;
+};
+
+class DFSPHCUDA {
+public:
+    DFSPHCUDA(const dfsph_params& params, const ParticleArrays& boundary, const ParticleArrays& fluid);
+    virtual ~DFSPHCUDA();
+
+    // DFSPHCUDA::step (DFSPH_CUDA.cpp:82-794)
+    virtual void step();
+    // DFSPHCUDA::reset (DFSPH_CUDA.cpp:796-817)
+    virtual void reset();
+
+    bool getEnableDivergenceSolver() const { return m_enableDivergenceSolver; }
+    void setEnableDivergenceSolver(bool val) { m_enableDivergenceSolver = val; }
+
+    unsigned int getIterations() const { return m_iterations; }
+    unsigned int getIterationsV() const { return m_iterationsV; }
+    double getDensityError() const { return m_densityError; }
+    double getDivergenceError() const { return m_divergenceError; }
+    int getRemovedLastStep() const { return m_removedLastStep; }
+
+    // DFSPHCUDA::zeroFluidVelocities
+    void zeroFluidVelocities() { m_data.zeroFluidVelocities(); }
+    // DFSPHCUDA::handleSimulationSave / handleSimulationLoad (DFSPH_CUDA.cpp:1887-1979):
+    // binary snapshot with positions, velocities, masses, ids, kappa, kappaV and
+    // the step history, so that a resumed run continues bit for bit.
+    void handleSimulationSave(const std::string& path) const;
+    void handleSimulationLoad(const std::string& path);
+
+    DFSPHCData& data() { return m_data; }
+    const DFSPHCData& data() const { return m_data; }
+
+    // tolerances and caps (TimeStep members, DFSPH_CUDA.h:43-49)
+    double m_maxError;
+    unsigned int m_maxIterations;
+    double m_maxErrorV;
+    unsigned int m_maxIterationsV;
+
+protected:
+    DFSPHCData m_data;
+    ParticleArrays m_initialFluid;
+    unsigned int m_iterations = 0;
+    unsigned int m_iterationsV = 0;
+    double m_densityError = 0.0;
+    double m_divergenceError = 0.0;
+    int m_removedLastStep = 0;
+    bool m_enableDivergenceSolver = true;
+    int count_steps = 0;
+};
+
+}  // namespace SPH

@@ -1,0 +1,919 @@
+// dfsph_capi.hip -- the C ABI of libdfsph_hip.so (declared in include/dfsph_hip.h).
+//
+// A handle owns every device buffer of one simulation (the role of
+// SPH::DFSPHCData and its UnifiedParticleSet members, DFSPH_c_arrays_structure.h:96-496)
+// and one HIP stream.  Convergence of the Jacobi loops is decided on the
+// device (FP64 reductions, dfsph_kernels.hip); the host queues iterations in
+// batches and reads the loop state back asynchronously, instead of the
+// reference's blocking device-to-host copy per iteration (DFSPH_core_cuda.cu:779,1325).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dfsph_hip.h"
+#include "dfsph_launch.hpp"
+#include "dfsph_mlp_api.hpp"
+
+using dfsph::DevCtl;
+using dfsph::DevTime;
+using dfsph::SimConst;
+using dfsph::SolveCtl;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return fail(DFSPH_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHK(x)                   \
+    do {                         \
+        int r_ = (x);            \
+        if (r_ != DFSPH_OK) return r_; \
+    } while (0)
+
+constexpr double kCellMargin = 1.001;   // cell edge = 1.001 h_k (conservative binning)
+
+struct Handle {
+    dfsph_params p{};
+    hipStream_t s = nullptr;
+    SimConst c{};
+    bool grid_ready = false;
+    uint32_t n = 0, nb = 0, cap = 0, tmp_cap = 0;
+    uint32_t ncell_alloc = 0;
+
+    // fluid (double-buffered persistent fields)
+    float4* pos[2] = {nullptr, nullptr};
+    float* kap[2] = {nullptr, nullptr};
+    float* kapv[2] = {nullptr, nullptr};
+    uint32_t* id[2] = {nullptr, nullptr};
+    int pc = 0;                       // current buffer of pos / kappa / kappaV / id
+    float4* vel[2] = {nullptr, nullptr};
+    int vc = 0;                       // current velocity buffer
+    float *rho = nullptr, *alpha = nullptr, *radv = nullptr, *kk = nullptr;
+    float4* acc = nullptr;
+    uint32_t *cnt = nullptr, *nbr = nullptr;
+    // sort / scan scratch
+    uint32_t *key = nullptr, *slot = nullptr, *perm = nullptr, *hist = nullptr, *cstart = nullptr;
+    uint32_t *keep = nullptr, *rank = nullptr;
+    uint8_t* keep8 = nullptr;
+    uint32_t* scan_tmp = nullptr;
+    size_t scan_words = 0;
+
+    // boundary (static, cell-sorted once)
+    float4* bpos = nullptr;
+    uint32_t* bcstart = nullptr;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+
+    // device-side control
+    DevCtl* ctl = nullptr;
+    DevTime* tm = nullptr;
+    double* partial = nullptr;
+    float* partial_f = nullptr;
+    uint32_t partial_n = 0;
+    DevCtl* hctl = nullptr;           // pinned
+    DevTime* htm = nullptr;           // pinned
+    SolveCtl* hslot = nullptr;        // pinned, 2 slots
+    hipEvent_t ev[2] = {nullptr, nullptr};
+
+    bool pend_vel = false;            // deferred cuda_update_vel (fused into the pressure warm start)
+    bool lists_valid = false;
+    bool acc_valid = false;
+
+    dfsph::MlpState* mlp = nullptr;   // learned warm start
+
+    ~Handle();
+};
+
+template <class T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+Handle::~Handle()
+{
+    if (s) (void)hipStreamSynchronize(s);
+    for (int b = 0; b < 2; ++b) {
+        dfree(pos[b]); dfree(kap[b]); dfree(kapv[b]); dfree(id[b]); dfree(vel[b]);
+        if (ev[b]) (void)hipEventDestroy(ev[b]);
+    }
+    dfree(rho); dfree(alpha); dfree(radv); dfree(kk); dfree(acc); dfree(cnt); dfree(nbr);
+    dfree(key); dfree(slot); dfree(perm); dfree(hist); dfree(cstart); dfree(keep); dfree(rank);
+    dfree(keep8); dfree(scan_tmp); dfree(bpos); dfree(bcstart);
+    dfree(ctl); dfree(tm); dfree(partial); dfree(partial_f);
+    if (hctl) (void)hipHostFree(hctl);
+    if (htm) (void)hipHostFree(htm);
+    if (hslot) (void)hipHostFree(hslot);
+    if (mlp) dfsph::mlp_destroy(mlp);
+    if (s) (void)hipStreamDestroy(s);
+}
+
+template <class T>
+int dalloc(T*& p, size_t count)
+{
+    dfree(p);
+    if (count == 0) count = 1;
+    HIPCHK(hipMalloc((void**)&p, count * sizeof(T)));
+    return DFSPH_OK;
+}
+
+Handle* H(void* h) { return static_cast<Handle*>(h); }
+
+int ensure_scan(Handle* h, uint32_t n)
+{
+    const size_t w = dfsph::scan_temp_words(n);
+    if (w > h->scan_words) {
+        CHK(dalloc(h->scan_tmp, w));
+        h->scan_words = w;
+    }
+    return DFSPH_OK;
+}
+
+int ensure_tmp(Handle* h, uint32_t n)
+{
+    if (n <= h->tmp_cap && h->key) return DFSPH_OK;
+    CHK(dalloc(h->key, n));
+    CHK(dalloc(h->slot, n));
+    CHK(dalloc(h->perm, n));
+    CHK(dalloc(h->keep, (size_t)n + 1));
+    CHK(dalloc(h->rank, (size_t)n + 1));
+    CHK(dalloc(h->keep8, n));
+    h->tmp_cap = n;
+    CHK(ensure_scan(h, n + 1));
+    return DFSPH_OK;
+}
+
+int ensure_partials(Handle* h, uint32_t n)
+{
+    const uint32_t g = dfsph::grid_blocks_xcd(n);
+    if (g <= h->partial_n && h->partial) return DFSPH_OK;
+    CHK(dalloc(h->partial, g));
+    CHK(dalloc(h->partial_f, g));
+    h->partial_n = g;
+    return DFSPH_OK;
+}
+
+// counting sort of n points into the cell table cs (ncell+1 entries);
+// perm[k] = index of the point at sorted slot k (stable)
+int counting_sort(Handle* h, const float4* pts, uint32_t n, uint32_t* cs)
+{
+    const uint32_t nc = h->c.ncell;
+    dfsph::launch_fill_u32(h->s, h->hist, 0u, nc + 1);
+    dfsph::launch_cell_hist(h->s, h->c, pts, n, h->key, h->slot, h->hist);
+    dfsph::launch_exclusive_scan(h->s, h->hist, cs, nc + 1, h->scan_tmp);
+    dfsph::launch_scatter_perm(h->s, h->key, h->slot, cs, n, h->perm);
+    dfsph::launch_cell_sort(h->s, cs, nc, h->perm);
+    HIPCHK(hipGetLastError());
+    return DFSPH_OK;
+}
+
+int flush_vel(Handle* h)
+{
+    if (!h->pend_vel) return DFSPH_OK;
+    dfsph::launch_update_vel(h->s, h->tm, h->n, h->acc, h->vel[h->vc], h->kap[h->pc]);
+    h->pend_vel = false;
+    HIPCHK(hipGetLastError());
+    return DFSPH_OK;
+}
+
+int sync(Handle* h)
+{
+    HIPCHK(hipStreamSynchronize(h->s));
+    return DFSPH_OK;
+}
+
+int read_ctl(Handle* h)
+{
+    HIPCHK(hipMemcpyAsync(h->hctl, h->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipMemcpyAsync(h->htm, h->tm, sizeof(DevTime), hipMemcpyDeviceToHost, h->s));
+    return sync(h);
+}
+
+int neighbor_search(Handle* h, bool sort)
+{
+    if (!h->grid_ready) return fail(DFSPH_ERR_STATE, "no boundary set uploaded");
+    CHK(flush_vel(h));
+    dfsph::launch_nbr_ctl_init(h->s, h->ctl);
+    if (h->n == 0) { h->lists_valid = true; return DFSPH_OK; }
+    CHK(counting_sort(h, h->pos[h->pc], h->n, h->cstart));
+    if (sort) {
+        const int o = h->pc ^ 1, vo = h->vc ^ 1;
+        dfsph::launch_gather_fluid(h->s, h->n, h->perm, h->pos[h->pc], h->vel[h->vc], h->kap[h->pc],
+                                   h->kapv[h->pc], h->id[h->pc], h->pos[o], h->vel[vo], h->kap[o],
+                                   h->kapv[o], h->id[o]);
+        h->pc = o;
+        h->vc = vo;
+    }
+    dfsph::launch_neighbors(h->s, h->c, !sort, h->pos[h->pc], h->cstart, h->perm, h->bpos, h->bcstart, h->n,
+                            h->nbr, h->cnt, h->ctl);
+    HIPCHK(hipGetLastError());
+    h->lists_valid = true;
+    h->acc_valid = false;
+    return DFSPH_OK;
+}
+
+// Queue Jacobi iterations in batches; the device decides when the loop ends
+// (kernels of later iterations exit at once).  Two readback slots keep one
+// batch in flight while the previous batch's loop state is inspected.
+int jacobi_loop(Handle* h, bool div, uint32_t max_iter, double max_error, uint32_t* iters, double* err)
+{
+    SolveCtl* sc = div ? &h->ctl->div : &h->ctl->dens;
+    const uint32_t min_iter = div ? h->p.min_iter_v : h->p.min_iter;
+    dfsph::launch_ctl_init(h->s, sc, max_error * 0.01 * h->p.density0, div ? 1 : 0, h->tm, (double)h->n,
+                           min_iter, max_iter);
+    if (h->n == 0 || max_iter == 0) {
+        if (iters) *iters = 0;
+        if (err) *err = 0.0;
+        return DFSPH_OK;
+    }
+    auto jacobi = [&]() {
+        if (div)
+            dfsph::launch_jacobi(h->s, true, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->kk,
+                                 h->kapv[h->pc], h->vel[h->vc], sc);
+        else
+            dfsph::launch_jacobi(h->s, false, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->kk,
+                                 nullptr, h->vel[h->vc], sc);
+    };
+    auto loop_end = [&]() {
+        if (div)
+            dfsph::launch_div_source(h->s, false, h->c, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->vel[h->vc],
+                                     h->alpha, h->radv, h->kk, h->kapv[h->pc], sc, h->partial);
+        else
+            dfsph::launch_density_source(h->s, false, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
+                                         h->vel[h->vc], h->rho, h->alpha, h->radv, h->kk, h->kap[h->pc], sc,
+                                         h->partial);
+    };
+    // The minimum iterations are always run.  After that, the velocity sweep
+    // of the next iteration is queued before the host reads the loop state of
+    // the current one, so the readback hides behind it; the source sweep is
+    // queued only once the loop is known to continue.  The iteration count is
+    // exact, and at most one velocity sweep per solve exits at once.
+    // with min_iter == 0 the reference loop does not start (its error starts at 0)
+    const uint32_t first = std::min(min_iter, max_iter);
+    if (first == 0) {
+        if (iters) *iters = 0;
+        if (err) *err = 0.0;
+        return DFSPH_OK;
+    }
+    for (uint32_t k = 0; k < first; ++k) { jacobi(); loop_end(); }
+    uint32_t queued = first;
+    SolveCtl res{};
+    for (;;) {
+        HIPCHK(hipMemcpyAsync(h->hslot, sc, sizeof(SolveCtl), hipMemcpyDeviceToHost, h->s));
+        HIPCHK(hipEventRecord(h->ev[0], h->s));
+        const bool spec = queued < max_iter;
+        if (spec) jacobi();
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventSynchronize(h->ev[0]));
+        res = h->hslot[0];
+        if (res.done || !spec) break;
+        loop_end();
+        ++queued;
+    }
+    if (!res.done) return fail(DFSPH_ERR_STATE, "Jacobi loop ended without a decision");
+    if (iters) *iters = res.iters;
+    if (err) *err = res.err;
+    return DFSPH_OK;
+}
+
+int divergence_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* iters, double* err)
+{
+    CHK(flush_vel(h));
+    if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale: call dfsph_neighbor_search");
+    int ws = h->p.warm_start_mode;
+    const float* kws = h->kapv[h->pc];
+    if (ws == DFSPH_WARMSTART_LEARNED) {
+        if (!h->mlp) return fail(DFSPH_ERR_NO_MODEL, "learned warm start selected but no model loaded");
+        CHK(dfsph::mlp_predict_divergence(h->mlp, h->s, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
+                                          h->vel[h->vc], h->kap[h->pc], h->kapv[h->pc], &kws));
+    }
+    const int vo = h->vc ^ 1;
+    dfsph::launch_div_ws_init(h->s, ws, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->vel[h->vc],
+                              h->vel[vo], kws, h->rho, h->alpha, h->radv);
+    h->vc = vo;
+    dfsph::launch_div_source(h->s, true, h->c, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->vel[h->vc],
+                             h->alpha, h->radv, h->kk, h->kapv[h->pc], nullptr, nullptr);
+    HIPCHK(hipGetLastError());
+    h->acc_valid = false;
+    return jacobi_loop(h, true, max_iter, max_error, iters, err);
+}
+
+int density_only(Handle* h)
+{
+    // rho and alpha without the divergence solver (the reference leaves them stale)
+    const int vo = h->vc ^ 1;
+    dfsph::launch_div_ws_init(h->s, DFSPH_WARMSTART_NONE, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
+                              h->vel[h->vc], h->vel[vo], nullptr, h->rho, h->alpha, h->radv);
+    h->vc = vo;
+    HIPCHK(hipGetLastError());
+    return DFSPH_OK;
+}
+
+int external_forces(Handle* h)
+{
+    CHK(flush_vel(h));
+    if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale: call dfsph_neighbor_search");
+    dfsph::launch_step_ctl_init(h->s, h->ctl, h->n);
+    dfsph::launch_xsph(h->s, h->c, h->tm, h->n, h->pos[h->pc], h->nbr, h->cnt, h->vel[h->vc], h->rho, h->acc,
+                       h->ctl, h->partial_f);
+    HIPCHK(hipGetLastError());
+    h->acc_valid = true;
+    return DFSPH_OK;
+}
+
+int pressure_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* iters, double* err)
+{
+    if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale: call dfsph_neighbor_search");
+    const int ws = h->p.warm_start_mode;
+    const float* kws = h->kap[h->pc];
+    if (ws == DFSPH_WARMSTART_LEARNED) {
+        if (!h->mlp) return fail(DFSPH_ERR_NO_MODEL, "learned warm start selected but no model loaded");
+        CHK(dfsph::mlp_predict_pressure(h->mlp, h->s, &kws));
+    }
+    if (ws == DFSPH_WARMSTART_NONE) {
+        CHK(flush_vel(h));
+    } else {
+        const bool fused = h->pend_vel;
+        dfsph::launch_pressure_ws(h->s, ws, fused, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
+                                  h->acc, h->vel[h->vc], kws, fused ? 1 : 0);
+        h->pend_vel = false;
+    }
+    dfsph::launch_density_source(h->s, true, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
+                                 h->vel[h->vc], h->rho, h->alpha, h->radv, h->kk, h->kap[h->pc], nullptr, nullptr);
+    HIPCHK(hipGetLastError());
+    h->acc_valid = false;
+    return jacobi_loop(h, false, max_iter, max_error, iters, err);
+}
+
+int compact(Handle* h, uint32_t* removed)
+{
+    const uint32_t n = h->n;
+    dfsph::launch_exclusive_scan(h->s, h->keep, h->rank, n + 1, h->scan_tmp);
+    uint32_t kept = 0;
+    HIPCHK(hipMemcpyAsync(&h->hctl->pad[0], h->rank + n, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s));
+    CHK(sync(h));
+    kept = h->hctl->pad[0];
+    if (removed) *removed = n - kept;
+    if (kept == n) return DFSPH_OK;
+    const int o = h->pc ^ 1, vo = h->vc ^ 1;
+    dfsph::launch_compact(h->s, n, h->keep, h->rank, h->pos[h->pc], h->vel[h->vc], h->kap[h->pc], h->kapv[h->pc],
+                          h->id[h->pc], h->pos[o], h->vel[vo], h->kap[o], h->kapv[o], h->id[o]);
+    HIPCHK(hipGetLastError());
+    h->pc = o;
+    h->vc = vo;
+    h->n = kept;
+    h->lists_valid = false;
+    h->acc_valid = false;
+    return DFSPH_OK;
+}
+
+int check_positions(Handle* h, int mode, uint32_t* count)
+{
+    CHK(flush_vel(h));
+    dfsph::launch_keep_inside(h->s, h->n, h->pos[h->pc], h->bmin, h->bmax, h->keep);
+    if (mode == 2) return compact(h, count);
+    dfsph::launch_exclusive_scan(h->s, h->keep, h->rank, h->n + 1, h->scan_tmp);
+    HIPCHK(hipMemcpyAsync(&h->hctl->pad[0], h->rank + h->n, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s));
+    CHK(sync(h));
+    if (count) *count = h->n - h->hctl->pad[0];
+    return DFSPH_OK;
+}
+
+int alloc_fluid(Handle* h, uint32_t cap)
+{
+    for (int b = 0; b < 2; ++b) {
+        CHK(dalloc(h->pos[b], cap));
+        CHK(dalloc(h->vel[b], cap));
+        CHK(dalloc(h->kap[b], cap));
+        CHK(dalloc(h->kapv[b], cap));
+        CHK(dalloc(h->id[b], cap));
+    }
+    CHK(dalloc(h->rho, cap));
+    CHK(dalloc(h->alpha, cap));
+    CHK(dalloc(h->radv, cap));
+    CHK(dalloc(h->kk, cap));
+    CHK(dalloc(h->acc, cap));
+    CHK(dalloc(h->cnt, cap));
+    CHK(dalloc(h->nbr, (size_t)cap * h->p.max_neighbors));
+    CHK(ensure_tmp(h, std::max(cap, h->nb)));
+    CHK(ensure_partials(h, cap));
+    h->cap = cap;
+    h->c.cap = cap;
+    // zero-initialise the persistent fields beyond the active count
+    for (int b = 0; b < 2; ++b) {
+        HIPCHK(hipMemsetAsync(h->kap[b], 0, sizeof(float) * cap, h->s));
+        HIPCHK(hipMemsetAsync(h->kapv[b], 0, sizeof(float) * cap, h->s));
+    }
+    HIPCHK(hipMemsetAsync(h->cnt, 0, sizeof(uint32_t) * cap, h->s));
+    return DFSPH_OK;
+}
+
+int write_fluid_range(Handle* h, uint32_t at, const float* xyz, const float* vel, const float* mass,
+                      const uint32_t* id, uint32_t m)
+{
+    const double d = 2.0 * h->p.particle_radius;
+    const float m0 = (float)(0.8 * d * d * d * h->p.density0);
+    std::vector<float4> P(m), V(m);
+    std::vector<uint32_t> I(m);
+    std::vector<float> Z(m, 0.0f);
+    for (uint32_t i = 0; i < m; ++i) {
+        P[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], mass ? mass[i] : m0);
+        V[i] = vel ? make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0f) : make_float4(0, 0, 0, 0);
+        I[i] = id ? id[i] : at + i;
+    }
+    HIPCHK(hipMemcpyAsync(h->pos[h->pc] + at, P.data(), sizeof(float4) * m, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->vel[h->vc] + at, V.data(), sizeof(float4) * m, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->id[h->pc] + at, I.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->kap[h->pc] + at, Z.data(), sizeof(float) * m, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->kapv[h->pc] + at, Z.data(), sizeof(float) * m, hipMemcpyHostToDevice, h->s));
+    return sync(h);
+}
+
+}  // namespace
+
+namespace dfsph {
+// accessors used by the learned warm start (dfsph_mlp.hip)
+MlpView handle_view(void* hv)
+{
+    Handle* h = H(hv);
+    MlpView v;
+    v.s = h->s;
+    v.c = h->c;
+    v.tm = h->tm;
+    v.n = h->n;
+    v.pos = h->pos[h->pc];
+    v.bpos = h->bpos;
+    v.nbr = h->nbr;
+    v.cnt = h->cnt;
+    v.vel = h->vel[h->vc];
+    v.rho = h->rho;
+    v.alpha = h->alpha;
+    v.radv = h->radv;
+    v.kappa = h->kap[h->pc];
+    v.kappaV = h->kapv[h->pc];
+    v.mlp = &h->mlp;
+    return v;
+}
+void handle_set_error(const char* msg) { g_err = msg; }
+}  // namespace dfsph
+
+extern "C" {
+
+void dfsph_default_params(dfsph_params* p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->particle_radius = 0.025;
+    p->density0 = 1000.0;
+    p->viscosity = 0.01;
+    p->gravity[1] = -9.81;
+    p->dt = 0.001;
+    p->max_iter_v = 100;
+    p->max_iter = 100;
+    p->min_iter_v = 3;
+    p->min_iter = 2;
+    p->max_error_v = 0.1;
+    p->max_error = 0.01;
+    p->use_cfl = 0;
+    p->cfl_factor = 1.0;
+    p->dt_min = 1.0e-4;
+    p->dt_max = 5.0e-3;
+    p->enable_divergence = 1;
+    p->check_positions_mode = 2;
+    p->max_neighbors = 90;
+    p->fluid_capacity = 0;
+    p->warm_start_mode = DFSPH_WARMSTART_REFERENCE;
+    p->check_interval = 4;
+}
+
+const char* dfsph_last_error(void) { return g_err.c_str(); }
+
+int dfsph_create(const dfsph_params* p, void** out)
+{
+    if (!p || !out) return fail(DFSPH_ERR_ARG, "null argument");
+    if (!(p->particle_radius > 0.0) || !(p->density0 > 0.0) || !(p->dt > 0.0))
+        return fail(DFSPH_ERR_ARG, "particle_radius, density0 and dt must be positive");
+    if (p->max_neighbors == 0 || p->max_neighbors > 255) return fail(DFSPH_ERR_ARG, "max_neighbors must be in [1, 255]");
+    if (p->warm_start_mode < 0 || p->warm_start_mode > 2) return fail(DFSPH_ERR_ARG, "bad warm_start_mode");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(DFSPH_ERR_HIP, "no HIP device");
+    Handle* h = new Handle();
+    h->p = *p;
+    if (h->p.check_interval == 0) h->p.check_interval = 4;
+    int rc = DFSPH_OK;
+    do {
+        if (hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking) != hipSuccess) { rc = fail(DFSPH_ERR_HIP, "stream"); break; }
+        if ((rc = dalloc(h->ctl, 1)) != DFSPH_OK) break;
+        if ((rc = dalloc(h->tm, 1)) != DFSPH_OK) break;
+        if (hipHostMalloc((void**)&h->hctl, sizeof(DevCtl)) != hipSuccess ||
+            hipHostMalloc((void**)&h->htm, sizeof(DevTime)) != hipSuccess ||
+            hipHostMalloc((void**)&h->hslot, 2 * sizeof(SolveCtl)) != hipSuccess) {
+            rc = fail(DFSPH_ERR_HIP, "pinned allocation");
+            break;
+        }
+        for (int b = 0; b < 2; ++b)
+            if (hipEventCreateWithFlags(&h->ev[b], hipEventDisableTiming) != hipSuccess) rc = fail(DFSPH_ERR_HIP, "event");
+        if (rc) break;
+        if (hipMemsetAsync(h->ctl, 0, sizeof(DevCtl), h->s) != hipSuccess) { rc = fail(DFSPH_ERR_HIP, "memset"); break; }
+        dfsph::launch_time_reset(h->s, h->tm, h->p.dt);
+        const double hk = 4.0 * h->p.particle_radius;
+        const double kc = 8.0 / (M_PI * hk * hk * hk), lc = 48.0 / (M_PI * hk * hk * hk);
+        SimConst& c = h->c;
+        c.hk = (float)hk;
+        c.hk2 = c.hk * c.hk;
+        c.inv_hk = (float)(1.0 / hk);
+        c.kcoef = (float)kc;
+        c.lh2 = (float)(lc / (hk * hk));
+        c.lh = (float)(lc / hk);
+        c.W0 = (float)kc;
+        c.density0 = (float)h->p.density0;
+        c.gx = (float)h->p.gravity[0];
+        c.gy = (float)h->p.gravity[1];
+        c.gz = (float)h->p.gravity[2];
+        c.viscosity = (float)h->p.viscosity;
+        c.max_nbr = h->p.max_neighbors;
+        if (hipStreamSynchronize(h->s) != hipSuccess) { rc = fail(DFSPH_ERR_HIP, "sync"); break; }
+    } while (0);
+    if (rc != DFSPH_OK) { delete h; return rc; }
+    *out = h;
+    return DFSPH_OK;
+}
+
+int dfsph_destroy(void* hv)
+{
+    delete H(hv);
+    return DFSPH_OK;
+}
+
+int dfsph_upload_boundary(void* hv, const float* xyz, const float* psi, uint32_t n)
+{
+    Handle* h = H(hv);
+    if (!h || !xyz || n == 0) return fail(DFSPH_ERR_ARG, "boundary needs at least one particle");
+    if (h->grid_ready) return fail(DFSPH_ERR_STATE, "the boundary set is static and was already uploaded");
+    // initGridOffset (DFSPH_c_arrays_structure.cpp:1050-1098) with a grid sized
+    // to the boundary bounding box (no 256^3 cap)
+    for (int a = 0; a < 3; ++a) { h->bmin[a] = xyz[a]; h->bmax[a] = xyz[a]; }
+    for (uint32_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            h->bmin[a] = std::min(h->bmin[a], xyz[3 * i + a]);
+            h->bmax[a] = std::max(h->bmax[a], xyz[3 * i + a]);
+        }
+    const double cell = (double)h->c.hk * kCellMargin;
+    uint64_t ncell = 1;
+    for (int a = 0; a < 3; ++a) {
+        const double org = (double)h->bmin[a] - 2.0 * cell;
+        h->c.org[a] = (float)org;
+        h->c.dim[a] = (int)std::floor(((double)h->bmax[a] - org) / cell) + 3;
+        ncell *= (uint64_t)h->c.dim[a];
+    }
+    if (ncell >= (1ull << 31)) return fail(DFSPH_ERR_ARG, "domain too large for the cell grid");
+    h->c.inv_cell = (float)(1.0 / cell);
+    h->c.ncell = (uint32_t)ncell;
+    h->nb = n;
+    CHK(dalloc(h->hist, ncell + 1));
+    CHK(dalloc(h->cstart, ncell + 1));
+    CHK(dalloc(h->bcstart, ncell + 1));
+    CHK(ensure_scan(h, (uint32_t)ncell + 1));
+    CHK(ensure_tmp(h, std::max(n, h->cap)));
+    std::vector<float4> P(n);
+    for (uint32_t i = 0; i < n; ++i) P[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], psi ? psi[i] : 0.0f);
+    float4* raw = nullptr;
+    CHK(dalloc(raw, n));
+    CHK(dalloc(h->bpos, n));
+    HIPCHK(hipMemcpyAsync(raw, P.data(), sizeof(float4) * n, hipMemcpyHostToDevice, h->s));
+    int rc = counting_sort(h, raw, n, h->bcstart);
+    if (rc == DFSPH_OK) {
+        dfsph::launch_gather_f4(h->s, n, h->perm, raw, h->bpos);
+        if (!psi) {
+            float* tmp = nullptr;
+            rc = dalloc(tmp, n);
+            if (rc == DFSPH_OK) {
+                dfsph::launch_boundary_psi(h->s, h->c, h->bpos, h->bcstart, n, tmp);
+                dfsph::launch_set_w(h->s, h->bpos, tmp, n);
+                rc = sync(h);
+                dfree(tmp);
+            }
+        }
+    }
+    if (rc == DFSPH_OK) rc = sync(h);
+    dfree(raw);
+    if (rc != DFSPH_OK) return rc;
+    h->grid_ready = true;
+    h->ncell_alloc = (uint32_t)ncell;
+    return DFSPH_OK;
+}
+
+int dfsph_upload_fluid(void* hv, const float* xyz, const float* vel, const float* mass, const uint32_t* id,
+                       uint32_t n)
+{
+    Handle* h = H(hv);
+    if (!h || (!xyz && n)) return fail(DFSPH_ERR_ARG, "null argument");
+    if (!h->grid_ready) return fail(DFSPH_ERR_STATE, "upload the boundary set first");
+    uint32_t cap = h->p.fluid_capacity ? h->p.fluid_capacity : (uint32_t)std::ceil(1.3 * (double)n);
+    if (cap < n) return fail(DFSPH_ERR_CAPACITY, "fluid_capacity is smaller than the particle count");
+    if (cap < 64) cap = 64;
+    if (cap != h->cap) CHK(alloc_fluid(h, cap));
+    h->pc = 0;
+    h->vc = 0;
+    h->pend_vel = false;
+    h->n = n;
+    if (n) CHK(write_fluid_range(h, 0, xyz, vel, mass, id, n));
+    dfsph::launch_time_reset(h->s, h->tm, h->p.dt);
+    h->lists_valid = false;
+    h->acc_valid = false;
+    return sync(h);
+}
+
+int dfsph_neighbor_search(void* hv, int sort) { return neighbor_search(H(hv), sort != 0); }
+
+int dfsph_compute_density_factor(void* hv)
+{
+    Handle* h = H(hv);
+    CHK(flush_vel(h));
+    if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale: call dfsph_neighbor_search");
+    return density_only(h);
+}
+
+int dfsph_divergence_solve(void* hv, uint32_t max_iter, double max_error, uint32_t* iters, double* err)
+{
+    return divergence_solve(H(hv), max_iter, max_error, iters, err);
+}
+
+int dfsph_external_forces(void* hv) { return external_forces(H(hv)); }
+
+int dfsph_cfl(void* hv, double min_dt, double cfl_factor, double max_dt, double* dt_out)
+{
+    Handle* h = H(hv);
+    if (!h->acc_valid) CHK(external_forces(h));
+    dfsph::launch_time_update(h->s, h->tm, 0.0, h->ctl, 1, cfl_factor, 2.0 * h->p.particle_radius, min_dt, max_dt, 0.0);
+    HIPCHK(hipGetLastError());
+    CHK(read_ctl(h));
+    if (dt_out) *dt_out = h->htm->hfd;
+    return DFSPH_OK;
+}
+
+int dfsph_update_time_step(void* hv, double h_future)
+{
+    Handle* h = H(hv);
+    if (!(h_future > 0.0)) return fail(DFSPH_ERR_ARG, "time step must be positive");
+    CHK(flush_vel(h));
+    dfsph::launch_time_update(h->s, h->tm, h_future, h->ctl, 0, 0, 0, 0, 0, 0);
+    HIPCHK(hipGetLastError());
+    return DFSPH_OK;
+}
+
+int dfsph_update_vel(void* hv)
+{
+    Handle* h = H(hv);
+    CHK(flush_vel(h));
+    h->pend_vel = true;    // applied by the next pressure warm start, or flushed
+    h->acc_valid = false;
+    return DFSPH_OK;
+}
+
+int dfsph_pressure_solve(void* hv, uint32_t max_iter, double max_error, uint32_t* iters, double* err)
+{
+    return pressure_solve(H(hv), max_iter, max_error, iters, err);
+}
+
+int dfsph_update_pos(void* hv)
+{
+    Handle* h = H(hv);
+    CHK(flush_vel(h));
+    dfsph::launch_update_pos(h->s, h->tm, h->n, h->vel[h->vc], h->pos[h->pc], h->bmin, h->bmax, h->ctl);
+    HIPCHK(hipGetLastError());
+    h->lists_valid = false;
+    h->acc_valid = false;
+    return DFSPH_OK;
+}
+
+int dfsph_step_end(void* hv)
+{
+    Handle* h = H(hv);
+    dfsph::launch_time_step_end(h->s, h->tm);
+    HIPCHK(hipGetLastError());
+    return DFSPH_OK;
+}
+
+int dfsph_set_time(void* hv, double hh, double h_past, double time)
+{
+    Handle* h = H(hv);
+    if (!(hh > 0.0) || !(h_past > 0.0)) return fail(DFSPH_ERR_ARG, "time steps must be positive");
+    CHK(flush_vel(h));
+    dfsph::launch_time_set(h->s, h->tm, hh, h_past, time);
+    HIPCHK(hipGetLastError());
+    return sync(h);
+}
+
+int dfsph_check_particles_positions(void* hv, int mode, uint32_t* count)
+{
+    Handle* h = H(hv);
+    if (mode != 0 && mode != 2) return fail(DFSPH_ERR_ARG, "mode must be 0 (count) or 2 (remove)");
+    return check_positions(h, mode, count);
+}
+
+int dfsph_remove_particles(void* hv, const uint8_t* keep, uint32_t* n_out)
+{
+    Handle* h = H(hv);
+    if (!keep && h->n) return fail(DFSPH_ERR_ARG, "null keep array");
+    CHK(flush_vel(h));
+    if (h->n) {
+        HIPCHK(hipMemcpyAsync(h->keep8, keep, h->n, hipMemcpyHostToDevice, h->s));
+        dfsph::launch_keep_from_u8(h->s, h->n, h->keep8, h->keep);
+        CHK(compact(h, nullptr));
+    }
+    if (n_out) *n_out = h->n;
+    return DFSPH_OK;
+}
+
+int dfsph_add_particles(void* hv, const float* xyz, const float* vel, const float* mass, const uint32_t* id,
+                        uint32_t m)
+{
+    Handle* h = H(hv);
+    if (!xyz && m) return fail(DFSPH_ERR_ARG, "null positions");
+    if (h->n + m > h->cap) return fail(DFSPH_ERR_CAPACITY, "adding particles beyond the fluid capacity");
+    CHK(flush_vel(h));
+    if (m) CHK(write_fluid_range(h, h->n, xyz, vel, mass, id, m));
+    h->n += m;
+    h->lists_valid = false;
+    h->acc_valid = false;
+    return DFSPH_OK;
+}
+
+int dfsph_step(void* hv, dfsph_step_stats* st)
+{
+    Handle* h = H(hv);
+    if (!h->grid_ready) return fail(DFSPH_ERR_STATE, "no boundary set uploaded");
+    dfsph_step_stats loc{};
+    CHK(neighbor_search(h, true));
+    if (h->p.enable_divergence) CHK(divergence_solve(h, h->p.max_iter_v, h->p.max_error_v, &loc.iters_v, &loc.err_v));
+    else CHK(density_only(h));
+    CHK(external_forces(h));
+    // h_future: CFL estimate, or the current step kept constant (the reference
+    // takes TimeManager's step size, DFSPH_CUDA.cpp:299-306)
+    dfsph::launch_time_update(h->s, h->tm, 0.0, h->ctl, h->p.use_cfl ? 1 : 2, h->p.cfl_factor,
+                              2.0 * h->p.particle_radius, h->p.dt_min, h->p.dt_max, 0.0);
+    h->pend_vel = true;
+    CHK(pressure_solve(h, h->p.max_iter, h->p.max_error, &loc.iters, &loc.err));
+    CHK(dfsph_update_pos(h));
+    CHK(dfsph_step_end(h));
+    CHK(read_ctl(h));
+    loc.dt = h->htm->hpd;
+    loc.time = h->htm->time;
+    loc.max_neighbors_seen = h->hctl->max_seen;
+    loc.overflow = h->hctl->overflow;
+    if (h->p.check_positions_mode == 2 && h->hctl->escaped) {
+        CHK(check_positions(h, 2, &loc.removed));
+    } else if (h->p.check_positions_mode == 0) {
+        loc.removed = 0;
+    }
+    loc.n_fluid = h->n;
+    if (st) *st = loc;
+    if (loc.overflow)
+        return fail(DFSPH_ERR_NEIGHBOR_OVERFLOW, std::to_string(loc.overflow) + " particles exceed max_neighbors");
+    return DFSPH_OK;
+}
+
+int dfsph_download(void* hv, int field, void* dst, uint32_t count)
+{
+    Handle* h = H(hv);
+    if (!dst && count) return fail(DFSPH_ERR_ARG, "null destination");
+    CHK(flush_vel(h));
+    const uint32_t n = (field == DFSPH_FIELD_BOUNDARY_POS || field == DFSPH_FIELD_BOUNDARY_PSI) ? h->nb : h->n;
+    const bool vec = field == DFSPH_FIELD_POS || field == DFSPH_FIELD_VEL || field == DFSPH_FIELD_ACC ||
+                     field == DFSPH_FIELD_BOUNDARY_POS;
+    if (count != (vec ? 3 * n : n)) return fail(DFSPH_ERR_ARG, "count does not match the field size");
+    if (n == 0) return DFSPH_OK;
+    CHK(sync(h));
+    auto get_f4 = [&](const float4* src, int comp) -> int {
+        std::vector<float4> tmp(n);
+        HIPCHK(hipMemcpy(tmp.data(), src, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        float* o = static_cast<float*>(dst);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (comp < 0) { o[3 * i] = tmp[i].x; o[3 * i + 1] = tmp[i].y; o[3 * i + 2] = tmp[i].z; }
+            else o[i] = tmp[i].w;
+        }
+        return DFSPH_OK;
+    };
+    auto get_raw = [&](const void* src, size_t bytes) -> int {
+        HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return DFSPH_OK;
+    };
+    switch (field) {
+    case DFSPH_FIELD_POS: return get_f4(h->pos[h->pc], -1);
+    case DFSPH_FIELD_VEL: return get_f4(h->vel[h->vc], -1);
+    case DFSPH_FIELD_ACC: return get_f4(h->acc, -1);
+    case DFSPH_FIELD_MASS: return get_f4(h->pos[h->pc], 3);
+    case DFSPH_FIELD_DENSITY: return get_raw(h->rho, 4ull * n);
+    case DFSPH_FIELD_FACTOR: return get_raw(h->alpha, 4ull * n);
+    case DFSPH_FIELD_DENSITY_ADV: return get_raw(h->radv, 4ull * n);
+    case DFSPH_FIELD_KAPPA: return get_raw(h->kap[h->pc], 4ull * n);
+    case DFSPH_FIELD_KAPPAV: return get_raw(h->kapv[h->pc], 4ull * n);
+    case DFSPH_FIELD_ID: return get_raw(h->id[h->pc], 4ull * n);
+    case DFSPH_FIELD_NBR_FLUID:
+    case DFSPH_FIELD_NBR_BOUNDARY: {
+        std::vector<uint32_t> c(n);
+        HIPCHK(hipMemcpy(c.data(), h->cnt, 4ull * n, hipMemcpyDeviceToHost));
+        uint32_t* o = static_cast<uint32_t*>(dst);
+        for (uint32_t i = 0; i < n; ++i) o[i] = field == DFSPH_FIELD_NBR_FLUID ? (c[i] & 0xffffu) : (c[i] >> 16);
+        return DFSPH_OK;
+    }
+    case DFSPH_FIELD_BOUNDARY_POS: return get_f4(h->bpos, -1);
+    case DFSPH_FIELD_BOUNDARY_PSI: return get_f4(h->bpos, 3);
+    default: return fail(DFSPH_ERR_ARG, "unknown field");
+    }
+}
+
+int dfsph_upload_field(void* hv, int field, const void* src, uint32_t count)
+{
+    Handle* h = H(hv);
+    CHK(flush_vel(h));
+    const uint32_t n = h->n;
+    const bool vec = field == DFSPH_FIELD_POS || field == DFSPH_FIELD_VEL;
+    if (count != (vec ? 3 * n : n)) return fail(DFSPH_ERR_ARG, "count does not match the field size");
+    if (n == 0) return DFSPH_OK;
+    CHK(sync(h));
+    const float* f = static_cast<const float*>(src);
+    if (field == DFSPH_FIELD_POS || field == DFSPH_FIELD_VEL || field == DFSPH_FIELD_MASS) {
+        float4* dptr = field == DFSPH_FIELD_VEL ? h->vel[h->vc] : h->pos[h->pc];
+        std::vector<float4> tmp(n);
+        HIPCHK(hipMemcpy(tmp.data(), dptr, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) {
+            if (field == DFSPH_FIELD_MASS) tmp[i].w = f[i];
+            else { tmp[i].x = f[3 * i]; tmp[i].y = f[3 * i + 1]; tmp[i].z = f[3 * i + 2]; }
+        }
+        HIPCHK(hipMemcpy(dptr, tmp.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        if (field != DFSPH_FIELD_VEL) h->lists_valid = false;
+        h->acc_valid = false;
+        return DFSPH_OK;
+    }
+    if (field == DFSPH_FIELD_KAPPA) { HIPCHK(hipMemcpy(h->kap[h->pc], src, 4ull * n, hipMemcpyHostToDevice)); return DFSPH_OK; }
+    if (field == DFSPH_FIELD_KAPPAV) { HIPCHK(hipMemcpy(h->kapv[h->pc], src, 4ull * n, hipMemcpyHostToDevice)); return DFSPH_OK; }
+    return fail(DFSPH_ERR_ARG, "field cannot be uploaded");
+}
+
+int dfsph_get_neighbors(void* hv, uint32_t* n_fluid, uint32_t* n_boundary, uint32_t* list, uint32_t max)
+{
+    Handle* h = H(hv);
+    if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale");
+    CHK(sync(h));
+    const uint32_t n = h->n, mx = h->p.max_neighbors, cap = h->cap;
+    if (n == 0) return DFSPH_OK;
+    std::vector<uint32_t> c(n), L((size_t)mx * cap);
+    HIPCHK(hipMemcpy(c.data(), h->cnt, 4ull * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(L.data(), h->nbr, 4ull * mx * cap, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t f = c[i] & 0xffffu, b = c[i] >> 16;
+        if (n_fluid) n_fluid[i] = f;
+        if (n_boundary) n_boundary[i] = b;
+        if (list)
+            for (uint32_t k = 0; k < f + b && k < max; ++k) list[(size_t)i * max + k] = L[(size_t)k * cap + i];
+    }
+    return DFSPH_OK;
+}
+
+int dfsph_get_info(void* hv, dfsph_info* info)
+{
+    Handle* h = H(hv);
+    if (!info) return fail(DFSPH_ERR_ARG, "null info");
+    CHK(flush_vel(h));
+    CHK(read_ctl(h));
+    std::memset(info, 0, sizeof(*info));
+    info->n_fluid = h->n;
+    info->n_boundary = h->nb;
+    info->fluid_capacity = h->cap;
+    info->max_neighbors = h->p.max_neighbors;
+    for (int a = 0; a < 3; ++a) {
+        info->grid_dim[a] = (uint32_t)h->c.dim[a];
+        info->grid_origin[a] = h->c.org[a];
+        info->bbox_min[a] = h->bmin[a];
+        info->bbox_max[a] = h->bmax[a];
+    }
+    info->cell_size = 1.0 / (double)h->c.inv_cell;
+    info->h = h->htm->hd;
+    info->h_future = h->htm->hfd;
+    info->h_past = h->htm->hpd;
+    info->time = h->htm->time;
+    info->max_neighbors_seen = h->hctl->max_seen;
+    info->overflow = h->hctl->overflow;
+    info->escaped = h->hctl->escaped;
+    return DFSPH_OK;
+}
+
+int dfsph_synchronize(void* hv) { return sync(H(hv)); }
+
+}  // extern "C"

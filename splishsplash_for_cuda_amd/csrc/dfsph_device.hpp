@@ -1,0 +1,236 @@
+// dfsph_device.hpp -- device-side helpers shared by the DFSPH kernels (gfx950).
+//
+// Data layout in HBM (structure of arrays, particles in cell-sorted order):
+//   pos   float4 (x, y, z, mass)        fluid; boundary: (x, y, z, psi)
+//   vel   float4 (vx, vy, vz, 0)
+//   rho, alpha, radv, k, kappa, kappaV  float
+//   nbr   uint32 [max_neighbors][capacity]  (interleaved: slot k of particle i
+//         at k*capacity + i, so a wavefront reads 64 consecutive words)
+//   cnt   uint32  fluid count | boundary count << 16
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dfsph {
+
+constexpr int kBlock = 256;        // 4 wavefronts of 64
+constexpr int kNumXcd = 8;         // MI355X: 8 XCDs, blocks dispatched round robin
+constexpr float kEps = 1.0e-5f;    // m_eps (DFSPH_define_cuda.h:7)
+
+// Constants of one simulation, passed by value to every kernel.
+struct SimConst {
+    float hk, hk2, inv_hk;         // kernel support h_k = 4r, h_k^2, 1/h_k
+    float kcoef;                   // 8 / (pi h_k^3)
+    float lh2;                     // (48 / (pi h_k^3)) / h_k^2
+    float lh;                      // (48 / (pi h_k^3)) / h_k
+    float W0;                      // W(0)
+    float density0;
+    float gx, gy, gz;
+    float viscosity;
+    float org[3];                  // grid origin
+    float inv_cell;                // 1 / cell edge (cell edge = 1.001 h_k)
+    int dim[3];                    // cells per axis; cell id = x + dim0 (z + dim2 y)
+    uint32_t ncell;
+    uint32_t cap;                  // neighbor list stride (fluid capacity)
+    uint32_t max_nbr;
+};
+
+// Time-step scalars live on the device so a whole step can be queued
+// without host round trips (DFSPHCData h, h_future, h_past, ratios,
+// DFSPH_c_arrays_structure.h:345-356).
+struct DevTime {
+    float h, h_future, h_past;
+    float inv_h, inv_h_future;
+    float ratio, ratio2;           // h_ratio_to_past, h_ratio_to_past2
+    float pad;
+    double hd, hfd, hpd;           // the same in FP64
+    double time;                   // simulated time (FP64 accumulator)
+};
+
+// Device-side control of one Jacobi loop: the loop-end kernel's last block
+// turns the FP64 block partial sums into the average error and decides
+// whether the loop continues, with the reference's condition
+// while ((err > eta) || (it < min)) && (it < max)  (DFSPH_core_cuda.cu:828,1378).
+struct SolveCtl {
+    double sum, err, eta;
+    double n_div;                  // particle count the error is averaged over
+    uint32_t iters, done, min_iter, max_iter;
+    uint32_t ticket, pad;
+};
+
+struct DevCtl {
+    SolveCtl div, dens;
+    float cfl_max;                 // max |v + a h|^2
+    uint32_t cfl_ticket;
+    uint32_t escaped;              // particles outside the bounding box
+    uint32_t max_seen;             // largest neighbor count of the last search
+    uint32_t overflow;             // particles whose list was truncated
+    uint32_t n_active;             // active fluid particles (device copy)
+    uint32_t pad[2];
+};
+
+// Blocks are dispatched round robin over the 8 XCDs, each with its own L2.
+// Remapping gives every XCD a contiguous range of logical blocks, so
+// neighbor gathers of cell-sorted particles stay in one L2.  gridDim.x must
+// be a multiple of 8; logical ids >= the needed count are idle.
+__device__ __forceinline__ uint32_t xcd_block()
+{
+    const uint32_t per = gridDim.x / kNumXcd;
+    return (blockIdx.x % kNumXcd) * per + blockIdx.x / kNumXcd;
+}
+
+__device__ __forceinline__ uint32_t xcd_index()
+{
+    return xcd_block() * blockDim.x + threadIdx.x;
+}
+
+// The neighbor predicate: strict |xi - xj|^2 < h_k^2 in FP32
+// (DFSPH_macro_cuda.h:124,149) with a fixed evaluation order
+// d2 = fma(dz, dz, fma(dy, dy, dx*dx)); the oracle evaluates the same.
+__device__ __forceinline__ float dist2(float dx, float dy, float dz)
+{
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// Cubic spline (CubicKernelPerso, SPH_kernels.h:42-94) evaluated from the
+// squared distance.  q <= 1/2: W = k (6q^3 - 6q^2 + 1), grad = l (3q - 2) / h^2 r;
+// q > 1/2: W = 2k (1-q)^3, grad = -l (1-q)^2 / (|r| h) r; zero for q > 1 and
+// for |r| <= 1e-6 (gradient).
+struct PairK {
+    float W;    // kernel value
+    float g;    // gradient coefficient: gradW = g * (xi - xj)
+};
+
+__device__ __forceinline__ float kernel_grad_coef(const SimConst& c, float r2)
+{
+    const float r2c = fmaxf(r2, 1.0e-24f);
+    const float ir = __builtin_amdgcn_rsqf(r2c);
+    const float r = r2c * ir;
+    const float q = r * c.inv_hk;
+    const float f = 1.0f - q;
+    const float inner = c.lh2 * __builtin_fmaf(3.0f, q, -2.0f);
+    const float outer = -c.lh * f * f * ir;
+    float g = (q <= 0.5f) ? inner : outer;
+    g = (q <= 1.0f && r > 1.0e-6f) ? g : 0.0f;
+    return g;
+}
+
+__device__ __forceinline__ PairK kernel_pair(const SimConst& c, float r2)
+{
+    const float r2c = fmaxf(r2, 1.0e-24f);
+    const float ir = __builtin_amdgcn_rsqf(r2c);
+    const float r = r2c * ir;
+    const float q = r * c.inv_hk;
+    const float f = 1.0f - q;
+    PairK o;
+    const float w_in = c.kcoef * __builtin_fmaf(6.0f * q * q, q - 1.0f, 1.0f);
+    const float w_out = 2.0f * c.kcoef * f * f * f;
+    o.W = (q <= 0.5f) ? w_in : w_out;
+    o.W = (q <= 1.0f) ? o.W : 0.0f;
+    const float inner = c.lh2 * __builtin_fmaf(3.0f, q, -2.0f);
+    const float outer = -c.lh * f * f * ir;
+    float g = (q <= 0.5f) ? inner : outer;
+    o.g = (q <= 1.0f && r > 1.0e-6f) ? g : 0.0f;
+    return o;
+}
+
+__device__ __forceinline__ float kernel_W(const SimConst& c, float r2)
+{
+    return kernel_pair(c, r2).W;
+}
+
+__device__ __forceinline__ int cell_coord(const SimConst& c, float v, int a)
+{
+    int k = (int)floorf((v - c.org[a]) * c.inv_cell);
+    k = k < 0 ? 0 : k;
+    k = k >= c.dim[a] ? c.dim[a] - 1 : k;
+    return k;
+}
+
+__device__ __forceinline__ uint32_t cell_id(const SimConst& c, float x, float y, float z)
+{
+    const int cx = cell_coord(c, x, 0), cy = cell_coord(c, y, 1), cz = cell_coord(c, z, 2);
+    return (uint32_t)cx + (uint32_t)c.dim[0] * ((uint32_t)cz + (uint32_t)c.dim[2] * (uint32_t)cy);
+}
+
+// ---- wavefront / block reductions (wave64) ----
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { const uint32_t w = __shfl_xor(v, o, 64); v = v > w ? v : w; }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block sum in a fixed order (wave tree, then waves 0..3): deterministic.
+__device__ __forceinline__ double block_sum(double v, double* lds4)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds4[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+        for (int k = 0; k < nw; ++k) t += lds4[k];
+    }
+    return t;   // valid in thread 0
+}
+
+__device__ __forceinline__ double load_agent(const double* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_agent(double* p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ float load_agent_f(const float* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_agent_f(float* p, float v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns true in every thread of the block that finished last.  Thread 0 of
+// each block has published its partial result before calling.
+__device__ __forceinline__ bool last_block_done(uint32_t* ticket, bool* lds_flag)
+{
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        *lds_flag = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    const bool last = *lds_flag;
+    if (last) __threadfence();
+    return last;
+}
+
+}  // namespace dfsph

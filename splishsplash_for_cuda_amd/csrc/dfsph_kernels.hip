@@ -1,0 +1,1065 @@
+// dfsph_kernels.hip -- hand-written gfx950 kernels of the DFSPH time step.
+//
+// One thread per particle, particles in cell-sorted order, 256-thread
+// blocks with the XCD-aware block remap of dfsph_device.hpp.  Every kernel
+// cites the reference kernel(s) it replaces (paths relative to the
+// reference repository root).
+#include "dfsph_device.hpp"
+#include "dfsph_launch.hpp"
+
+namespace dfsph {
+
+// ------------------------------------------------------------------------
+// small utilities
+// ------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_f32(float* p, float v, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Loop-control init (the host-side locals of cuda_divergenceSolve /
+// cuda_pressureSolve, DFSPH_core_cuda.cu:785-828, 1338-1378).
+// eta = max_error * 0.01 * rho0 (/ h for the divergence solver).
+__global__ void k_ctl_init(SolveCtl* s, double eta_base, int divide_by_h, const DevTime* t,
+                           double n_div, uint32_t min_iter, uint32_t max_iter)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        s->sum = 0.0;
+        s->err = 0.0;
+        s->eta = divide_by_h ? eta_base / t->hd : eta_base;
+        s->n_div = n_div;
+        s->iters = 0;
+        s->done = (max_iter == 0) ? 1u : 0u;
+        s->min_iter = min_iter;
+        s->max_iter = max_iter;
+        s->ticket = 0;
+    }
+}
+
+__global__ void k_step_ctl_init(DevCtl* c, uint32_t n_active)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c->cfl_max = 0.0f;
+        c->cfl_ticket = 0;
+        c->escaped = 0;
+        c->n_active = n_active;
+    }
+}
+
+__global__ void k_nbr_ctl_init(DevCtl* c)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) { c->max_seen = 0; c->overflow = 0; }
+}
+
+static __device__ void time_fill(DevTime* t, double h, double hf, double hp)
+{
+    t->hd = h; t->hfd = hf; t->hpd = hp;
+    t->h = (float)h; t->h_future = (float)hf; t->h_past = (float)hp;
+    t->inv_h = (float)(1.0 / h);
+    t->inv_h_future = (float)(1.0 / hf);
+}
+
+// DFSPHCData::reset time bookkeeping (DFSPH_c_arrays_structure.cpp:1000-1004)
+__global__ void k_time_reset(DevTime* t, double h)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        time_fill(t, h, h, h);
+        t->ratio = 1.0f; t->ratio2 = 1.0f;
+        t->time = 0.0;
+    }
+}
+
+// DFSPHCData::updateTimeStep (DFSPH_c_arrays_structure.cpp:1100-1106).  mode 0: the
+// given h_future; mode 1: the CFL estimate of cuda_CFL (DFSPH_core_cuda.cu:3380-3387),
+// h = cfl_factor * 0.4 * 2r / sqrt(max |v + a h|^2) clamped to [dt_min, dt_max];
+// mode 2: keep the current h (constant step, DFSPH_CUDA.cpp:302-306).
+__global__ void k_time_update(DevTime* t, double hf, const DevCtl* ctl, int mode,
+                              double cfl_factor, double diameter, double dt_min, double dt_max,
+                              double cfl_seed)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (mode == 2) hf = t->hd;
+        if (mode == 1) {
+            double m = (double)ctl->cfl_max;
+            m = m > cfl_seed ? m : cfl_seed;
+            double h = cfl_factor * 0.4 * (diameter / sqrt(m));
+            if (!(h < dt_max)) h = dt_max;
+            if (h < dt_min) h = dt_min;
+            hf = h;
+        }
+        const double h = t->hd;
+        time_fill(t, h, hf, t->hpd);
+        t->ratio = (float)(h / hf);
+        t->ratio2 = (float)((h * h) / (hf * hf));
+    }
+}
+
+// resume from a snapshot: h, h_past, time; h_future = h and the ratios of the
+// last updateTimeStep (h_past / h)
+__global__ void k_time_set(DevTime* t, double h, double hp, double time)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        time_fill(t, h, h, hp);
+        t->ratio = (float)(hp / h);
+        t->ratio2 = (float)((hp * hp) / (h * h));
+        t->time = time;
+    }
+}
+
+// DFSPHCData::onSimulationStepEnd (DFSPH_c_arrays_structure.cpp:1108-1118) and the
+// TimeManager advance time += h (DFSPH_CUDA.cpp:449-452, with the new h).
+__global__ void k_time_step_end(DevTime* t)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const double hp = t->hd, h = t->hfd;
+        time_fill(t, h, h, hp);
+        t->time += h;
+    }
+}
+
+// ------------------------------------------------------------------------
+// exclusive scan of uint32 (replaces cub::DeviceScan::ExclusiveSum,
+// DFSPH_core_cuda.cu:1920): tiles of 2048 in LDS, deterministic.
+// ------------------------------------------------------------------------
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ __forceinline__ uint32_t wave_incl_scan_u(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t w = __shfl_up(v, o, 64);
+        if (lane >= o) v += w;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* in, uint32_t* out, uint32_t n,
+                                                        uint32_t* tile_sums)
+{
+    __shared__ uint32_t tile[kScanTile];
+    __shared__ uint32_t wsum[kBlock / 64];
+    const uint32_t base = blockIdx.x * kScanTile;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t g = base + k * kBlock + threadIdx.x;
+        tile[k * kBlock + threadIdx.x] = g < n ? in[g] : 0u;
+    }
+    __syncthreads();
+    uint32_t loc[kScanItems];
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        loc[k] = run;
+        run += tile[threadIdx.x * kScanItems + k];
+    }
+    const uint32_t incl = wave_incl_scan_u(run);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (int k = 0; k < w; ++k) woff += wsum[k];
+    const uint32_t excl = woff + incl - run;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) tile[threadIdx.x * kScanItems + k] = excl + loc[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t g = base + k * kBlock + threadIdx.x;
+        if (g < n) out[g] = tile[k * kBlock + threadIdx.x];
+    }
+    if (threadIdx.x == kBlock - 1) {
+        uint32_t tot = 0;
+        for (int k = 0; k < kBlock / 64; ++k) tot += wsum[k];
+        tile_sums[blockIdx.x] = tot;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* tile_off)
+{
+    const uint32_t off = tile_off[blockIdx.x];
+    const uint32_t base = blockIdx.x * kScanTile;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t g = base + k * kBlock + threadIdx.x;
+        if (g < n) out[g] += off;
+    }
+}
+
+// ------------------------------------------------------------------------
+// neighbor search (cuda_neighborsSearch, DFSPH_core_cuda.cu:1833-2129, 2219-2531)
+// ------------------------------------------------------------------------
+
+// DFSPH_computeGridIdx_kernel (DFSPH_core_cuda.cu:1833-1848): cell id and the
+// particle's arrival slot in its cell.  Particles outside the grid are
+// clamped into the border cells.
+__global__ void k_cell_hist(SimConst c, const float4* pos, uint32_t n, uint32_t* key,
+                            uint32_t* slot, uint32_t* hist)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos[i];
+    const uint32_t k = cell_id(c, p.x, p.y, p.z);
+    key[i] = k;
+    slot[i] = atomicAdd(hist + k, 1u);
+}
+
+// DFSPH_CountingSortIds_kernel (DFSPH_core_cuda.cu:1851-1858)
+__global__ void k_scatter_perm(const uint32_t* key, const uint32_t* slot, const uint32_t* cstart,
+                               uint32_t n, uint32_t* perm)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    perm[cstart[key[i]] + slot[i]] = i;
+}
+
+// The arrival slots come from atomics, so the order inside a cell varies from
+// run to run (DFSPH_core_cuda.cu:1845-1846).  Sorting each cell's few entries
+// by original index makes the counting sort stable and the whole step
+// bit-reproducible.
+__global__ void k_cell_sort(const uint32_t* cstart, uint32_t ncell, uint32_t* perm)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const uint32_t s = cstart[c], e = cstart[c + 1];
+    for (uint32_t a = s + 1; a < e; ++a) {
+        const uint32_t v = perm[a];
+        uint32_t b = a;
+        while (b > s && perm[b - 1] > v) { perm[b] = perm[b - 1]; --b; }
+        perm[b] = v;
+    }
+}
+
+// cuda_sortData (DFSPH_core_cuda.cu:2057-2129): one gather of every persistent
+// per-particle field into the alternate buffers (no device-to-device copies).
+__global__ void k_gather_fluid(uint32_t n, const uint32_t* perm,
+                               const float4* pos_i, const float4* vel_i, const float* kap_i,
+                               const float* kapv_i, const uint32_t* id_i,
+                               float4* pos_o, float4* vel_o, float* kap_o, float* kapv_o, uint32_t* id_o)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t o = perm[k];
+    pos_o[k] = pos_i[o];
+    vel_o[k] = vel_i[o];
+    kap_o[k] = kap_i[o];
+    kapv_o[k] = kapv_i[o];
+    id_o[k] = id_i[o];
+}
+
+__global__ void k_gather_f4(uint32_t n, const uint32_t* perm, const float4* in, float4* out)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    out[k] = in[perm[k]];
+}
+
+// DFSPH_neighborsSearch_kernel<true> with ITER_NEIGHBORS_FROM_STRUCTURE_LINEAR_ADVANCED
+// (DFSPH_core_cuda.cu:2219-2531, DFSPH_macro_cuda.h:136-155): for each of the 3x3
+// (y, z) rows, the contiguous 3-cell x strip of the fluid table (self excluded),
+// then the boundary table; interleaved storage (DFSPH_macro_cuda.h:210-214).
+// INDIRECT: the fluid is not reordered and the table indexes it through perm
+// (the need_sort == false path).
+template <bool INDIRECT>
+__global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* pos, const uint32_t* cs_f,
+                                                      const uint32_t* perm, const float4* bpos,
+                                                      const uint32_t* cs_b, uint32_t n, uint32_t* nbr,
+                                                      uint32_t* cnt, DevCtl* ctl)
+{
+    const uint32_t i = xcd_index();
+    uint32_t total = 0, trunc = 0;
+    if (i < n) {
+        const float4 p = pos[i];
+        const int cx = cell_coord(c, p.x, 0), cy = cell_coord(c, p.y, 1), cz = cell_coord(c, p.z, 2);
+        const int x0 = cx > 0 ? cx - 1 : 0;
+        const int x1 = cx + 1 < c.dim[0] ? cx + 1 : c.dim[0] - 1;
+        const uint32_t maxn = c.max_nbr, stride = c.cap;
+        uint32_t* out = nbr + i;
+        uint32_t nf = 0, nb = 0;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = cy + dy;
+            if (y < 0 || y >= c.dim[1]) continue;
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int z = cz + dz;
+                if (z < 0 || z >= c.dim[2]) continue;
+                const uint32_t row = (uint32_t)c.dim[0] * ((uint32_t)z + (uint32_t)c.dim[2] * (uint32_t)y);
+                const uint32_t s = cs_f[row + x0], e = cs_f[row + x1 + 1];
+                for (uint32_t q = s; q < e; ++q) {
+                    const uint32_t j = INDIRECT ? perm[q] : q;
+                    if (j == i) continue;
+                    const float4 pj = pos[j];
+                    if (dist2(p.x - pj.x, p.y - pj.y, p.z - pj.z) < c.hk2) {
+                        if (nf < maxn) out[(size_t)nf * stride] = j;
+                        ++nf;
+                    }
+                }
+            }
+        }
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = cy + dy;
+            if (y < 0 || y >= c.dim[1]) continue;
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int z = cz + dz;
+                if (z < 0 || z >= c.dim[2]) continue;
+                const uint32_t row = (uint32_t)c.dim[0] * ((uint32_t)z + (uint32_t)c.dim[2] * (uint32_t)y);
+                const uint32_t s = cs_b[row + x0], e = cs_b[row + x1 + 1];
+                for (uint32_t j = s; j < e; ++j) {
+                    const float4 pj = bpos[j];
+                    if (dist2(p.x - pj.x, p.y - pj.y, p.z - pj.z) < c.hk2) {
+                        const uint32_t at = nf + nb;
+                        if (at < maxn) out[(size_t)at * stride] = j;
+                        ++nb;
+                    }
+                }
+            }
+        }
+        total = nf + nb;
+        const uint32_t sf = nf < maxn ? nf : maxn;
+        const uint32_t sb = nb < maxn - sf ? nb : maxn - sf;
+        trunc = total > maxn ? 1u : 0u;
+        cnt[i] = sf | (sb << 16);
+    }
+    const uint32_t wmax = wave_max_u(total);
+    const uint32_t wtr = wave_sum_u(trunc);
+    if ((threadIdx.x & 63) == 0) {
+        if (wmax) atomicMax(&ctl->max_seen, wmax);
+        if (wtr) atomicAdd(&ctl->overflow, wtr);
+    }
+}
+
+// FluidModel::computeBoundaryPsi (FluidModel.cpp:233-258) on the device:
+// psi_i = rho0 / (W0 + sum_{j != i, |xij| < h} W(xij)) over the boundary set.
+__global__ void k_boundary_psi(SimConst c, const float4* bpos, const uint32_t* cs_b, uint32_t nb, float* psi)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const float4 p = bpos[i];
+    const int cx = cell_coord(c, p.x, 0), cy = cell_coord(c, p.y, 1), cz = cell_coord(c, p.z, 2);
+    const int x0 = cx > 0 ? cx - 1 : 0;
+    const int x1 = cx + 1 < c.dim[0] ? cx + 1 : c.dim[0] - 1;
+    float delta = c.W0;
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int y = cy + dy;
+        if (y < 0 || y >= c.dim[1]) continue;
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int z = cz + dz;
+            if (z < 0 || z >= c.dim[2]) continue;
+            const uint32_t row = (uint32_t)c.dim[0] * ((uint32_t)z + (uint32_t)c.dim[2] * (uint32_t)y);
+            for (uint32_t j = cs_b[row + x0]; j < cs_b[row + x1 + 1]; ++j) {
+                if (j == i) continue;
+                const float4 q = bpos[j];
+                const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+                if (r2 < c.hk2) delta += kernel_W(c, r2);
+            }
+        }
+    }
+    psi[i] = c.density0 / delta;
+}
+
+__global__ void k_set_w(float4* p, const float* w, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i].w = w[i];
+}
+
+// ------------------------------------------------------------------------
+// solver sweeps
+// ------------------------------------------------------------------------
+
+// Final step of every reducing sweep: the last block sums the block
+// partials in a fixed order, forms err = sum / N and evaluates the loop
+// condition of cuda_divergenceSolve / cuda_pressureSolve.
+__device__ __forceinline__ void finish_solve(SolveCtl* ctl, double* partial, double bsum,
+                                             double* lds4, bool* flag)
+{
+    if (threadIdx.x == 0) store_agent(partial + blockIdx.x, bsum);
+    if (!last_block_done(&ctl->ticket, flag)) return;
+    double acc = 0.0;
+    for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x) acc += load_agent(partial + k);
+    __syncthreads();
+    const double tot = block_sum(acc, lds4);
+    if (threadIdx.x == 0) {
+        const double err = ctl->n_div > 0.0 ? tot / ctl->n_div : 0.0;
+        const uint32_t it = ctl->iters + 1;
+        ctl->sum = tot;
+        ctl->err = err;
+        ctl->iters = it;
+        const bool cont = ((err > ctl->eta) || (it < ctl->min_iter)) && (it < ctl->max_iter);
+        ctl->done = cont ? 0u : 1u;
+        ctl->ticket = 0;
+    }
+}
+
+// Fused density + factor + divergence source, then the divergence warm start:
+// DFSPH_divergence_warmstart_init_kernel (DFSPH_core_cuda.cu:220-376) followed by
+// DFSPH_divergence_compute_kernel<true> (:495-517, divergenceSolveParticle :400-481).
+// rho_i = m_i W0 + sum m_j W_ij; alpha_i = -(1/h) / max(sum |m_j gradW|^2 + |sum m_j gradW|^2, eps);
+// rho_adv_i = 0 if fewer than 20 neighbors else max(sum m_j (v_i - v_j).gradW, 0).
+// Warm start (only where rho_adv_i > 0): v_i += h sum (k_i + k_j) m_j gradW with
+// k = max(kappaV h_ratio / 2, -0.5) (WS == 0), the predicted value (WS == 1),
+// or nothing (WS == 2).  Velocities are read from vel_i and written to vel_o.
+template <int WS>
+__global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTime* t, uint32_t n,
+                                                        const float4* __restrict__ pos,
+                                                        const float4* __restrict__ bpos,
+                                                        const uint32_t* __restrict__ nbr,
+                                                        const uint32_t* __restrict__ cnt,
+                                                        const float4* __restrict__ vel_i,
+                                                        float4* __restrict__ vel_o,
+                                                        const float* __restrict__ kws,
+                                                        float* __restrict__ rho, float* __restrict__ alpha,
+                                                        float* __restrict__ radv)
+{
+    const uint32_t i = xcd_index();
+    if (i >= n) return;
+    const float ratio = t->ratio, h = t->h, inv_h = t->inv_h;
+    const float4 pi = pos[i];
+    const float4 vi = vel_i[i];
+    const uint32_t cn = cnt[i];
+    const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
+    const uint32_t stride = c.cap;
+    auto ws_of = [&](uint32_t j) -> float {
+        if (WS == 0) return fmaxf(kws[j] * ratio / 2.0f, -0.5f);
+        if (WS == 1) return kws[j];
+        return 0.0f;
+    };
+    const float ki = ws_of(i);
+    float dens = pi.w * c.W0, sg2 = 0.0f, dr = 0.0f;
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
+    const uint32_t* p = nbr + i;
+    for (uint32_t k = 0; k < nf; ++k, p += stride) {
+        const uint32_t j = *p;
+        const float4 pj = pos[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const PairK kp = kernel_pair(c, dist2(dx, dy, dz));
+        dens += pj.w * kp.W;
+        const float mg = pj.w * kp.g;
+        const float px = mg * dx, py = mg * dy, pz = mg * dz;
+        sg2 += px * px + py * py + pz * pz;
+        gx += px; gy += py; gz += pz;
+        const float4 vj = vel_i[j];
+        dr += (vi.x - vj.x) * px + (vi.y - vj.y) * py + (vi.z - vj.z) * pz;
+        if (WS != 2) {
+            const float ks = ki + ws_of(j);
+            const float s = fabsf(ks) > kEps ? ks : 0.0f;
+            ax += s * px; ay += s * py; az += s * pz;
+        }
+    }
+    float bx = 0.0f, by = 0.0f, bz = 0.0f;
+    for (uint32_t k = 0; k < nb; ++k, p += stride) {
+        const uint32_t j = *p;
+        const float4 pj = bpos[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const PairK kp = kernel_pair(c, dist2(dx, dy, dz));
+        dens += pj.w * kp.W;
+        const float mg = pj.w * kp.g;
+        const float px = mg * dx, py = mg * dy, pz = mg * dz;
+        sg2 += px * px + py * py + pz * pz;
+        gx += px; gy += py; gz += pz;
+        dr += vi.x * px + vi.y * py + vi.z * pz;
+        bx += px; by += py; bz += pz;
+    }
+    sg2 += gx * gx + gy * gy + gz * gz;
+    const float a = -inv_h / fmaxf(sg2, kEps);
+    const float r = (nf + nb < 20u) ? 0.0f : fmaxf(dr, 0.0f);
+    rho[i] = dens;
+    alpha[i] = a;
+    radv[i] = r;
+    float4 vo = vi;
+    if (WS != 2 && r > 0.0f) {
+        if (fabsf(ki) > kEps) { ax += ki * bx; ay += ki * by; az += ki * bz; }
+        vo.x += h * ax; vo.y += h * ay; vo.z += h * az;
+    }
+    vel_o[i] = vo;
+}
+
+// Divergence source from the current velocities: computeDensityChange
+// (DFSPH_core_cuda.cu:546-614) in DFSPH_divergence_init_kernel (:618-709, INIT:
+// kappaV = 0) and DFSPH_divergence_loop_end_kernel (:732-748) + the FP64 error
+// reduction that replaces cub::DeviceReduce::Sum + cudaMemcpy (:774-781).
+// Also stores k_i = rho_adv_i * alpha_i for the next Jacobi sweep.
+template <bool INIT>
+__global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, const float4* __restrict__ pos,
+                                                       const float4* __restrict__ bpos,
+                                                       const uint32_t* __restrict__ nbr,
+                                                       const uint32_t* __restrict__ cnt,
+                                                       const float4* __restrict__ vel,
+                                                       const float* __restrict__ alpha,
+                                                       float* __restrict__ radv, float* __restrict__ kk,
+                                                       float* __restrict__ kappaV, SolveCtl* ctl,
+                                                       double* partial)
+{
+    __shared__ double lds4[kBlock / 64];
+    __shared__ bool flag;
+    if (!INIT && ctl->done) return;
+    const uint32_t i = xcd_index();
+    float r = 0.0f;
+    if (i < n) {
+        const float4 pi = pos[i];
+        const float4 vi = vel[i];
+        const uint32_t cn = cnt[i];
+        const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
+        const uint32_t* p = nbr + i;
+        float dr = 0.0f;
+        for (uint32_t k = 0; k < nf; ++k, p += c.cap) {
+            const uint32_t j = *p;
+            const float4 pj = pos[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float mg = pj.w * kernel_grad_coef(c, dist2(dx, dy, dz));
+            const float4 vj = vel[j];
+            dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+        }
+        for (uint32_t k = 0; k < nb; ++k, p += c.cap) {
+            const uint32_t j = *p;
+            const float4 pj = bpos[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float mg = pj.w * kernel_grad_coef(c, dist2(dx, dy, dz));
+            dr += mg * (vi.x * dx + vi.y * dy + vi.z * dz);
+        }
+        r = (nf + nb < 20u) ? 0.0f : fmaxf(dr, 0.0f);
+        radv[i] = r;
+        kk[i] = r * alpha[i];
+        if (INIT) kappaV[i] = 0.0f;
+    }
+    if (!INIT) {
+        const double bs = block_sum((double)r, lds4);
+        finish_solve(ctl, partial, bs, lds4, &flag);
+    }
+}
+
+// Jacobi velocity update: divergenceSolveParticle<false> (DFSPH_core_cuda.cu:400-481,
+// DIV: kappaV += k_i, step h) and pressureSolveParticle<false> (:867-951, step h_future):
+// v_i += step (sum_f [|k_i + k_j| > eps] (k_i + k_j) m_j gradW + [|k_i| > eps] sum_b k_i psi_j gradW)
+template <bool DIV>
+__global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t, uint32_t n,
+                                                   const float4* __restrict__ pos,
+                                                   const float4* __restrict__ bpos,
+                                                   const uint32_t* __restrict__ nbr,
+                                                   const uint32_t* __restrict__ cnt,
+                                                   const float* __restrict__ kk, float* __restrict__ kappaV,
+                                                   float4* __restrict__ vel, const SolveCtl* ctl)
+{
+    if (ctl->done) return;
+    const uint32_t i = xcd_index();
+    if (i >= n) return;
+    const float ki = kk[i];
+    if (DIV) kappaV[i] += ki;
+    const float4 pi = pos[i];
+    const uint32_t cn = cnt[i];
+    const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
+    const uint32_t* p = nbr + i;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    for (uint32_t k = 0; k < nf; ++k, p += c.cap) {
+        const uint32_t j = *p;
+        const float4 pj = pos[j];
+        const float ks = ki + kk[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float g = kernel_grad_coef(c, dist2(dx, dy, dz));
+        const float s = fabsf(ks) > kEps ? ks * pj.w * g : 0.0f;
+        ax += s * dx; ay += s * dy; az += s * dz;
+    }
+    if (fabsf(ki) > kEps) {
+        for (uint32_t k = 0; k < nb; ++k, p += c.cap) {
+            const uint32_t j = *p;
+            const float4 pj = bpos[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float s = ki * pj.w * kernel_grad_coef(c, dist2(dx, dy, dz));
+            ax += s * dx; ay += s * dy; az += s * dz;
+        }
+    }
+    const float step = DIV ? t->h : t->h_future;
+    float4 v = vel[i];
+    v.x += step * ax; v.y += step * ay; v.z += step * az;
+    vel[i] = v;
+}
+
+// DFSPH_viscosityXSPH_kernel (DFSPH_core_cuda.cu:1423-1510):
+// a_i = g - sum_f (1/h) nu (m_j / rho_j) (v_i - v_j) W_ij, fused with the CFL
+// maximum of |v + a h|^2 (DFSPH_CFLVelSquaredNorm_kernel, :3273-3278).
+__global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, uint32_t n,
+                                                 const float4* __restrict__ pos,
+                                                 const uint32_t* __restrict__ nbr,
+                                                 const uint32_t* __restrict__ cnt,
+                                                 const float4* __restrict__ vel,
+                                                 const float* __restrict__ rho, float4* __restrict__ acc,
+                                                 DevCtl* ctl, float* partial)
+{
+    __shared__ float lds4[kBlock / 64];
+    __shared__ bool flag;
+    const uint32_t i = xcd_index();
+    float m2 = 0.0f;
+    if (i < n) {
+        const float coef = t->inv_h * c.viscosity;
+        const float4 pi = pos[i];
+        const float4 vi = vel[i];
+        const uint32_t nf = cnt[i] & 0xffffu;
+        const uint32_t* p = nbr + i;
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        for (uint32_t k = 0; k < nf; ++k, p += c.cap) {
+            const uint32_t j = *p;
+            const float4 pj = pos[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float w = kernel_W(c, dist2(dx, dy, dz));
+            const float4 vj = vel[j];
+            const float s = coef * (pj.w / rho[j]) * w;
+            ax -= s * (vi.x - vj.x); ay -= s * (vi.y - vj.y); az -= s * (vi.z - vj.z);
+        }
+        const float4 a = make_float4(c.gx + ax, c.gy + ay, c.gz + az, 0.0f);
+        acc[i] = a;
+        const float h = t->h;
+        const float wx = vi.x + a.x * h, wy = vi.y + a.y * h, wz = vi.z + a.z * h;
+        m2 = wx * wx + wy * wy + wz * wz;
+    }
+    float m = wave_max(m2);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds4[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float bm = lds4[0];
+        for (int k = 1; k < kBlock / 64; ++k) bm = fmaxf(bm, lds4[k]);
+        store_agent_f(partial + blockIdx.x, bm);
+    }
+    if (!last_block_done(&ctl->cfl_ticket, &flag)) return;
+    float mm = 0.0f;
+    for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x) mm = fmaxf(mm, load_agent_f(partial + k));
+    mm = wave_max(mm);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds4[w] = mm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float bm = lds4[0];
+        for (int k = 1; k < kBlock / 64; ++k) bm = fmaxf(bm, lds4[k]);
+        ctl->cfl_max = bm;
+        ctl->cfl_ticket = 0;
+    }
+}
+
+// DFSPH_update_vel_kernel (DFSPH_core_cuda.cu:3047-3070): v += h a and the
+// pressure warm-start rescale kappa = max(kappa h^2 / h_f^2, -0.5).
+__global__ void k_update_vel(const DevTime* t, uint32_t n, const float4* acc, float4* vel, float* kappa)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float h = t->h;
+    const float4 a = acc[i];
+    float4 v = vel[i];
+    v.x += h * a.x; v.y += h * a.y; v.z += h * a.z;
+    vel[i] = v;
+    kappa[i] = fmaxf(kappa[i] * t->ratio2, -0.5f);
+}
+
+// Pressure warm start cuda_pressure_compute<true> (DFSPH_core_cuda.cu:965-1000,
+// pressureSolveParticle<true> :867-951), optionally fused with the velocity
+// update of DFSPH_update_vel_kernel (APPLY_ACC):
+// v_i += h a_i;  v_i += h_f (sum_f [|k_i+k_j|>eps] (k_i+k_j) m_j gradW + [|k_i|>eps] sum_b k_i psi_j gradW)
+// with k = max(kappa ratio2, -0.5) (WS == 0; ratio2 = 1 when already applied),
+// the predicted value (WS == 1).  No gate (unlike the divergence warm start).
+template <int WS, bool APPLY_ACC>
+__global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTime* t, uint32_t n,
+                                                        const float4* __restrict__ pos,
+                                                        const float4* __restrict__ bpos,
+                                                        const uint32_t* __restrict__ nbr,
+                                                        const uint32_t* __restrict__ cnt,
+                                                        const float4* __restrict__ acc,
+                                                        float4* __restrict__ vel,
+                                                        const float* __restrict__ kws, int rescale)
+{
+    const uint32_t i = xcd_index();
+    if (i >= n) return;
+    const float ratio2 = rescale ? t->ratio2 : 1.0f;
+    auto ws_of = [&](uint32_t j) -> float {
+        if (WS == 0) return fmaxf(kws[j] * ratio2, -0.5f);
+        return kws[j];
+    };
+    const float ki = ws_of(i);
+    const float4 pi = pos[i];
+    const uint32_t cn = cnt[i];
+    const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
+    const uint32_t* p = nbr + i;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    for (uint32_t k = 0; k < nf; ++k, p += c.cap) {
+        const uint32_t j = *p;
+        const float4 pj = pos[j];
+        const float ks = ki + ws_of(j);
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float g = kernel_grad_coef(c, dist2(dx, dy, dz));
+        const float s = fabsf(ks) > kEps ? ks * pj.w * g : 0.0f;
+        ax += s * dx; ay += s * dy; az += s * dz;
+    }
+    if (fabsf(ki) > kEps) {
+        for (uint32_t k = 0; k < nb; ++k, p += c.cap) {
+            const uint32_t j = *p;
+            const float4 pj = bpos[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float s = ki * pj.w * kernel_grad_coef(c, dist2(dx, dy, dz));
+            ax += s * dx; ay += s * dy; az += s * dz;
+        }
+    }
+    float4 v = vel[i];
+    if (APPLY_ACC) {
+        const float h = t->h;
+        const float4 a = acc[i];
+        v.x += h * a.x; v.y += h * a.y; v.z += h * a.z;
+    }
+    const float hf = t->h_future;
+    v.x += hf * ax; v.y += hf * ay; v.z += hf * az;
+    vel[i] = v;
+}
+
+// Density source computeDensityAdv (DFSPH_core_cuda.cu:1165-1215):
+// rho_adv = max(rho + h_f sum m_j (v_i - v_j).gradW - rho0, 0); kappa += rho_adv alpha.
+// INIT: DFSPH_pressure_init_kernel (:1217-1262): alpha *= 1/h_f, kappa = 0 first.
+// Otherwise DFSPH_pressure_loop_end_kernel (:1264-1284) + FP64 error reduction
+// (replaces cub Sum + cudaMemcpy, :1321-1325).
+template <bool INIT>
+__global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const DevTime* t, uint32_t n,
+                                                           const float4* __restrict__ pos,
+                                                           const float4* __restrict__ bpos,
+                                                           const uint32_t* __restrict__ nbr,
+                                                           const uint32_t* __restrict__ cnt,
+                                                           const float4* __restrict__ vel,
+                                                           const float* __restrict__ rho,
+                                                           float* __restrict__ alpha, float* __restrict__ radv,
+                                                           float* __restrict__ kk, float* __restrict__ kappa,
+                                                           SolveCtl* ctl, double* partial)
+{
+    __shared__ double lds4[kBlock / 64];
+    __shared__ bool flag;
+    if (!INIT && ctl->done) return;
+    const uint32_t i = xcd_index();
+    float r = 0.0f;
+    if (i < n) {
+        const float hf = t->h_future;
+        const float4 pi = pos[i];
+        const float4 vi = vel[i];
+        const uint32_t cn = cnt[i];
+        const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
+        const uint32_t* p = nbr + i;
+        float dr = 0.0f;
+        for (uint32_t k = 0; k < nf; ++k, p += c.cap) {
+            const uint32_t j = *p;
+            const float4 pj = pos[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float mg = pj.w * kernel_grad_coef(c, dist2(dx, dy, dz));
+            const float4 vj = vel[j];
+            dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+        }
+        for (uint32_t k = 0; k < nb; ++k, p += c.cap) {
+            const uint32_t j = *p;
+            const float4 pj = bpos[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float mg = pj.w * kernel_grad_coef(c, dist2(dx, dy, dz));
+            dr += mg * (vi.x * dx + vi.y * dy + vi.z * dz);
+        }
+        float a = alpha[i];
+        if (INIT) { a *= t->inv_h_future; alpha[i] = a; }
+        r = fmaxf(rho[i] + hf * dr - c.density0, 0.0f);
+        radv[i] = r;
+        const float ka = r * a;
+        kk[i] = ka;
+        kappa[i] = INIT ? ka : kappa[i] + ka;
+    }
+    if (!INIT) {
+        const double bs = block_sum((double)r, lds4);
+        finish_solve(ctl, partial, bs, lds4, &flag);
+    }
+}
+
+// DFSPH_update_pos_kernel (DFSPH_core_cuda.cu:3127-3214): x += h v, fused with the
+// escape count of tag_particles_outside_bounding_box_kernel
+// (SPH_other_systems_cuda.cu:1244-1281): outside if any coordinate leaves the box.
+__global__ void k_update_pos(const DevTime* t, uint32_t n, const float4* vel, float4* pos,
+                             float bx0, float by0, float bz0, float bx1, float by1, float bz1, DevCtl* ctl)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t out = 0;
+    if (i < n) {
+        const float h = t->h;
+        const float4 v = vel[i];
+        float4 x = pos[i];
+        x.x += h * v.x; x.y += h * v.y; x.z += h * v.z;
+        pos[i] = x;
+        out = (x.x > bx1 || x.y > by1 || x.z > bz1 || x.x < bx0 || x.y < by0 || x.z < bz0) ? 1u : 0u;
+    }
+    const uint32_t s = wave_sum_u(out);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&ctl->escaped, s);
+}
+
+// keep flags for the compaction (tag_particles_outside_bounding_box_kernel)
+__global__ void k_keep_inside(uint32_t n, const float4* pos, float bx0, float by0, float bz0,
+                              float bx1, float by1, float bz1, uint32_t* keep)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) { keep[n] = 0; return; }
+    const float4 x = pos[i];
+    const bool out = x.x > bx1 || x.y > by1 || x.z > bz1 || x.x < bx0 || x.y < by0 || x.z < bz0;
+    keep[i] = out ? 0u : 1u;
+}
+
+__global__ void k_keep_from_u8(uint32_t n, const uint8_t* k8, uint32_t* keep)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    keep[i] = (i < n && k8[i]) ? 1u : 0u;
+}
+
+// Stable compaction: the kept particles move to their exclusive-scan rank
+// (replaces the radix sort by tag + cuda_sortData of remove_tagged_particles,
+// SPH_other_systems_cuda.cu:1606-1641).
+__global__ void k_compact(uint32_t n, const uint32_t* keep, const uint32_t* rank,
+                          const float4* pos_i, const float4* vel_i, const float* kap_i,
+                          const float* kapv_i, const uint32_t* id_i,
+                          float4* pos_o, float4* vel_o, float* kap_o, float* kapv_o, uint32_t* id_o)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !keep[i]) return;
+    const uint32_t o = rank[i];
+    pos_o[o] = pos_i[i];
+    vel_o[o] = vel_i[i];
+    kap_o[o] = kap_i[i];
+    kapv_o[o] = kapv_i[i];
+    id_o[o] = id_i[i];
+}
+
+// ------------------------------------------------------------------------
+// host launch wrappers
+// ------------------------------------------------------------------------
+static inline uint32_t nblk(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+static inline uint32_t nblk_xcd(uint32_t n)
+{
+    const uint32_t b = nblk(n);
+    return ((b + kNumXcd - 1) / kNumXcd) * kNumXcd;
+}
+
+uint32_t grid_blocks_xcd(uint32_t n) { return nblk_xcd(n ? n : 1); }
+
+void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint32_t n)
+{
+    if (n) k_fill_u32<<<nblk(n), kBlock, 0, s>>>(p, v, n);
+}
+
+void launch_fill_f32(hipStream_t s, float* p, float v, uint32_t n)
+{
+    if (n) k_fill_f32<<<nblk(n), kBlock, 0, s>>>(p, v, n);
+}
+
+size_t scan_temp_words(uint32_t n)
+{
+    size_t words = 0;
+    uint32_t m = n;
+    while (m > 1) {
+        const uint32_t tiles = (m + kScanTile - 1) / kScanTile;
+        words += 2 * (size_t)tiles + 2;
+        m = tiles;
+    }
+    return words + 4;
+}
+
+void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp)
+{
+    if (n == 0) return;
+    const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
+    uint32_t* sums = tmp;
+    uint32_t* offs = tmp + tiles + 1;
+    k_scan_tiles<<<tiles, kBlock, 0, s>>>(in, out, n, sums);
+    if (tiles > 1) {
+        launch_exclusive_scan(s, sums, offs, tiles, tmp + 2 * (size_t)tiles + 2);
+        k_scan_add<<<tiles, kBlock, 0, s>>>(out, n, offs);
+    }
+}
+
+void launch_ctl_init(hipStream_t s, SolveCtl* ctl, double eta_base, int divide_by_h, const DevTime* t,
+                     double n_div, uint32_t min_iter, uint32_t max_iter)
+{
+    k_ctl_init<<<1, 64, 0, s>>>(ctl, eta_base, divide_by_h, t, n_div, min_iter, max_iter);
+}
+
+void launch_step_ctl_init(hipStream_t s, DevCtl* c, uint32_t n_active)
+{
+    k_step_ctl_init<<<1, 64, 0, s>>>(c, n_active);
+}
+
+void launch_nbr_ctl_init(hipStream_t s, DevCtl* c) { k_nbr_ctl_init<<<1, 64, 0, s>>>(c); }
+
+void launch_time_reset(hipStream_t s, DevTime* t, double h) { k_time_reset<<<1, 64, 0, s>>>(t, h); }
+
+void launch_time_update(hipStream_t s, DevTime* t, double hf, const DevCtl* ctl, int mode,
+                        double cfl_factor, double diameter, double dt_min, double dt_max, double seed)
+{
+    k_time_update<<<1, 64, 0, s>>>(t, hf, ctl, mode, cfl_factor, diameter, dt_min, dt_max, seed);
+}
+
+void launch_time_step_end(hipStream_t s, DevTime* t) { k_time_step_end<<<1, 64, 0, s>>>(t); }
+
+void launch_time_set(hipStream_t s, DevTime* t, double h, double hp, double time)
+{
+    k_time_set<<<1, 64, 0, s>>>(t, h, hp, time);
+}
+
+void launch_cell_hist(hipStream_t s, const SimConst& c, const float4* pos, uint32_t n, uint32_t* key,
+                      uint32_t* slot, uint32_t* hist)
+{
+    if (n) k_cell_hist<<<nblk(n), kBlock, 0, s>>>(c, pos, n, key, slot, hist);
+}
+
+void launch_scatter_perm(hipStream_t s, const uint32_t* key, const uint32_t* slot, const uint32_t* cstart,
+                         uint32_t n, uint32_t* perm)
+{
+    if (n) k_scatter_perm<<<nblk(n), kBlock, 0, s>>>(key, slot, cstart, n, perm);
+}
+
+void launch_cell_sort(hipStream_t s, const uint32_t* cstart, uint32_t ncell, uint32_t* perm)
+{
+    k_cell_sort<<<nblk(ncell), kBlock, 0, s>>>(cstart, ncell, perm);
+}
+
+void launch_gather_fluid(hipStream_t s, uint32_t n, const uint32_t* perm, const float4* pi, const float4* vi,
+                         const float* ki, const float* kvi, const uint32_t* idi, float4* po, float4* vo,
+                         float* ko, float* kvo, uint32_t* ido)
+{
+    if (n) k_gather_fluid<<<nblk(n), kBlock, 0, s>>>(n, perm, pi, vi, ki, kvi, idi, po, vo, ko, kvo, ido);
+}
+
+void launch_gather_f4(hipStream_t s, uint32_t n, const uint32_t* perm, const float4* in, float4* out)
+{
+    if (n) k_gather_f4<<<nblk(n), kBlock, 0, s>>>(n, perm, in, out);
+}
+
+void launch_neighbors(hipStream_t s, const SimConst& c, bool indirect, const float4* pos, const uint32_t* cs_f,
+                      const uint32_t* perm, const float4* bpos, const uint32_t* cs_b, uint32_t n,
+                      uint32_t* nbr, uint32_t* cnt, DevCtl* ctl)
+{
+    if (!n) return;
+    if (indirect)
+        k_neighbors<true><<<nblk_xcd(n), kBlock, 0, s>>>(c, pos, cs_f, perm, bpos, cs_b, n, nbr, cnt, ctl);
+    else
+        k_neighbors<false><<<nblk_xcd(n), kBlock, 0, s>>>(c, pos, cs_f, perm, bpos, cs_b, n, nbr, cnt, ctl);
+}
+
+void launch_boundary_psi(hipStream_t s, const SimConst& c, const float4* bpos, const uint32_t* cs_b,
+                         uint32_t nb, float* psi)
+{
+    if (nb) k_boundary_psi<<<nblk(nb), kBlock, 0, s>>>(c, bpos, cs_b, nb, psi);
+}
+
+void launch_set_w(hipStream_t s, float4* p, const float* w, uint32_t n)
+{
+    if (n) k_set_w<<<nblk(n), kBlock, 0, s>>>(p, w, n);
+}
+
+void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime* t, uint32_t n,
+                        const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
+                        const float4* vel_i, float4* vel_o, const float* kws, float* rho, float* alpha,
+                        float* radv)
+{
+    if (!n) return;
+    const uint32_t g = nblk_xcd(n);
+    if (ws == 0)
+        k_div_ws_init<0><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel_i, vel_o, kws, rho, alpha, radv);
+    else if (ws == 1)
+        k_div_ws_init<1><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel_i, vel_o, kws, rho, alpha, radv);
+    else
+        k_div_ws_init<2><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel_i, vel_o, kws, rho, alpha, radv);
+}
+
+void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
+                       const float4* bpos, const uint32_t* nbr, const uint32_t* cnt, const float4* vel,
+                       const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial)
+{
+    const uint32_t g = nblk_xcd(n ? n : 1);
+    if (init)
+        k_div_source<true><<<g, kBlock, 0, s>>>(c, n, pos, bpos, nbr, cnt, vel, alpha, radv, kk, kappaV, ctl, partial);
+    else
+        k_div_source<false><<<g, kBlock, 0, s>>>(c, n, pos, bpos, nbr, cnt, vel, alpha, radv, kk, kappaV, ctl, partial);
+}
+
+void launch_jacobi(hipStream_t s, bool div, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
+                   const float4* bpos, const uint32_t* nbr, const uint32_t* cnt, const float* kk, float* kappaV,
+                   float4* vel, const SolveCtl* ctl)
+{
+    if (!n) return;
+    const uint32_t g = nblk_xcd(n);
+    if (div)
+        k_jacobi<true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, kk, kappaV, vel, ctl);
+    else
+        k_jacobi<false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, kk, kappaV, vel, ctl);
+}
+
+void launch_xsph(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
+                 const uint32_t* nbr, const uint32_t* cnt, const float4* vel, const float* rho, float4* acc,
+                 DevCtl* ctl, float* partial)
+{
+    k_xsph<<<nblk_xcd(n ? n : 1), kBlock, 0, s>>>(c, t, n, pos, nbr, cnt, vel, rho, acc, ctl, partial);
+}
+
+void launch_update_vel(hipStream_t s, const DevTime* t, uint32_t n, const float4* acc, float4* vel, float* kappa)
+{
+    if (n) k_update_vel<<<nblk(n), kBlock, 0, s>>>(t, n, acc, vel, kappa);
+}
+
+void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c, const DevTime* t, uint32_t n,
+                        const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
+                        const float4* acc, float4* vel, const float* kws, int rescale)
+{
+    if (!n) return;
+    const uint32_t g = nblk_xcd(n);
+    if (ws == 1) {
+        if (apply_acc)
+            k_pressure_ws<1, true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale);
+        else
+            k_pressure_ws<1, false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale);
+    } else {
+        if (apply_acc)
+            k_pressure_ws<0, true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale);
+        else
+            k_pressure_ws<0, false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale);
+    }
+}
+
+void launch_density_source(hipStream_t s, bool init, const SimConst& c, const DevTime* t, uint32_t n,
+                           const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
+                           const float4* vel, const float* rho, float* alpha, float* radv, float* kk,
+                           float* kappa, SolveCtl* ctl, double* partial)
+{
+    const uint32_t g = nblk_xcd(n ? n : 1);
+    if (init)
+        k_density_source<true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel, rho, alpha, radv, kk, kappa, ctl, partial);
+    else
+        k_density_source<false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel, rho, alpha, radv, kk, kappa, ctl, partial);
+}
+
+void launch_update_pos(hipStream_t s, const DevTime* t, uint32_t n, const float4* vel, float4* pos,
+                       const float* bmin, const float* bmax, DevCtl* ctl)
+{
+    if (n)
+        k_update_pos<<<nblk(n), kBlock, 0, s>>>(t, n, vel, pos, bmin[0], bmin[1], bmin[2], bmax[0], bmax[1], bmax[2], ctl);
+}
+
+void launch_keep_inside(hipStream_t s, uint32_t n, const float4* pos, const float* bmin, const float* bmax,
+                        uint32_t* keep)
+{
+    k_keep_inside<<<nblk(n + 1), kBlock, 0, s>>>(n, pos, bmin[0], bmin[1], bmin[2], bmax[0], bmax[1], bmax[2], keep);
+}
+
+void launch_keep_from_u8(hipStream_t s, uint32_t n, const uint8_t* k8, uint32_t* keep)
+{
+    k_keep_from_u8<<<nblk(n + 1), kBlock, 0, s>>>(n, k8, keep);
+}
+
+void launch_compact(hipStream_t s, uint32_t n, const uint32_t* keep, const uint32_t* rank, const float4* pi,
+                    const float4* vi, const float* ki, const float* kvi, const uint32_t* idi, float4* po,
+                    float4* vo, float* ko, float* kvo, uint32_t* ido)
+{
+    if (n) k_compact<<<nblk(n), kBlock, 0, s>>>(n, keep, rank, pi, vi, ki, kvi, idi, po, vo, ko, kvo, ido);
+}
+
+}  // namespace dfsph

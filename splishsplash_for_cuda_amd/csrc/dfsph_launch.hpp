@@ -1,0 +1,74 @@
+// dfsph_launch.hpp -- host launch wrappers of dfsph_kernels.hip (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "dfsph_device.hpp"
+
+namespace dfsph {
+
+uint32_t grid_blocks_xcd(uint32_t n);
+void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint32_t n);
+void launch_fill_f32(hipStream_t s, float* p, float v, uint32_t n);
+size_t scan_temp_words(uint32_t n);
+void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp);
+
+void launch_ctl_init(hipStream_t s, SolveCtl* ctl, double eta_base, int divide_by_h, const DevTime* t,
+                     double n_div, uint32_t min_iter, uint32_t max_iter);
+void launch_step_ctl_init(hipStream_t s, DevCtl* c, uint32_t n_active);
+void launch_nbr_ctl_init(hipStream_t s, DevCtl* c);
+void launch_time_reset(hipStream_t s, DevTime* t, double h);
+void launch_time_update(hipStream_t s, DevTime* t, double hf, const DevCtl* ctl, int mode,
+                        double cfl_factor, double diameter, double dt_min, double dt_max, double seed);
+void launch_time_step_end(hipStream_t s, DevTime* t);
+void launch_time_set(hipStream_t s, DevTime* t, double h, double hp, double time);
+
+void launch_cell_hist(hipStream_t s, const SimConst& c, const float4* pos, uint32_t n, uint32_t* key,
+                      uint32_t* slot, uint32_t* hist);
+void launch_scatter_perm(hipStream_t s, const uint32_t* key, const uint32_t* slot, const uint32_t* cstart,
+                         uint32_t n, uint32_t* perm);
+void launch_cell_sort(hipStream_t s, const uint32_t* cstart, uint32_t ncell, uint32_t* perm);
+void launch_gather_fluid(hipStream_t s, uint32_t n, const uint32_t* perm, const float4* pi, const float4* vi,
+                         const float* ki, const float* kvi, const uint32_t* idi, float4* po, float4* vo,
+                         float* ko, float* kvo, uint32_t* ido);
+void launch_gather_f4(hipStream_t s, uint32_t n, const uint32_t* perm, const float4* in, float4* out);
+void launch_neighbors(hipStream_t s, const SimConst& c, bool indirect, const float4* pos, const uint32_t* cs_f,
+                      const uint32_t* perm, const float4* bpos, const uint32_t* cs_b, uint32_t n,
+                      uint32_t* nbr, uint32_t* cnt, DevCtl* ctl);
+void launch_boundary_psi(hipStream_t s, const SimConst& c, const float4* bpos, const uint32_t* cs_b,
+                         uint32_t nb, float* psi);
+void launch_set_w(hipStream_t s, float4* p, const float* w, uint32_t n);
+
+void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime* t, uint32_t n,
+                        const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
+                        const float4* vel_i, float4* vel_o, const float* kws, float* rho, float* alpha,
+                        float* radv);
+void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
+                       const float4* bpos, const uint32_t* nbr, const uint32_t* cnt, const float4* vel,
+                       const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial);
+void launch_jacobi(hipStream_t s, bool div, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
+                   const float4* bpos, const uint32_t* nbr, const uint32_t* cnt, const float* kk, float* kappaV,
+                   float4* vel, const SolveCtl* ctl);
+void launch_xsph(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
+                 const uint32_t* nbr, const uint32_t* cnt, const float4* vel, const float* rho, float4* acc,
+                 DevCtl* ctl, float* partial);
+void launch_update_vel(hipStream_t s, const DevTime* t, uint32_t n, const float4* acc, float4* vel, float* kappa);
+void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c, const DevTime* t, uint32_t n,
+                        const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
+                        const float4* acc, float4* vel, const float* kws, int rescale);
+void launch_density_source(hipStream_t s, bool init, const SimConst& c, const DevTime* t, uint32_t n,
+                           const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
+                           const float4* vel, const float* rho, float* alpha, float* radv, float* kk,
+                           float* kappa, SolveCtl* ctl, double* partial);
+void launch_update_pos(hipStream_t s, const DevTime* t, uint32_t n, const float4* vel, float4* pos,
+                       const float* bmin, const float* bmax, DevCtl* ctl);
+void launch_keep_inside(hipStream_t s, uint32_t n, const float4* pos, const float* bmin, const float* bmax,
+                        uint32_t* keep);
+void launch_keep_from_u8(hipStream_t s, uint32_t n, const uint8_t* k8, uint32_t* keep);
+void launch_compact(hipStream_t s, uint32_t n, const uint32_t* keep, const uint32_t* rank, const float4* pi,
+                    const float4* vi, const float* ki, const float* kvi, const uint32_t* idi, float4* po,
+                    float4* vo, float* ko, float* kvo, uint32_t* ido);
+
+}  // namespace dfsph

@@ -1,0 +1,293 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Tolerances (FP32 device arithmetic vs the FP64 oracle, same inputs):
+  neighbor lists       bit-exact as sorted sets (FP32 predicate in both)
+  boundary psi         rel 2e-6
+  density              rel 2e-6 of rho0
+  factor (alpha)       rel 1e-4 of max |alpha|
+  divergence source    1e-4 of max |rho_adv| + 1e-3 rho0/h * 1e-6
+  velocities           5e-5 of (max |v| + g h) after a full step
+  positions            1e-6 of the particle radius after one step
+  iteration counts     equal
+"""
+import numpy as np
+import pytest
+
+import splishsplash_for_cuda_amd as S
+from splishsplash_for_cuda_amd import scene
+from gpu_util import boundary_index_map, gpu_neighbor_sets, max_rel, oracle_neighbor_sets
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(sc, psi=None, **kw):
+    p = S.default_params(particle_radius=sc.particle_radius, **kw)
+    s = S.Solver(p)
+    s.upload_boundary(sc.boundary, psi)
+    s.upload_fluid(sc.fluid)
+    return s
+
+
+def _oracle(om, sc, **kw):
+    o = om.Oracle(particle_radius=sc.particle_radius, fp32_predicate=1, **kw)
+    o.set_boundary(sc.boundary.astype(np.float64))
+    o.set_fluid(sc.fluid.astype(np.float64))
+    return o
+
+
+@pytest.mark.parametrize("sort", [True, False])
+def test_neighbor_lists_bit_exact_config1(oracle_mod, sort):
+    sc = scene.config1()
+    s = _solver(sc)
+    s.neighbor_search(sort=sort)
+    inf = s.info()
+    assert inf.overflow == 0
+    hk = np.float32(4 * sc.particle_radius)
+    nf, nb, L, mx = oracle_mod.neighbors_fp32(sc.fluid, sc.boundary, hk, brute=True)
+    want = oracle_neighbor_sets(nf, nb, L)
+    got = gpu_neighbor_sets(s, boundary_index_map(s, sc.boundary))
+    assert len(got) == len(want) == sc.n_fluid
+    diff = [i for i in want if got[i] != want[i]]
+    assert not diff, "%d particles differ, first %d" % (len(diff), diff[0])
+    assert inf.max_neighbors_seen == mx
+
+
+def test_neighbor_lists_bit_exact_random_cloud(oracle_mod):
+    """Ragged densities, duplicates and pairs at exactly h."""
+    rng = np.random.default_rng(21)
+    r = 0.02
+    hk = np.float32(4 * r)
+    fl = rng.uniform(0.1, 0.5, size=(6000, 3)).astype(np.float32)
+    fl[:300] = rng.uniform(0.1, 0.14, size=(300, 3)).astype(np.float32)   # crowded corner
+    fl[400] = fl[401]
+    fl[500] = fl[501] + np.float32([hk, 0, 0])
+    b = scene.box_surface((0, 0, 0), (0.6, 0.6, 0.6), 2 * r).astype(np.float32)
+    sc = scene.Scene("cloud", r, fl, b)
+    s = _solver(sc, max_neighbors=255)
+    s.neighbor_search()
+    nf, nb, L, mx = oracle_mod.neighbors_fp32(fl, b, hk, brute=True, max_nbr=255)
+    assert mx <= 255
+    got = gpu_neighbor_sets(s, boundary_index_map(s, b))
+    want = oracle_neighbor_sets(nf, nb, L)
+    assert got == want
+
+
+def test_boundary_psi_matches_oracle(oracle_mod):
+    sc = scene.config1()
+    s = _solver(sc)
+    o = oracle_mod.Oracle(particle_radius=sc.particle_radius)
+    o.set_boundary(sc.boundary.astype(np.float64))
+    psi_o = o.boundary_psi(sc.n_boundary)
+    bmap = boundary_index_map(s, sc.boundary)
+    psi_g = s.download(S.FIELD_BOUNDARY_PSI)
+    assert max_rel(psi_g, psi_o[bmap]) < 2e-6
+
+
+def _by_id(state):
+    return state
+
+
+def _phase_compare(om, sc, dt, steps, report):
+    o = _oracle(om, sc, dt=dt)
+    psi = o.boundary_psi(sc.n_boundary)
+    s = _solver(sc, psi=psi.astype(np.float32), dt=dt)
+    rho0, g = 1000.0, 9.81
+    worst = {}
+    for k in range(steps):
+        s.neighbor_search()
+        o.neighbor_search()
+        itv_g, _ = s.divergence_solve()
+        itv_o, _ = o.divergence_solve()
+        gs, os_ = s.state_by_id(), o.state()
+        assert itv_g == itv_o, (k, itv_g, itv_o)
+        vscale = np.abs(os_["vel"]).max() + g * dt
+        e = {
+            "density": max_rel(gs["density"], os_["density"], rho0),
+            "factor": max_rel(gs["factor"], os_["factor"]),
+            "vel_div": max_rel(gs["vel"], os_["vel"], vscale),
+        }
+        s.external_forces()
+        o.external_forces()
+        e["acc"] = max_rel(s.state_by_id()["acc"], o.get(om.F_ACC), g)
+        s.update_time_step(dt)
+        o.update_time_step(dt)
+        s.update_vel()
+        o.update_vel()
+        it_g, _ = s.pressure_solve()
+        it_o, _ = o.pressure_solve()
+        assert it_g == it_o, (k, it_g, it_o)
+        s.update_pos()
+        o.update_pos()
+        s.step_end()
+        o.step_end()
+        gs, os_ = s.state_by_id(), o.state()
+        vscale = np.abs(os_["vel"]).max() + g * dt
+        e["vel"] = max_rel(gs["vel"], os_["vel"], vscale)
+        e["pos"] = max_rel(gs["pos"], os_["pos"], sc.particle_radius)
+        e["kappa"] = max_rel(gs["kappa"], os_["kappa"])
+        e["kappaV"] = max_rel(gs["kappaV"], os_["kappaV"])
+        for kk, v in e.items():
+            worst[kk] = max(worst.get(kk, 0.0), v)
+        report.append((k, itv_g, it_g, e))
+    return worst
+
+
+def test_one_step_phases_match_oracle(oracle_mod):
+    sc = scene.small_dam_break()
+    report = []
+    w = _phase_compare(oracle_mod, sc, 0.002, 1, report)
+    print("one-step worst relative errors:", w)
+    assert w["density"] < 2e-6
+    assert w["factor"] < 1e-4
+    assert w["vel_div"] < 5e-5
+    assert w["acc"] < 1e-5
+    assert w["vel"] < 5e-5
+    assert w["pos"] < 1e-6
+
+
+def test_short_trajectory_matches_oracle(oracle_mod):
+    """Config-1 dam break, first 10 steps: field errors stay small."""
+    sc = scene.config1()
+    report = []
+    w = _phase_compare(oracle_mod, sc, 0.002, 10, report)
+    for row in report:
+        print(row)
+    print("10-step worst relative errors:", w)
+    assert w["density"] < 2e-5
+    assert w["vel"] < 1e-3
+    assert w["pos"] < 1e-4
+
+
+def test_full_step_entry_point_matches_phase_sequence():
+    """dfsph_step (fused path) gives the same state as the phase-by-phase ABI calls."""
+    sc = scene.small_dam_break()
+    a = _solver(sc, dt=0.002)
+    b = _solver(sc, dt=0.002)
+    for _ in range(5):
+        a.step()
+        b.neighbor_search()
+        b.divergence_solve()
+        b.external_forces()
+        b.update_time_step(0.002)
+        b.update_vel()
+        b.pressure_solve()
+        b.update_pos()
+        b.step_end()
+        b.check_particles_positions(2)
+    sa, sb = a.state_by_id(), b.state_by_id()
+    for k in ("pos", "vel", "kappa", "kappaV", "density"):
+        assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_host_class_step_matches_abi_step():
+    """SPH::DFSPHCUDA::step() (C++ host over the C ABI) == dfsph_step."""
+    sc = scene.small_dam_break()
+    p = S.default_params(particle_radius=sc.particle_radius, dt=0.002)
+    host = S.HostSolver(p, sc.boundary, sc.fluid)
+    abi = _solver(sc, dt=0.002)
+    for _ in range(5):
+        st = host.step()
+        sa = abi.step()
+        assert (st.iters_v, st.iters) == (sa.iters_v, sa.iters)
+    x1, x2 = host.view.state_by_id(), abi.state_by_id()
+    assert np.array_equal(x1["pos"], x2["pos"]) and np.array_equal(x1["vel"], x2["vel"])
+    host.close()
+
+
+def test_runs_are_bit_reproducible():
+    sc = scene.config1()
+    out = []
+    for _ in range(2):
+        s = _solver(sc, dt=0.002)
+        for _ in range(5):
+            s.step()
+        out.append(s.state_by_id())
+    for k in ("id", "pos", "vel", "kappa", "kappaV"):
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_checkpoint_resume_is_bit_exact(tmp_path):
+    sc = scene.small_dam_break()
+    p = S.default_params(particle_radius=sc.particle_radius, dt=0.002)
+    a = S.HostSolver(p, sc.boundary, sc.fluid)
+    for _ in range(3):
+        a.step()
+    path = str(tmp_path / "snap.bin")
+    a.save(path)
+    for _ in range(4):
+        a.step()
+    b = S.HostSolver(p, sc.boundary, sc.fluid)
+    b.load(path)
+    for _ in range(4):
+        b.step()
+    sa, sb = a.view.state_by_id(), b.view.state_by_id()
+    for k in ("pos", "vel", "kappa", "kappaV"):
+        assert np.array_equal(sa[k], sb[k]), k
+    a.close()
+    b.close()
+
+
+def test_escaped_particle_is_removed_order_kept():
+    sc = scene.small_dam_break()
+    s = _solver(sc, dt=0.002)
+    s.step()
+    st = s.state_by_id()
+    x = st["pos"].copy()
+    # push particle id 7 out of the tank through the ceiling (device order upload)
+    ids = s.download(S.FIELD_ID)
+    pos_dev = s.download(S.FIELD_POS).copy()
+    slot = int(np.nonzero(ids == 7)[0][0])
+    pos_dev[slot, 1] = 50.0
+    s.upload_field(S.FIELD_POS, pos_dev)
+    before = s.download(S.FIELD_ID)
+    removed = s.check_particles_positions(2)
+    after = s.download(S.FIELD_ID)
+    assert removed == 1
+    assert after.tolist() == [i for i in before.tolist() if i != 7]
+    assert s.n == sc.n_fluid - 1
+    st = s.step()
+    assert st.n_fluid == sc.n_fluid - 1
+    del x
+
+
+def test_remove_and_add_particles():
+    sc = scene.small_dam_break()
+    s = _solver(sc, dt=0.002)
+    n = sc.n_fluid
+    keep = np.ones(n, np.uint8)
+    keep[::5] = 0
+    ids = s.download(S.FIELD_ID)
+    assert s.remove_particles(keep) == n - (n + 4) // 5
+    assert s.download(S.FIELD_ID).tolist() == ids[keep.astype(bool)].tolist()
+    s.add_particles(np.array([[0.3, 0.9, 0.3]], np.float32), ids=np.array([99999], np.uint32))
+    assert s.download(S.FIELD_ID)[-1] == 99999
+    s.step()
+
+
+def test_neighbor_overflow_is_reported():
+    r = 0.025
+    rng = np.random.default_rng(2)
+    fl = (np.float32(0.5) + rng.uniform(-0.01, 0.01, size=(200, 3))).astype(np.float32)
+    b = scene.box_surface((0, 0, 0), (1, 1, 1), 2 * r).astype(np.float32)
+    s = _solver(scene.Scene("crowd", r, fl, b), max_neighbors=90)
+    with pytest.raises(S.DFSPHError) as e:
+        s.step()
+    assert e.value.code == 4
+    assert s.info().overflow == 200
+
+
+def test_cfl_matches_oracle(oracle_mod):
+    sc = scene.small_dam_break()
+    o = _oracle(oracle_mod, sc, dt=0.002, dt_min=1e-5, dt_max=0.05)
+    psi = o.boundary_psi(sc.n_boundary)
+    s = _solver(sc, psi=psi.astype(np.float32), dt=0.002)
+    vel = np.tile(np.float32([2.0, -1.0, 0.5]), (sc.n_fluid, 1))
+    o.set_fluid(sc.fluid.astype(np.float64), vel.astype(np.float64))
+    s.upload_fluid(sc.fluid, vel)
+    for x in (s, o):
+        x.neighbor_search()
+        x.divergence_solve()
+        x.external_forces()
+    hg = s.cfl(1e-5, 1.0, 0.05)
+    ho = o.cfl()
+    assert hg == pytest.approx(ho, rel=1e-5)
