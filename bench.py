@@ -1,0 +1,208 @@
+"""Benchmark of the DFSPH time step on one MI355X.
+
+Workload (default): the plain DFSPH dam break of 1,048,576 fluid particles
+(128 x 128 x 64 block, r = 0.01 m) in a 6 x 3 x 2 m open-top tank with
+600,801 wall particles at spacing r; FP32 bulk arithmetic, FP64 convergence
+reductions, divergence and density tolerances 0.1 %, CFL 0.4 with dt in
+[0.1, 5] ms.  A "step" is one DFSPHCUDA::step() (neighbor search, both
+solvers, forces, integration, position check) over the whole particle set.
+
+Prints ONE JSON line: particle-steps/s over the timed steps, the roofline of
+the dominant kernel (algorithmic bytes per launch / its average duration from
+HIP events recorded on the solver's stream during the timed steps), and the
+CPU baseline (the FP64 OpenMP oracle in the reference CPU semantics, timed on
+a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import splishsplash_for_cuda_amd as S  # noqa: E402
+from splishsplash_for_cuda_amd import scene  # noqa: E402
+from splishsplash_for_cuda_amd._native import KERNEL_NAMES  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, AMD published
+
+# Compulsory per-particle state of each sweep in bytes (FP32 SoA; uniform
+# fluid mass not counted), the 4 B index per stored neighbor and the 16 B
+# boundary position + psi are added per launch:
+#   B = N_f * S_k + 4 * (sum n_fluid + sum n_boundary) + 16 * N_b
+SWEEP_STATE_BYTES = {
+    "div_ws_init": 52,    # x12 v12 kappaV4 r; rho alpha rho_adv 12 w; v12 w
+    "div_init": 36,       # x12 v12 alpha4 r; rho_adv4 k4 w  (kappaV := 0 not counted)
+    "div_jacobi": 48,     # x12 k4 r; kappaV8 rw; v24 rw
+    "div_loop_end": 36,   # x12 v12 alpha4 r; rho_adv4 k4 w
+    "xsph": 40,           # x12 v12 rho4 r; acc12 w (fluid neighbors only, no boundary term)
+    "pressure_ws": 52,    # x12 acc12 kappa4 r; v24 rw
+    "dens_init": 48,      # x12 v12 rho4 r; alpha8 rw; rho_adv4 kappa4 k4 w
+    "dens_jacobi": 40,    # x12 k4 r; v24 rw
+    "dens_loop_end": 48,  # x12 v12 rho4 alpha4 r; rho_adv4 k4 w; kappa8 rw
+}
+
+
+def kernel_bytes(name, n_f, n_b, sum_nf, sum_nb):
+    s = SWEEP_STATE_BYTES[name]
+    if name == "xsph":
+        return n_f * s + 4 * sum_nf
+    return n_f * s + 4 * (sum_nf + sum_nb) + 16 * n_b
+
+
+def make_params(sc, mode, max_neighbors):
+    return S.default_params(
+        particle_radius=sc.particle_radius, dt=0.001, use_cfl=1, cfl_factor=1.0, dt_min=1e-4, dt_max=5e-3,
+        max_error=0.1, max_error_v=0.1, max_neighbors=max_neighbors,
+        warm_start_mode=S.WARMSTART_LEARNED if mode == "learned" else S.WARMSTART_REFERENCE)
+
+
+def cpu_baseline(sc, budget_s, max_steps):
+    """FP64 OpenMP oracle in the reference CPU semantics (TimeStepDFSPH) on the
+    same scene: whole steps until budget_s of CPU time has been used."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as om
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = om.Oracle(particle_radius=sc.particle_radius, dt=0.001, mode=om.MODE_CPU, use_cfl=1, cfl_factor=1.0,
+                  dt_min=1e-4, dt_max=5e-3, cfl_seed=0.1, max_error=0.1, max_error_v=0.1, min_iter_v=1,
+                  max_neighbors=160, remove_escaped=1)
+    o.set_boundary(sc.boundary.astype(np.float64))
+    o.set_fluid(sc.fluid.astype(np.float64))
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        o.step()
+        steps += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    t = time.perf_counter() - t0
+    return {"value": sc.n_fluid * steps / t, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "first %d step(s) of the same %d-particle scene, FP64 restatement of the reference CPU "
+                      "TimeStepDFSPH with OpenMP (%.1f s)" % (steps, sc.n_fluid, t)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c4"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "learned"])
+    ap.add_argument("--path", default="host", choices=["host", "abi"],
+                    help="host: SPH::DFSPHCUDA::step() (C++); abi: dfsph_step")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the solver is single-GPU (multi-GPU slab decomposition is "
+              "design-only); rerun with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    sc = {"c1": scene.config1, "c2": scene.config2, "c4": scene.config4}[args.config]()
+    max_nbr = 90 if args.config == "c1" else 128
+    p = make_params(sc, args.mode, max_nbr)
+    if args.path == "host":
+        hs = S.HostSolver(p, sc.boundary, sc.fluid)
+        solver = hs.view
+        step = hs.step
+    else:
+        solver = S.Solver(p)
+        solver.upload_boundary(sc.boundary)
+        solver.upload_fluid(sc.fluid)
+        step = solver.step
+
+    for _ in range(args.warmup):
+        step()
+    # find the dominant sweep over a few profiled steps (outside the timed region)
+    solver.profile_enable((1 << len(KERNEL_NAMES)) - 1)
+    solver.profile_reset()
+    for _ in range(5):
+        step()
+    prof = solver.profile()
+    solver.profile_enable(0)
+    dominant = max((k for k in SWEEP_STATE_BYTES), key=lambda k: prof[k][1])
+
+    # timed region: event timing of the dominant sweep only
+    solver.profile_enable(1 << KERNEL_NAMES.index(dominant))
+    solver.profile_reset()
+    inf0 = solver.info()
+    iters_v, iters = [], []
+    solver.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = step()
+        iters_v.append(st.iters_v)
+        iters.append(st.iters)
+    solver.synchronize()
+    elapsed = time.perf_counter() - t0
+    inf1 = solver.info()
+    launches, ms = solver.profile()[dominant]
+    solver.profile_enable(0)
+
+    n_f = (inf0.n_fluid + inf1.n_fluid) / 2.0
+    sum_nf = (inf0.neighbors_fluid_total + inf1.neighbors_fluid_total) / 2.0
+    sum_nb = (inf0.neighbors_boundary_total + inf1.neighbors_boundary_total) / 2.0
+    nbytes = kernel_bytes(dominant, n_f, inf1.n_boundary, sum_nf, sum_nb)
+    avg_s = ms / 1e3 / max(launches, 1)
+    achieved = nbytes / avg_s / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (dominant, args.config))
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("bytes_per_launch")
+
+    value = sc.n_fluid * args.steps / elapsed
+    line = {
+        "metric": "particle-steps/s at ρ-err≤0.1%; pressure iters/step learned vs plain DFSPH",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "%s: %s DFSPH dam break, %d fluid + %d boundary particles, r=%g m, tolerances 0.1%%, "
+                        "CFL 0.4 dt in [0.1,5] ms, steps %d-%d of the run"
+                        % (args.config.upper(), args.mode, sc.n_fluid, sc.n_boundary, sc.particle_radius,
+                           args.warmup + 5, args.warmup + 5 + args.steps),
+            "path": "SPH::DFSPHCUDA::step()" if args.path == "host" else "dfsph_step",
+            "iters_per_step": {"divergence_mean": float(np.mean(iters_v)), "density_mean": float(np.mean(iters)),
+                               "divergence_p95": float(np.percentile(iters_v, 95)),
+                               "density_p95": float(np.percentile(iters, 95))},
+            "max_neighbors": max_nbr,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": nbytes,
+            "avg_launch_us": avg_s * 1e6,
+            "launches": launches,
+        },
+        "kernel_share": {k: round(v[1] / max(sum(x[1] for x in prof.values()), 1e-9), 4) for k, v in prof.items()},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(sc, args.cpu_budget, 1000)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+    if args.path == "host":
+        hs.close()
+
+
+if __name__ == "__main__":
+    main()

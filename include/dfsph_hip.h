@@ -73,6 +73,8 @@ typedef struct dfsph_step_stats {
     uint32_t removed;         /* particles removed by the position check */
     uint32_t max_neighbors_seen;
     uint32_t overflow;        /* particles whose list was truncated    */
+    uint64_t neighbors_fluid_total;    /* stored neighbors of this step's lists */
+    uint64_t neighbors_boundary_total;
 } dfsph_step_stats;
 
 typedef struct dfsph_info {
@@ -86,6 +88,8 @@ typedef struct dfsph_info {
     uint32_t overflow;             /* particles whose list was truncated        */
     uint32_t escaped;              /* outside the box after the last position update */
     uint32_t pad;
+    uint64_t neighbors_fluid_total;    /* stored fluid neighbors, all particles    */
+    uint64_t neighbors_boundary_total; /* stored boundary neighbors, all particles */
 } dfsph_info;
 
 /* fields for dfsph_download / dfsph_upload_field */
@@ -204,6 +208,30 @@ int dfsph_get_neighbors(void* handle, uint32_t* n_fluid, uint32_t* n_boundary,
 
 int dfsph_get_info(void* handle, dfsph_info* info);
 int dfsph_synchronize(void* handle);
+
+/* Per-kernel HIP-event timing, the device-side counterpart of the phase timers
+ * of DFSPHCUDA::step (DFSPH_CUDA.cpp:185-194, 480-495) and SegmentedTiming
+ * (SPlisHSPlasH/Utilities/SegmentedTiming.h:22-84).  mask: one bit per
+ * DFSPH_KERNEL_* id; an event pair is recorded around each launch of those
+ * kernels on the handle's stream (0 turns timing off).  Velocity sweeps that
+ * exit at once because their loop had already converged are not counted. */
+#define DFSPH_KERNEL_SORT 0          /* cell hash, scan, counting sort, reorder */
+#define DFSPH_KERNEL_NEIGHBORS 1     /* neighbor list build                     */
+#define DFSPH_KERNEL_DIV_WS_INIT 2   /* density + factor + source + warm start  */
+#define DFSPH_KERNEL_DIV_INIT 3      /* divergence source (init)                */
+#define DFSPH_KERNEL_DIV_JACOBI 4    /* divergence velocity sweep               */
+#define DFSPH_KERNEL_DIV_LOOP_END 5  /* divergence source + FP64 reduction      */
+#define DFSPH_KERNEL_XSPH 6          /* XSPH + gravity + CFL maximum            */
+#define DFSPH_KERNEL_PRESSURE_WS 7   /* velocity update + pressure warm start   */
+#define DFSPH_KERNEL_DENS_INIT 8     /* density source (init)                   */
+#define DFSPH_KERNEL_DENS_JACOBI 9   /* density velocity sweep                  */
+#define DFSPH_KERNEL_DENS_LOOP_END 10 /* density source + FP64 reduction        */
+#define DFSPH_KERNEL_UPDATE_POS 11   /* position update + escape count          */
+#define DFSPH_KERNEL_COUNT 12
+int dfsph_profile_enable(void* handle, uint32_t mask);
+int dfsph_profile_reset(void* handle);
+/* launches and summed duration (ms) of one kernel id since the last reset */
+int dfsph_profile_read(void* handle, uint32_t kernel, uint32_t* launches, double* total_ms);
 
 /* ---- learned warm start (no reference counterpart; see dfsph_mlp.h) ---- */
 

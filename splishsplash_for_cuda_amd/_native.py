@@ -83,6 +83,8 @@ class DFSPHStepStats(C.Structure):
         ("removed", C.c_uint32),
         ("max_neighbors_seen", C.c_uint32),
         ("overflow", C.c_uint32),
+        ("neighbors_fluid_total", C.c_uint64),
+        ("neighbors_boundary_total", C.c_uint64),
     ]
 
 
@@ -105,6 +107,8 @@ class DFSPHInfo(C.Structure):
         ("overflow", C.c_uint32),
         ("escaped", C.c_uint32),
         ("pad", C.c_uint32),
+        ("neighbors_fluid_total", C.c_uint64),
+        ("neighbors_boundary_total", C.c_uint64),
     ]
 
 
@@ -157,7 +161,13 @@ HIP_SIGNATURES = {
     "dfsph_get_neighbors": (_I, [_P, _U32P, _U32P, _U32P, _U32]),
     "dfsph_get_info": (_I, [_P, C.POINTER(DFSPHInfo)]),
     "dfsph_synchronize": (_I, [_P]),
+    "dfsph_profile_enable": (_I, [_P, _U32]),
+    "dfsph_profile_reset": (_I, [_P]),
+    "dfsph_profile_read": (_I, [_P, _U32, _U32P, C.POINTER(_D)]),
 }
+
+KERNEL_NAMES = ["sort", "neighbors", "div_ws_init", "div_init", "div_jacobi", "div_loop_end", "xsph",
+                "pressure_ws", "dens_init", "dens_jacobi", "dens_loop_end", "update_pos"]
 
 HOST_SIGNATURES = {
     "dfsph_cuda_new": (C.c_void_p, [C.POINTER(DFSPHParams), _F32P, _U32, _F32P, _F32P, _U32, C.c_char_p, _U32]),
@@ -361,6 +371,23 @@ class Solver:
 
     def synchronize(self):
         _check(self.lib.dfsph_synchronize(self.h), "dfsph_synchronize")
+
+    # ---- per-kernel event timing ----
+    def profile_enable(self, mask=(1 << 12) - 1):
+        _check(self.lib.dfsph_profile_enable(self.h, mask), "dfsph_profile_enable")
+
+    def profile_reset(self):
+        _check(self.lib.dfsph_profile_reset(self.h), "dfsph_profile_reset")
+
+    def profile(self):
+        """{kernel name: (launches, total ms)}"""
+        out = {}
+        for k, name in enumerate(KERNEL_NAMES):
+            n = C.c_uint32()
+            ms = C.c_double()
+            _check(self.lib.dfsph_profile_read(self.h, k, C.byref(n), C.byref(ms)), "dfsph_profile_read")
+            out[name] = (n.value, ms.value)
+        return out
 
     # ---- data out ----
     def info(self) -> DFSPHInfo:

@@ -95,8 +95,67 @@ struct Handle {
 
     dfsph::MlpState* mlp = nullptr;   // learned warm start
 
+    // per-kernel event timing (dfsph_profile_*)
+    struct ProfRec {
+        int kind;
+        hipEvent_t a, b;
+    };
+    uint32_t prof_mask = 0;
+    std::vector<ProfRec> prof_pending;
+    std::vector<hipEvent_t> prof_pool;
+    uint32_t prof_launches[DFSPH_KERNEL_COUNT] = {};
+    double prof_ms[DFSPH_KERNEL_COUNT] = {};
+
     ~Handle();
 };
+
+// Event pair around one launch of kernel `kind` when it is being timed;
+// returns the record index or -1.
+int pbeg(Handle* h, int kind)
+{
+    if (!(h->prof_mask & (1u << kind))) return -1;
+    hipEvent_t e[2];
+    for (int k = 0; k < 2; ++k) {
+        if (!h->prof_pool.empty()) {
+            e[k] = h->prof_pool.back();
+            h->prof_pool.pop_back();
+        } else if (hipEventCreateWithFlags(&e[k], hipEventDisableSystemFence) != hipSuccess) {
+            // timing events without the system-scope fence: that fence writes the
+            // L2 back at every record and would slow the timed kernel itself
+            return -1;
+        }
+    }
+    (void)hipEventRecord(e[0], h->s);
+    h->prof_pending.push_back({kind, e[0], e[1]});
+    return (int)h->prof_pending.size() - 1;
+}
+
+void pend(Handle* h, int idx)
+{
+    if (idx >= 0) (void)hipEventRecord(h->prof_pending[idx].b, h->s);
+}
+
+void pdrop(Handle* h, int idx)
+{
+    if (idx >= 0) h->prof_pending[idx].kind = -1;
+}
+
+// after a stream synchronisation: turn the recorded pairs into durations
+void prof_collect(Handle* h)
+{
+    for (auto& r : h->prof_pending) {
+        if (r.kind >= 0) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+                h->prof_launches[r.kind]++;
+                h->prof_ms[r.kind] += ms;
+            }
+        }
+        h->prof_pool.push_back(r.a);
+        h->prof_pool.push_back(r.b);
+    }
+    h->prof_pending.clear();
+}
 
 template <class T>
 void dfree(T*& p)
@@ -120,6 +179,11 @@ Handle::~Handle()
     if (htm) (void)hipHostFree(htm);
     if (hslot) (void)hipHostFree(hslot);
     if (mlp) dfsph::mlp_destroy(mlp);
+    for (auto& r : prof_pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : prof_pool) (void)hipEventDestroy(e);
     if (s) (void)hipStreamDestroy(s);
 }
 
@@ -210,6 +274,7 @@ int neighbor_search(Handle* h, bool sort)
     CHK(flush_vel(h));
     dfsph::launch_nbr_ctl_init(h->s, h->ctl);
     if (h->n == 0) { h->lists_valid = true; return DFSPH_OK; }
+    const int ps = pbeg(h, DFSPH_KERNEL_SORT);
     CHK(counting_sort(h, h->pos[h->pc], h->n, h->cstart));
     if (sort) {
         const int o = h->pc ^ 1, vo = h->vc ^ 1;
@@ -219,8 +284,11 @@ int neighbor_search(Handle* h, bool sort)
         h->pc = o;
         h->vc = vo;
     }
+    pend(h, ps);
+    const int pn = pbeg(h, DFSPH_KERNEL_NEIGHBORS);
     dfsph::launch_neighbors(h->s, h->c, !sort, h->pos[h->pc], h->cstart, h->perm, h->bpos, h->bcstart, h->n,
                             h->nbr, h->cnt, h->ctl);
+    pend(h, pn);
     HIPCHK(hipGetLastError());
     h->lists_valid = true;
     h->acc_valid = false;
@@ -241,15 +309,19 @@ int jacobi_loop(Handle* h, bool div, uint32_t max_iter, double max_error, uint32
         if (err) *err = 0.0;
         return DFSPH_OK;
     }
-    auto jacobi = [&]() {
+    auto jacobi = [&]() -> int {
+        const int pr = pbeg(h, div ? DFSPH_KERNEL_DIV_JACOBI : DFSPH_KERNEL_DENS_JACOBI);
         if (div)
             dfsph::launch_jacobi(h->s, true, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->kk,
                                  h->kapv[h->pc], h->vel[h->vc], sc);
         else
             dfsph::launch_jacobi(h->s, false, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->kk,
                                  nullptr, h->vel[h->vc], sc);
+        pend(h, pr);
+        return pr;
     };
     auto loop_end = [&]() {
+        const int pr = pbeg(h, div ? DFSPH_KERNEL_DIV_LOOP_END : DFSPH_KERNEL_DENS_LOOP_END);
         if (div)
             dfsph::launch_div_source(h->s, false, h->c, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->vel[h->vc],
                                      h->alpha, h->radv, h->kk, h->kapv[h->pc], sc, h->partial);
@@ -257,6 +329,7 @@ int jacobi_loop(Handle* h, bool div, uint32_t max_iter, double max_error, uint32
             dfsph::launch_density_source(h->s, false, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
                                          h->vel[h->vc], h->rho, h->alpha, h->radv, h->kk, h->kap[h->pc], sc,
                                          h->partial);
+        pend(h, pr);
     };
     // The minimum iterations are always run.  After that, the velocity sweep
     // of the next iteration is queued before the host reads the loop state of
@@ -277,10 +350,11 @@ int jacobi_loop(Handle* h, bool div, uint32_t max_iter, double max_error, uint32
         HIPCHK(hipMemcpyAsync(h->hslot, sc, sizeof(SolveCtl), hipMemcpyDeviceToHost, h->s));
         HIPCHK(hipEventRecord(h->ev[0], h->s));
         const bool spec = queued < max_iter;
-        if (spec) jacobi();
+        const int pr = spec ? jacobi() : -1;
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventSynchronize(h->ev[0]));
         res = h->hslot[0];
+        if (res.done) pdrop(h, pr);   // that sweep exited at once
         if (res.done || !spec) break;
         loop_end();
         ++queued;
@@ -303,11 +377,15 @@ int divergence_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* i
                                           h->vel[h->vc], h->kap[h->pc], h->kapv[h->pc], &kws));
     }
     const int vo = h->vc ^ 1;
+    int pr = pbeg(h, DFSPH_KERNEL_DIV_WS_INIT);
     dfsph::launch_div_ws_init(h->s, ws, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->vel[h->vc],
                               h->vel[vo], kws, h->rho, h->alpha, h->radv);
+    pend(h, pr);
     h->vc = vo;
+    pr = pbeg(h, DFSPH_KERNEL_DIV_INIT);
     dfsph::launch_div_source(h->s, true, h->c, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->vel[h->vc],
                              h->alpha, h->radv, h->kk, h->kapv[h->pc], nullptr, nullptr);
+    pend(h, pr);
     HIPCHK(hipGetLastError());
     h->acc_valid = false;
     return jacobi_loop(h, true, max_iter, max_error, iters, err);
@@ -329,8 +407,10 @@ int external_forces(Handle* h)
     CHK(flush_vel(h));
     if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale: call dfsph_neighbor_search");
     dfsph::launch_step_ctl_init(h->s, h->ctl, h->n);
+    const int pr = pbeg(h, DFSPH_KERNEL_XSPH);
     dfsph::launch_xsph(h->s, h->c, h->tm, h->n, h->pos[h->pc], h->nbr, h->cnt, h->vel[h->vc], h->rho, h->acc,
                        h->ctl, h->partial_f);
+    pend(h, pr);
     HIPCHK(hipGetLastError());
     h->acc_valid = true;
     return DFSPH_OK;
@@ -349,12 +429,16 @@ int pressure_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* ite
         CHK(flush_vel(h));
     } else {
         const bool fused = h->pend_vel;
+        const int pw = pbeg(h, DFSPH_KERNEL_PRESSURE_WS);
         dfsph::launch_pressure_ws(h->s, ws, fused, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
                                   h->acc, h->vel[h->vc], kws, fused ? 1 : 0);
+        pend(h, pw);
         h->pend_vel = false;
     }
+    const int pr = pbeg(h, DFSPH_KERNEL_DENS_INIT);
     dfsph::launch_density_source(h->s, true, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
                                  h->vel[h->vc], h->rho, h->alpha, h->radv, h->kk, h->kap[h->pc], nullptr, nullptr);
+    pend(h, pr);
     HIPCHK(hipGetLastError());
     h->acc_valid = false;
     return jacobi_loop(h, false, max_iter, max_error, iters, err);
@@ -694,7 +778,9 @@ int dfsph_update_pos(void* hv)
 {
     Handle* h = H(hv);
     CHK(flush_vel(h));
+    const int pr = pbeg(h, DFSPH_KERNEL_UPDATE_POS);
     dfsph::launch_update_pos(h->s, h->tm, h->n, h->vel[h->vc], h->pos[h->pc], h->bmin, h->bmax, h->ctl);
+    pend(h, pr);
     HIPCHK(hipGetLastError());
     h->lists_valid = false;
     h->acc_valid = false;
@@ -776,6 +862,9 @@ int dfsph_step(void* hv, dfsph_step_stats* st)
     loc.time = h->htm->time;
     loc.max_neighbors_seen = h->hctl->max_seen;
     loc.overflow = h->hctl->overflow;
+    loc.neighbors_fluid_total = h->hctl->nbr_fluid;
+    loc.neighbors_boundary_total = h->hctl->nbr_boundary;
+    prof_collect(h);
     if (h->p.check_positions_mode == 2 && h->hctl->escaped) {
         CHK(check_positions(h, 2, &loc.removed));
     } else if (h->p.check_positions_mode == 0) {
@@ -911,9 +1000,37 @@ int dfsph_get_info(void* hv, dfsph_info* info)
     info->max_neighbors_seen = h->hctl->max_seen;
     info->overflow = h->hctl->overflow;
     info->escaped = h->hctl->escaped;
+    info->neighbors_fluid_total = h->hctl->nbr_fluid;
+    info->neighbors_boundary_total = h->hctl->nbr_boundary;
     return DFSPH_OK;
 }
 
 int dfsph_synchronize(void* hv) { return sync(H(hv)); }
+
+int dfsph_profile_enable(void* hv, uint32_t mask)
+{
+    H(hv)->prof_mask = mask & ((1u << DFSPH_KERNEL_COUNT) - 1u);
+    return DFSPH_OK;
+}
+
+int dfsph_profile_reset(void* hv)
+{
+    Handle* h = H(hv);
+    CHK(sync(h));
+    prof_collect(h);
+    for (int k = 0; k < DFSPH_KERNEL_COUNT; ++k) { h->prof_launches[k] = 0; h->prof_ms[k] = 0.0; }
+    return DFSPH_OK;
+}
+
+int dfsph_profile_read(void* hv, uint32_t kernel, uint32_t* launches, double* total_ms)
+{
+    Handle* h = H(hv);
+    if (kernel >= DFSPH_KERNEL_COUNT) return fail(DFSPH_ERR_ARG, "unknown kernel id");
+    CHK(sync(h));
+    prof_collect(h);
+    if (launches) *launches = h->prof_launches[kernel];
+    if (total_ms) *total_ms = h->prof_ms[kernel];
+    return DFSPH_OK;
+}
 
 }  // extern "C"

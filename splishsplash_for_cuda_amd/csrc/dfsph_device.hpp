@@ -48,9 +48,9 @@ struct DevTime {
     double time;                   // simulated time (FP64 accumulator)
 };
 
-// Device-side control of one Jacobi loop: the loop-end kernel's last block
-// turns the FP64 block partial sums into the average error and decides
-// whether the loop continues, with the reference's condition
+// Device-side control of one Jacobi loop: a one-block kernel after each
+// loop-end sweep turns the FP64 block partial sums into the average error and
+// decides whether the loop continues, with the reference's condition
 // while ((err > eta) || (it < min)) && (it < max)  (DFSPH_core_cuda.cu:828,1378).
 struct SolveCtl {
     double sum, err, eta;
@@ -68,6 +68,8 @@ struct DevCtl {
     uint32_t overflow;             // particles whose list was truncated
     uint32_t n_active;             // active fluid particles (device copy)
     uint32_t pad[2];
+    unsigned long long nbr_fluid;  // stored fluid neighbors of the last search
+    unsigned long long nbr_boundary;
 };
 
 // Blocks are dispatched round robin over the 8 XCDs, each with its own L2.
@@ -196,41 +198,6 @@ __device__ __forceinline__ double block_sum(double v, double* lds4)
         for (int k = 0; k < nw; ++k) t += lds4[k];
     }
     return t;   // valid in thread 0
-}
-
-__device__ __forceinline__ double load_agent(const double* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void store_agent(double* p, double v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ float load_agent_f(const float* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void store_agent_f(float* p, float v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Returns true in every thread of the block that finished last.  Thread 0 of
-// each block has published its partial result before calling.
-__device__ __forceinline__ bool last_block_done(uint32_t* ticket, bool* lds_flag)
-{
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        *lds_flag = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    const bool last = *lds_flag;
-    if (last) __threadfence();
-    return last;
 }
 
 }  // namespace dfsph

@@ -55,7 +55,12 @@ __global__ void k_step_ctl_init(DevCtl* c, uint32_t n_active)
 
 __global__ void k_nbr_ctl_init(DevCtl* c)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) { c->max_seen = 0; c->overflow = 0; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c->max_seen = 0;
+        c->overflow = 0;
+        c->nbr_fluid = 0;
+        c->nbr_boundary = 0;
+    }
 }
 
 static __device__ void time_fill(DevTime* t, double h, double hf, double hp)
@@ -275,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
                                                       uint32_t* cnt, DevCtl* ctl)
 {
     const uint32_t i = xcd_index();
-    uint32_t total = 0, trunc = 0;
+    uint32_t total = 0, trunc = 0, stored_f = 0, stored_b = 0;
     if (i < n) {
         const float4 p = pos[i];
         const int cx = cell_coord(c, p.x, 0), cy = cell_coord(c, p.y, 1), cz = cell_coord(c, p.z, 2);
@@ -326,12 +331,18 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
         const uint32_t sb = nb < maxn - sf ? nb : maxn - sf;
         trunc = total > maxn ? 1u : 0u;
         cnt[i] = sf | (sb << 16);
+        stored_f = sf;
+        stored_b = sb;
     }
     const uint32_t wmax = wave_max_u(total);
     const uint32_t wtr = wave_sum_u(trunc);
+    const uint32_t wsf = wave_sum_u(stored_f);
+    const uint32_t wsb = wave_sum_u(stored_b);
     if ((threadIdx.x & 63) == 0) {
         if (wmax) atomicMax(&ctl->max_seen, wmax);
         if (wtr) atomicAdd(&ctl->overflow, wtr);
+        if (wsf) atomicAdd(&ctl->nbr_fluid, (unsigned long long)wsf);
+        if (wsb) atomicAdd(&ctl->nbr_boundary, (unsigned long long)wsb);
     }
 }
 
@@ -374,17 +385,25 @@ __global__ void k_set_w(float4* p, const float* w, uint32_t n)
 // solver sweeps
 // ------------------------------------------------------------------------
 
-// Final step of every reducing sweep: the last block sums the block
-// partials in a fixed order, forms err = sum / N and evaluates the loop
-// condition of cuda_divergenceSolve / cuda_pressureSolve.
-__device__ __forceinline__ void finish_solve(SolveCtl* ctl, double* partial, double bsum,
-                                             double* lds4, bool* flag)
+// Reducing sweeps write one FP64 partial per block with a plain store; the
+// separate one-block kernel below sums them.  (A "last block finishes"
+// reduction inside the sweep needs a device-scope release/acquire per block,
+// which on MI355X writes back and invalidates the XCD's L2 every time and
+// throws away the neighbor data the other blocks are gathering.)
+__device__ __forceinline__ void store_partial(double* partial, double bsum)
 {
-    if (threadIdx.x == 0) store_agent(partial + blockIdx.x, bsum);
-    if (!last_block_done(&ctl->ticket, flag)) return;
+    if (threadIdx.x == 0) partial[blockIdx.x] = bsum;
+}
+
+// Sums the block partials in a fixed order (deterministic), forms
+// err = sum / N and evaluates the loop condition of cuda_divergenceSolve /
+// cuda_pressureSolve: continue while ((err > eta) || (it < min)) && (it < max).
+__global__ __launch_bounds__(kBlock) void k_solve_finalize(SolveCtl* ctl, const double* partial, uint32_t nparts)
+{
+    __shared__ double lds4[kBlock / 64];
+    if (ctl->done) return;
     double acc = 0.0;
-    for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x) acc += load_agent(partial + k);
-    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nparts; k += blockDim.x) acc += partial[k];
     const double tot = block_sum(acc, lds4);
     if (threadIdx.x == 0) {
         const double err = ctl->n_div > 0.0 ? tot / ctl->n_div : 0.0;
@@ -394,7 +413,6 @@ __device__ __forceinline__ void finish_solve(SolveCtl* ctl, double* partial, dou
         ctl->iters = it;
         const bool cont = ((err > ctl->eta) || (it < ctl->min_iter)) && (it < ctl->max_iter);
         ctl->done = cont ? 0u : 1u;
-        ctl->ticket = 0;
     }
 }
 
@@ -498,7 +516,6 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
                                                        double* partial)
 {
     __shared__ double lds4[kBlock / 64];
-    __shared__ bool flag;
     if (!INIT && ctl->done) return;
     const uint32_t i = xcd_index();
     float r = 0.0f;
@@ -531,7 +548,7 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
     }
     if (!INIT) {
         const double bs = block_sum((double)r, lds4);
-        finish_solve(ctl, partial, bs, lds4, &flag);
+        store_partial(partial, bs);
     }
 }
 
@@ -593,7 +610,6 @@ __global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, u
                                                  DevCtl* ctl, float* partial)
 {
     __shared__ float lds4[kBlock / 64];
-    __shared__ bool flag;
     const uint32_t i = xcd_index();
     float m2 = 0.0f;
     if (i < n) {
@@ -625,20 +641,23 @@ __global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, u
     if (threadIdx.x == 0) {
         float bm = lds4[0];
         for (int k = 1; k < kBlock / 64; ++k) bm = fmaxf(bm, lds4[k]);
-        store_agent_f(partial + blockIdx.x, bm);
+        partial[blockIdx.x] = bm;
     }
-    if (!last_block_done(&ctl->cfl_ticket, &flag)) return;
+}
+
+// maximum of the XSPH kernel's block maxima (the CFL velocity bound)
+__global__ __launch_bounds__(kBlock) void k_cfl_finalize(DevCtl* ctl, const float* partial, uint32_t nparts)
+{
+    __shared__ float lds4[kBlock / 64];
     float mm = 0.0f;
-    for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x) mm = fmaxf(mm, load_agent_f(partial + k));
+    for (uint32_t k = threadIdx.x; k < nparts; k += blockDim.x) mm = fmaxf(mm, partial[k]);
     mm = wave_max(mm);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) lds4[w] = mm;
+    if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = mm;
     __syncthreads();
     if (threadIdx.x == 0) {
         float bm = lds4[0];
         for (int k = 1; k < kBlock / 64; ++k) bm = fmaxf(bm, lds4[k]);
         ctl->cfl_max = bm;
-        ctl->cfl_ticket = 0;
     }
 }
 
@@ -732,7 +751,6 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
                                                            SolveCtl* ctl, double* partial)
 {
     __shared__ double lds4[kBlock / 64];
-    __shared__ bool flag;
     if (!INIT && ctl->done) return;
     const uint32_t i = xcd_index();
     float r = 0.0f;
@@ -769,7 +787,7 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
     }
     if (!INIT) {
         const double bs = block_sum((double)r, lds4);
-        finish_solve(ctl, partial, bs, lds4, &flag);
+        store_partial(partial, bs);
     }
 }
 
@@ -976,10 +994,12 @@ void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, 
                        const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial)
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
-    if (init)
+    if (init) {
         k_div_source<true><<<g, kBlock, 0, s>>>(c, n, pos, bpos, nbr, cnt, vel, alpha, radv, kk, kappaV, ctl, partial);
-    else
+    } else {
         k_div_source<false><<<g, kBlock, 0, s>>>(c, n, pos, bpos, nbr, cnt, vel, alpha, radv, kk, kappaV, ctl, partial);
+        k_solve_finalize<<<1, kBlock, 0, s>>>(ctl, partial, g);
+    }
 }
 
 void launch_jacobi(hipStream_t s, bool div, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
@@ -998,7 +1018,9 @@ void launch_xsph(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n,
                  const uint32_t* nbr, const uint32_t* cnt, const float4* vel, const float* rho, float4* acc,
                  DevCtl* ctl, float* partial)
 {
-    k_xsph<<<nblk_xcd(n ? n : 1), kBlock, 0, s>>>(c, t, n, pos, nbr, cnt, vel, rho, acc, ctl, partial);
+    const uint32_t g = nblk_xcd(n ? n : 1);
+    k_xsph<<<g, kBlock, 0, s>>>(c, t, n, pos, nbr, cnt, vel, rho, acc, ctl, partial);
+    k_cfl_finalize<<<1, kBlock, 0, s>>>(ctl, partial, g);
 }
 
 void launch_update_vel(hipStream_t s, const DevTime* t, uint32_t n, const float4* acc, float4* vel, float* kappa)
@@ -1031,10 +1053,12 @@ void launch_density_source(hipStream_t s, bool init, const SimConst& c, const De
                            float* kappa, SolveCtl* ctl, double* partial)
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
-    if (init)
+    if (init) {
         k_density_source<true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel, rho, alpha, radv, kk, kappa, ctl, partial);
-    else
+    } else {
         k_density_source<false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel, rho, alpha, radv, kk, kappa, ctl, partial);
+        k_solve_finalize<<<1, kBlock, 0, s>>>(ctl, partial, g);
+    }
 }
 
 void launch_update_pos(hipStream_t s, const DevTime* t, uint32_t n, const float4* vel, float4* pos,

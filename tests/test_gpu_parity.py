@@ -57,8 +57,9 @@ def test_neighbor_lists_bit_exact_random_cloud(oracle_mod):
     rng = np.random.default_rng(21)
     r = 0.02
     hk = np.float32(4 * r)
-    fl = rng.uniform(0.1, 0.5, size=(6000, 3)).astype(np.float32)
-    fl[:300] = rng.uniform(0.1, 0.14, size=(300, 3)).astype(np.float32)   # crowded corner
+    # about 85 neighbors in the bulk, about 180 in the crowded corner (cap 255)
+    fl = rng.uniform(0.1, 0.5, size=(2500, 3)).astype(np.float32)
+    fl[:150] = rng.uniform(0.1, 0.25, size=(150, 3)).astype(np.float32)   # crowded corner
     fl[400] = fl[401]
     fl[500] = fl[501] + np.float32([hk, 0, 0])
     b = scene.box_surface((0, 0, 0), (0.6, 0.6, 0.6), 2 * r).astype(np.float32)
