@@ -233,7 +233,42 @@ int dfsph_profile_reset(void* handle);
 /* launches and summed duration (ms) of one kernel id since the last reset */
 int dfsph_profile_read(void* handle, uint32_t kernel, uint32_t* launches, double* total_ms);
 
-/* ---- learned warm start (no reference counterpart; see dfsph_mlp.h) ---- */
+/* ---- learned warm start -------------------------------------------------
+ * No reference counterpart: these entry points add a per-particle MLP
+ * (12 -> 64 -> 64 -> 64 -> 2, SiLU; BF16 weights on the matrix cores, FP32
+ * accumulate) that predicts the warm starts the reference forms at
+ * DFSPH_core_cuda.cu:240 (kappaV) and :3063-3068 (kappa).
+ * Parameters use one flat FP32 layout of DFSPH_MLP_NPARAM values:
+ *   W1[64x12] b1[64] W2[64x64] b2[64] W3[64x64] b3[64] W4[2x64] b4[2]
+ * (row-major, out x in); norm holds in_mean[12] in_std[12] out_mean[2] out_std[2]. */
+#define DFSPH_MLP_NPARAM 9282
+#define DFSPH_MLP_NNORM 28
+#define DFSPH_MLP_NFEAT 12
+/* set / read the parameters (norm == NULL: identity normalisation) */
+int dfsph_mlp_set_params(void* handle, const float* params, const float* norm);
+int dfsph_mlp_get_params(void* handle, float* params, float* norm);
+/* y[n][2] = MLP(x[n][12]) on the GPU (normalise: apply the stored normalisation) */
+int dfsph_mlp_forward(void* handle, const float* x, uint32_t n, float* y, int normalise);
+/* loss = sum 0.5 |MLP(x) - t|^2 / n and its gradient (DFSPH_MLP_NPARAM floats),
+ * computed by the training kernels (forward + backward on the matrix cores) */
+int dfsph_mlp_loss_grad(void* handle, const float* x, const float* t, uint32_t n, float* grad, double* loss,
+                        int normalise);
+/* Training set harvest: while capacity is not reached, every step (dfsph_step or
+ * the phase calls) appends each stride-th particle's 12 features (computed after
+ * the neighbor search) and the step's converged totals (kappaV h, kappa h_f^2).
+ * capacity 0 stops harvesting. */
+int dfsph_mlp_harvest(void* handle, uint32_t capacity, uint32_t stride);
+int dfsph_mlp_dataset_size(void* handle, uint32_t* n);
+int dfsph_mlp_dataset_download(void* handle, float* x, float* t, uint32_t n);
+/* replace the training set (x[n][12], t[n][2]) */
+int dfsph_mlp_dataset_upload(void* handle, const float* x, const float* t, uint32_t n);
+/* normalisation = mean / std of the training set's features and targets */
+int dfsph_mlp_fit_normalization(void* handle);
+/* mini-batch Adam (batch a multiple of 64) on the GPU's training kernels */
+int dfsph_mlp_train(void* handle, uint32_t steps, uint32_t batch, float lr, uint32_t seed, float* loss_first,
+                    float* loss_last);
+/* features and predictions of the current step (device order, n particles) */
+int dfsph_mlp_step_features(void* handle, float* feat, float* pred_v, float* pred_p, uint32_t n);
 
 #ifdef __cplusplus
 }

@@ -164,7 +164,22 @@ HIP_SIGNATURES = {
     "dfsph_profile_enable": (_I, [_P, _U32]),
     "dfsph_profile_reset": (_I, [_P]),
     "dfsph_profile_read": (_I, [_P, _U32, _U32P, C.POINTER(_D)]),
+    "dfsph_mlp_set_params": (_I, [_P, _F32P, _F32P]),
+    "dfsph_mlp_get_params": (_I, [_P, _F32P, _F32P]),
+    "dfsph_mlp_forward": (_I, [_P, _F32P, _U32, _F32P, _I]),
+    "dfsph_mlp_loss_grad": (_I, [_P, _F32P, _F32P, _U32, _F32P, C.POINTER(_D), _I]),
+    "dfsph_mlp_harvest": (_I, [_P, _U32, _U32]),
+    "dfsph_mlp_dataset_size": (_I, [_P, _U32P]),
+    "dfsph_mlp_dataset_download": (_I, [_P, _F32P, _F32P, _U32]),
+    "dfsph_mlp_dataset_upload": (_I, [_P, _F32P, _F32P, _U32]),
+    "dfsph_mlp_fit_normalization": (_I, [_P]),
+    "dfsph_mlp_train": (_I, [_P, _U32, _U32, C.c_float, _U32, _F32P, _F32P]),
+    "dfsph_mlp_step_features": (_I, [_P, _F32P, _F32P, _F32P, _U32]),
 }
+
+MLP_NPARAM = 9282
+MLP_NNORM = 28
+MLP_NFEAT = 12
 
 KERNEL_NAMES = ["sort", "neighbors", "div_ws_init", "div_init", "div_jacobi", "div_loop_end", "xsph",
                 "pressure_ws", "dens_init", "dens_jacobi", "dens_loop_end", "update_pos"]
@@ -371,6 +386,73 @@ class Solver:
 
     def synchronize(self):
         _check(self.lib.dfsph_synchronize(self.h), "dfsph_synchronize")
+
+    # ---- learned warm start model ----
+    def mlp_set_params(self, params, norm=None):
+        w = np.ascontiguousarray(params, np.float32)
+        assert w.size == MLP_NPARAM
+        nr = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        _check(self.lib.dfsph_mlp_set_params(self.h, _ptr(w, C.c_float), _ptr(nr, C.c_float)), "dfsph_mlp_set_params")
+
+    def mlp_get_params(self):
+        w = np.zeros(MLP_NPARAM, np.float32)
+        nr = np.zeros(MLP_NNORM, np.float32)
+        _check(self.lib.dfsph_mlp_get_params(self.h, _ptr(w, C.c_float), _ptr(nr, C.c_float)), "dfsph_mlp_get_params")
+        return w, nr
+
+    def mlp_forward(self, x, normalise=True):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_NFEAT)
+        y = np.zeros((x.shape[0], 2), np.float32)
+        _check(self.lib.dfsph_mlp_forward(self.h, _ptr(x, C.c_float), x.shape[0], _ptr(y, C.c_float),
+                                          1 if normalise else 0), "dfsph_mlp_forward")
+        return y
+
+    def mlp_loss_grad(self, x, t, normalise=False):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_NFEAT)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 2)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = C.c_double()
+        _check(self.lib.dfsph_mlp_loss_grad(self.h, _ptr(x, C.c_float), _ptr(t, C.c_float), x.shape[0],
+                                            _ptr(g, C.c_float), C.byref(loss), 1 if normalise else 0),
+               "dfsph_mlp_loss_grad")
+        return loss.value, g
+
+    def mlp_harvest(self, capacity, stride=1):
+        _check(self.lib.dfsph_mlp_harvest(self.h, capacity, stride), "dfsph_mlp_harvest")
+
+    def mlp_dataset(self):
+        n = C.c_uint32()
+        _check(self.lib.dfsph_mlp_dataset_size(self.h, C.byref(n)), "dfsph_mlp_dataset_size")
+        x = np.zeros((n.value, MLP_NFEAT), np.float32)
+        t = np.zeros((n.value, 2), np.float32)
+        if n.value:
+            _check(self.lib.dfsph_mlp_dataset_download(self.h, _ptr(x, C.c_float), _ptr(t, C.c_float), n.value),
+                   "dfsph_mlp_dataset_download")
+        return x, t
+
+    def mlp_dataset_upload(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_NFEAT)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 2)
+        _check(self.lib.dfsph_mlp_dataset_upload(self.h, _ptr(x, C.c_float), _ptr(t, C.c_float), x.shape[0]),
+               "dfsph_mlp_dataset_upload")
+
+    def mlp_fit_normalization(self):
+        _check(self.lib.dfsph_mlp_fit_normalization(self.h), "dfsph_mlp_fit_normalization")
+
+    def mlp_train(self, steps, batch=4096, lr=2e-3, seed=0):
+        a = C.c_float()
+        b = C.c_float()
+        _check(self.lib.dfsph_mlp_train(self.h, steps, batch, lr, seed, C.byref(a), C.byref(b)), "dfsph_mlp_train")
+        return a.value, b.value
+
+    def mlp_step_features(self):
+        n = self.n
+        f = np.zeros((n, MLP_NFEAT), np.float32)
+        pv = np.zeros(n, np.float32)
+        pp = np.zeros(n, np.float32)
+        _check(self.lib.dfsph_mlp_step_features(self.h, _ptr(f, C.c_float), _ptr(pv, C.c_float), _ptr(pp, C.c_float),
+                                                n), "dfsph_mlp_step_features")
+        return f, pv, pp
 
     # ---- per-kernel event timing ----
     def profile_enable(self, mask=(1 << 12) - 1):

@@ -60,8 +60,14 @@ def build(verbose: bool = False) -> None:
          + [os.path.join(CSRC, s) for s in HOST_SOURCES]
          + ["-L" + HERE, "-ldfsph_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp])
     os.replace(tmp, host_lib)
+
+    # plain copy kernel of known size, for calibrating rocprofv3 byte counters
+    calib = os.path.join(HERE, "copy_calibration")
+    _run([HIPCC, "--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          os.path.join(CSRC, "copy_calibration.hip"), "-o", calib + ".tmp"])
+    os.replace(calib + ".tmp", calib)
     if verbose:
-        print("built", lib, host_lib)
+        print("built", lib, host_lib, calib)
 
 
 if __name__ == "__main__":

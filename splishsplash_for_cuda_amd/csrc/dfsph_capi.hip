@@ -59,6 +59,8 @@ struct Handle {
     float4* pos[2] = {nullptr, nullptr};
     float* kap[2] = {nullptr, nullptr};
     float* kapv[2] = {nullptr, nullptr};
+    float* ktot[2] = {nullptr, nullptr};      // step totals (kappa h_f^2), learned warm start I/O
+    float* kvtot[2] = {nullptr, nullptr};     // step totals (kappaV h)
     uint32_t* id[2] = {nullptr, nullptr};
     int pc = 0;                       // current buffer of pos / kappa / kappaV / id
     float4* vel[2] = {nullptr, nullptr};
@@ -92,6 +94,7 @@ struct Handle {
     bool pend_vel = false;            // deferred cuda_update_vel (fused into the pressure warm start)
     bool lists_valid = false;
     bool acc_valid = false;
+    bool pred_valid = false;          // features / predictions of this step computed
 
     dfsph::MlpState* mlp = nullptr;   // learned warm start
 
@@ -169,6 +172,7 @@ Handle::~Handle()
     if (s) (void)hipStreamSynchronize(s);
     for (int b = 0; b < 2; ++b) {
         dfree(pos[b]); dfree(kap[b]); dfree(kapv[b]); dfree(id[b]); dfree(vel[b]);
+        dfree(ktot[b]); dfree(kvtot[b]);
         if (ev[b]) (void)hipEventDestroy(ev[b]);
     }
     dfree(rho); dfree(alpha); dfree(radv); dfree(kk); dfree(acc); dfree(cnt); dfree(nbr);
@@ -246,6 +250,11 @@ int counting_sort(Handle* h, const float4* pts, uint32_t n, uint32_t* cs)
     return DFSPH_OK;
 }
 
+dfsph::FluidPtrs fluid_ptrs(Handle* h, int b, int vb)
+{
+    return dfsph::FluidPtrs{h->pos[b], h->vel[vb], h->kap[b], h->kapv[b], h->ktot[b], h->kvtot[b], h->id[b]};
+}
+
 int flush_vel(Handle* h)
 {
     if (!h->pend_vel) return DFSPH_OK;
@@ -273,14 +282,13 @@ int neighbor_search(Handle* h, bool sort)
     if (!h->grid_ready) return fail(DFSPH_ERR_STATE, "no boundary set uploaded");
     CHK(flush_vel(h));
     dfsph::launch_nbr_ctl_init(h->s, h->ctl);
+    h->pred_valid = false;
     if (h->n == 0) { h->lists_valid = true; return DFSPH_OK; }
     const int ps = pbeg(h, DFSPH_KERNEL_SORT);
     CHK(counting_sort(h, h->pos[h->pc], h->n, h->cstart));
     if (sort) {
         const int o = h->pc ^ 1, vo = h->vc ^ 1;
-        dfsph::launch_gather_fluid(h->s, h->n, h->perm, h->pos[h->pc], h->vel[h->vc], h->kap[h->pc],
-                                   h->kapv[h->pc], h->id[h->pc], h->pos[o], h->vel[vo], h->kap[o],
-                                   h->kapv[o], h->id[o]);
+        dfsph::launch_gather_fluid(h->s, h->n, h->perm, fluid_ptrs(h, h->pc, h->vc), fluid_ptrs(h, o, vo));
         h->pc = o;
         h->vc = vo;
     }
@@ -365,21 +373,39 @@ int jacobi_loop(Handle* h, bool div, uint32_t max_iter, double max_error, uint32
     return DFSPH_OK;
 }
 
+// Features of the learned warm start (once per step, after the neighbor
+// search) and, in learned mode, the MLP's predictions of both warm starts.
+// Features are also computed while a training set is being harvested.
+int learned_predict(Handle* h)
+{
+    const bool learned = h->p.warm_start_mode == DFSPH_WARMSTART_LEARNED;
+    const bool harvest = h->mlp && dfsph::mlp_harvesting(h->mlp);
+    if (!learned && !harvest) return DFSPH_OK;
+    if (h->pred_valid) return DFSPH_OK;
+    if (learned && !(h->mlp && dfsph::mlp_loaded(h->mlp)))
+        return fail(DFSPH_ERR_NO_MODEL, "learned warm start selected but no model loaded");
+    CHK(dfsph::mlp_ensure_particles(h->mlp, h->cap));
+    dfsph::launch_features(h->s, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->vel[h->vc],
+                           h->ktot[h->pc], h->kvtot[h->pc], (float)(2.0 * h->p.particle_radius),
+                           dfsph::mlp_feat(h->mlp));
+    HIPCHK(hipGetLastError());
+    if (learned) CHK(dfsph::mlp_infer_particles(h->mlp, h->s, h->n));
+    h->pred_valid = true;
+    return DFSPH_OK;
+}
+
 int divergence_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* iters, double* err)
 {
     CHK(flush_vel(h));
     if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale: call dfsph_neighbor_search");
-    int ws = h->p.warm_start_mode;
+    const int ws = h->p.warm_start_mode;
     const float* kws = h->kapv[h->pc];
-    if (ws == DFSPH_WARMSTART_LEARNED) {
-        if (!h->mlp) return fail(DFSPH_ERR_NO_MODEL, "learned warm start selected but no model loaded");
-        CHK(dfsph::mlp_predict_divergence(h->mlp, h->s, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
-                                          h->vel[h->vc], h->kap[h->pc], h->kapv[h->pc], &kws));
-    }
+    CHK(learned_predict(h));
+    if (ws == DFSPH_WARMSTART_LEARNED) kws = dfsph::mlp_pred_v(h->mlp);
     const int vo = h->vc ^ 1;
     int pr = pbeg(h, DFSPH_KERNEL_DIV_WS_INIT);
     dfsph::launch_div_ws_init(h->s, ws, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->vel[h->vc],
-                              h->vel[vo], kws, h->rho, h->alpha, h->radv);
+                              h->vel[vo], kws, h->rho, h->alpha, h->radv, h->kvtot[h->pc]);
     pend(h, pr);
     h->vc = vo;
     pr = pbeg(h, DFSPH_KERNEL_DIV_INIT);
@@ -388,15 +414,20 @@ int divergence_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* i
     pend(h, pr);
     HIPCHK(hipGetLastError());
     h->acc_valid = false;
-    return jacobi_loop(h, true, max_iter, max_error, iters, err);
+    CHK(jacobi_loop(h, true, max_iter, max_error, iters, err));
+    // the step's divergence total (applied warm start + increments) in kappaV h
+    dfsph::launch_finish_totals(h->s, h->tm, h->n, 0, h->kapv[h->pc], h->kvtot[h->pc]);
+    HIPCHK(hipGetLastError());
+    return DFSPH_OK;
 }
 
 int density_only(Handle* h)
 {
     // rho and alpha without the divergence solver (the reference leaves them stale)
+    CHK(learned_predict(h));
     const int vo = h->vc ^ 1;
     dfsph::launch_div_ws_init(h->s, DFSPH_WARMSTART_NONE, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
-                              h->vel[h->vc], h->vel[vo], nullptr, h->rho, h->alpha, h->radv);
+                              h->vel[h->vc], h->vel[vo], nullptr, h->rho, h->alpha, h->radv, h->kvtot[h->pc]);
     h->vc = vo;
     HIPCHK(hipGetLastError());
     return DFSPH_OK;
@@ -422,16 +453,17 @@ int pressure_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* ite
     const int ws = h->p.warm_start_mode;
     const float* kws = h->kap[h->pc];
     if (ws == DFSPH_WARMSTART_LEARNED) {
-        if (!h->mlp) return fail(DFSPH_ERR_NO_MODEL, "learned warm start selected but no model loaded");
-        CHK(dfsph::mlp_predict_pressure(h->mlp, h->s, &kws));
+        CHK(learned_predict(h));
+        kws = dfsph::mlp_pred_p(h->mlp);
     }
     if (ws == DFSPH_WARMSTART_NONE) {
         CHK(flush_vel(h));
+        dfsph::launch_fill_f32(h->s, h->ktot[h->pc], 0.0f, h->n);
     } else {
         const bool fused = h->pend_vel;
         const int pw = pbeg(h, DFSPH_KERNEL_PRESSURE_WS);
         dfsph::launch_pressure_ws(h->s, ws, fused, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
-                                  h->acc, h->vel[h->vc], kws, fused ? 1 : 0);
+                                  h->acc, h->vel[h->vc], kws, fused ? 1 : 0, h->ktot[h->pc]);
         pend(h, pw);
         h->pend_vel = false;
     }
@@ -441,7 +473,13 @@ int pressure_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* ite
     pend(h, pr);
     HIPCHK(hipGetLastError());
     h->acc_valid = false;
-    return jacobi_loop(h, false, max_iter, max_error, iters, err);
+    CHK(jacobi_loop(h, false, max_iter, max_error, iters, err));
+    // the step's density total (applied warm start + increments) in kappa h_f^2
+    dfsph::launch_finish_totals(h->s, h->tm, h->n, 1, h->kap[h->pc], h->ktot[h->pc]);
+    if (h->mlp && dfsph::mlp_harvesting(h->mlp))
+        CHK(dfsph::mlp_harvest_step(h->mlp, h->s, h->n, h->kvtot[h->pc], h->ktot[h->pc]));
+    HIPCHK(hipGetLastError());
+    return DFSPH_OK;
 }
 
 int compact(Handle* h, uint32_t* removed)
@@ -455,9 +493,9 @@ int compact(Handle* h, uint32_t* removed)
     if (removed) *removed = n - kept;
     if (kept == n) return DFSPH_OK;
     const int o = h->pc ^ 1, vo = h->vc ^ 1;
-    dfsph::launch_compact(h->s, n, h->keep, h->rank, h->pos[h->pc], h->vel[h->vc], h->kap[h->pc], h->kapv[h->pc],
-                          h->id[h->pc], h->pos[o], h->vel[vo], h->kap[o], h->kapv[o], h->id[o]);
+    dfsph::launch_compact(h->s, n, h->keep, h->rank, fluid_ptrs(h, h->pc, h->vc), fluid_ptrs(h, o, vo));
     HIPCHK(hipGetLastError());
+    h->pred_valid = false;
     h->pc = o;
     h->vc = vo;
     h->n = kept;
@@ -485,6 +523,8 @@ int alloc_fluid(Handle* h, uint32_t cap)
         CHK(dalloc(h->vel[b], cap));
         CHK(dalloc(h->kap[b], cap));
         CHK(dalloc(h->kapv[b], cap));
+        CHK(dalloc(h->ktot[b], cap));
+        CHK(dalloc(h->kvtot[b], cap));
         CHK(dalloc(h->id[b], cap));
     }
     CHK(dalloc(h->rho, cap));
@@ -502,6 +542,8 @@ int alloc_fluid(Handle* h, uint32_t cap)
     for (int b = 0; b < 2; ++b) {
         HIPCHK(hipMemsetAsync(h->kap[b], 0, sizeof(float) * cap, h->s));
         HIPCHK(hipMemsetAsync(h->kapv[b], 0, sizeof(float) * cap, h->s));
+        HIPCHK(hipMemsetAsync(h->ktot[b], 0, sizeof(float) * cap, h->s));
+        HIPCHK(hipMemsetAsync(h->kvtot[b], 0, sizeof(float) * cap, h->s));
     }
     HIPCHK(hipMemsetAsync(h->cnt, 0, sizeof(uint32_t) * cap, h->s));
     return DFSPH_OK;
@@ -525,6 +567,9 @@ int write_fluid_range(Handle* h, uint32_t at, const float* xyz, const float* vel
     HIPCHK(hipMemcpyAsync(h->id[h->pc] + at, I.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipMemcpyAsync(h->kap[h->pc] + at, Z.data(), sizeof(float) * m, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipMemcpyAsync(h->kapv[h->pc] + at, Z.data(), sizeof(float) * m, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->ktot[h->pc] + at, Z.data(), sizeof(float) * m, hipMemcpyHostToDevice, h->s));
+    HIPCHK(hipMemcpyAsync(h->kvtot[h->pc] + at, Z.data(), sizeof(float) * m, hipMemcpyHostToDevice, h->s));
+    h->pred_valid = false;
     return sync(h);
 }
 
@@ -537,19 +582,6 @@ MlpView handle_view(void* hv)
     Handle* h = H(hv);
     MlpView v;
     v.s = h->s;
-    v.c = h->c;
-    v.tm = h->tm;
-    v.n = h->n;
-    v.pos = h->pos[h->pc];
-    v.bpos = h->bpos;
-    v.nbr = h->nbr;
-    v.cnt = h->cnt;
-    v.vel = h->vel[h->vc];
-    v.rho = h->rho;
-    v.alpha = h->alpha;
-    v.radv = h->radv;
-    v.kappa = h->kap[h->pc];
-    v.kappaV = h->kapv[h->pc];
     v.mlp = &h->mlp;
     return v;
 }

@@ -72,6 +72,21 @@ struct DevCtl {
     unsigned long long nbr_boundary;
 };
 
+// The persistent per-particle fields that move with a particle when the
+// set is reordered or compacted.  kappa / kappaV are the reference's
+// warm-start accumulators (this step's solver increments); ktot / kvtot are
+// the step's totals (applied warm start + increments) in dt-invariant units
+// (kappa h_f^2, kappaV h), the inputs and targets of the learned warm start.
+struct FluidPtrs {
+    float4* pos;
+    float4* vel;
+    float* kap;
+    float* kapv;
+    float* ktot;
+    float* kvtot;
+    uint32_t* id;
+};
+
 // Blocks are dispatched round robin over the 8 XCDs, each with its own L2.
 // Remapping gives every XCD a contiguous range of logical blocks, so
 // neighbor gathers of cell-sorted particles stay in one L2.  gridDim.x must

@@ -245,19 +245,22 @@ __global__ void k_cell_sort(const uint32_t* cstart, uint32_t ncell, uint32_t* pe
 
 // cuda_sortData (DFSPH_core_cuda.cu:2057-2129): one gather of every persistent
 // per-particle field into the alternate buffers (no device-to-device copies).
-__global__ void k_gather_fluid(uint32_t n, const uint32_t* perm,
-                               const float4* pos_i, const float4* vel_i, const float* kap_i,
-                               const float* kapv_i, const uint32_t* id_i,
-                               float4* pos_o, float4* vel_o, float* kap_o, float* kapv_o, uint32_t* id_o)
+__device__ __forceinline__ void copy_fluid(const FluidPtrs& a, uint32_t from, const FluidPtrs& b, uint32_t to)
+{
+    b.pos[to] = a.pos[from];
+    b.vel[to] = a.vel[from];
+    b.kap[to] = a.kap[from];
+    b.kapv[to] = a.kapv[from];
+    b.ktot[to] = a.ktot[from];
+    b.kvtot[to] = a.kvtot[from];
+    b.id[to] = a.id[from];
+}
+
+__global__ void k_gather_fluid(uint32_t n, const uint32_t* perm, FluidPtrs in, FluidPtrs out)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const uint32_t o = perm[k];
-    pos_o[k] = pos_i[o];
-    vel_o[k] = vel_i[o];
-    kap_o[k] = kap_i[o];
-    kapv_o[k] = kapv_i[o];
-    id_o[k] = id_i[o];
+    copy_fluid(in, perm[k], out, k);
 }
 
 __global__ void k_gather_f4(uint32_t n, const uint32_t* perm, const float4* in, float4* out)
@@ -422,8 +425,10 @@ __global__ __launch_bounds__(kBlock) void k_solve_finalize(SolveCtl* ctl, const 
 // rho_i = m_i W0 + sum m_j W_ij; alpha_i = -(1/h) / max(sum |m_j gradW|^2 + |sum m_j gradW|^2, eps);
 // rho_adv_i = 0 if fewer than 20 neighbors else max(sum m_j (v_i - v_j).gradW, 0).
 // Warm start (only where rho_adv_i > 0): v_i += h sum (k_i + k_j) m_j gradW with
-// k = max(kappaV h_ratio / 2, -0.5) (WS == 0), the predicted value (WS == 1),
-// or nothing (WS == 2).  Velocities are read from vel_i and written to vel_o.
+// k = max(kappaV h_ratio / 2, -0.5) (WS == 0), the learned prediction of
+// kappaV h (dt-invariant) / h, clamped like the reference (WS == 1), or nothing
+// (WS == 2).  Velocities are read from vel_i and written to vel_o.  The
+// applied k_i starts the particle's total of the step in kvtot (when given).
 template <int WS>
 __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTime* t, uint32_t n,
                                                         const float4* __restrict__ pos,
@@ -434,7 +439,7 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
                                                         float4* __restrict__ vel_o,
                                                         const float* __restrict__ kws,
                                                         float* __restrict__ rho, float* __restrict__ alpha,
-                                                        float* __restrict__ radv)
+                                                        float* __restrict__ radv, float* __restrict__ kvtot)
 {
     const uint32_t i = xcd_index();
     if (i >= n) return;
@@ -446,10 +451,11 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
     const uint32_t stride = c.cap;
     auto ws_of = [&](uint32_t j) -> float {
         if (WS == 0) return fmaxf(kws[j] * ratio / 2.0f, -0.5f);
-        if (WS == 1) return kws[j];
+        if (WS == 1) return fmaxf(kws[j] * inv_h, -0.5f);
         return 0.0f;
     };
     const float ki = ws_of(i);
+    if (kvtot) kvtot[i] = ki;
     float dens = pi.w * c.W0, sg2 = 0.0f, dr = 0.0f;
     float gx = 0.0f, gy = 0.0f, gz = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
     const uint32_t* p = nbr + i;
@@ -680,7 +686,9 @@ __global__ void k_update_vel(const DevTime* t, uint32_t n, const float4* acc, fl
 // update of DFSPH_update_vel_kernel (APPLY_ACC):
 // v_i += h a_i;  v_i += h_f (sum_f [|k_i+k_j|>eps] (k_i+k_j) m_j gradW + [|k_i|>eps] sum_b k_i psi_j gradW)
 // with k = max(kappa ratio2, -0.5) (WS == 0; ratio2 = 1 when already applied),
-// the predicted value (WS == 1).  No gate (unlike the divergence warm start).
+// or the learned prediction of kappa h_f^2 (dt-invariant) / h_f^2, clamped like
+// the reference (WS == 1).  No gate (unlike the divergence warm start).  The
+// applied k_i starts the particle's total of the step in ktot (when given).
 template <int WS, bool APPLY_ACC>
 __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTime* t, uint32_t n,
                                                         const float4* __restrict__ pos,
@@ -689,16 +697,19 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
                                                         const uint32_t* __restrict__ cnt,
                                                         const float4* __restrict__ acc,
                                                         float4* __restrict__ vel,
-                                                        const float* __restrict__ kws, int rescale)
+                                                        const float* __restrict__ kws, int rescale,
+                                                        float* __restrict__ ktot)
 {
     const uint32_t i = xcd_index();
     if (i >= n) return;
     const float ratio2 = rescale ? t->ratio2 : 1.0f;
+    const float ihf2 = t->inv_h_future * t->inv_h_future;
     auto ws_of = [&](uint32_t j) -> float {
         if (WS == 0) return fmaxf(kws[j] * ratio2, -0.5f);
-        return kws[j];
+        return fmaxf(kws[j] * ihf2, -0.5f);
     };
     const float ki = ws_of(i);
+    if (ktot) ktot[i] = ki;
     const float4 pi = pos[i];
     const uint32_t cn = cnt[i];
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
@@ -833,19 +844,104 @@ __global__ void k_keep_from_u8(uint32_t n, const uint8_t* k8, uint32_t* keep)
 // Stable compaction: the kept particles move to their exclusive-scan rank
 // (replaces the radix sort by tag + cuda_sortData of remove_tagged_particles,
 // SPH_other_systems_cuda.cu:1606-1641).
-__global__ void k_compact(uint32_t n, const uint32_t* keep, const uint32_t* rank,
-                          const float4* pos_i, const float4* vel_i, const float* kap_i,
-                          const float* kapv_i, const uint32_t* id_i,
-                          float4* pos_o, float4* vel_o, float* kap_o, float* kapv_o, uint32_t* id_o)
+__global__ void k_compact(uint32_t n, const uint32_t* keep, const uint32_t* rank, FluidPtrs in, FluidPtrs out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !keep[i]) return;
-    const uint32_t o = rank[i];
-    pos_o[o] = pos_i[i];
-    vel_o[o] = vel_i[i];
-    kap_o[o] = kap_i[i];
-    kapv_o[o] = kapv_i[i];
-    id_o[o] = id_i[i];
+    copy_fluid(in, i, out, rank[i]);
+}
+
+// End of a solve: the step's total = applied warm start + the solver's
+// accumulated increments, in dt-invariant units (scale h for kappaV, h_f^2 for kappa).
+__global__ void k_finish_totals(const DevTime* t, uint32_t n, int pressure, const float* inc, float* tot)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float s = pressure ? t->h_future * t->h_future : t->h;
+    tot[i] = (tot[i] + inc[i]) * s;
+}
+
+// Features of the learned warm start, one neighbor sweep over the current state
+// (before the divergence warm start):
+//  0 rho / rho0 - 1            1 divergence source h / rho0 (unclamped)
+//  2 (n_fluid + n_boundary) / 40   3 |v| h / (2r)
+//  4 previous total kappa h_f^2    5 previous total kappaV h
+//  6 boundary fraction n_b / n     7-9 mean neighbor offset / h_k
+// 10 mean neighbor distance / h_k 11 fluid share of the density
+__global__ __launch_bounds__(kBlock) void k_features(SimConst c, const DevTime* t, uint32_t n,
+                                                     const float4* __restrict__ pos,
+                                                     const float4* __restrict__ bpos,
+                                                     const uint32_t* __restrict__ nbr,
+                                                     const uint32_t* __restrict__ cnt,
+                                                     const float4* __restrict__ vel,
+                                                     const float* __restrict__ ktot,
+                                                     const float* __restrict__ kvtot, float diameter,
+                                                     float* __restrict__ feat)
+{
+    const uint32_t i = xcd_index();
+    if (i >= n) return;
+    const float4 pi = pos[i];
+    const float4 vi = vel[i];
+    const uint32_t cn = cnt[i];
+    const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
+    const uint32_t* p = nbr + i;
+    float dens = pi.w * c.W0, dfl = 0.0f, dr = 0.0f;
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f, od = 0.0f;
+    for (uint32_t k = 0; k < nf; ++k, p += c.cap) {
+        const uint32_t j = *p;
+        const float4 pj = pos[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dist2(dx, dy, dz);
+        const PairK kp = kernel_pair(c, r2);
+        dfl += pj.w * kp.W;
+        const float mg = pj.w * kp.g;
+        const float4 vj = vel[j];
+        dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+        ox -= dx; oy -= dy; oz -= dz;
+        od += __builtin_sqrtf(r2);
+    }
+    float dbd = 0.0f;
+    for (uint32_t k = 0; k < nb; ++k, p += c.cap) {
+        const uint32_t j = *p;
+        const float4 pj = bpos[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dist2(dx, dy, dz);
+        const PairK kp = kernel_pair(c, r2);
+        dbd += pj.w * kp.W;
+        dr += pj.w * kp.g * (vi.x * dx + vi.y * dy + vi.z * dz);
+        ox -= dx; oy -= dy; oz -= dz;
+        od += __builtin_sqrtf(r2);
+    }
+    dens += dfl + dbd;
+    const float nn = (float)(nf + nb);
+    const float inv_n = nn > 0.0f ? 1.0f / nn : 0.0f;
+    const float s = c.inv_hk * inv_n;
+    float* f = feat + (size_t)i * 12;
+    f[0] = dens / c.density0 - 1.0f;
+    f[1] = dr * t->h / c.density0;
+    f[2] = nn * (1.0f / 40.0f);
+    f[3] = __builtin_sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z) * t->h / diameter;
+    f[4] = ktot[i];
+    f[5] = kvtot[i];
+    f[6] = (float)nb * inv_n;
+    f[7] = ox * s;
+    f[8] = oy * s;
+    f[9] = oz * s;
+    f[10] = od * s;
+    f[11] = dfl / dens;
+}
+
+// training set: append the features of the step and the step's totals
+__global__ void k_harvest(uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap, const float* feat,
+                          const float* kvtot, const float* ktot, float* ds_x, float* ds_t)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = k * stride;
+    if (i >= n || offset + k >= cap) return;
+    const size_t o = offset + k;
+    for (int a = 0; a < 12; ++a) ds_x[o * 12 + a] = feat[(size_t)i * 12 + a];
+    ds_t[o * 2 + 0] = kvtot[i];
+    ds_t[o * 2 + 1] = ktot[i];
 }
 
 // ------------------------------------------------------------------------
@@ -940,11 +1036,9 @@ void launch_cell_sort(hipStream_t s, const uint32_t* cstart, uint32_t ncell, uin
     k_cell_sort<<<nblk(ncell), kBlock, 0, s>>>(cstart, ncell, perm);
 }
 
-void launch_gather_fluid(hipStream_t s, uint32_t n, const uint32_t* perm, const float4* pi, const float4* vi,
-                         const float* ki, const float* kvi, const uint32_t* idi, float4* po, float4* vo,
-                         float* ko, float* kvo, uint32_t* ido)
+void launch_gather_fluid(hipStream_t s, uint32_t n, const uint32_t* perm, const FluidPtrs& in, const FluidPtrs& out)
 {
-    if (n) k_gather_fluid<<<nblk(n), kBlock, 0, s>>>(n, perm, pi, vi, ki, kvi, idi, po, vo, ko, kvo, ido);
+    if (n) k_gather_fluid<<<nblk(n), kBlock, 0, s>>>(n, perm, in, out);
 }
 
 void launch_gather_f4(hipStream_t s, uint32_t n, const uint32_t* perm, const float4* in, float4* out)
@@ -977,16 +1071,16 @@ void launch_set_w(hipStream_t s, float4* p, const float* w, uint32_t n)
 void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime* t, uint32_t n,
                         const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
                         const float4* vel_i, float4* vel_o, const float* kws, float* rho, float* alpha,
-                        float* radv)
+                        float* radv, float* kvtot)
 {
     if (!n) return;
     const uint32_t g = nblk_xcd(n);
     if (ws == 0)
-        k_div_ws_init<0><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel_i, vel_o, kws, rho, alpha, radv);
+        k_div_ws_init<0><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel_i, vel_o, kws, rho, alpha, radv, kvtot);
     else if (ws == 1)
-        k_div_ws_init<1><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel_i, vel_o, kws, rho, alpha, radv);
+        k_div_ws_init<1><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel_i, vel_o, kws, rho, alpha, radv, kvtot);
     else
-        k_div_ws_init<2><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel_i, vel_o, kws, rho, alpha, radv);
+        k_div_ws_init<2><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel_i, vel_o, kws, rho, alpha, radv, kvtot);
 }
 
 void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
@@ -1030,20 +1124,20 @@ void launch_update_vel(hipStream_t s, const DevTime* t, uint32_t n, const float4
 
 void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c, const DevTime* t, uint32_t n,
                         const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
-                        const float4* acc, float4* vel, const float* kws, int rescale)
+                        const float4* acc, float4* vel, const float* kws, int rescale, float* ktot)
 {
     if (!n) return;
     const uint32_t g = nblk_xcd(n);
     if (ws == 1) {
         if (apply_acc)
-            k_pressure_ws<1, true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale);
+            k_pressure_ws<1, true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale, ktot);
         else
-            k_pressure_ws<1, false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale);
+            k_pressure_ws<1, false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale, ktot);
     } else {
         if (apply_acc)
-            k_pressure_ws<0, true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale);
+            k_pressure_ws<0, true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale, ktot);
         else
-            k_pressure_ws<0, false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale);
+            k_pressure_ws<0, false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale, ktot);
     }
 }
 
@@ -1079,11 +1173,29 @@ void launch_keep_from_u8(hipStream_t s, uint32_t n, const uint8_t* k8, uint32_t*
     k_keep_from_u8<<<nblk(n + 1), kBlock, 0, s>>>(n, k8, keep);
 }
 
-void launch_compact(hipStream_t s, uint32_t n, const uint32_t* keep, const uint32_t* rank, const float4* pi,
-                    const float4* vi, const float* ki, const float* kvi, const uint32_t* idi, float4* po,
-                    float4* vo, float* ko, float* kvo, uint32_t* ido)
+void launch_compact(hipStream_t s, uint32_t n, const uint32_t* keep, const uint32_t* rank, const FluidPtrs& in,
+                    const FluidPtrs& out)
 {
-    if (n) k_compact<<<nblk(n), kBlock, 0, s>>>(n, keep, rank, pi, vi, ki, kvi, idi, po, vo, ko, kvo, ido);
+    if (n) k_compact<<<nblk(n), kBlock, 0, s>>>(n, keep, rank, in, out);
+}
+
+void launch_finish_totals(hipStream_t s, const DevTime* t, uint32_t n, int pressure, const float* inc, float* tot)
+{
+    if (n) k_finish_totals<<<nblk(n), kBlock, 0, s>>>(t, n, pressure, inc, tot);
+}
+
+void launch_features(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
+                     const float4* bpos, const uint32_t* nbr, const uint32_t* cnt, const float4* vel,
+                     const float* ktot, const float* kvtot, float diameter, float* feat)
+{
+    if (n) k_features<<<nblk_xcd(n), kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel, ktot, kvtot, diameter, feat);
+}
+
+void launch_harvest(hipStream_t s, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap, const float* feat,
+                    const float* kvtot, const float* ktot, float* ds_x, float* ds_t)
+{
+    const uint32_t m = (n + stride - 1) / stride;
+    if (m) k_harvest<<<nblk(m), kBlock, 0, s>>>(n, stride, offset, cap, feat, kvtot, ktot, ds_x, ds_t);
 }
 
 }  // namespace dfsph

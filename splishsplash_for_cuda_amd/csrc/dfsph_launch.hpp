@@ -30,9 +30,7 @@ void launch_cell_hist(hipStream_t s, const SimConst& c, const float4* pos, uint3
 void launch_scatter_perm(hipStream_t s, const uint32_t* key, const uint32_t* slot, const uint32_t* cstart,
                          uint32_t n, uint32_t* perm);
 void launch_cell_sort(hipStream_t s, const uint32_t* cstart, uint32_t ncell, uint32_t* perm);
-void launch_gather_fluid(hipStream_t s, uint32_t n, const uint32_t* perm, const float4* pi, const float4* vi,
-                         const float* ki, const float* kvi, const uint32_t* idi, float4* po, float4* vo,
-                         float* ko, float* kvo, uint32_t* ido);
+void launch_gather_fluid(hipStream_t s, uint32_t n, const uint32_t* perm, const FluidPtrs& in, const FluidPtrs& out);
 void launch_gather_f4(hipStream_t s, uint32_t n, const uint32_t* perm, const float4* in, float4* out);
 void launch_neighbors(hipStream_t s, const SimConst& c, bool indirect, const float4* pos, const uint32_t* cs_f,
                       const uint32_t* perm, const float4* bpos, const uint32_t* cs_b, uint32_t n,
@@ -44,7 +42,7 @@ void launch_set_w(hipStream_t s, float4* p, const float* w, uint32_t n);
 void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime* t, uint32_t n,
                         const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
                         const float4* vel_i, float4* vel_o, const float* kws, float* rho, float* alpha,
-                        float* radv);
+                        float* radv, float* kvtot);
 void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
                        const float4* bpos, const uint32_t* nbr, const uint32_t* cnt, const float4* vel,
                        const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial);
@@ -57,7 +55,7 @@ void launch_xsph(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n,
 void launch_update_vel(hipStream_t s, const DevTime* t, uint32_t n, const float4* acc, float4* vel, float* kappa);
 void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c, const DevTime* t, uint32_t n,
                         const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
-                        const float4* acc, float4* vel, const float* kws, int rescale);
+                        const float4* acc, float4* vel, const float* kws, int rescale, float* ktot);
 void launch_density_source(hipStream_t s, bool init, const SimConst& c, const DevTime* t, uint32_t n,
                            const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
                            const float4* vel, const float* rho, float* alpha, float* radv, float* kk,
@@ -67,8 +65,13 @@ void launch_update_pos(hipStream_t s, const DevTime* t, uint32_t n, const float4
 void launch_keep_inside(hipStream_t s, uint32_t n, const float4* pos, const float* bmin, const float* bmax,
                         uint32_t* keep);
 void launch_keep_from_u8(hipStream_t s, uint32_t n, const uint8_t* k8, uint32_t* keep);
-void launch_compact(hipStream_t s, uint32_t n, const uint32_t* keep, const uint32_t* rank, const float4* pi,
-                    const float4* vi, const float* ki, const float* kvi, const uint32_t* idi, float4* po,
-                    float4* vo, float* ko, float* kvo, uint32_t* ido);
+void launch_compact(hipStream_t s, uint32_t n, const uint32_t* keep, const uint32_t* rank, const FluidPtrs& in,
+                    const FluidPtrs& out);
+void launch_finish_totals(hipStream_t s, const DevTime* t, uint32_t n, int pressure, const float* inc, float* tot);
+void launch_features(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
+                     const float4* bpos, const uint32_t* nbr, const uint32_t* cnt, const float4* vel,
+                     const float* ktot, const float* kvtot, float diameter, float* feat);
+void launch_harvest(hipStream_t s, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap, const float* feat,
+                    const float* kvtot, const float* ktot, float* ds_x, float* ds_t);
 
 }  // namespace dfsph

@@ -11,36 +11,26 @@ namespace dfsph {
 
 struct MlpState;
 
-// Views of the solver buffers the learned warm start reads (current order).
+// Solver-handle access used by dfsph_mlp.hip.
 struct MlpView {
     hipStream_t s;
-    SimConst c;
-    const DevTime* tm;
-    uint32_t n;
-    const float4* pos;
-    const float4* bpos;
-    const uint32_t* nbr;
-    const uint32_t* cnt;
-    const float4* vel;
-    const float* rho;
-    const float* alpha;
-    const float* radv;
-    float* kappa;
-    float* kappaV;
     MlpState** mlp;
 };
 
 MlpView handle_view(void* handle);
 void handle_set_error(const char* msg);
 
-// Predict the divergence warm start for every particle (features from the
-// current state, then the MLP); *kws_out receives the device array of
-// predicted kappaV values (already in the units the warm-start sweep uses).
-int mlp_predict_divergence(MlpState* m, hipStream_t s, const SimConst& c, const DevTime* tm, uint32_t n,
-                           const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
-                           const float4* vel, const float* kappa, const float* kappaV, const float** kws_out);
-// The pressure warm start predicted by the same pass.
-int mlp_predict_pressure(MlpState* m, hipStream_t s, const float** kws_out);
+bool mlp_loaded(const MlpState* m);
+bool mlp_harvesting(const MlpState* m);
+// per-particle buffers for up to cap particles
+int mlp_ensure_particles(MlpState* m, uint32_t cap);
+float* mlp_feat(MlpState* m);            // [cap][12] features of the current step
+// predictions from the features of the first n particles
+int mlp_infer_particles(MlpState* m, hipStream_t s, uint32_t n);
+const float* mlp_pred_v(const MlpState* m);   // kappaV h     (dt-invariant)
+const float* mlp_pred_p(const MlpState* m);   // kappa h_f^2  (dt-invariant)
+// append (features, totals) of the step to the training set
+int mlp_harvest_step(MlpState* m, hipStream_t s, uint32_t n, const float* kvtot, const float* ktot);
 void mlp_destroy(MlpState* m);
 
 }  // namespace dfsph
