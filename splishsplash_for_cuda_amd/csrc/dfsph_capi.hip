@@ -533,7 +533,11 @@ int alloc_fluid(Handle* h, uint32_t cap)
     CHK(dalloc(h->kk, cap));
     CHK(dalloc(h->acc, cap));
     CHK(dalloc(h->cnt, cap));
-    CHK(dalloc(h->nbr, (size_t)cap * h->p.max_neighbors));
+    // kNbrSlack extra slots: the sweeps load list entries in batches past a
+    // particle's count (masked); zeroed so those reads see valid indices
+    const size_t nslots = (size_t)cap * (h->p.max_neighbors + dfsph::kNbrSlack);
+    CHK(dalloc(h->nbr, nslots));
+    HIPCHK(hipMemsetAsync(h->nbr, 0, sizeof(uint32_t) * nslots, h->s));
     CHK(ensure_tmp(h, std::max(cap, h->nb)));
     CHK(ensure_partials(h, cap));
     h->cap = cap;

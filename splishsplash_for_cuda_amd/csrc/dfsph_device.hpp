@@ -171,6 +171,46 @@ __device__ __forceinline__ uint32_t cell_id(const SimConst& c, float x, float y,
     return (uint32_t)cx + (uint32_t)c.dim[0] * ((uint32_t)cz + (uint32_t)c.dim[2] * (uint32_t)cy);
 }
 
+// Neighbor loop with memory-level parallelism: U list entries are loaded at
+// once (U coalesced loads of the interleaved list), then the U neighbors'
+// gathers can all be in flight before the first one is used.  Without the
+// batching every neighbor costs two dependent memory round trips (list entry,
+// then the gather).  The list loads are unconditional: the last batch reads up
+// to U-1 slots past the particle's entries, which the list buffer's
+// kNbrSlack extra slots keep in bounds; those entries are replaced by `pad`
+// (a valid particle index) and passed with valid == false, so the callee
+// zeroes their contributions.  Callees mask with keep() on values already
+// computed, so the body has no branch the compiler could sink a gather into.
+#ifndef DFSPH_NBR_UNROLL
+#define DFSPH_NBR_UNROLL 4
+#endif
+constexpr int kNbrSlack = 8;       // extra list slots past max_neighbors
+static_assert(DFSPH_NBR_UNROLL >= 1 && DFSPH_NBR_UNROLL <= kNbrSlack + 1, "unroll exceeds the list slack");
+
+template <int U = DFSPH_NBR_UNROLL, class F>
+__device__ __forceinline__ void for_each_nbr(const uint32_t* __restrict__ p, uint32_t stride, uint32_t count,
+                                             uint32_t pad, F&& f)
+{
+    for (uint32_t k = 0; k < count; k += U) {
+        uint32_t j[U];
+        bool v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t e = p[(size_t)(k + u) * stride];
+            v[u] = k + u < count;
+            j[u] = v[u] ? e : pad;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) f(j[u], v[u]);
+    }
+}
+
+// x where c holds, else 0 (a select of an already computed value)
+__device__ __forceinline__ float keep(bool c, float x)
+{
+    return c ? x : 0.0f;
+}
+
 // ---- wavefront / block reductions (wave64) ----
 __device__ __forceinline__ double wave_sum(double v)
 {

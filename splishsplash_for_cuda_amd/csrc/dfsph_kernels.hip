@@ -458,14 +458,13 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
     if (kvtot) kvtot[i] = ki;
     float dens = pi.w * c.W0, sg2 = 0.0f, dr = 0.0f;
     float gx = 0.0f, gy = 0.0f, gz = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
-    const uint32_t* p = nbr + i;
-    for (uint32_t k = 0; k < nf; ++k, p += stride) {
-        const uint32_t j = *p;
+    for_each_nbr(nbr + i, stride, nf, i, [&](uint32_t j, bool valid) {
         const float4 pj = pos[j];
+        const float m = keep(valid, pj.w);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const PairK kp = kernel_pair(c, dist2(dx, dy, dz));
-        dens += pj.w * kp.W;
-        const float mg = pj.w * kp.g;
+        dens += m * kp.W;
+        const float mg = m * kp.g;
         const float px = mg * dx, py = mg * dy, pz = mg * dz;
         sg2 += px * px + py * py + pz * pz;
         gx += px; gy += py; gz += pz;
@@ -473,24 +472,24 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
         dr += (vi.x - vj.x) * px + (vi.y - vj.y) * py + (vi.z - vj.z) * pz;
         if (WS != 2) {
             const float ks = ki + ws_of(j);
-            const float s = fabsf(ks) > kEps ? ks : 0.0f;
+            const float s = keep(fabsf(ks) > kEps, ks);
             ax += s * px; ay += s * py; az += s * pz;
         }
-    }
+    });
     float bx = 0.0f, by = 0.0f, bz = 0.0f;
-    for (uint32_t k = 0; k < nb; ++k, p += stride) {
-        const uint32_t j = *p;
+    for_each_nbr(nbr + i + (size_t)nf * stride, stride, nb, 0u, [&](uint32_t j, bool valid) {
         const float4 pj = bpos[j];
+        const float m = keep(valid, pj.w);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const PairK kp = kernel_pair(c, dist2(dx, dy, dz));
-        dens += pj.w * kp.W;
-        const float mg = pj.w * kp.g;
+        dens += m * kp.W;
+        const float mg = m * kp.g;
         const float px = mg * dx, py = mg * dy, pz = mg * dz;
         sg2 += px * px + py * py + pz * pz;
         gx += px; gy += py; gz += pz;
         dr += vi.x * px + vi.y * py + vi.z * pz;
         bx += px; by += py; bz += pz;
-    }
+    });
     sg2 += gx * gx + gy * gy + gz * gz;
     const float a = -inv_h / fmaxf(sg2, kEps);
     const float r = (nf + nb < 20u) ? 0.0f : fmaxf(dr, 0.0f);
@@ -530,23 +529,20 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
         const float4 vi = vel[i];
         const uint32_t cn = cnt[i];
         const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
-        const uint32_t* p = nbr + i;
         float dr = 0.0f;
-        for (uint32_t k = 0; k < nf; ++k, p += c.cap) {
-            const uint32_t j = *p;
+        for_each_nbr(nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) {
             const float4 pj = pos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float mg = pj.w * kernel_grad_coef(c, dist2(dx, dy, dz));
+            const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
             const float4 vj = vel[j];
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
-        }
-        for (uint32_t k = 0; k < nb; ++k, p += c.cap) {
-            const uint32_t j = *p;
+        });
+        for_each_nbr(nbr + i + (size_t)nf * c.cap, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
             const float4 pj = bpos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float mg = pj.w * kernel_grad_coef(c, dist2(dx, dy, dz));
+            const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
             dr += mg * (vi.x * dx + vi.y * dy + vi.z * dz);
-        }
+        });
         r = (nf + nb < 20u) ? 0.0f : fmaxf(dr, 0.0f);
         radv[i] = r;
         kk[i] = r * alpha[i];
@@ -578,25 +574,22 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t,
     const float4 pi = pos[i];
     const uint32_t cn = cnt[i];
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
-    const uint32_t* p = nbr + i;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    for (uint32_t k = 0; k < nf; ++k, p += c.cap) {
-        const uint32_t j = *p;
+    for_each_nbr(nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) {
         const float4 pj = pos[j];
         const float ks = ki + kk[j];
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float g = kernel_grad_coef(c, dist2(dx, dy, dz));
-        const float s = fabsf(ks) > kEps ? ks * pj.w * g : 0.0f;
+        const float s = keep(valid && fabsf(ks) > kEps, ks * pj.w * g);
         ax += s * dx; ay += s * dy; az += s * dz;
-    }
+    });
     if (fabsf(ki) > kEps) {
-        for (uint32_t k = 0; k < nb; ++k, p += c.cap) {
-            const uint32_t j = *p;
+        for_each_nbr(nbr + i + (size_t)nf * c.cap, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
             const float4 pj = bpos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float s = ki * pj.w * kernel_grad_coef(c, dist2(dx, dy, dz));
+            const float s = keep(valid, ki * pj.w * kernel_grad_coef(c, dist2(dx, dy, dz)));
             ax += s * dx; ay += s * dy; az += s * dz;
-        }
+        });
     }
     const float step = DIV ? t->h : t->h_future;
     float4 v = vel[i];
@@ -623,17 +616,15 @@ __global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, u
         const float4 pi = pos[i];
         const float4 vi = vel[i];
         const uint32_t nf = cnt[i] & 0xffffu;
-        const uint32_t* p = nbr + i;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        for (uint32_t k = 0; k < nf; ++k, p += c.cap) {
-            const uint32_t j = *p;
+        for_each_nbr(nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) {
             const float4 pj = pos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float w = kernel_W(c, dist2(dx, dy, dz));
             const float4 vj = vel[j];
-            const float s = coef * (pj.w / rho[j]) * w;
+            const float s = keep(valid, coef * (pj.w / rho[j]) * w);
             ax -= s * (vi.x - vj.x); ay -= s * (vi.y - vj.y); az -= s * (vi.z - vj.z);
-        }
+        });
         const float4 a = make_float4(c.gx + ax, c.gy + ay, c.gz + az, 0.0f);
         acc[i] = a;
         const float h = t->h;
@@ -713,25 +704,22 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
     const float4 pi = pos[i];
     const uint32_t cn = cnt[i];
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
-    const uint32_t* p = nbr + i;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    for (uint32_t k = 0; k < nf; ++k, p += c.cap) {
-        const uint32_t j = *p;
+    for_each_nbr(nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) {
         const float4 pj = pos[j];
         const float ks = ki + ws_of(j);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float g = kernel_grad_coef(c, dist2(dx, dy, dz));
-        const float s = fabsf(ks) > kEps ? ks * pj.w * g : 0.0f;
+        const float s = keep(valid && fabsf(ks) > kEps, ks * pj.w * g);
         ax += s * dx; ay += s * dy; az += s * dz;
-    }
+    });
     if (fabsf(ki) > kEps) {
-        for (uint32_t k = 0; k < nb; ++k, p += c.cap) {
-            const uint32_t j = *p;
+        for_each_nbr(nbr + i + (size_t)nf * c.cap, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
             const float4 pj = bpos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float s = ki * pj.w * kernel_grad_coef(c, dist2(dx, dy, dz));
+            const float s = keep(valid, ki * pj.w * kernel_grad_coef(c, dist2(dx, dy, dz)));
             ax += s * dx; ay += s * dy; az += s * dz;
-        }
+        });
     }
     float4 v = vel[i];
     if (APPLY_ACC) {
@@ -771,23 +759,20 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
         const float4 vi = vel[i];
         const uint32_t cn = cnt[i];
         const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
-        const uint32_t* p = nbr + i;
         float dr = 0.0f;
-        for (uint32_t k = 0; k < nf; ++k, p += c.cap) {
-            const uint32_t j = *p;
+        for_each_nbr(nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) {
             const float4 pj = pos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float mg = pj.w * kernel_grad_coef(c, dist2(dx, dy, dz));
+            const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
             const float4 vj = vel[j];
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
-        }
-        for (uint32_t k = 0; k < nb; ++k, p += c.cap) {
-            const uint32_t j = *p;
+        });
+        for_each_nbr(nbr + i + (size_t)nf * c.cap, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
             const float4 pj = bpos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float mg = pj.w * kernel_grad_coef(c, dist2(dx, dy, dz));
+            const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
             dr += mg * (vi.x * dx + vi.y * dy + vi.z * dz);
-        }
+        });
         float a = alpha[i];
         if (INIT) { a *= t->inv_h_future; alpha[i] = a; }
         r = fmaxf(rho[i] + hf * dr - c.density0, 0.0f);
@@ -884,34 +869,34 @@ __global__ __launch_bounds__(kBlock) void k_features(SimConst c, const DevTime* 
     const float4 vi = vel[i];
     const uint32_t cn = cnt[i];
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
-    const uint32_t* p = nbr + i;
     float dens = pi.w * c.W0, dfl = 0.0f, dr = 0.0f;
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, od = 0.0f;
-    for (uint32_t k = 0; k < nf; ++k, p += c.cap) {
-        const uint32_t j = *p;
+    for_each_nbr(nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) {
         const float4 pj = pos[j];
+        const float m = keep(valid, 1.0f);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2(dx, dy, dz);
         const PairK kp = kernel_pair(c, r2);
-        dfl += pj.w * kp.W;
-        const float mg = pj.w * kp.g;
+        const float mj = m * pj.w;
+        dfl += mj * kp.W;
         const float4 vj = vel[j];
-        dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
-        ox -= dx; oy -= dy; oz -= dz;
-        od += __builtin_sqrtf(r2);
-    }
+        dr += mj * kp.g * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+        ox -= m * dx; oy -= m * dy; oz -= m * dz;
+        od += m * __builtin_sqrtf(r2);
+    });
     float dbd = 0.0f;
-    for (uint32_t k = 0; k < nb; ++k, p += c.cap) {
-        const uint32_t j = *p;
+    for_each_nbr(nbr + i + (size_t)nf * c.cap, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
         const float4 pj = bpos[j];
+        const float m = keep(valid, 1.0f);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2(dx, dy, dz);
         const PairK kp = kernel_pair(c, r2);
-        dbd += pj.w * kp.W;
-        dr += pj.w * kp.g * (vi.x * dx + vi.y * dy + vi.z * dz);
-        ox -= dx; oy -= dy; oz -= dz;
-        od += __builtin_sqrtf(r2);
-    }
+        const float mj = m * pj.w;
+        dbd += mj * kp.W;
+        dr += mj * kp.g * (vi.x * dx + vi.y * dy + vi.z * dz);
+        ox -= m * dx; oy -= m * dy; oz -= m * dz;
+        od += m * __builtin_sqrtf(r2);
+    });
     dens += dfl + dbd;
     const float nn = (float)(nf + nb);
     const float inv_n = nn > 0.0f ? 1.0f / nn : 0.0f;
