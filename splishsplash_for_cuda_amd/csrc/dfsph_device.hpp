@@ -182,7 +182,7 @@ __device__ __forceinline__ uint32_t cell_id(const SimConst& c, float x, float y,
 // zeroes their contributions.  Callees mask with keep() on values already
 // computed, so the body has no branch the compiler could sink a gather into.
 #ifndef DFSPH_NBR_UNROLL
-#define DFSPH_NBR_UNROLL 4
+#define DFSPH_NBR_UNROLL 8
 #endif
 constexpr int kNbrSlack = 8;       // extra list slots past max_neighbors
 static_assert(DFSPH_NBR_UNROLL >= 1 && DFSPH_NBR_UNROLL <= kNbrSlack + 1, "unroll exceeds the list slack");

@@ -675,9 +675,13 @@ int dfsph_mlp_fit_normalization(void* handle)
             s += v;
             s2 += v * v;
         }
+        // no absolute floor on the spread: the dt-invariant totals are O(1e-9)
+        // in SI units; only a constant column falls back to its magnitude or 1
         const double mean = s / n, var = std::max(s2 / n - mean * mean, 0.0);
+        double dev = std::sqrt(var);
+        if (!(dev > 1e-7 * std::fabs(mean)) || !(dev > 1e-30)) dev = std::max(std::fabs(mean), 1.0);
         *mu = (float)mean;
-        *sd = (float)std::max(std::sqrt(var), 1e-6);
+        *sd = (float)dev;
     };
     for (int k = 0; k < dfsph::kIn; ++k) stats(x, dfsph::kIn, k, &nrm[k], &nrm[dfsph::kIn + k]);
     for (int k = 0; k < dfsph::kOut; ++k)

@@ -6,7 +6,9 @@
 # One kernel-trace + stats run, then one run per hardware counter (each in a
 # run of its own, as rocprofv3 does not split counters over passes), and the
 # same counter runs over the plain copy kernel of known size used to calibrate
-# FETCH_SIZE / WRITE_SIZE.  Every step has its own time limit and the script
+# FETCH_SIZE / WRITE_SIZE.  EXTRA_COUNTERS adds passes: a space-separated list
+# whose entries join the counters of one pass with '+' (within the per-block
+# limits of one pass).  Every step has its own time limit and the script
 # stops at the first failure.  Summaries: python splishsplash_for_cuda_amd/rocprof_summary.py OUTDIR
 set -u
 OUT=${1:?output directory}
@@ -32,7 +34,7 @@ done
 
 for m in FETCH_SIZE WRITE_SIZE ${EXTRA_COUNTERS:-}; do
     echo "== bench $m"
-    timeout -s KILL 240 rocprofv3 --pmc $m --output-format csv -d "$OUT/bench_$m" -o bench -- \
+    timeout -s KILL 240 rocprofv3 --pmc ${m//+/ } --output-format csv -d "$OUT/bench_$m" -o bench -- \
         python3 bench.py "${BENCH_ARGS[@]}" > "$OUT/bench_$m.log" 2>&1 || { echo "bench $m failed"; tail -5 "$OUT/bench_$m.log"; exit 1; }
 done
 echo "done: $OUT"

@@ -138,6 +138,31 @@ def main():
                            "kernel": short(name)}
             lines.append("| %s | %d | %.1f | %.1f | %.1f | %s |" % (sw, len(fv), fb / 1e6, wb / 1e6, (fb + wb) / 1e6, vbs))
         lines.append("")
+    # further counter passes (EXTRA_COUNTERS): per sweep averages of every counter
+    extra = sorted(e for e in os.listdir(d) if e.startswith("bench_") and e not in
+                   ("bench_FETCH_SIZE", "bench_WRITE_SIZE", "bench_VALUBusy")) if os.path.isdir(d) else []
+    for e in extra:
+        path = os.path.join(d, e, "bench_counter_collection.csv")
+        if not os.path.exists(path):
+            continue
+        per = defaultdict(lambda: defaultdict(list))
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                sw = sweep_of(row["Kernel_Name"])
+                if sw:
+                    per[sw][row["Counter_Name"]].append(float(row["Counter_Value"]))
+        names = sorted({c for v in per.values() for c in v})
+        if not names:
+            continue
+        lines.append("## Counters: %s (average per dispatch)" % e[len("bench_"):].replace("+", ", "))
+        lines.append("")
+        lines.append("| sweep | " + " | ".join(names) + " |")
+        lines.append("|---|" + "---|" * len(names))
+        for sw in SWEEPS:
+            if sw in per:
+                vals = [per[sw].get(c, []) for c in names]
+                lines.append("| %s | " % sw + " | ".join("%.4g" % (sum(v) / len(v)) if v else "-" for v in vals) + " |")
+        lines.append("")
     text = "\n".join(lines)
     print(text)
     if args.write_profiles:

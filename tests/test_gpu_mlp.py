@@ -65,3 +65,26 @@ def test_mlp_gradient_matches_oracle(oracle_mod):
         rel = np.linalg.norm(a - b) / np.linalg.norm(b)
         cos = float(np.dot(a, b) / (np.linalg.norm(a) * np.linalg.norm(b)))
         assert rel < 5e-2 and cos > 0.998, (k, rel, cos)
+
+
+def test_normalization_is_scale_free():
+    """Columns of the training set are standardised by their own mean and spread
+    (the dt-invariant totals are O(1e-9) in SI units); a constant column gets
+    a spread of max(|mean|, 1)."""
+    rng = np.random.default_rng(7)
+    x = rng.normal(size=(2000, 12)).astype(np.float32)
+    x[:, 4] *= 3e-9
+    x[:, 5] = 0.25                         # constant column
+    t = np.stack([rng.normal(-6e-6, 7e-6, 2000), rng.normal(-1e-9, 7e-9, 2000)], 1).astype(np.float32)
+    s = S.Solver()
+    s.mlp_set_params(np.zeros(MLP_NPARAM, np.float32))
+    s.mlp_dataset_upload(x, t)
+    s.mlp_fit_normalization()
+    _, nrm = s.mlp_get_params()
+    mu, sd = nrm[:12], nrm[12:24]
+    tmu, tsd = nrm[24:26], nrm[26:28]
+    np.testing.assert_allclose(mu, x.astype(np.float64).mean(0), rtol=1e-4, atol=1e-12)
+    np.testing.assert_allclose(sd[[0, 1, 2, 3, 4]], x[:, [0, 1, 2, 3, 4]].astype(np.float64).std(0), rtol=1e-3)
+    assert sd[5] == pytest.approx(1.0)
+    np.testing.assert_allclose(tmu, t.astype(np.float64).mean(0), rtol=1e-4)
+    np.testing.assert_allclose(tsd, t.astype(np.float64).std(0), rtol=1e-3)
