@@ -33,9 +33,10 @@ from splishsplash_for_cuda_amd._native import KERNEL_NAMES  # noqa: E402
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, AMD published
 
 # Compulsory per-particle state of each sweep in bytes (FP32 SoA; uniform
-# fluid mass not counted), the 4 B index per stored neighbor and the 16 B
-# boundary position + psi are added per launch:
-#   B = N_f * S_k + 4 * (sum n_fluid + sum n_boundary) + 16 * N_b
+# fluid mass not counted), the index per stored neighbor (2 B in the 16-bit
+# lists of LDS-staged blocks, 4 B in 32-bit lists) and the 16 B boundary
+# position + psi are added per launch:
+#   B = N_f * S_k + 2 * sum n_fluid,staged + 4 * (sum n_fluid,unstaged + sum n_boundary) + 16 * N_b
 SWEEP_STATE_BYTES = {
     "div_ws_init": 52,    # x12 v12 kappaV4 r; rho alpha rho_adv 12 w; v12 w
     "div_init": 36,       # x12 v12 alpha4 r; rho_adv4 k4 w  (kappaV := 0 not counted)
@@ -49,11 +50,13 @@ SWEEP_STATE_BYTES = {
 }
 
 
-def kernel_bytes(name, n_f, n_b, sum_nf, sum_nb):
+def kernel_bytes(name, n_f, n_b, sum_nf, sum_nb, sum_nf_unstaged=None):
     s = SWEEP_STATE_BYTES[name]
+    u = sum_nf if sum_nf_unstaged is None else sum_nf_unstaged
+    idx_f = 2 * (sum_nf - u) + 4 * u
     if name == "xsph":
-        return n_f * s + 4 * sum_nf
-    return n_f * s + 4 * (sum_nf + sum_nb) + 16 * n_b
+        return n_f * s + idx_f
+    return n_f * s + idx_f + 4 * sum_nb + 16 * n_b
 
 
 def make_params(sc, mode, max_neighbors):
@@ -96,6 +99,8 @@ def main():
     ap.add_argument("--mode", default="plain", choices=["plain", "learned"])
     ap.add_argument("--path", default="host", choices=["host", "abi"],
                     help="host: SPH::DFSPHCUDA::step() (C++); abi: dfsph_step")
+    ap.add_argument("--min-iters", default="3,2",
+                    help="minimum divergence,density Jacobi iterations (the reference GPU solver's are 3,2)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -107,6 +112,7 @@ def main():
     sc = {"c1": scene.config1, "c2": scene.config2, "c4": scene.config4}[args.config]()
     max_nbr = 90 if args.config == "c1" else 128
     p = make_params(sc, args.mode, max_nbr)
+    p.min_iter_v, p.min_iter = (int(v) for v in args.min_iters.split(","))
     if args.path == "host":
         hs = S.HostSolver(p, sc.boundary, sc.fluid)
         solver = hs.view
@@ -116,6 +122,9 @@ def main():
         solver.upload_boundary(sc.boundary)
         solver.upload_fluid(sc.fluid)
         step = solver.step
+    if args.mode == "learned":
+        from splishsplash_for_cuda_amd import learned
+        learned.attach(solver)
 
     for _ in range(args.warmup):
         step()
@@ -148,7 +157,8 @@ def main():
     n_f = (inf0.n_fluid + inf1.n_fluid) / 2.0
     sum_nf = (inf0.neighbors_fluid_total + inf1.neighbors_fluid_total) / 2.0
     sum_nb = (inf0.neighbors_boundary_total + inf1.neighbors_boundary_total) / 2.0
-    nbytes = kernel_bytes(dominant, n_f, inf1.n_boundary, sum_nf, sum_nb)
+    sum_nfu = (inf0.neighbors_fluid_unstaged + inf1.neighbors_fluid_unstaged) / 2.0
+    nbytes = kernel_bytes(dominant, n_f, inf1.n_boundary, sum_nf, sum_nb, sum_nfu)
     avg_s = ms / 1e3 / max(launches, 1)
     achieved = nbytes / avg_s / 1e9
     traffic = None
@@ -180,6 +190,9 @@ def main():
                                "divergence_p95": float(np.percentile(iters_v, 95)),
                                "density_p95": float(np.percentile(iters, 95))},
             "max_neighbors": max_nbr,
+            "min_iters": [int(p.min_iter_v), int(p.min_iter)],
+            "unstaged_blocks": int(inf1.unstaged_blocks),
+            "fluid_neighbors_unstaged_frac": float(sum_nfu / max(sum_nf, 1.0)),
         },
         "roofline": {
             "bound": "hbm",

@@ -87,9 +87,11 @@ typedef struct dfsph_info {
     uint32_t max_neighbors_seen;   /* largest count of the last neighbor search */
     uint32_t overflow;             /* particles whose list was truncated        */
     uint32_t escaped;              /* outside the box after the last position update */
-    uint32_t pad;
+    uint32_t unstaged_blocks;      /* 256-particle blocks whose neighbor set exceeded
+                                      the LDS stage (32-bit global lists)      */
     uint64_t neighbors_fluid_total;    /* stored fluid neighbors, all particles    */
     uint64_t neighbors_boundary_total; /* stored boundary neighbors, all particles */
+    uint64_t neighbors_fluid_unstaged; /* of the fluid ones, in 32-bit global lists */
 } dfsph_info;
 
 /* fields for dfsph_download / dfsph_upload_field */

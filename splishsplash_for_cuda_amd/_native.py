@@ -106,9 +106,10 @@ class DFSPHInfo(C.Structure):
         ("max_neighbors_seen", C.c_uint32),
         ("overflow", C.c_uint32),
         ("escaped", C.c_uint32),
-        ("pad", C.c_uint32),
+        ("unstaged_blocks", C.c_uint32),
         ("neighbors_fluid_total", C.c_uint64),
         ("neighbors_boundary_total", C.c_uint64),
+        ("neighbors_fluid_unstaged", C.c_uint64),
     ]
 
 

@@ -68,6 +68,8 @@ struct Handle {
     float *rho = nullptr, *alpha = nullptr, *radv = nullptr, *kk = nullptr;
     float4* acc = nullptr;
     uint32_t *cnt = nullptr, *nbr = nullptr;
+    uint16_t* nbr16 = nullptr;          // staged fluid lists
+    uint32_t *stage_n = nullptr, *stg = nullptr;   // per-block staged sets
     // sort / scan scratch
     uint32_t *key = nullptr, *slot = nullptr, *perm = nullptr, *hist = nullptr, *cstart = nullptr;
     uint32_t *keep = nullptr, *rank = nullptr;
@@ -175,7 +177,7 @@ Handle::~Handle()
         dfree(ktot[b]); dfree(kvtot[b]);
         if (ev[b]) (void)hipEventDestroy(ev[b]);
     }
-    dfree(rho); dfree(alpha); dfree(radv); dfree(kk); dfree(acc); dfree(cnt); dfree(nbr);
+    dfree(rho); dfree(alpha); dfree(radv); dfree(kk); dfree(acc); dfree(cnt); dfree(nbr); dfree(nbr16); dfree(stage_n); dfree(stg);
     dfree(key); dfree(slot); dfree(perm); dfree(hist); dfree(cstart); dfree(keep); dfree(rank);
     dfree(keep8); dfree(scan_tmp); dfree(bpos); dfree(bcstart);
     dfree(ctl); dfree(tm); dfree(partial); dfree(partial_f);
@@ -201,6 +203,11 @@ int dalloc(T*& p, size_t count)
 }
 
 Handle* H(void* h) { return static_cast<Handle*>(h); }
+
+dfsph::NbrLists lists(const Handle* h)
+{
+    return dfsph::NbrLists{h->nbr, h->nbr16, h->cnt, h->stage_n, h->stg};
+}
 
 int ensure_scan(Handle* h, uint32_t n)
 {
@@ -295,7 +302,7 @@ int neighbor_search(Handle* h, bool sort)
     pend(h, ps);
     const int pn = pbeg(h, DFSPH_KERNEL_NEIGHBORS);
     dfsph::launch_neighbors(h->s, h->c, !sort, h->pos[h->pc], h->cstart, h->perm, h->bpos, h->bcstart, h->n,
-                            h->nbr, h->cnt, h->ctl);
+                            h->nbr, h->nbr16, h->cnt, h->stage_n, h->stg, h->ctl);
     pend(h, pn);
     HIPCHK(hipGetLastError());
     h->lists_valid = true;
@@ -320,10 +327,10 @@ int jacobi_loop(Handle* h, bool div, uint32_t max_iter, double max_error, uint32
     auto jacobi = [&]() -> int {
         const int pr = pbeg(h, div ? DFSPH_KERNEL_DIV_JACOBI : DFSPH_KERNEL_DENS_JACOBI);
         if (div)
-            dfsph::launch_jacobi(h->s, true, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->kk,
+            dfsph::launch_jacobi(h->s, true, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h), h->kk,
                                  h->kapv[h->pc], h->vel[h->vc], sc);
         else
-            dfsph::launch_jacobi(h->s, false, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->kk,
+            dfsph::launch_jacobi(h->s, false, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h), h->kk,
                                  nullptr, h->vel[h->vc], sc);
         pend(h, pr);
         return pr;
@@ -331,10 +338,10 @@ int jacobi_loop(Handle* h, bool div, uint32_t max_iter, double max_error, uint32
     auto loop_end = [&]() {
         const int pr = pbeg(h, div ? DFSPH_KERNEL_DIV_LOOP_END : DFSPH_KERNEL_DENS_LOOP_END);
         if (div)
-            dfsph::launch_div_source(h->s, false, h->c, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->vel[h->vc],
+            dfsph::launch_div_source(h->s, false, h->c, h->n, h->pos[h->pc], h->bpos, lists(h), h->vel[h->vc],
                                      h->alpha, h->radv, h->kk, h->kapv[h->pc], sc, h->partial);
         else
-            dfsph::launch_density_source(h->s, false, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
+            dfsph::launch_density_source(h->s, false, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h),
                                          h->vel[h->vc], h->rho, h->alpha, h->radv, h->kk, h->kap[h->pc], sc,
                                          h->partial);
         pend(h, pr);
@@ -385,7 +392,7 @@ int learned_predict(Handle* h)
     if (learned && !(h->mlp && dfsph::mlp_loaded(h->mlp)))
         return fail(DFSPH_ERR_NO_MODEL, "learned warm start selected but no model loaded");
     CHK(dfsph::mlp_ensure_particles(h->mlp, h->cap));
-    dfsph::launch_features(h->s, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->vel[h->vc],
+    dfsph::launch_features(h->s, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h), h->vel[h->vc],
                            h->ktot[h->pc], h->kvtot[h->pc], (float)(2.0 * h->p.particle_radius),
                            dfsph::mlp_feat(h->mlp));
     HIPCHK(hipGetLastError());
@@ -404,12 +411,12 @@ int divergence_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* i
     if (ws == DFSPH_WARMSTART_LEARNED) kws = dfsph::mlp_pred_v(h->mlp);
     const int vo = h->vc ^ 1;
     int pr = pbeg(h, DFSPH_KERNEL_DIV_WS_INIT);
-    dfsph::launch_div_ws_init(h->s, ws, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->vel[h->vc],
+    dfsph::launch_div_ws_init(h->s, ws, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h), h->vel[h->vc],
                               h->vel[vo], kws, h->rho, h->alpha, h->radv, h->kvtot[h->pc]);
     pend(h, pr);
     h->vc = vo;
     pr = pbeg(h, DFSPH_KERNEL_DIV_INIT);
-    dfsph::launch_div_source(h->s, true, h->c, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt, h->vel[h->vc],
+    dfsph::launch_div_source(h->s, true, h->c, h->n, h->pos[h->pc], h->bpos, lists(h), h->vel[h->vc],
                              h->alpha, h->radv, h->kk, h->kapv[h->pc], nullptr, nullptr);
     pend(h, pr);
     HIPCHK(hipGetLastError());
@@ -426,7 +433,7 @@ int density_only(Handle* h)
     // rho and alpha without the divergence solver (the reference leaves them stale)
     CHK(learned_predict(h));
     const int vo = h->vc ^ 1;
-    dfsph::launch_div_ws_init(h->s, DFSPH_WARMSTART_NONE, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
+    dfsph::launch_div_ws_init(h->s, DFSPH_WARMSTART_NONE, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h),
                               h->vel[h->vc], h->vel[vo], nullptr, h->rho, h->alpha, h->radv, h->kvtot[h->pc]);
     h->vc = vo;
     HIPCHK(hipGetLastError());
@@ -439,7 +446,7 @@ int external_forces(Handle* h)
     if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale: call dfsph_neighbor_search");
     dfsph::launch_step_ctl_init(h->s, h->ctl, h->n);
     const int pr = pbeg(h, DFSPH_KERNEL_XSPH);
-    dfsph::launch_xsph(h->s, h->c, h->tm, h->n, h->pos[h->pc], h->nbr, h->cnt, h->vel[h->vc], h->rho, h->acc,
+    dfsph::launch_xsph(h->s, h->c, h->tm, h->n, h->pos[h->pc], lists(h), h->vel[h->vc], h->rho, h->acc,
                        h->ctl, h->partial_f);
     pend(h, pr);
     HIPCHK(hipGetLastError());
@@ -462,13 +469,13 @@ int pressure_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* ite
     } else {
         const bool fused = h->pend_vel;
         const int pw = pbeg(h, DFSPH_KERNEL_PRESSURE_WS);
-        dfsph::launch_pressure_ws(h->s, ws, fused, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
+        dfsph::launch_pressure_ws(h->s, ws, fused, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h),
                                   h->acc, h->vel[h->vc], kws, fused ? 1 : 0, h->ktot[h->pc]);
         pend(h, pw);
         h->pend_vel = false;
     }
     const int pr = pbeg(h, DFSPH_KERNEL_DENS_INIT);
-    dfsph::launch_density_source(h->s, true, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, h->nbr, h->cnt,
+    dfsph::launch_density_source(h->s, true, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h),
                                  h->vel[h->vc], h->rho, h->alpha, h->radv, h->kk, h->kap[h->pc], nullptr, nullptr);
     pend(h, pr);
     HIPCHK(hipGetLastError());
@@ -487,9 +494,9 @@ int compact(Handle* h, uint32_t* removed)
     const uint32_t n = h->n;
     dfsph::launch_exclusive_scan(h->s, h->keep, h->rank, n + 1, h->scan_tmp);
     uint32_t kept = 0;
-    HIPCHK(hipMemcpyAsync(&h->hctl->pad[0], h->rank + n, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipMemcpyAsync(&h->hctl->pad, h->rank + n, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s));
     CHK(sync(h));
-    kept = h->hctl->pad[0];
+    kept = h->hctl->pad;
     if (removed) *removed = n - kept;
     if (kept == n) return DFSPH_OK;
     const int o = h->pc ^ 1, vo = h->vc ^ 1;
@@ -510,9 +517,9 @@ int check_positions(Handle* h, int mode, uint32_t* count)
     dfsph::launch_keep_inside(h->s, h->n, h->pos[h->pc], h->bmin, h->bmax, h->keep);
     if (mode == 2) return compact(h, count);
     dfsph::launch_exclusive_scan(h->s, h->keep, h->rank, h->n + 1, h->scan_tmp);
-    HIPCHK(hipMemcpyAsync(&h->hctl->pad[0], h->rank + h->n, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s));
+    HIPCHK(hipMemcpyAsync(&h->hctl->pad, h->rank + h->n, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s));
     CHK(sync(h));
-    if (count) *count = h->n - h->hctl->pad[0];
+    if (count) *count = h->n - h->hctl->pad;
     return DFSPH_OK;
 }
 
@@ -538,6 +545,12 @@ int alloc_fluid(Handle* h, uint32_t cap)
     const size_t nslots = (size_t)cap * (h->p.max_neighbors + dfsph::kNbrSlack);
     CHK(dalloc(h->nbr, nslots));
     HIPCHK(hipMemsetAsync(h->nbr, 0, sizeof(uint32_t) * nslots, h->s));
+    CHK(dalloc(h->nbr16, nslots));
+    HIPCHK(hipMemsetAsync(h->nbr16, 0, sizeof(uint16_t) * nslots, h->s));
+    // staged sets: one per block of the XCD-padded grid
+    const uint32_t nblocks = dfsph::grid_blocks_xcd(cap);
+    CHK(dalloc(h->stage_n, nblocks));
+    CHK(dalloc(h->stg, (size_t)nblocks * dfsph::kStageMax));
     CHK(ensure_tmp(h, std::max(cap, h->nb)));
     CHK(ensure_partials(h, cap));
     h->cap = cap;
@@ -693,12 +706,15 @@ int dfsph_upload_boundary(void* hv, const float* xyz, const float* psi, uint32_t
             h->bmax[a] = std::max(h->bmax[a], xyz[3 * i + a]);
         }
     const double cell = (double)h->c.hk * kCellMargin;
-    uint64_t ncell = 1;
+    // cell keys: 8x8x8-cell tiles, 512 keys per tile (the key table covers
+    // the grid rounded up to whole tiles)
+    uint64_t ncell = 512;
     for (int a = 0; a < 3; ++a) {
         const double org = (double)h->bmin[a] - 2.0 * cell;
         h->c.org[a] = (float)org;
         h->c.dim[a] = (int)std::floor(((double)h->bmax[a] - org) / cell) + 3;
-        ncell *= (uint64_t)h->c.dim[a];
+        h->c.tiles[a] = (h->c.dim[a] + 7) / 8;
+        ncell *= (uint64_t)h->c.tiles[a];
     }
     if (ncell >= (1ull << 31)) return fail(DFSPH_ERR_ARG, "domain too large for the cell grid");
     h->c.inv_cell = (float)(1.0 / cell);
@@ -1000,7 +1016,17 @@ int dfsph_get_neighbors(void* hv, uint32_t* n_fluid, uint32_t* n_boundary, uint3
     if (n == 0) return DFSPH_OK;
     std::vector<uint32_t> c(n), L((size_t)mx * cap);
     HIPCHK(hipMemcpy(c.data(), h->cnt, 4ull * n, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(L.data(), h->nbr, 4ull * mx * cap, hipMemcpyDeviceToHost));
+    if (list) {
+        // staged (16-bit) and unstaged lists as 32-bit global ids
+        uint32_t* g = nullptr;
+        CHK(dalloc(g, (size_t)mx * cap));
+        dfsph::launch_expand_lists(h->s, n, cap, lists(h), g);
+        int rc = sync(h);
+        if (rc == DFSPH_OK && hipMemcpy(L.data(), g, 4ull * mx * cap, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(DFSPH_ERR_HIP, "hipMemcpy of the neighbor lists failed");
+        dfree(g);
+        if (rc != DFSPH_OK) return rc;
+    }
     for (uint32_t i = 0; i < n; ++i) {
         const uint32_t f = c[i] & 0xffffu, b = c[i] >> 16;
         if (n_fluid) n_fluid[i] = f;
@@ -1038,6 +1064,8 @@ int dfsph_get_info(void* hv, dfsph_info* info)
     info->escaped = h->hctl->escaped;
     info->neighbors_fluid_total = h->hctl->nbr_fluid;
     info->neighbors_boundary_total = h->hctl->nbr_boundary;
+    info->unstaged_blocks = h->hctl->unstaged_blocks;
+    info->neighbors_fluid_unstaged = h->hctl->nbr_fluid_unstaged;
     return DFSPH_OK;
 }
 

@@ -4,9 +4,17 @@
 //   pos   float4 (x, y, z, mass)        fluid; boundary: (x, y, z, psi)
 //   vel   float4 (vx, vy, vz, 0)
 //   rho, alpha, radv, k, kappa, kappaV  float
-//   nbr   uint32 [max_neighbors][capacity]  (interleaved: slot k of particle i
-//         at k*capacity + i, so a wavefront reads 64 consecutive words)
+//   nbr16 uint16 [max_neighbors + slack][capacity]  fluid lists of staged
+//         blocks: indices into the block's staged set (interleaved: slot k of
+//         particle i at k*capacity + i, so a wavefront reads 64 consecutive words)
+//   nbr   uint32 [max_neighbors + slack][capacity]  boundary lists (from slot 0
+//         in staged blocks) and the fluid-then-boundary lists of unstaged blocks
 //   cnt   uint32  fluid count | boundary count << 16
+//   stg   uint32 [blocks][kStageMax]  global ids of each block's staged set
+//
+// Cells are ordered in 8x8x8-cell tiles (tiles x-fastest, then z, then y),
+// Morton order inside a tile, so 256 consecutive particles occupy a compact
+// region and the neighbor set of a block is about 4.5 times the block.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,8 +38,9 @@ struct SimConst {
     float viscosity;
     float org[3];                  // grid origin
     float inv_cell;                // 1 / cell edge (cell edge = 1.001 h_k)
-    int dim[3];                    // cells per axis; cell id = x + dim0 (z + dim2 y)
-    uint32_t ncell;
+    int dim[3];                    // cells per axis
+    int tiles[3];                  // 8-cell tiles per axis (dim rounded up / 8)
+    uint32_t ncell;                // cell keys: tiles0 tiles1 tiles2 512
     uint32_t cap;                  // neighbor list stride (fluid capacity)
     uint32_t max_nbr;
 };
@@ -67,9 +76,30 @@ struct DevCtl {
     uint32_t max_seen;             // largest neighbor count of the last search
     uint32_t overflow;             // particles whose list was truncated
     uint32_t n_active;             // active fluid particles (device copy)
-    uint32_t pad[2];
+    uint32_t unstaged_blocks;      // blocks of the last search that use global lists
+    uint32_t pad;
     unsigned long long nbr_fluid;  // stored fluid neighbors of the last search
     unsigned long long nbr_boundary;
+    unsigned long long nbr_fluid_unstaged;  // of which in 32-bit (unstaged) lists
+};
+
+// ---- LDS-staged neighbor sets ----
+// A sweep block (256 consecutive particles) loads the positions / velocities
+// of every fluid particle in the cells adjacent to its own cells into LDS once
+// (coalesced), and its fluid lists hold 16-bit indices into that set.  Blocks
+// whose set exceeds kStageMax (sparse splashes, the unsorted search) keep
+// 32-bit global lists.
+constexpr uint32_t kStageMax = 1600;       // staged particles per block
+constexpr uint32_t kUnstaged = 0xffffffffu;
+constexpr uint32_t kAdjMax = 2048;         // 27 x unique own cells per block
+constexpr uint32_t kKeyInvalid = 0xffffffffu;
+
+struct NbrLists {
+    const uint32_t* nbr;           // 32-bit lists (see the layout above)
+    const uint16_t* nbr16;         // 16-bit staged fluid lists
+    const uint32_t* cnt;
+    const uint32_t* stage_n;       // per block: staged count or kUnstaged
+    const uint32_t* stg;           // per block: global ids of the staged set
 };
 
 // The persistent per-particle fields that move with a particle when the
@@ -165,10 +195,43 @@ __device__ __forceinline__ int cell_coord(const SimConst& c, float v, int a)
     return k;
 }
 
+// 3-bit value -> bits 0, 3, 6
+__device__ __forceinline__ uint32_t spread3(uint32_t v)
+{
+    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4);
+}
+
+// Cell key: tile index (x fastest, then z, then y) * 512 + Morton index in the tile.
+__device__ __forceinline__ uint32_t cell_key(const SimConst& c, int x, int y, int z)
+{
+    const uint32_t tile = (uint32_t)(x >> 3) +
+                          (uint32_t)c.tiles[0] * ((uint32_t)(z >> 3) + (uint32_t)c.tiles[2] * (uint32_t)(y >> 3));
+    const uint32_t local = spread3((uint32_t)x & 7u) | (spread3((uint32_t)y & 7u) << 1) | (spread3((uint32_t)z & 7u) << 2);
+    return (tile << 9) | local;
+}
+
 __device__ __forceinline__ uint32_t cell_id(const SimConst& c, float x, float y, float z)
 {
-    const int cx = cell_coord(c, x, 0), cy = cell_coord(c, y, 1), cz = cell_coord(c, z, 2);
-    return (uint32_t)cx + (uint32_t)c.dim[0] * ((uint32_t)cz + (uint32_t)c.dim[2] * (uint32_t)cy);
+    return cell_key(c, cell_coord(c, x, 0), cell_coord(c, y, 1), cell_coord(c, z, 2));
+}
+
+// The 27 cells around (cx, cy, cz) inside the grid, in a fixed order.
+template <class F>
+__device__ __forceinline__ void for_each_adjacent_cell(const SimConst& c, int cx, int cy, int cz, F&& f)
+{
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int y = cy + dy;
+        if (y < 0 || y >= c.dim[1]) continue;
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int z = cz + dz;
+            if (z < 0 || z >= c.dim[2]) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = cx + dx;
+                if (x < 0 || x >= c.dim[0]) continue;
+                f(cell_key(c, x, y, z));
+            }
+        }
+    }
 }
 
 // Neighbor loop with memory-level parallelism: U list entries are loaded at
@@ -205,6 +268,39 @@ __device__ __forceinline__ void for_each_nbr(const uint32_t* __restrict__ p, uin
     }
 }
 
+// The same over a 16-bit staged list; invalid entries read staged slot 0.
+template <int U = DFSPH_NBR_UNROLL, class F>
+__device__ __forceinline__ void for_each_nbr16(const uint16_t* __restrict__ p, uint32_t stride, uint32_t count,
+                                               F&& f)
+{
+    for (uint32_t k = 0; k < count; k += U) {
+        uint32_t j[U];
+        bool v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t e = p[(size_t)(k + u) * stride];
+            v[u] = k + u < count;
+            j[u] = v[u] ? e : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) f(j[u], v[u]);
+    }
+}
+
+// Loads a block's staged set: st(k, global id) for k < count, then a block
+// barrier (reached by every thread).  Returns the staged count or kUnstaged.
+template <class St>
+__device__ __forceinline__ uint32_t stage_block(const NbrLists& L, uint32_t b, St&& st)
+{
+    const uint32_t sn = L.stage_n[b];
+    if (sn != kUnstaged) {
+        const uint32_t* g = L.stg + (size_t)b * kStageMax;
+        for (uint32_t k = threadIdx.x; k < sn; k += blockDim.x) st(k, g[k]);
+    }
+    __syncthreads();
+    return sn;
+}
+
 // x where c holds, else 0 (a select of an already computed value)
 __device__ __forceinline__ float keep(bool c, float x)
 {
@@ -238,6 +334,55 @@ __device__ __forceinline__ uint32_t wave_sum_u(uint32_t v)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan_u(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t w = __shfl_up(v, o, 64);
+        if (lane >= o) v += w;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan (kBlock threads); returns the block total to
+// every thread.  s_w: kBlock / 64 words of LDS.  Contains two barriers.
+__device__ __forceinline__ uint32_t block_excl_scan_u(uint32_t v, uint32_t* excl, uint32_t* s_w)
+{
+    const uint32_t incl = wave_incl_scan_u(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) s_w[w] = incl;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kBlock / 64; ++k) {
+        const uint32_t s = s_w[k];
+        off += k < w ? s : 0u;
+        tot += s;
+    }
+    *excl = off + incl - v;
+    __syncthreads();
+    return tot;
+}
+
+// Ascending bitonic sort of a[0, P) in LDS, P a power of two (kBlock threads).
+__device__ __forceinline__ void bitonic_sort_lds(uint32_t* a, uint32_t P)
+{
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = threadIdx.x; t < P; t += kBlock) {
+                const uint32_t x = t ^ j;
+                if (x > t) {
+                    const uint32_t va = a[t], vb = a[x];
+                    const bool up = (t & k) == 0;
+                    if ((va > vb) == up) { a[t] = vb; a[x] = va; }
+                }
+            }
+            __syncthreads();
+        }
+    }
 }
 
 // Block sum in a fixed order (wave tree, then waves 0..3): deterministic.

@@ -60,6 +60,8 @@ __global__ void k_nbr_ctl_init(DevCtl* c)
         c->overflow = 0;
         c->nbr_fluid = 0;
         c->nbr_boundary = 0;
+        c->nbr_fluid_unstaged = 0;
+        c->unstaged_blocks = 0;
     }
 }
 
@@ -135,17 +137,6 @@ __global__ void k_time_step_end(DevTime* t)
 // ------------------------------------------------------------------------
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
-
-__device__ __forceinline__ uint32_t wave_incl_scan_u(uint32_t v)
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t w = __shfl_up(v, o, 64);
-        if (lane >= o) v += w;
-    }
-    return v;
-}
 
 __global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* in, uint32_t* out, uint32_t n,
                                                         uint32_t* tile_sums)
@@ -270,37 +261,166 @@ __global__ void k_gather_f4(uint32_t n, const uint32_t* perm, const float4* in, 
     out[k] = in[perm[k]];
 }
 
-// DFSPH_neighborsSearch_kernel<true> with ITER_NEIGHBORS_FROM_STRUCTURE_LINEAR_ADVANCED
-// (DFSPH_core_cuda.cu:2219-2531, DFSPH_macro_cuda.h:136-155): for each of the 3x3
-// (y, z) rows, the contiguous 3-cell x strip of the fluid table (self excluded),
-// then the boundary table; interleaved storage (DFSPH_macro_cuda.h:210-214).
+// DFSPH_neighborsSearch_kernel<true> (DFSPH_core_cuda.cu:2219-2531,
+// DFSPH_macro_cuda.h:136-155): every fluid particle within h (strict, FP32) of
+// particle i except i itself, over the 27 cells around its cell, then the
+// boundary particles the same way; interleaved storage
+// (DFSPH_macro_cuda.h:210-214), at most max_neighbors in total (fluid first).
+//
+// Each block also builds its staged set: the sorted unique keys of the cells
+// adjacent to its particles' cells (a bitonic sort in LDS), their particle
+// ranges in the cell table and the prefix offsets.  If the set fits
+// kStageMax, the block's positions are staged in LDS, the candidate tests read
+// them from there and the fluid lists are 16-bit indices into the set;
+// otherwise the block keeps 32-bit global lists (stage_n[b] = kUnstaged).
 // INDIRECT: the fluid is not reordered and the table indexes it through perm
-// (the need_sort == false path).
+// (the need_sort == false path); such blocks are never staged.
 template <bool INDIRECT>
-__global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* pos, const uint32_t* cs_f,
-                                                      const uint32_t* perm, const float4* bpos,
-                                                      const uint32_t* cs_b, uint32_t n, uint32_t* nbr,
-                                                      uint32_t* cnt, DevCtl* ctl)
+__global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* __restrict__ pos,
+                                                      const uint32_t* __restrict__ cs_f,
+                                                      const uint32_t* __restrict__ perm,
+                                                      const float4* __restrict__ bpos,
+                                                      const uint32_t* __restrict__ cs_b, uint32_t n,
+                                                      uint32_t* __restrict__ nbr, uint16_t* __restrict__ nbr16,
+                                                      uint32_t* __restrict__ cnt, uint32_t* __restrict__ stage_n,
+                                                      uint32_t* __restrict__ stg, DevCtl* ctl)
 {
-    const uint32_t i = xcd_index();
-    uint32_t total = 0, trunc = 0, stored_f = 0, stored_b = 0;
-    if (i < n) {
-        const float4 p = pos[i];
-        const int cx = cell_coord(c, p.x, 0), cy = cell_coord(c, p.y, 1), cz = cell_coord(c, p.z, 2);
-        const int x0 = cx > 0 ? cx - 1 : 0;
-        const int x1 = cx + 1 < c.dim[0] ? cx + 1 : c.dim[0] - 1;
+    __shared__ uint32_t s_key[kAdjMax];        // adjacent cell keys, then the unique ones
+    __shared__ uint32_t s_off[kAdjMax + 1];    // staged offset of each unique cell
+    __shared__ uint32_t s_src[kAdjMax];        // first table slot of each unique cell
+    __shared__ float4 s_pos[kStageMax];
+    __shared__ uint32_t s_w[kBlock / 64];
+    const uint32_t b = xcd_block();
+    const uint32_t i = b * kBlock + threadIdx.x;
+    const bool act = i < n;
+    float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    int cx = 0, cy = 0, cz = 0;
+    uint32_t key = kKeyInvalid;
+    if (act) {
+        p = pos[i];
+        cx = cell_coord(c, p.x, 0);
+        cy = cell_coord(c, p.y, 1);
+        cz = cell_coord(c, p.z, 2);
+        key = cell_key(c, cx, cy, cz);
+    }
+    bool staged = false;
+    uint32_t U = 0, S = 0;
+    if (!INDIRECT) {
+        // unique own cells: keys are non-decreasing along the block
+        s_key[threadIdx.x] = key;
+        __syncthreads();
+        const bool first = act && (threadIdx.x == 0 || s_key[threadIdx.x - 1] != key);
+        uint32_t uown = 0;
+        const uint32_t nown = block_excl_scan_u(first ? 1u : 0u, &uown, s_w);
+        if (nown * 27u <= kAdjMax) {
+            const uint32_t m = nown * 27u;
+            if (first) {
+                uint32_t* o = s_key + uown * 27u;
+                uint32_t q = 0;
+                for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) { o[q++] = nk; });
+                for (; q < 27u; ++q) o[q] = kKeyInvalid;
+            }
+            uint32_t P = 64;
+            while (P < m) P <<= 1;
+            for (uint32_t k = m + threadIdx.x; k < P; k += kBlock) s_key[k] = kKeyInvalid;
+            __syncthreads();
+            bitonic_sort_lds(s_key, P);
+            // unique keys, compacted in place (reads, barrier, writes)
+            const uint32_t E = P > kBlock ? P / kBlock : 1u;
+            const uint32_t k0 = threadIdx.x * E;
+            uint32_t loc[kAdjMax / kBlock];
+            uint32_t nloc = 0;
+#pragma unroll
+            for (uint32_t e = 0; e < kAdjMax / kBlock; ++e) {
+                loc[e] = kKeyInvalid;
+                const uint32_t k = k0 + e;
+                if (e < E && k < P) {
+                    const uint32_t v = s_key[k];
+                    const uint32_t prev = k > 0 ? s_key[k - 1] : kKeyInvalid;
+                    if (v != kKeyInvalid && v != prev) { loc[e] = v; ++nloc; }
+                }
+            }
+            uint32_t base = 0;
+            U = block_excl_scan_u(nloc, &base, s_w);
+#pragma unroll
+            for (uint32_t e = 0; e < kAdjMax / kBlock; ++e)
+                if (loc[e] != kKeyInvalid) s_key[base++] = loc[e];
+            __syncthreads();
+            // particle counts of the unique cells -> staged offsets
+            const uint32_t E2 = (U + kBlock - 1) / kBlock;
+            const uint32_t u0 = threadIdx.x * E2;
+            uint32_t sum = 0;
+            for (uint32_t e = 0; e < E2; ++e) {
+                const uint32_t u = u0 + e;
+                if (u < U) {
+                    const uint32_t a = cs_f[s_key[u]], z = cs_f[s_key[u] + 1];
+                    s_src[u] = a;
+                    sum += z - a;
+                }
+            }
+            uint32_t off = 0;
+            S = block_excl_scan_u(sum, &off, s_w);
+            for (uint32_t e = 0; e < E2; ++e) {
+                const uint32_t u = u0 + e;
+                if (u < U) {
+                    s_off[u] = off;
+                    off += cs_f[s_key[u] + 1] - s_src[u];
+                }
+            }
+            if (threadIdx.x == 0) s_off[U] = S;
+            staged = S <= kStageMax;
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) stage_n[b] = staged ? S : kUnstaged;
+    if (staged) {
+        uint32_t* g_out = stg + (size_t)b * kStageMax;
+        for (uint32_t k = threadIdx.x; k < S; k += kBlock) {
+            uint32_t lo = 0, hi = U;            // last u with s_off[u] <= k
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_off[mid] <= k) lo = mid; else hi = mid;
+            }
+            const uint32_t g = s_src[lo] + (k - s_off[lo]);
+            s_pos[k] = pos[g];
+            g_out[k] = g;
+        }
+        __syncthreads();
+    }
+    uint32_t total = 0, trunc = 0, stored_f = 0, stored_b = 0, stored_u32 = 0;
+    if (act) {
         const uint32_t maxn = c.max_nbr, stride = c.cap;
-        uint32_t* out = nbr + i;
         uint32_t nf = 0, nb = 0;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int y = cy + dy;
-            if (y < 0 || y >= c.dim[1]) continue;
-            for (int dz = -1; dz <= 1; ++dz) {
-                const int z = cz + dz;
-                if (z < 0 || z >= c.dim[2]) continue;
-                const uint32_t row = (uint32_t)c.dim[0] * ((uint32_t)z + (uint32_t)c.dim[2] * (uint32_t)y);
-                const uint32_t s = cs_f[row + x0], e = cs_f[row + x1 + 1];
-                for (uint32_t q = s; q < e; ++q) {
+        if (staged) {
+            auto find = [&](uint32_t nk) -> uint32_t {
+                uint32_t lo = 0, hi = U;         // lower bound of nk
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_key[mid] < nk) lo = mid + 1; else hi = mid;
+                }
+                return lo;
+            };
+            const uint32_t io = find(key);     // the own cell is always in the set
+            const uint32_t self = io < U ? s_off[io] + (i - s_src[io]) : kKeyInvalid;
+            uint16_t* out = nbr16 + i;
+            for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) {
+                const uint32_t u = find(nk);
+                if (u >= U || s_key[u] != nk) return;
+                const uint32_t e = s_off[u + 1];
+                for (uint32_t l = s_off[u]; l < e; ++l) {
+                    if (l == self) continue;
+                    const float4 q = s_pos[l];
+                    if (dist2(p.x - q.x, p.y - q.y, p.z - q.z) < c.hk2) {
+                        if (nf < maxn) out[(size_t)nf * stride] = (uint16_t)l;
+                        ++nf;
+                    }
+                }
+            });
+        } else {
+            uint32_t* out = nbr + i;
+            for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) {
+                const uint32_t s0 = cs_f[nk], e = cs_f[nk + 1];
+                for (uint32_t q = s0; q < e; ++q) {
                     const uint32_t j = INDIRECT ? perm[q] : q;
                     if (j == i) continue;
                     const float4 pj = pos[j];
@@ -309,26 +429,19 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
                         ++nf;
                     }
                 }
-            }
+            });
         }
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int y = cy + dy;
-            if (y < 0 || y >= c.dim[1]) continue;
-            for (int dz = -1; dz <= 1; ++dz) {
-                const int z = cz + dz;
-                if (z < 0 || z >= c.dim[2]) continue;
-                const uint32_t row = (uint32_t)c.dim[0] * ((uint32_t)z + (uint32_t)c.dim[2] * (uint32_t)y);
-                const uint32_t s = cs_b[row + x0], e = cs_b[row + x1 + 1];
-                for (uint32_t j = s; j < e; ++j) {
-                    const float4 pj = bpos[j];
-                    if (dist2(p.x - pj.x, p.y - pj.y, p.z - pj.z) < c.hk2) {
-                        const uint32_t at = nf + nb;
-                        if (at < maxn) out[(size_t)at * stride] = j;
-                        ++nb;
-                    }
+        uint32_t* bout = nbr + i + (size_t)(staged ? 0u : nf) * stride;
+        for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) {
+            const uint32_t s0 = cs_b[nk], e = cs_b[nk + 1];
+            for (uint32_t j = s0; j < e; ++j) {
+                const float4 pj = bpos[j];
+                if (dist2(p.x - pj.x, p.y - pj.y, p.z - pj.z) < c.hk2) {
+                    if (nf + nb < maxn) bout[(size_t)nb * stride] = j;
+                    ++nb;
                 }
             }
-        }
+        });
         total = nf + nb;
         const uint32_t sf = nf < maxn ? nf : maxn;
         const uint32_t sb = nb < maxn - sf ? nb : maxn - sf;
@@ -336,16 +449,39 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
         cnt[i] = sf | (sb << 16);
         stored_f = sf;
         stored_b = sb;
+        stored_u32 = staged ? 0u : sf;
     }
     const uint32_t wmax = wave_max_u(total);
     const uint32_t wtr = wave_sum_u(trunc);
     const uint32_t wsf = wave_sum_u(stored_f);
     const uint32_t wsb = wave_sum_u(stored_b);
+    const uint32_t wsu = wave_sum_u(stored_u32);
     if ((threadIdx.x & 63) == 0) {
         if (wmax) atomicMax(&ctl->max_seen, wmax);
         if (wtr) atomicAdd(&ctl->overflow, wtr);
         if (wsf) atomicAdd(&ctl->nbr_fluid, (unsigned long long)wsf);
         if (wsb) atomicAdd(&ctl->nbr_boundary, (unsigned long long)wsb);
+        if (wsu) atomicAdd(&ctl->nbr_fluid_unstaged, (unsigned long long)wsu);
+    }
+    if (threadIdx.x == 0 && !staged && n > b * kBlock) atomicAdd(&ctl->unstaged_blocks, 1u);
+}
+
+// Every list of the last search as 32-bit global ids, fluid then boundary
+// (the format of the reference's interleaved storage), for readback.
+__global__ __launch_bounds__(kBlock) void k_expand_lists(uint32_t n, uint32_t cap, NbrLists L, uint32_t* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b = i / kBlock;
+    const uint32_t cn = L.cnt[i];
+    const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
+    const uint32_t sn = L.stage_n[b];
+    if (sn != kUnstaged) {
+        const uint32_t* g = L.stg + (size_t)b * kStageMax;
+        for (uint32_t k = 0; k < nf; ++k) out[(size_t)k * cap + i] = g[L.nbr16[(size_t)k * cap + i]];
+        for (uint32_t k = 0; k < nb; ++k) out[(size_t)(nf + k) * cap + i] = L.nbr[(size_t)k * cap + i];
+    } else {
+        for (uint32_t k = 0; k < nf + nb; ++k) out[(size_t)k * cap + i] = L.nbr[(size_t)k * cap + i];
     }
 }
 
@@ -357,24 +493,15 @@ __global__ void k_boundary_psi(SimConst c, const float4* bpos, const uint32_t* c
     if (i >= nb) return;
     const float4 p = bpos[i];
     const int cx = cell_coord(c, p.x, 0), cy = cell_coord(c, p.y, 1), cz = cell_coord(c, p.z, 2);
-    const int x0 = cx > 0 ? cx - 1 : 0;
-    const int x1 = cx + 1 < c.dim[0] ? cx + 1 : c.dim[0] - 1;
     float delta = c.W0;
-    for (int dy = -1; dy <= 1; ++dy) {
-        const int y = cy + dy;
-        if (y < 0 || y >= c.dim[1]) continue;
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int z = cz + dz;
-            if (z < 0 || z >= c.dim[2]) continue;
-            const uint32_t row = (uint32_t)c.dim[0] * ((uint32_t)z + (uint32_t)c.dim[2] * (uint32_t)y);
-            for (uint32_t j = cs_b[row + x0]; j < cs_b[row + x1 + 1]; ++j) {
-                if (j == i) continue;
-                const float4 q = bpos[j];
-                const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
-                if (r2 < c.hk2) delta += kernel_W(c, r2);
-            }
+    for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) {
+        for (uint32_t j = cs_b[nk]; j < cs_b[nk + 1]; ++j) {
+            if (j == i) continue;
+            const float4 q = bpos[j];
+            const float r2 = dist2(p.x - q.x, p.y - q.y, p.z - q.z);
+            if (r2 < c.hk2) delta += kernel_W(c, r2);
         }
-    }
+    });
     psi[i] = c.density0 / delta;
 }
 
@@ -432,34 +559,39 @@ __global__ __launch_bounds__(kBlock) void k_solve_finalize(SolveCtl* ctl, const 
 template <int WS>
 __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTime* t, uint32_t n,
                                                         const float4* __restrict__ pos,
-                                                        const float4* __restrict__ bpos,
-                                                        const uint32_t* __restrict__ nbr,
-                                                        const uint32_t* __restrict__ cnt,
+                                                        const float4* __restrict__ bpos, NbrLists L,
                                                         const float4* __restrict__ vel_i,
                                                         float4* __restrict__ vel_o,
                                                         const float* __restrict__ kws,
                                                         float* __restrict__ rho, float* __restrict__ alpha,
                                                         float* __restrict__ radv, float* __restrict__ kvtot)
 {
-    const uint32_t i = xcd_index();
-    if (i >= n) return;
+    __shared__ float4 sP[kStageMax];
+    __shared__ float4 sV[kStageMax];           // (v, warm-start k) of the staged set
     const float ratio = t->ratio, h = t->h, inv_h = t->inv_h;
-    const float4 pi = pos[i];
-    const float4 vi = vel_i[i];
-    const uint32_t cn = cnt[i];
-    const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
-    const uint32_t stride = c.cap;
     auto ws_of = [&](uint32_t j) -> float {
         if (WS == 0) return fmaxf(kws[j] * ratio / 2.0f, -0.5f);
         if (WS == 1) return fmaxf(kws[j] * inv_h, -0.5f);
         return 0.0f;
     };
+    const uint32_t b = xcd_block();
+    const uint32_t i = b * kBlock + threadIdx.x;
+    const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) {
+        sP[k] = pos[g];
+        const float4 v = vel_i[g];
+        sV[k] = make_float4(v.x, v.y, v.z, ws_of(g));
+    });
+    if (i >= n) return;
+    const float4 pi = pos[i];
+    const float4 vi = vel_i[i];
+    const uint32_t cn = L.cnt[i];
+    const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
+    const uint32_t stride = c.cap;
     const float ki = ws_of(i);
     if (kvtot) kvtot[i] = ki;
     float dens = pi.w * c.W0, sg2 = 0.0f, dr = 0.0f;
     float gx = 0.0f, gy = 0.0f, gz = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
-    for_each_nbr(nbr + i, stride, nf, i, [&](uint32_t j, bool valid) {
-        const float4 pj = pos[j];
+    auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
         const float m = keep(valid, pj.w);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const PairK kp = kernel_pair(c, dist2(dx, dy, dz));
@@ -468,16 +600,24 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
         const float px = mg * dx, py = mg * dy, pz = mg * dz;
         sg2 += px * px + py * py + pz * pz;
         gx += px; gy += py; gz += pz;
-        const float4 vj = vel_i[j];
         dr += (vi.x - vj.x) * px + (vi.y - vj.y) * py + (vi.z - vj.z) * pz;
         if (WS != 2) {
-            const float ks = ki + ws_of(j);
+            const float ks = ki + vj.w;
             const float s = keep(fabsf(ks) > kEps, ks);
             ax += s * px; ay += s * py; az += s * pz;
         }
-    });
+    };
+    if (sn != kUnstaged) {
+        for_each_nbr16(L.nbr16 + i, stride, nf, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); });
+    } else {
+        for_each_nbr(L.nbr + i, stride, nf, i, [&](uint32_t j, bool valid) {
+            const float4 v = vel_i[j];
+            fluid(pos[j], make_float4(v.x, v.y, v.z, ws_of(j)), valid);
+        });
+    }
     float bx = 0.0f, by = 0.0f, bz = 0.0f;
-    for_each_nbr(nbr + i + (size_t)nf * stride, stride, nb, 0u, [&](uint32_t j, bool valid) {
+    const uint32_t* bl = L.nbr + i + (size_t)(sn != kUnstaged ? 0u : nf) * stride;
+    for_each_nbr(bl, stride, nb, 0u, [&](uint32_t j, bool valid) {
         const float4 pj = bpos[j];
         const float m = keep(valid, pj.w);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -511,9 +651,7 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
 // Also stores k_i = rho_adv_i * alpha_i for the next Jacobi sweep.
 template <bool INIT>
 __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, const float4* __restrict__ pos,
-                                                       const float4* __restrict__ bpos,
-                                                       const uint32_t* __restrict__ nbr,
-                                                       const uint32_t* __restrict__ cnt,
+                                                       const float4* __restrict__ bpos, NbrLists L,
                                                        const float4* __restrict__ vel,
                                                        const float* __restrict__ alpha,
                                                        float* __restrict__ radv, float* __restrict__ kk,
@@ -521,23 +659,30 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
                                                        double* partial)
 {
     __shared__ double lds4[kBlock / 64];
+    __shared__ float4 sP[kStageMax];
+    __shared__ float4 sV[kStageMax];
     if (!INIT && ctl->done) return;
-    const uint32_t i = xcd_index();
+    const uint32_t b = xcd_block();
+    const uint32_t i = b * kBlock + threadIdx.x;
+    const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) { sP[k] = pos[g]; sV[k] = vel[g]; });
     float r = 0.0f;
     if (i < n) {
         const float4 pi = pos[i];
         const float4 vi = vel[i];
-        const uint32_t cn = cnt[i];
+        const uint32_t cn = L.cnt[i];
         const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
         float dr = 0.0f;
-        for_each_nbr(nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) {
-            const float4 pj = pos[j];
+        auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
-            const float4 vj = vel[j];
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
-        });
-        for_each_nbr(nbr + i + (size_t)nf * c.cap, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
+        };
+        if (sn != kUnstaged)
+            for_each_nbr16(L.nbr16 + i, c.cap, nf, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); });
+        else
+            for_each_nbr(L.nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) { fluid(pos[j], vel[j], valid); });
+        const uint32_t* bl = L.nbr + i + (size_t)(sn != kUnstaged ? 0u : nf) * c.cap;
+        for_each_nbr(bl, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
             const float4 pj = bpos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
@@ -560,31 +705,37 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
 template <bool DIV>
 __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t, uint32_t n,
                                                    const float4* __restrict__ pos,
-                                                   const float4* __restrict__ bpos,
-                                                   const uint32_t* __restrict__ nbr,
-                                                   const uint32_t* __restrict__ cnt,
+                                                   const float4* __restrict__ bpos, NbrLists L,
                                                    const float* __restrict__ kk, float* __restrict__ kappaV,
                                                    float4* __restrict__ vel, const SolveCtl* ctl)
 {
+    __shared__ float4 sP[kStageMax];
+    __shared__ float sK[kStageMax];
     if (ctl->done) return;
-    const uint32_t i = xcd_index();
+    const uint32_t b = xcd_block();
+    const uint32_t i = b * kBlock + threadIdx.x;
+    const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) { sP[k] = pos[g]; sK[k] = kk[g]; });
     if (i >= n) return;
     const float ki = kk[i];
     if (DIV) kappaV[i] += ki;
     const float4 pi = pos[i];
-    const uint32_t cn = cnt[i];
+    const uint32_t cn = L.cnt[i];
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    for_each_nbr(nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) {
-        const float4 pj = pos[j];
-        const float ks = ki + kk[j];
+    auto fluid = [&](const float4& pj, float kj, bool valid) {
+        const float ks = ki + kj;
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float g = kernel_grad_coef(c, dist2(dx, dy, dz));
         const float s = keep(valid && fabsf(ks) > kEps, ks * pj.w * g);
         ax += s * dx; ay += s * dy; az += s * dz;
-    });
+    };
+    if (sn != kUnstaged)
+        for_each_nbr16(L.nbr16 + i, c.cap, nf, [&](uint32_t l, bool valid) { fluid(sP[l], sK[l], valid); });
+    else
+        for_each_nbr(L.nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) { fluid(pos[j], kk[j], valid); });
     if (fabsf(ki) > kEps) {
-        for_each_nbr(nbr + i + (size_t)nf * c.cap, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
+        const uint32_t* bl = L.nbr + i + (size_t)(sn != kUnstaged ? 0u : nf) * c.cap;
+        for_each_nbr(bl, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
             const float4 pj = bpos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float s = keep(valid, ki * pj.w * kernel_grad_coef(c, dist2(dx, dy, dz)));
@@ -601,30 +752,44 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t,
 // a_i = g - sum_f (1/h) nu (m_j / rho_j) (v_i - v_j) W_ij, fused with the CFL
 // maximum of |v + a h|^2 (DFSPH_CFLVelSquaredNorm_kernel, :3273-3278).
 __global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, uint32_t n,
-                                                 const float4* __restrict__ pos,
-                                                 const uint32_t* __restrict__ nbr,
-                                                 const uint32_t* __restrict__ cnt,
+                                                 const float4* __restrict__ pos, NbrLists L,
                                                  const float4* __restrict__ vel,
                                                  const float* __restrict__ rho, float4* __restrict__ acc,
                                                  DevCtl* ctl, float* partial)
 {
     __shared__ float lds4[kBlock / 64];
-    const uint32_t i = xcd_index();
+    __shared__ float4 sP[kStageMax];
+    __shared__ float4 sV[kStageMax];           // (v, m / rho) of the staged set
+    const uint32_t b = xcd_block();
+    const uint32_t i = b * kBlock + threadIdx.x;
+    const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) {
+        const float4 p = pos[g];
+        const float4 v = vel[g];
+        sP[k] = p;
+        sV[k] = make_float4(v.x, v.y, v.z, p.w / rho[g]);
+    });
     float m2 = 0.0f;
     if (i < n) {
         const float coef = t->inv_h * c.viscosity;
         const float4 pi = pos[i];
         const float4 vi = vel[i];
-        const uint32_t nf = cnt[i] & 0xffffu;
+        const uint32_t nf = L.cnt[i] & 0xffffu;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        for_each_nbr(nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) {
-            const float4 pj = pos[j];
+        auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float w = kernel_W(c, dist2(dx, dy, dz));
-            const float4 vj = vel[j];
-            const float s = keep(valid, coef * (pj.w / rho[j]) * w);
+            const float s = keep(valid, coef * vj.w * w);
             ax -= s * (vi.x - vj.x); ay -= s * (vi.y - vj.y); az -= s * (vi.z - vj.z);
-        });
+        };
+        if (sn != kUnstaged) {
+            for_each_nbr16(L.nbr16 + i, c.cap, nf, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); });
+        } else {
+            for_each_nbr(L.nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) {
+                const float4 p = pos[j];
+                const float4 v = vel[j];
+                fluid(p, make_float4(v.x, v.y, v.z, p.w / rho[j]), valid);
+            });
+        }
         const float4 a = make_float4(c.gx + ax, c.gy + ay, c.gz + az, 0.0f);
         acc[i] = a;
         const float h = t->h;
@@ -683,38 +848,44 @@ __global__ void k_update_vel(const DevTime* t, uint32_t n, const float4* acc, fl
 template <int WS, bool APPLY_ACC>
 __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTime* t, uint32_t n,
                                                         const float4* __restrict__ pos,
-                                                        const float4* __restrict__ bpos,
-                                                        const uint32_t* __restrict__ nbr,
-                                                        const uint32_t* __restrict__ cnt,
+                                                        const float4* __restrict__ bpos, NbrLists L,
                                                         const float4* __restrict__ acc,
                                                         float4* __restrict__ vel,
                                                         const float* __restrict__ kws, int rescale,
                                                         float* __restrict__ ktot)
 {
-    const uint32_t i = xcd_index();
-    if (i >= n) return;
+    __shared__ float4 sP[kStageMax];
+    __shared__ float sK[kStageMax];
     const float ratio2 = rescale ? t->ratio2 : 1.0f;
     const float ihf2 = t->inv_h_future * t->inv_h_future;
     auto ws_of = [&](uint32_t j) -> float {
         if (WS == 0) return fmaxf(kws[j] * ratio2, -0.5f);
         return fmaxf(kws[j] * ihf2, -0.5f);
     };
+    const uint32_t b = xcd_block();
+    const uint32_t i = b * kBlock + threadIdx.x;
+    const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) { sP[k] = pos[g]; sK[k] = ws_of(g); });
+    if (i >= n) return;
     const float ki = ws_of(i);
     if (ktot) ktot[i] = ki;
     const float4 pi = pos[i];
-    const uint32_t cn = cnt[i];
+    const uint32_t cn = L.cnt[i];
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    for_each_nbr(nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) {
-        const float4 pj = pos[j];
-        const float ks = ki + ws_of(j);
+    auto fluid = [&](const float4& pj, float kj, bool valid) {
+        const float ks = ki + kj;
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float g = kernel_grad_coef(c, dist2(dx, dy, dz));
         const float s = keep(valid && fabsf(ks) > kEps, ks * pj.w * g);
         ax += s * dx; ay += s * dy; az += s * dz;
-    });
+    };
+    if (sn != kUnstaged)
+        for_each_nbr16(L.nbr16 + i, c.cap, nf, [&](uint32_t l, bool valid) { fluid(sP[l], sK[l], valid); });
+    else
+        for_each_nbr(L.nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) { fluid(pos[j], ws_of(j), valid); });
     if (fabsf(ki) > kEps) {
-        for_each_nbr(nbr + i + (size_t)nf * c.cap, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
+        const uint32_t* bl = L.nbr + i + (size_t)(sn != kUnstaged ? 0u : nf) * c.cap;
+        for_each_nbr(bl, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
             const float4 pj = bpos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float s = keep(valid, ki * pj.w * kernel_grad_coef(c, dist2(dx, dy, dz)));
@@ -740,9 +911,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
 template <bool INIT>
 __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const DevTime* t, uint32_t n,
                                                            const float4* __restrict__ pos,
-                                                           const float4* __restrict__ bpos,
-                                                           const uint32_t* __restrict__ nbr,
-                                                           const uint32_t* __restrict__ cnt,
+                                                           const float4* __restrict__ bpos, NbrLists L,
                                                            const float4* __restrict__ vel,
                                                            const float* __restrict__ rho,
                                                            float* __restrict__ alpha, float* __restrict__ radv,
@@ -750,24 +919,31 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
                                                            SolveCtl* ctl, double* partial)
 {
     __shared__ double lds4[kBlock / 64];
+    __shared__ float4 sP[kStageMax];
+    __shared__ float4 sV[kStageMax];
     if (!INIT && ctl->done) return;
-    const uint32_t i = xcd_index();
+    const uint32_t b = xcd_block();
+    const uint32_t i = b * kBlock + threadIdx.x;
+    const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) { sP[k] = pos[g]; sV[k] = vel[g]; });
     float r = 0.0f;
     if (i < n) {
         const float hf = t->h_future;
         const float4 pi = pos[i];
         const float4 vi = vel[i];
-        const uint32_t cn = cnt[i];
+        const uint32_t cn = L.cnt[i];
         const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
         float dr = 0.0f;
-        for_each_nbr(nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) {
-            const float4 pj = pos[j];
+        auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
-            const float4 vj = vel[j];
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
-        });
-        for_each_nbr(nbr + i + (size_t)nf * c.cap, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
+        };
+        if (sn != kUnstaged)
+            for_each_nbr16(L.nbr16 + i, c.cap, nf, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); });
+        else
+            for_each_nbr(L.nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) { fluid(pos[j], vel[j], valid); });
+        const uint32_t* bl = L.nbr + i + (size_t)(sn != kUnstaged ? 0u : nf) * c.cap;
+        for_each_nbr(bl, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
             const float4 pj = bpos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
@@ -855,37 +1031,42 @@ __global__ void k_finish_totals(const DevTime* t, uint32_t n, int pressure, cons
 // 10 mean neighbor distance / h_k 11 fluid share of the density
 __global__ __launch_bounds__(kBlock) void k_features(SimConst c, const DevTime* t, uint32_t n,
                                                      const float4* __restrict__ pos,
-                                                     const float4* __restrict__ bpos,
-                                                     const uint32_t* __restrict__ nbr,
-                                                     const uint32_t* __restrict__ cnt,
+                                                     const float4* __restrict__ bpos, NbrLists L,
                                                      const float4* __restrict__ vel,
                                                      const float* __restrict__ ktot,
                                                      const float* __restrict__ kvtot, float diameter,
                                                      float* __restrict__ feat)
 {
-    const uint32_t i = xcd_index();
+    __shared__ float4 sP[kStageMax];
+    __shared__ float4 sV[kStageMax];
+    const uint32_t b = xcd_block();
+    const uint32_t i = b * kBlock + threadIdx.x;
+    const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) { sP[k] = pos[g]; sV[k] = vel[g]; });
     if (i >= n) return;
     const float4 pi = pos[i];
     const float4 vi = vel[i];
-    const uint32_t cn = cnt[i];
+    const uint32_t cn = L.cnt[i];
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
     float dens = pi.w * c.W0, dfl = 0.0f, dr = 0.0f;
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, od = 0.0f;
-    for_each_nbr(nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) {
-        const float4 pj = pos[j];
+    auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
         const float m = keep(valid, 1.0f);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2(dx, dy, dz);
         const PairK kp = kernel_pair(c, r2);
         const float mj = m * pj.w;
         dfl += mj * kp.W;
-        const float4 vj = vel[j];
         dr += mj * kp.g * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         ox -= m * dx; oy -= m * dy; oz -= m * dz;
         od += m * __builtin_sqrtf(r2);
-    });
+    };
+    if (sn != kUnstaged)
+        for_each_nbr16(L.nbr16 + i, c.cap, nf, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); });
+    else
+        for_each_nbr(L.nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) { fluid(pos[j], vel[j], valid); });
     float dbd = 0.0f;
-    for_each_nbr(nbr + i + (size_t)nf * c.cap, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
+    const uint32_t* bl = L.nbr + i + (size_t)(sn != kUnstaged ? 0u : nf) * c.cap;
+    for_each_nbr(bl, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
         const float4 pj = bpos[j];
         const float m = keep(valid, 1.0f);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -1033,13 +1214,20 @@ void launch_gather_f4(hipStream_t s, uint32_t n, const uint32_t* perm, const flo
 
 void launch_neighbors(hipStream_t s, const SimConst& c, bool indirect, const float4* pos, const uint32_t* cs_f,
                       const uint32_t* perm, const float4* bpos, const uint32_t* cs_b, uint32_t n,
-                      uint32_t* nbr, uint32_t* cnt, DevCtl* ctl)
+                      uint32_t* nbr, uint16_t* nbr16, uint32_t* cnt, uint32_t* stage_n, uint32_t* stg, DevCtl* ctl)
 {
     if (!n) return;
     if (indirect)
-        k_neighbors<true><<<nblk_xcd(n), kBlock, 0, s>>>(c, pos, cs_f, perm, bpos, cs_b, n, nbr, cnt, ctl);
+        k_neighbors<true><<<nblk_xcd(n), kBlock, 0, s>>>(c, pos, cs_f, perm, bpos, cs_b, n, nbr, nbr16, cnt, stage_n,
+                                                         stg, ctl);
     else
-        k_neighbors<false><<<nblk_xcd(n), kBlock, 0, s>>>(c, pos, cs_f, perm, bpos, cs_b, n, nbr, cnt, ctl);
+        k_neighbors<false><<<nblk_xcd(n), kBlock, 0, s>>>(c, pos, cs_f, perm, bpos, cs_b, n, nbr, nbr16, cnt, stage_n,
+                                                          stg, ctl);
+}
+
+void launch_expand_lists(hipStream_t s, uint32_t n, uint32_t cap, const NbrLists& L, uint32_t* out)
+{
+    if (n) k_expand_lists<<<nblk(n), kBlock, 0, s>>>(n, cap, L, out);
 }
 
 void launch_boundary_psi(hipStream_t s, const SimConst& c, const float4* bpos, const uint32_t* cs_b,
@@ -1054,51 +1242,51 @@ void launch_set_w(hipStream_t s, float4* p, const float* w, uint32_t n)
 }
 
 void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime* t, uint32_t n,
-                        const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
+                        const float4* pos, const float4* bpos, const NbrLists& L,
                         const float4* vel_i, float4* vel_o, const float* kws, float* rho, float* alpha,
                         float* radv, float* kvtot)
 {
     if (!n) return;
     const uint32_t g = nblk_xcd(n);
     if (ws == 0)
-        k_div_ws_init<0><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel_i, vel_o, kws, rho, alpha, radv, kvtot);
+        k_div_ws_init<0><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel_i, vel_o, kws, rho, alpha, radv, kvtot);
     else if (ws == 1)
-        k_div_ws_init<1><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel_i, vel_o, kws, rho, alpha, radv, kvtot);
+        k_div_ws_init<1><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel_i, vel_o, kws, rho, alpha, radv, kvtot);
     else
-        k_div_ws_init<2><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel_i, vel_o, kws, rho, alpha, radv, kvtot);
+        k_div_ws_init<2><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel_i, vel_o, kws, rho, alpha, radv, kvtot);
 }
 
 void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
-                       const float4* bpos, const uint32_t* nbr, const uint32_t* cnt, const float4* vel,
+                       const float4* bpos, const NbrLists& L, const float4* vel,
                        const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial)
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
     if (init) {
-        k_div_source<true><<<g, kBlock, 0, s>>>(c, n, pos, bpos, nbr, cnt, vel, alpha, radv, kk, kappaV, ctl, partial);
+        k_div_source<true><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial);
     } else {
-        k_div_source<false><<<g, kBlock, 0, s>>>(c, n, pos, bpos, nbr, cnt, vel, alpha, radv, kk, kappaV, ctl, partial);
+        k_div_source<false><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial);
         k_solve_finalize<<<1, kBlock, 0, s>>>(ctl, partial, g);
     }
 }
 
 void launch_jacobi(hipStream_t s, bool div, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
-                   const float4* bpos, const uint32_t* nbr, const uint32_t* cnt, const float* kk, float* kappaV,
+                   const float4* bpos, const NbrLists& L, const float* kk, float* kappaV,
                    float4* vel, const SolveCtl* ctl)
 {
     if (!n) return;
     const uint32_t g = nblk_xcd(n);
     if (div)
-        k_jacobi<true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, kk, kappaV, vel, ctl);
+        k_jacobi<true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, kk, kappaV, vel, ctl);
     else
-        k_jacobi<false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, kk, kappaV, vel, ctl);
+        k_jacobi<false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, kk, kappaV, vel, ctl);
 }
 
 void launch_xsph(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
-                 const uint32_t* nbr, const uint32_t* cnt, const float4* vel, const float* rho, float4* acc,
+                 const NbrLists& L, const float4* vel, const float* rho, float4* acc,
                  DevCtl* ctl, float* partial)
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
-    k_xsph<<<g, kBlock, 0, s>>>(c, t, n, pos, nbr, cnt, vel, rho, acc, ctl, partial);
+    k_xsph<<<g, kBlock, 0, s>>>(c, t, n, pos, L, vel, rho, acc, ctl, partial);
     k_cfl_finalize<<<1, kBlock, 0, s>>>(ctl, partial, g);
 }
 
@@ -1108,34 +1296,34 @@ void launch_update_vel(hipStream_t s, const DevTime* t, uint32_t n, const float4
 }
 
 void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c, const DevTime* t, uint32_t n,
-                        const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
+                        const float4* pos, const float4* bpos, const NbrLists& L,
                         const float4* acc, float4* vel, const float* kws, int rescale, float* ktot)
 {
     if (!n) return;
     const uint32_t g = nblk_xcd(n);
     if (ws == 1) {
         if (apply_acc)
-            k_pressure_ws<1, true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale, ktot);
+            k_pressure_ws<1, true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, acc, vel, kws, rescale, ktot);
         else
-            k_pressure_ws<1, false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale, ktot);
+            k_pressure_ws<1, false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, acc, vel, kws, rescale, ktot);
     } else {
         if (apply_acc)
-            k_pressure_ws<0, true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale, ktot);
+            k_pressure_ws<0, true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, acc, vel, kws, rescale, ktot);
         else
-            k_pressure_ws<0, false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, acc, vel, kws, rescale, ktot);
+            k_pressure_ws<0, false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, acc, vel, kws, rescale, ktot);
     }
 }
 
 void launch_density_source(hipStream_t s, bool init, const SimConst& c, const DevTime* t, uint32_t n,
-                           const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
+                           const float4* pos, const float4* bpos, const NbrLists& L,
                            const float4* vel, const float* rho, float* alpha, float* radv, float* kk,
                            float* kappa, SolveCtl* ctl, double* partial)
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
     if (init) {
-        k_density_source<true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel, rho, alpha, radv, kk, kappa, ctl, partial);
+        k_density_source<true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl, partial);
     } else {
-        k_density_source<false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel, rho, alpha, radv, kk, kappa, ctl, partial);
+        k_density_source<false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl, partial);
         k_solve_finalize<<<1, kBlock, 0, s>>>(ctl, partial, g);
     }
 }
@@ -1170,10 +1358,10 @@ void launch_finish_totals(hipStream_t s, const DevTime* t, uint32_t n, int press
 }
 
 void launch_features(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
-                     const float4* bpos, const uint32_t* nbr, const uint32_t* cnt, const float4* vel,
+                     const float4* bpos, const NbrLists& L, const float4* vel,
                      const float* ktot, const float* kvtot, float diameter, float* feat)
 {
-    if (n) k_features<<<nblk_xcd(n), kBlock, 0, s>>>(c, t, n, pos, bpos, nbr, cnt, vel, ktot, kvtot, diameter, feat);
+    if (n) k_features<<<nblk_xcd(n), kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, ktot, kvtot, diameter, feat);
 }
 
 void launch_harvest(hipStream_t s, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap, const float* feat,

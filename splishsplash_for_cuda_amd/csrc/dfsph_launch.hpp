@@ -34,30 +34,32 @@ void launch_gather_fluid(hipStream_t s, uint32_t n, const uint32_t* perm, const 
 void launch_gather_f4(hipStream_t s, uint32_t n, const uint32_t* perm, const float4* in, float4* out);
 void launch_neighbors(hipStream_t s, const SimConst& c, bool indirect, const float4* pos, const uint32_t* cs_f,
                       const uint32_t* perm, const float4* bpos, const uint32_t* cs_b, uint32_t n,
-                      uint32_t* nbr, uint32_t* cnt, DevCtl* ctl);
+                      uint32_t* nbr, uint16_t* nbr16, uint32_t* cnt, uint32_t* stage_n, uint32_t* stg,
+                      DevCtl* ctl);
+void launch_expand_lists(hipStream_t s, uint32_t n, uint32_t cap, const NbrLists& L, uint32_t* out);
 void launch_boundary_psi(hipStream_t s, const SimConst& c, const float4* bpos, const uint32_t* cs_b,
                          uint32_t nb, float* psi);
 void launch_set_w(hipStream_t s, float4* p, const float* w, uint32_t n);
 
 void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime* t, uint32_t n,
-                        const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
+                        const float4* pos, const float4* bpos, const NbrLists& L,
                         const float4* vel_i, float4* vel_o, const float* kws, float* rho, float* alpha,
                         float* radv, float* kvtot);
 void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
-                       const float4* bpos, const uint32_t* nbr, const uint32_t* cnt, const float4* vel,
+                       const float4* bpos, const NbrLists& L, const float4* vel,
                        const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial);
 void launch_jacobi(hipStream_t s, bool div, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
-                   const float4* bpos, const uint32_t* nbr, const uint32_t* cnt, const float* kk, float* kappaV,
+                   const float4* bpos, const NbrLists& L, const float* kk, float* kappaV,
                    float4* vel, const SolveCtl* ctl);
 void launch_xsph(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
-                 const uint32_t* nbr, const uint32_t* cnt, const float4* vel, const float* rho, float4* acc,
+                 const NbrLists& L, const float4* vel, const float* rho, float4* acc,
                  DevCtl* ctl, float* partial);
 void launch_update_vel(hipStream_t s, const DevTime* t, uint32_t n, const float4* acc, float4* vel, float* kappa);
 void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c, const DevTime* t, uint32_t n,
-                        const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
+                        const float4* pos, const float4* bpos, const NbrLists& L,
                         const float4* acc, float4* vel, const float* kws, int rescale, float* ktot);
 void launch_density_source(hipStream_t s, bool init, const SimConst& c, const DevTime* t, uint32_t n,
-                           const float4* pos, const float4* bpos, const uint32_t* nbr, const uint32_t* cnt,
+                           const float4* pos, const float4* bpos, const NbrLists& L,
                            const float4* vel, const float* rho, float* alpha, float* radv, float* kk,
                            float* kappa, SolveCtl* ctl, double* partial);
 void launch_update_pos(hipStream_t s, const DevTime* t, uint32_t n, const float4* vel, float4* pos,
@@ -69,7 +71,7 @@ void launch_compact(hipStream_t s, uint32_t n, const uint32_t* keep, const uint3
                     const FluidPtrs& out);
 void launch_finish_totals(hipStream_t s, const DevTime* t, uint32_t n, int pressure, const float* inc, float* tot);
 void launch_features(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
-                     const float4* bpos, const uint32_t* nbr, const uint32_t* cnt, const float4* vel,
+                     const float4* bpos, const NbrLists& L, const float4* vel,
                      const float* ktot, const float* kvtot, float diameter, float* feat);
 void launch_harvest(hipStream_t s, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap, const float* feat,
                     const float* kvtot, const float* ktot, float* ds_x, float* ds_t);
