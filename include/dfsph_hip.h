@@ -62,7 +62,13 @@ typedef struct dfsph_params {
     uint32_t fluid_capacity;  /* 0: 1.3 x the first upload (DFSPH_c_arrays_structure.cpp:170-172) */
     int32_t warm_start_mode;  /* DFSPH_WARMSTART_*                                            */
     uint32_t check_interval;  /* solver iterations queued per convergence readback (0: 4)     */
+    int32_t list_mode;        /* DFSPH_LISTS_*: neighbor-list form (results are identical)     */
 } dfsph_params;
+
+/* list_mode: how sweep blocks (256 particles) read their fluid neighbors */
+#define DFSPH_LISTS_AUTO 0       /* LDS-staged set when it fits, else 16-bit set ids, else 32-bit */
+#define DFSPH_LISTS_NO_LDS 1     /* 16-bit set ids read through the set's global ids               */
+#define DFSPH_LISTS_GLOBAL 2     /* 32-bit global ids only                                          */
 
 typedef struct dfsph_step_stats {
     uint32_t iters_v, iters;  /* divergence / density Jacobi iterations */
@@ -109,6 +115,7 @@ typedef struct dfsph_info {
 #define DFSPH_FIELD_NBR_BOUNDARY 11 /* uint32 boundary neighbor count */
 #define DFSPH_FIELD_BOUNDARY_POS 12 /* float xyz (3 n_boundary), cell-sorted */
 #define DFSPH_FIELD_BOUNDARY_PSI 13 /* float (n_boundary), cell-sorted */
+#define DFSPH_FIELD_TAG 14         /* uint32 dynamic-window / open-boundary tag (also uploadable) */
 
 void dfsph_default_params(dfsph_params* p);
 const char* dfsph_last_error(void);
@@ -234,6 +241,81 @@ int dfsph_profile_enable(void* handle, uint32_t mask);
 int dfsph_profile_reset(void* handle);
 /* launches and summed duration (ms) of one kernel id since the last reset */
 int dfsph_profile_read(void* handle, uint32_t kernel, uint32_t* launches, double* total_ms);
+
+/* ---- dynamic window: solver restriction and tags -------------------------
+ * The reference restricts its solver to a window of the fluid while it
+ * stabilises a newly loaded region (DFSPHCData::restriction_mode, count_active,
+ * count_active_neighbors, true_particle_count, DFSPH_c_arrays_structure.h:410-418,
+ * set at DynamicWindow.cu:3047-3048, 3336, 3736-3739).  Tags carry the
+ * reference's values (cuda_compilation_center.cu:2-11) and move with the
+ * particles through sorts and compactions. */
+#define DFSPH_TAG_ACTIVE 10000000u
+#define DFSPH_TAG_ACTIVE_NEIGHBORS 20000000u
+#define DFSPH_TAG_UNTAGGED 40000000u
+#define DFSPH_TAG_REMOVAL 310000000u
+/* mode 0: whole fluid.  mode 1: velocity / position updates on the particles
+ * tagged ACTIVE, density / factor / source / neighbor lists also on those
+ * tagged ACTIVE_NEIGHBORS (DFSPH_core_cuda.cu:225-231, 500-506).  mode 2: the
+ * same on the index prefixes [0, count_active) and
+ * [0, count_active + count_active_neighbors) (dfsph_step then searches without
+ * reordering).  true_particle_count >= 0 replaces N in the average errors
+ * (DFSPH_core_cuda.cu:791-795, 1339-1343).  Restricted-out particles keep
+ * their stored rho_adv in the error sums, as the reference's reductions do. */
+int dfsph_set_restriction(void* handle, int mode, uint32_t count_active, uint32_t count_active_neighbors,
+                          int64_t true_particle_count);
+/* tag_neighbors_of_tagged_kernel<untagged_only, true> (RestFluidLoader.cu:2070-2101):
+ * the stored fluid neighbors of every particle tagged tag_in get tag_out (only
+ * untagged ones if untagged_only); changed: particles whose tag changed.  The
+ * tags read are those before the call, so the result is order independent. */
+int dfsph_tag_neighbors(void* handle, uint32_t tag_in, uint32_t tag_out, int untagged_only, uint32_t* changed);
+/* count_particle_with_tag_kernel (RestFluidLoader.cu:2238-2252) */
+int dfsph_count_tagged(void* handle, uint32_t tag, uint32_t* count);
+/* Stable reorder: ACTIVE, then ACTIVE_NEIGHBORS, then the rest (the prefix
+ * order of restriction mode 2; the reference radix-sorts by tag,
+ * DynamicWindow.cu:2990-2998); returns both counts. */
+int dfsph_order_active_first(void* handle, uint32_t* count_active, uint32_t* count_active_neighbors);
+/* remove_tagged_particles (SPH_other_systems_cuda.cu:1606-1641) for one tag, order kept */
+int dfsph_remove_tagged(void* handle, uint32_t tag, uint32_t* removed);
+
+/* ---- open boundary ---------------------------------------------------------
+ * OpenBoundariesSimple (OpenBoundariesSimple.cu:117-621) with the analytic
+ * surfaces of BufferFluidSurface (BufferFluidSurface.cu:421-1070). */
+#define DFSPH_SURFACE_PLANE 0      /* inside: (p - o) . n >= 0                     */
+#define DFSPH_SURFACE_CUBOID 1     /* |p - center| < half per axis                 */
+#define DFSPH_SURFACE_CYLINDER 2   /* axis y: |dy| <= half[1], radial dist < radius */
+#define DFSPH_SURFACE_SPHERE 4     /* |p - center| < radius                         */
+#define DFSPH_SURFACE_STAR 5       /* prism along y, linear-sided star in x-z       */
+typedef struct dfsph_surface {
+    int32_t type;                  /* DFSPH_SURFACE_*                               */
+    int32_t reversed;              /* exterior becomes interior                     */
+    float o[3], n[3];              /* plane                                         */
+    float center[3], half[3];      /* cuboid; cylinder / star use half[1]           */
+    float radius, radius_internal; /* cylinder, sphere; star outer / inner radius   */
+    uint32_t points;               /* star                                          */
+    float direction[3];            /* star: unit direction of the first point (x-z) */
+} dfsph_surface;
+/* BufferFluidSurface::isinside on the device for n points (inside[i] = 0 / 1) */
+int dfsph_surface_inside(void* handle, const dfsph_surface* s, const float* xyz, uint32_t n, uint8_t* inside);
+/* OpenBoundariesSimple::init (OpenBoundariesSimple.cu:117-357): the surfaces and
+ * the inflow positions, of which those inside the interior shape or outside
+ * the fluid-surface shape are dropped; kept: the remaining positions. */
+int dfsph_open_boundary_init(void* handle, const dfsph_surface* boundary, const dfsph_surface* interior,
+                             const dfsph_surface* fluid_surface, const float* inflow_xyz, uint32_t n,
+                             uint32_t* kept);
+/* OpenBoundariesSimple::applyOpenBoundary (OpenBoundariesSimple.cu:541-621):
+ * window_displacement (may be NULL): the window's total displacement, the
+ * surfaces and inflow positions follow its change since the last call;
+ * inflow: inflow_with_predefined_positions_kernel (:360-508) appends, in
+ * inflow-position order, a particle at every position whose nearest fluid
+ * particle is farther than allowed_new_distance, whose density estimate is at
+ * most allowed_new_density and whose interpolated velocity does not point
+ * against the boundary's normal (mass of fluid particle 0, kappa = 0, ids
+ * after the largest ever used); outflow: outflow_basic_kernel (:517-534)
+ * removes the particles outside both the interior and the fluid-surface
+ * shape.  The reference's call passes 1.75 r and 700 (DFSPH_CUDA.cpp:3903-3906). */
+int dfsph_open_boundary_apply(void* handle, const double* window_displacement, float allowed_new_distance,
+                              float allowed_new_density, int use_inflow, int use_outflow, uint32_t* added,
+                              uint32_t* removed);
 
 /* ---- learned warm start -------------------------------------------------
  * No reference counterpart: these entry points add a per-particle MLP

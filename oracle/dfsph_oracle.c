@@ -47,7 +47,39 @@ struct or_state {
 
     double h, h_future, h_past, h_ratio_to_past, h_ratio_to_past2, time;
     double bbmin[3], bbmax[3];
+
+    /* solver restriction (GPU mode): DFSPHCData::restriction_mode,
+     * count_active, count_active_neighbors, true_particle_count */
+    int rmode;
+    uint32_t r_active, r_active_nbrs;
+    long long true_count;
+    uint32_t* tag;
 };
+
+#define OR_TAG_ACTIVE 10000000u            /* cuda_compilation_center.cu:6-9 */
+#define OR_TAG_ACTIVE_NEIGHBORS 20000000u
+#define OR_TAG_UNTAGGED 40000000u
+
+/* the restriction tests of the kernels (DFSPH_core_cuda.cu:225-231 "source"
+ * kernels: active + active neighbors; :500-506, 3052-3058, 3131-3137 "compute"
+ * kernels: active only) */
+static inline int r_compute(const or_state* s, uint32_t i)
+{
+    if (s->rmode == 1) return s->tag[i] == OR_TAG_ACTIVE;
+    if (s->rmode == 2) return i < s->r_active;
+    return 1;
+}
+static inline int r_source(const or_state* s, uint32_t i)
+{
+    if (s->rmode == 1) return s->tag[i] == OR_TAG_ACTIVE || s->tag[i] == OR_TAG_ACTIVE_NEIGHBORS;
+    if (s->rmode == 2) return i < s->r_active + s->r_active_nbrs;
+    return 1;
+}
+/* error average: true_particle_count when set (DFSPH_core_cuda.cu:791-795, 1339-1343) */
+static inline double r_count(const or_state* s)
+{
+    return s->true_count >= 0 ? (double)s->true_count : (double)s->n;
+}
 
 /* ------------------------------------------------------------------ */
 /* A3: cubic spline kernel (SPH_kernels.h:20-100, SPHKernels.h:13-92)  */
@@ -335,16 +367,18 @@ or_state* or_create(const or_params* p)
     s->h_ratio_to_past = 1.0;
     s->h_ratio_to_past2 = 1.0;
     s->time = 0.0;
+    s->rmode = 0;                               /* no restriction, N-particle averages */
+    s->true_count = -1;
     return s;
 }
 
 static void free_fluid(or_state* s)
 {
     free(s->x); free(s->v); free(s->m); free(s->rho); free(s->alpha); free(s->radv);
-    free(s->kappa); free(s->kappaV); free(s->acc); free(s->id);
+    free(s->kappa); free(s->kappaV); free(s->acc); free(s->id); free(s->tag);
     free(s->nf); free(s->nbc); free(s->nbr);
     s->x = s->v = s->m = s->rho = s->alpha = s->radv = s->kappa = s->kappaV = s->acc = NULL;
-    s->id = s->nf = s->nbc = s->nbr = NULL;
+    s->id = s->nf = s->nbc = s->nbr = s->tag = NULL;
 }
 
 void or_destroy(or_state* s)
@@ -371,6 +405,7 @@ int or_set_fluid(or_state* s, uint32_t n, const double* x, const double* v,
     s->kappa = (double*)calloc(n1, sizeof(double));
     s->kappaV = (double*)calloc(n1, sizeof(double));
     s->id = (uint32_t*)malloc(n1 * sizeof(uint32_t));
+    s->tag = (uint32_t*)malloc(n1 * sizeof(uint32_t));
     s->nf = (uint32_t*)calloc(n1, sizeof(uint32_t));
     s->nbc = (uint32_t*)calloc(n1, sizeof(uint32_t));
     s->nbr = (uint32_t*)calloc(n1 * s->p.max_neighbors, sizeof(uint32_t));
@@ -381,7 +416,20 @@ int or_set_fluid(or_state* s, uint32_t n, const double* x, const double* v,
     for (uint32_t i = 0; i < n; ++i) {
         s->m[i] = m ? m[i] : 0.8 * d * d * d * s->p.density0;
         s->id[i] = id ? id[i] : i;
+        s->tag[i] = OR_TAG_UNTAGGED;
     }
+    return 0;
+}
+
+int or_set_restriction(or_state* s, int mode, uint32_t count_active, uint32_t count_active_neighbors,
+                       long long true_particle_count, const uint32_t* tags)
+{
+    if (mode < 0 || mode > 2) return -1;
+    s->rmode = mode;
+    s->r_active = count_active;
+    s->r_active_nbrs = count_active_neighbors;
+    s->true_count = true_particle_count;
+    if (tags) memcpy(s->tag, tags, s->n * sizeof(uint32_t));
     return 0;
 }
 
@@ -532,6 +580,9 @@ int or_neighbor_search(or_state* s)
         s->max_seen = maxseen;
         grid_free(&gf); grid_free(&gb);
     }
+    /* restricted-out particles get no lists (DFSPH_core_cuda.cu:2545-2552) */
+    for (uint32_t i = 0; i < n; ++i)
+        if (!r_source(s, i)) { s->nf[i] = 0; s->nbc[i] = 0; }
     /* sort each particle's fluid and boundary sub-lists so sums have a fixed order */
     for (uint32_t i = 0; i < n; ++i) {
         if (s->nf[i] + s->nbc[i] > max) {
@@ -677,6 +728,7 @@ uint32_t or_divergence_solve(or_state* s, double* err_out)
 #pragma omp parallel for schedule(static)
     for (long il = 0; il < (long)n; ++il) {
         const uint32_t i = (uint32_t)il;
+        if (!r_source(s, i)) continue;
         if (gpu) {
             const double kv = s->kappaV[i] * s->h_ratio_to_past / 2.0;   /* core.cu:240 */
             s->kappaV[i] = kv > -0.5 ? kv : -0.5;
@@ -691,13 +743,14 @@ uint32_t or_divergence_solve(or_state* s, double* err_out)
 #pragma omp parallel for schedule(static)
     for (long il = 0; il < (long)n; ++il) {
         const uint32_t i = (uint32_t)il;
-        if (s->radv[i] > 0.0) jacobi_update(s, i, s->kappaV, h, dv + 3 * i);
+        if (s->radv[i] > 0.0 && r_compute(s, i)) jacobi_update(s, i, s->kappaV, h, dv + 3 * i);
         else dv[3 * i] = dv[3 * i + 1] = dv[3 * i + 2] = 0.0;
     }
     for (uint32_t i = 0; i < 3 * n; ++i) s->v[i] += dv[i];
     /* divergence init: kappaV = 0, source from the warm-started velocities */
 #pragma omp parallel for schedule(static)
     for (long il = 0; il < (long)n; ++il) {
+        if (!r_source(s, (uint32_t)il)) continue;
         s->kappaV[il] = 0.0;
         compute_density_change(s, (uint32_t)il);
     }
@@ -705,13 +758,20 @@ uint32_t or_divergence_solve(or_state* s, double* err_out)
     double err = 0.0;
     uint32_t it = 0;
     while ((err > eta || it < s->p.min_iter_v) && it < s->p.max_iter_v) {
-        for (uint32_t i = 0; i < n; ++i) { k[i] = s->radv[i] * s->alpha[i]; s->kappaV[i] += k[i]; }
+        for (uint32_t i = 0; i < n; ++i) {
+            k[i] = s->radv[i] * s->alpha[i];
+            if (r_compute(s, i)) s->kappaV[i] += k[i];
+        }
 #pragma omp parallel for schedule(static)
-        for (long il = 0; il < (long)n; ++il) jacobi_update(s, (uint32_t)il, k, h, dv + 3 * il);
+        for (long il = 0; il < (long)n; ++il) {
+            if (r_compute(s, (uint32_t)il)) jacobi_update(s, (uint32_t)il, k, h, dv + 3 * il);
+            else dv[3 * il] = dv[3 * il + 1] = dv[3 * il + 2] = 0.0;
+        }
         for (uint32_t i = 0; i < 3 * n; ++i) s->v[i] += dv[i];
 #pragma omp parallel for schedule(static)
-        for (long il = 0; il < (long)n; ++il) compute_density_change(s, (uint32_t)il);
-        err = n ? sum_radv(s) / (double)n : 0.0;
+        for (long il = 0; il < (long)n; ++il)
+            if (r_source(s, (uint32_t)il)) compute_density_change(s, (uint32_t)il);
+        err = n ? sum_radv(s) / r_count(s) : 0.0;
         it++;
     }
     if (!gpu) for (uint32_t i = 0; i < n; ++i) s->kappaV[i] *= h;        /* TimeStepDFSPH.cpp:658-661 */
@@ -728,6 +788,7 @@ void or_external_forces(or_state* s)
 #pragma omp parallel for schedule(static)
     for (long il = 0; il < (long)s->n; ++il) {
         const uint32_t i = (uint32_t)il;
+        if (!r_compute(s, i)) continue;          /* DFSPH_core_cuda.cu:1428-1434 */
         const uint32_t* row = ROW(i);
         const double* xi = XI(i);
         const double* vi = VI(i);
@@ -776,9 +837,11 @@ void or_update_time_step(or_state* s, double hf)
 /* A14: DFSPH_update_vel_kernel (DFSPH_core_cuda.cu:3047-3070) */
 void or_update_vel(or_state* s)
 {
-    for (uint32_t i = 0; i < 3 * s->n; ++i) s->v[i] += s->h * s->acc[i];
+    for (uint32_t i = 0; i < 3 * s->n; ++i)
+        if (r_compute(s, i / 3)) s->v[i] += s->h * s->acc[i];
     if (s->p.mode == OR_MODE_GPU) {
         for (uint32_t i = 0; i < s->n; ++i) {
+            if (!r_compute(s, i)) continue;
             const double kk = s->kappa[i] * s->h_ratio_to_past2;
             s->kappa[i] = kk > -0.5 ? kk : -0.5;
         }
@@ -811,12 +874,16 @@ uint32_t or_pressure_solve(or_state* s, double* err_out)
     }
     /* warm start: cuda_pressure_compute<true> (core.cu:965-1000), no gate */
 #pragma omp parallel for schedule(static)
-    for (long il = 0; il < (long)n; ++il) jacobi_update(s, (uint32_t)il, s->kappa, hf, dv + 3 * il);
+    for (long il = 0; il < (long)n; ++il) {
+        if (r_compute(s, (uint32_t)il)) jacobi_update(s, (uint32_t)il, s->kappa, hf, dv + 3 * il);
+        else dv[3 * il] = dv[3 * il + 1] = dv[3 * il + 2] = 0.0;
+    }
     for (uint32_t i = 0; i < 3 * n; ++i) s->v[i] += dv[i];
     /* init: kappa = 0, factor *= 1/h_f, rho_adv (core.cu:1217-1262) */
 #pragma omp parallel for schedule(static)
     for (long il = 0; il < (long)n; ++il) {
         const uint32_t i = (uint32_t)il;
+        if (!r_source(s, i)) continue;
         s->kappa[i] = 0.0;
         s->alpha[i] *= 1.0 / hf;
         compute_density_adv(s, i, hf);
@@ -831,15 +898,19 @@ uint32_t or_pressure_solve(or_state* s, double* err_out)
             if (!gpu) s->kappa[i] += k[i];                   /* TimeStepDFSPH.cpp:378 */
         }
 #pragma omp parallel for schedule(static)
-        for (long il = 0; il < (long)n; ++il) jacobi_update(s, (uint32_t)il, k, hf, dv + 3 * il);
+        for (long il = 0; il < (long)n; ++il) {
+            if (r_compute(s, (uint32_t)il)) jacobi_update(s, (uint32_t)il, k, hf, dv + 3 * il);
+            else dv[3 * il] = dv[3 * il + 1] = dv[3 * il + 2] = 0.0;
+        }
         for (uint32_t i = 0; i < 3 * n; ++i) s->v[i] += dv[i];
 #pragma omp parallel for schedule(static)
         for (long il = 0; il < (long)n; ++il) {
             const uint32_t i = (uint32_t)il;
+            if (!r_source(s, i)) continue;
             compute_density_adv(s, i, hf);
             if (gpu) s->kappa[i] += s->radv[i] * s->alpha[i];   /* core.cu:1209-1212 */
         }
-        err = n ? sum_radv(s) / (double)n : 0.0;
+        err = n ? sum_radv(s) / r_count(s) : 0.0;
         it++;
     }
     if (!gpu) for (uint32_t i = 0; i < n; ++i) s->kappa[i] *= hf * hf;   /* TimeStepDFSPH.cpp:455-459 */
@@ -851,7 +922,8 @@ uint32_t or_pressure_solve(or_state* s, double* err_out)
 /* A16: DFSPH_update_pos_kernel (DFSPH_core_cuda.cu:3127-3214) */
 void or_update_pos(or_state* s)
 {
-    for (uint32_t i = 0; i < 3 * s->n; ++i) s->x[i] += s->h * s->v[i];
+    for (uint32_t i = 0; i < 3 * s->n; ++i)
+        if (r_compute(s, i / 3)) s->x[i] += s->h * s->v[i];
 }
 
 /* DFSPHCData::onSimulationStepEnd (DFSPH_c_arrays_structure.cpp:1108-1118)
@@ -881,6 +953,7 @@ uint32_t or_check_positions(or_state* s, int remove)
             s->m[w] = s->m[i]; s->rho[w] = s->rho[i]; s->alpha[w] = s->alpha[i];
             s->radv[w] = s->radv[i]; s->kappa[w] = s->kappa[i]; s->kappaV[w] = s->kappaV[i];
             s->id[w] = s->id[i];
+            s->tag[w] = s->tag[i];
         }
         w++;
     }
@@ -913,6 +986,142 @@ int or_step(or_state* s, or_stats* st)
     loc.n_fluid = s->n;
     if (st) *st = loc;
     return ns;
+}
+
+/* ------------------------------------------------------------------ */
+/* A19: open boundary (OpenBoundariesSimple.cu) and BufferFluidSurface  */
+/* ------------------------------------------------------------------ */
+
+/* BufferFluidSurface::isinside (BufferFluidSurface.cu:775-880).  margin: the
+ * distance of the deciding quantity from its threshold (points with a tiny
+ * margin can flip between precisions). */
+int or_surface_inside(const or_surface* S, const double* p, double* margin)
+{
+    int r = 0;
+    double mg = 1e300;
+    const double vx = p[0] - S->center[0], vy = p[1] - S->center[1], vz = p[2] - S->center[2];
+    switch (S->type) {
+    case 0: {
+        const double d = (p[0] - S->o[0]) * S->n[0] + (p[1] - S->o[1]) * S->n[1] + (p[2] - S->o[2]) * S->n[2];
+        r = !(d < 0.0); mg = fabs(d); break;
+    }
+    case 1: {
+        const double a[3] = {fabs(vx), fabs(vy), fabs(vz)};
+        r = a[0] < S->half[0] && a[1] < S->half[1] && a[2] < S->half[2];
+        for (int c = 0; c < 3; ++c) mg = fmin(mg, fabs(S->half[c] - a[c]));
+        break;
+    }
+    case 2: {
+        const double l = sqrt(vx * vx + vz * vz);
+        mg = fabs(fabs(vy) - S->half[1]);
+        if (fabs(vy) > S->half[1]) { r = 0; break; }
+        r = l < S->radius; mg = fmin(mg, fabs(l - S->radius)); break;
+    }
+    case 4: {
+        const double l = sqrt(vx * vx + vy * vy + vz * vz);
+        r = l < S->radius; mg = fabs(l - S->radius); break;
+    }
+    case 5: {
+        mg = fabs(fabs(vy) - S->half[1]);
+        if (fabs(vy) > S->half[1]) { r = 0; break; }
+        const double l = sqrt(vx * vx + vz * vz);
+        mg = fmin(mg, fmin(fabs(l - S->radius_internal), fabs(l - S->radius)));
+        if (l < S->radius_internal) { r = 1; break; }
+        if (l > S->radius) { r = 0; break; }
+        /* angle to the first point folded into one period, then the border
+         * segment between the outer vertex (angle 0) and the inner one
+         * (half a period) or the reverse */
+        const double alpha = acos((vx * S->dir[0] + vz * S->dir[2]) / l);
+        const double a0 = 2.0 * M_PI / (double)S->points;
+        double k = alpha / a0;
+        while (k > 1.0) k -= 1.0;
+        const double x = k * a0;
+        double a1 = 0.0, r1 = S->radius, a2 = a0 / 2.0, r2 = S->radius_internal;
+        if (x > a0 / 2.0) { r1 = S->radius_internal; r2 = S->radius; a1 = a2; a2 = a1 + a0 / 2.0; }
+        const double x1 = r1 * cos(a1), y1 = r1 * sin(a1), x2 = r2 * cos(a2), y2 = r2 * sin(a2);
+        const double m = (y2 - y1) / (x2 - x1);
+        const double d = (-x1 * m + y1) / (-m * cos(x) + sin(x));
+        r = l <= d; mg = fmin(mg, fabs(l - d));
+        break;
+    }
+    default: r = 0;
+    }
+    if (margin) *margin = mg;
+    return S->reversed ? !r : r;
+}
+
+/* BufferFluidSurface::getNormal (BufferFluidSurface.cu:730-772) */
+static void surface_normal(const or_surface* S, const double* p, double* nrm)
+{
+    nrm[0] = nrm[1] = nrm[2] = 0.0;
+    if (S->type == 2 || S->type == 4 || S->type == 5)
+        for (int c = 0; c < 3; ++c) nrm[c] = S->center[c] - p[c];
+    if (S->reversed) for (int c = 0; c < 3; ++c) nrm[c] = -nrm[c];
+    const double l = sqrt(dot3(nrm, nrm));
+    if (l != 0.0) for (int c = 0; c < 3; ++c) nrm[c] /= l;
+}
+
+/* inflow_with_predefined_positions_kernel (OpenBoundariesSimple.cu:360-508),
+ * by brute force over the fluid and boundary (small cases).  For each
+ * candidate: nearest fluid distance within h (100 when none); when above
+ * min_dist, the (m W)-weighted fluid velocity (boundary: weight only) and
+ * the density W0 m + sum m_j W + sum psi_j W decide.  Returns the number
+ * accepted; vel (3 nc) is set for every candidate with a density estimate. */
+uint32_t or_inflow(const or_state* s, const or_surface* boundary, uint32_t nc, const double* cand, double min_dist,
+                   double max_density, double mass, uint8_t* accept, double* vel, double* dmin_out, double* dens_out)
+{
+    uint32_t cnt = 0;
+    for (uint32_t c = 0; c < nc; ++c) {
+        const double* p = cand + 3 * (size_t)c;
+        double dmin = 100.0, d[3];
+        for (uint32_t j = 0; j < s->n; ++j) {
+            sub3(p, XI(j), d);
+            const double r2 = dot3(d, d);
+            if (r2 < s->hk2) dmin = fmin(dmin, sqrt(r2));
+        }
+        double v[3] = {0, 0, 0}, dens = s->W0 * mass, sw = 0.0;
+        int ok = 0;
+        if (dmin > min_dist) {
+            for (uint32_t j = 0; j < s->n; ++j) {
+                sub3(p, XI(j), d);
+                if (dot3(d, d) < s->hk2) {
+                    const double w = kW(s, d) * s->m[j];
+                    dens += w; sw += w;
+                    for (int q = 0; q < 3; ++q) v[q] += s->v[3 * (size_t)j + q] * w;
+                }
+            }
+            for (uint32_t j = 0; j < s->nbn; ++j) {
+                sub3(p, XB(j), d);
+                if (dot3(d, d) < s->hk2) {
+                    const double w = kW(s, d) * s->psi[j];
+                    sw += w; dens += w;
+                }
+            }
+            if (sw > 0.0) for (int q = 0; q < 3; ++q) v[q] /= sw;
+            double nrm[3];
+            surface_normal(boundary, p, nrm);
+            ok = !(dens > max_density || dot3(v, nrm) < 0.0);
+        }
+        if (accept) accept[c] = (uint8_t)ok;
+        if (vel) for (int q = 0; q < 3; ++q) vel[3 * (size_t)c + q] = v[q];
+        if (dmin_out) dmin_out[c] = dmin;
+        if (dens_out) dens_out[c] = dens;
+        cnt += (uint32_t)ok;
+    }
+    return cnt;
+}
+
+/* outflow_basic_kernel (OpenBoundariesSimple.cu:517-534): keep[i] = 0 when
+ * particle i is outside both the interior and the fluid-surface shape */
+uint32_t or_outflow(const or_state* s, const or_surface* interior, const or_surface* surface, uint8_t* keep)
+{
+    uint32_t removed = 0;
+    for (uint32_t i = 0; i < s->n; ++i) {
+        const int out = !or_surface_inside(interior, XI(i), NULL) && !or_surface_inside(surface, XI(i), NULL);
+        keep[i] = (uint8_t)!out;
+        removed += (uint32_t)out;
+    }
+    return removed;
 }
 
 /* ------------------------------------------------------------------ */

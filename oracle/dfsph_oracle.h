@@ -118,6 +118,26 @@ uint32_t or_neighbors_fp32(uint32_t n_f, const float* xf, uint32_t n_b, const fl
                            float hk, int brute, uint32_t max,
                            uint32_t* nf, uint32_t* nb, uint32_t* list);
 
+/* solver restriction in GPU mode (restriction_mode, count_active,
+ * count_active_neighbors, true_particle_count, DFSPH_c_arrays_structure.h:410-418);
+ * tags (n, may be NULL) are the reference's tag values */
+int or_set_restriction(or_state* s, int mode, uint32_t count_active, uint32_t count_active_neighbors,
+                       long long true_particle_count, const uint32_t* tags);
+
+/* A19: BufferFluidSurface and the open boundary */
+typedef struct or_surface {
+    int type, reversed;       /* 0 plane, 1 cuboid, 2 cylinder, 4 sphere, 5 star */
+    double o[3], n[3], center[3], half[3];
+    double radius, radius_internal;
+    uint32_t points;
+    double dir[3];
+} or_surface;
+int or_surface_inside(const or_surface* S, const double* p, double* margin);
+uint32_t or_inflow(const or_state* s, const or_surface* boundary, uint32_t nc, const double* cand, double min_dist,
+                   double max_density, double mass, uint8_t* accept, double* vel, double* dmin_out,
+                   double* dens_out);
+uint32_t or_outflow(const or_state* s, const or_surface* interior, const or_surface* surface, uint8_t* keep);
+
 /* kernel evaluation (A3): analytic or table per flag */
 double or_kernel_W(double r, double hk, int table);
 void   or_kernel_gradW(const double* r3, double hk, int table, double* out3);

@@ -35,6 +35,31 @@ class OrParams(C.Structure):
     ]
 
 
+class OrSurface(C.Structure):
+    _fields_ = [("type", C.c_int), ("reversed", C.c_int), ("o", C.c_double * 3), ("n", C.c_double * 3),
+                ("center", C.c_double * 3), ("half", C.c_double * 3), ("radius", C.c_double),
+                ("radius_internal", C.c_double), ("points", C.c_uint32), ("dir", C.c_double * 3)]
+
+
+SURFACE_TYPES = {"plane": 0, "cuboid": 1, "cylinder": 2, "sphere": 4, "star": 5}
+
+
+def surface(kind, reversed=False, o=(0, 0, 0), n=(0, 1, 0), center=(0, 0, 0), half=(0, 0, 0), radius=0.0,
+            radius_internal=0.0, points=0, direction=(0, 0, 1)):
+    s = OrSurface()
+    s.type = SURFACE_TYPES[kind]
+    s.reversed = 1 if reversed else 0
+    s.o = (C.c_double * 3)(*o)
+    s.n = (C.c_double * 3)(*n)
+    s.center = (C.c_double * 3)(*center)
+    s.half = (C.c_double * 3)(*half)
+    s.radius = radius
+    s.radius_internal = radius_internal
+    s.points = points
+    s.dir = (C.c_double * 3)(*direction)
+    return s
+
+
 class OrStats(C.Structure):
     _fields_ = [("iters_v", C.c_uint32), ("iters", C.c_uint32), ("err_v", C.c_double), ("err", C.c_double),
                 ("dt_used", C.c_double), ("removed", C.c_uint32), ("n_fluid", C.c_uint32),
@@ -88,6 +113,12 @@ def lib():
             "or_kernel_gradW": (None, [_DP, C.c_double, C.c_int, _DP]),
             "or_mlp_forward": (None, [C.c_uint32, _DP, _DP, _DP]),
             "or_mlp_loss_grad": (C.c_double, [C.c_uint32, _DP, _DP, _DP, _DP]),
+            "or_set_restriction": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_longlong, _UP]),
+            "or_surface_inside": (C.c_int, [C.POINTER(OrSurface), _DP, _DP]),
+            "or_inflow": (C.c_uint32, [C.c_void_p, C.POINTER(OrSurface), C.c_uint32, _DP, C.c_double, C.c_double,
+                                       C.c_double, C.POINTER(C.c_uint8), _DP, _DP, _DP]),
+            "or_outflow": (C.c_uint32, [C.c_void_p, C.POINTER(OrSurface), C.POINTER(OrSurface),
+                                        C.POINTER(C.c_uint8)]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -223,6 +254,42 @@ class Oracle:
         st = OrStats()
         self.L.or_step(self.s, C.byref(st))
         return st
+
+    def set_restriction(self, mode, count_active=0, count_active_neighbors=0, true_particle_count=-1, tags=None):
+        t = None if tags is None else np.ascontiguousarray(tags, np.uint32)
+        self.L.or_set_restriction(self.s, mode, count_active, count_active_neighbors, true_particle_count,
+                                  None if t is None else t.ctypes.data_as(_UP))
+
+    def inflow(self, boundary, cand, min_dist, max_density, mass):
+        """(accept[nc] bool, vel[nc,3], nearest fluid distance[nc], density[nc])"""
+        c = _d(cand).reshape(-1)
+        nc = c.size // 3
+        acc = np.zeros(nc, np.uint8)
+        vel = np.zeros((nc, 3))
+        dmin = np.zeros(nc)
+        dens = np.zeros(nc)
+        self.L.or_inflow(self.s, C.byref(boundary), nc, c.ctypes.data_as(_DP), min_dist, max_density, mass,
+                         acc.ctypes.data_as(C.POINTER(C.c_uint8)), vel.ctypes.data_as(_DP), dmin.ctypes.data_as(_DP),
+                         dens.ctypes.data_as(_DP))
+        return acc.astype(bool), vel, dmin, dens
+
+    def outflow_keep(self, interior, surf):
+        keep = np.zeros(self.n, np.uint8)
+        self.L.or_outflow(self.s, C.byref(interior), C.byref(surf), keep.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return keep.astype(bool)
+
+
+def surface_inside(surf, xyz):
+    """(inside[n] bool, margin[n]) of or_surface_inside"""
+    x = _d(xyz).reshape(-1, 3)
+    ins = np.zeros(len(x), bool)
+    mg = np.zeros(len(x))
+    m = C.c_double()
+    for i in range(len(x)):
+        row = np.ascontiguousarray(x[i])
+        ins[i] = bool(lib().or_surface_inside(C.byref(surf), row.ctypes.data_as(_DP), C.byref(m)))
+        mg[i] = m.value
+    return ins, mg
 
 
 def neighbors_fp32(xf, xb, hk, brute=False, max_nbr=90):

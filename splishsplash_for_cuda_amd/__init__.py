@@ -9,4 +9,6 @@ from ._native import (  # noqa: F401
     FIELD_POS, FIELD_VEL, FIELD_MASS, FIELD_DENSITY, FIELD_FACTOR, FIELD_DENSITY_ADV, FIELD_KAPPA,
     FIELD_KAPPAV, FIELD_ACC, FIELD_ID, FIELD_NBR_FLUID, FIELD_NBR_BOUNDARY, FIELD_BOUNDARY_POS,
     FIELD_BOUNDARY_PSI, WARMSTART_LEARNED, WARMSTART_NONE, WARMSTART_REFERENCE,
+    LISTS_AUTO, LISTS_NO_LDS, LISTS_GLOBAL, FIELD_TAG, DFSPHSurface, surface,
+    TAG_ACTIVE, TAG_ACTIVE_NEIGHBORS, TAG_UNTAGGED, TAG_REMOVAL,
 )

@@ -27,6 +27,10 @@ WARMSTART_REFERENCE = 0
 WARMSTART_LEARNED = 1
 WARMSTART_NONE = 2
 
+LISTS_AUTO = 0
+LISTS_NO_LDS = 1
+LISTS_GLOBAL = 2
+
 FIELD_POS = 0
 FIELD_VEL = 1
 FIELD_MASS = 2
@@ -41,8 +45,9 @@ FIELD_NBR_FLUID = 10
 FIELD_NBR_BOUNDARY = 11
 FIELD_BOUNDARY_POS = 12
 FIELD_BOUNDARY_PSI = 13
+FIELD_TAG = 14
 _VECTOR_FIELDS = {FIELD_POS, FIELD_VEL, FIELD_ACC, FIELD_BOUNDARY_POS}
-_UINT_FIELDS = {FIELD_ID, FIELD_NBR_FLUID, FIELD_NBR_BOUNDARY}
+_UINT_FIELDS = {FIELD_ID, FIELD_NBR_FLUID, FIELD_NBR_BOUNDARY, FIELD_TAG}
 
 
 class DFSPHParams(C.Structure):
@@ -68,6 +73,7 @@ class DFSPHParams(C.Structure):
         ("fluid_capacity", C.c_uint32),
         ("warm_start_mode", C.c_int32),
         ("check_interval", C.c_uint32),
+        ("list_mode", C.c_int32),
     ]
 
 
@@ -127,6 +133,45 @@ class DFSPHCudaStats(C.Structure):
 
 
 # every function declared in include/dfsph_hip.h, with its ctypes signature
+class DFSPHSurface(C.Structure):
+    """dfsph_surface: a BufferFluidSurface (plane, cuboid, cylinder, sphere, star)."""
+    _fields_ = [
+        ("type", C.c_int32),
+        ("reversed", C.c_int32),
+        ("o", C.c_float * 3),
+        ("n", C.c_float * 3),
+        ("center", C.c_float * 3),
+        ("half", C.c_float * 3),
+        ("radius", C.c_float),
+        ("radius_internal", C.c_float),
+        ("points", C.c_uint32),
+        ("direction", C.c_float * 3),
+    ]
+
+
+SURFACE_PLANE, SURFACE_CUBOID, SURFACE_CYLINDER, SURFACE_SPHERE, SURFACE_STAR = 0, 1, 2, 4, 5
+TAG_ACTIVE, TAG_ACTIVE_NEIGHBORS, TAG_UNTAGGED, TAG_REMOVAL = 10000000, 20000000, 40000000, 310000000
+
+
+def surface(kind, reversed=False, o=(0, 0, 0), n=(0, 1, 0), center=(0, 0, 0), half=(0, 0, 0), radius=0.0,
+            radius_internal=0.0, points=0, direction=(0, 0, 1)):
+    """A DFSPHSurface; kind is 'plane', 'cuboid', 'cylinder', 'sphere' or 'star'
+    (cylinder / star: half height in half[1])."""
+    s = DFSPHSurface()
+    s.type = {"plane": SURFACE_PLANE, "cuboid": SURFACE_CUBOID, "cylinder": SURFACE_CYLINDER,
+              "sphere": SURFACE_SPHERE, "star": SURFACE_STAR}[kind]
+    s.reversed = 1 if reversed else 0
+    s.o = (C.c_float * 3)(*o)
+    s.n = (C.c_float * 3)(*n)
+    s.center = (C.c_float * 3)(*center)
+    s.half = (C.c_float * 3)(*half)
+    s.radius = radius
+    s.radius_internal = radius_internal
+    s.points = points
+    s.direction = (C.c_float * 3)(*direction)
+    return s
+
+
 _P = C.c_void_p
 _F32P = C.POINTER(C.c_float)
 _U32P = C.POINTER(C.c_uint32)
@@ -136,6 +181,15 @@ _U32 = C.c_uint32
 _I = C.c_int
 
 HIP_SIGNATURES = {
+    "dfsph_set_restriction": (_I, [_P, _I, _U32, _U32, C.c_int64]),
+    "dfsph_tag_neighbors": (_I, [_P, _U32, _U32, _I, _U32P]),
+    "dfsph_count_tagged": (_I, [_P, _U32, _U32P]),
+    "dfsph_order_active_first": (_I, [_P, _U32P, _U32P]),
+    "dfsph_remove_tagged": (_I, [_P, _U32, _U32P]),
+    "dfsph_surface_inside": (_I, [_P, C.POINTER(DFSPHSurface), _F32P, _U32, _U8P]),
+    "dfsph_open_boundary_init": (_I, [_P, C.POINTER(DFSPHSurface), C.POINTER(DFSPHSurface),
+                                      C.POINTER(DFSPHSurface), _F32P, _U32, _U32P]),
+    "dfsph_open_boundary_apply": (_I, [_P, C.POINTER(C.c_double), C.c_float, C.c_float, _I, _I, _U32P, _U32P]),
     "dfsph_default_params": (None, [C.POINTER(DFSPHParams)]),
     "dfsph_last_error": (C.c_char_p, []),
     "dfsph_create": (_I, [C.POINTER(DFSPHParams), C.POINTER(C.c_void_p)]),
@@ -315,7 +369,7 @@ class Solver:
                                             _ptr(i, C.c_uint32), x.size // 3), "dfsph_add_particles")
 
     def upload_field(self, field, values):
-        a = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
+        a = np.ascontiguousarray(values, dtype=np.uint32 if field in _UINT_FIELDS else np.float32).reshape(-1)
         _check(self.lib.dfsph_upload_field(self.h, field, a.ctypes.data_as(C.c_void_p), a.size), "dfsph_upload_field")
 
     # ---- the step's phases ----
@@ -377,6 +431,57 @@ class Solver:
         n = C.c_uint32()
         _check(self.lib.dfsph_remove_particles(self.h, _ptr(k, C.c_uint8), C.byref(n)), "dfsph_remove_particles")
         return n.value
+
+    # ---- dynamic window / open boundary ----
+    def set_restriction(self, mode, count_active=0, count_active_neighbors=0, true_particle_count=-1):
+        _check(self.lib.dfsph_set_restriction(self.h, mode, count_active, count_active_neighbors,
+                                              true_particle_count), "dfsph_set_restriction")
+
+    def tag_neighbors(self, tag_in, tag_out, untagged_only=True):
+        c = C.c_uint32()
+        _check(self.lib.dfsph_tag_neighbors(self.h, tag_in, tag_out, 1 if untagged_only else 0, C.byref(c)),
+               "dfsph_tag_neighbors")
+        return c.value
+
+    def count_tagged(self, tag):
+        c = C.c_uint32()
+        _check(self.lib.dfsph_count_tagged(self.h, tag, C.byref(c)), "dfsph_count_tagged")
+        return c.value
+
+    def order_active_first(self):
+        a, b = C.c_uint32(), C.c_uint32()
+        _check(self.lib.dfsph_order_active_first(self.h, C.byref(a), C.byref(b)), "dfsph_order_active_first")
+        return a.value, b.value
+
+    def remove_tagged(self, tag):
+        c = C.c_uint32()
+        _check(self.lib.dfsph_remove_tagged(self.h, tag, C.byref(c)), "dfsph_remove_tagged")
+        return c.value
+
+    def surface_inside(self, surf, xyz):
+        x = _f32(xyz).reshape(-1)
+        n = x.size // 3
+        out = np.zeros(n, np.uint8)
+        _check(self.lib.dfsph_surface_inside(self.h, C.byref(surf), _ptr(x, C.c_float), n, _ptr(out, C.c_uint8)),
+               "dfsph_surface_inside")
+        return out.astype(bool)
+
+    def open_boundary_init(self, boundary, interior, fluid_surface, inflow_xyz):
+        x = _f32(inflow_xyz).reshape(-1)
+        k = C.c_uint32()
+        _check(self.lib.dfsph_open_boundary_init(self.h, C.byref(boundary), C.byref(interior), C.byref(fluid_surface),
+                                                 _ptr(x, C.c_float), x.size // 3, C.byref(k)),
+               "dfsph_open_boundary_init")
+        return k.value
+
+    def open_boundary_apply(self, allowed_new_distance, allowed_new_density, use_inflow=True, use_outflow=True,
+                            window_displacement=None):
+        a, r = C.c_uint32(), C.c_uint32()
+        d = None if window_displacement is None else (C.c_double * 3)(*window_displacement)
+        _check(self.lib.dfsph_open_boundary_apply(self.h, d, allowed_new_distance, allowed_new_density,
+                                                  1 if use_inflow else 0, 1 if use_outflow else 0,
+                                                  C.byref(a), C.byref(r)), "dfsph_open_boundary_apply")
+        return a.value, r.value
 
     def step(self, raise_on_overflow=True):
         st = DFSPHStepStats()

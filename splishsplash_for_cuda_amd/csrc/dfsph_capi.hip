@@ -62,13 +62,27 @@ struct Handle {
     float* ktot[2] = {nullptr, nullptr};      // step totals (kappa h_f^2), learned warm start I/O
     float* kvtot[2] = {nullptr, nullptr};     // step totals (kappaV h)
     uint32_t* id[2] = {nullptr, nullptr};
+    uint32_t* tag[2] = {nullptr, nullptr};    // dynamic-window / open-boundary tags
+    // solver restriction (DFSPHCData::restriction_mode, count_active,
+    // count_active_neighbors, true_particle_count)
+    int rmode = 0;
+    uint32_t r_active = 0, r_active_nbrs = 0;
+    int64_t true_count = -1;
+    // open boundary: surfaces, inflow positions, applied window displacement
+    bool ob_ready = false;
+    dfsph::Surface ob_boundary{}, ob_interior{}, ob_surface{};
+    float4* ob_cand = nullptr;
+    uint32_t ob_n = 0;
+    double ob_disp[3] = {0.0, 0.0, 0.0};
+    uint32_t next_id = 0;             // first id not yet used by any particle
     int pc = 0;                       // current buffer of pos / kappa / kappaV / id
     float4* vel[2] = {nullptr, nullptr};
     int vc = 0;                       // current velocity buffer
     float *rho = nullptr, *alpha = nullptr, *radv = nullptr, *kk = nullptr;
     float4* acc = nullptr;
-    uint32_t *cnt = nullptr, *nbr = nullptr;
-    uint16_t* nbr16 = nullptr;          // staged fluid lists
+    uint32_t* cnt = nullptr;
+    uint16_t* nbr16 = nullptr;          // 16-bit fluid lists (blocks with a neighbor set)
+    uint32_t *nbrf = nullptr, *nbrb = nullptr;   // 32-bit fluid / boundary lists
     uint32_t *stage_n = nullptr, *stg = nullptr;   // per-block staged sets
     // sort / scan scratch
     uint32_t *key = nullptr, *slot = nullptr, *perm = nullptr, *hist = nullptr, *cstart = nullptr;
@@ -173,11 +187,11 @@ Handle::~Handle()
 {
     if (s) (void)hipStreamSynchronize(s);
     for (int b = 0; b < 2; ++b) {
-        dfree(pos[b]); dfree(kap[b]); dfree(kapv[b]); dfree(id[b]); dfree(vel[b]);
+        dfree(pos[b]); dfree(kap[b]); dfree(kapv[b]); dfree(id[b]); dfree(tag[b]); dfree(vel[b]);
         dfree(ktot[b]); dfree(kvtot[b]);
         if (ev[b]) (void)hipEventDestroy(ev[b]);
     }
-    dfree(rho); dfree(alpha); dfree(radv); dfree(kk); dfree(acc); dfree(cnt); dfree(nbr); dfree(nbr16); dfree(stage_n); dfree(stg);
+    dfree(rho); dfree(alpha); dfree(radv); dfree(kk); dfree(acc); dfree(cnt); dfree(nbrf); dfree(nbrb); dfree(nbr16); dfree(stage_n); dfree(stg); dfree(ob_cand);
     dfree(key); dfree(slot); dfree(perm); dfree(hist); dfree(cstart); dfree(keep); dfree(rank);
     dfree(keep8); dfree(scan_tmp); dfree(bpos); dfree(bcstart);
     dfree(ctl); dfree(tm); dfree(partial); dfree(partial_f);
@@ -204,9 +218,14 @@ int dalloc(T*& p, size_t count)
 
 Handle* H(void* h) { return static_cast<Handle*>(h); }
 
+dfsph::Restrict restriction(const Handle* h)
+{
+    return dfsph::Restrict{h->rmode, h->r_active, h->r_active + h->r_active_nbrs, h->tag[h->pc]};
+}
+
 dfsph::NbrLists lists(const Handle* h)
 {
-    return dfsph::NbrLists{h->nbr, h->nbr16, h->cnt, h->stage_n, h->stg};
+    return dfsph::NbrLists{h->nbr16, h->nbrf, h->nbrb, h->cnt, h->stage_n, h->stg, restriction(h)};
 }
 
 int ensure_scan(Handle* h, uint32_t n)
@@ -222,8 +241,8 @@ int ensure_scan(Handle* h, uint32_t n)
 int ensure_tmp(Handle* h, uint32_t n)
 {
     if (n <= h->tmp_cap && h->key) return DFSPH_OK;
-    CHK(dalloc(h->key, n));
-    CHK(dalloc(h->slot, n));
+    CHK(dalloc(h->key, (size_t)n + 1));     // n + 1: also class flags / ranks with a sentinel
+    CHK(dalloc(h->slot, (size_t)n + 1));
     CHK(dalloc(h->perm, n));
     CHK(dalloc(h->keep, (size_t)n + 1));
     CHK(dalloc(h->rank, (size_t)n + 1));
@@ -259,13 +278,13 @@ int counting_sort(Handle* h, const float4* pts, uint32_t n, uint32_t* cs)
 
 dfsph::FluidPtrs fluid_ptrs(Handle* h, int b, int vb)
 {
-    return dfsph::FluidPtrs{h->pos[b], h->vel[vb], h->kap[b], h->kapv[b], h->ktot[b], h->kvtot[b], h->id[b]};
+    return dfsph::FluidPtrs{h->pos[b], h->vel[vb], h->kap[b], h->kapv[b], h->ktot[b], h->kvtot[b], h->id[b], h->tag[b]};
 }
 
 int flush_vel(Handle* h)
 {
     if (!h->pend_vel) return DFSPH_OK;
-    dfsph::launch_update_vel(h->s, h->tm, h->n, h->acc, h->vel[h->vc], h->kap[h->pc]);
+    dfsph::launch_update_vel(h->s, h->tm, h->n, h->acc, h->vel[h->vc], h->kap[h->pc], restriction(h));
     h->pend_vel = false;
     HIPCHK(hipGetLastError());
     return DFSPH_OK;
@@ -302,7 +321,7 @@ int neighbor_search(Handle* h, bool sort)
     pend(h, ps);
     const int pn = pbeg(h, DFSPH_KERNEL_NEIGHBORS);
     dfsph::launch_neighbors(h->s, h->c, !sort, h->pos[h->pc], h->cstart, h->perm, h->bpos, h->bcstart, h->n,
-                            h->nbr, h->nbr16, h->cnt, h->stage_n, h->stg, h->ctl);
+                            h->nbr16, h->nbrf, h->nbrb, h->cnt, h->stage_n, h->stg, restriction(h), h->ctl);
     pend(h, pn);
     HIPCHK(hipGetLastError());
     h->lists_valid = true;
@@ -317,7 +336,9 @@ int jacobi_loop(Handle* h, bool div, uint32_t max_iter, double max_error, uint32
 {
     SolveCtl* sc = div ? &h->ctl->div : &h->ctl->dens;
     const uint32_t min_iter = div ? h->p.min_iter_v : h->p.min_iter;
-    dfsph::launch_ctl_init(h->s, sc, max_error * 0.01 * h->p.density0, div ? 1 : 0, h->tm, (double)h->n,
+    // error average over true_particle_count when set (DFSPH_core_cuda.cu:791-795, 1339-1343)
+    const double n_div = h->true_count >= 0 ? (double)h->true_count : (double)h->n;
+    dfsph::launch_ctl_init(h->s, sc, max_error * 0.01 * h->p.density0, div ? 1 : 0, h->tm, n_div,
                            min_iter, max_iter);
     if (h->n == 0 || max_iter == 0) {
         if (iters) *iters = 0;
@@ -533,6 +554,8 @@ int alloc_fluid(Handle* h, uint32_t cap)
         CHK(dalloc(h->ktot[b], cap));
         CHK(dalloc(h->kvtot[b], cap));
         CHK(dalloc(h->id[b], cap));
+        CHK(dalloc(h->tag[b], cap));
+        dfsph::launch_fill_u32(h->s, h->tag[b], dfsph::kTagUntagged, cap);
     }
     CHK(dalloc(h->rho, cap));
     CHK(dalloc(h->alpha, cap));
@@ -540,17 +563,21 @@ int alloc_fluid(Handle* h, uint32_t cap)
     CHK(dalloc(h->kk, cap));
     CHK(dalloc(h->acc, cap));
     CHK(dalloc(h->cnt, cap));
-    // kNbrSlack extra slots: the sweeps load list entries in batches past a
-    // particle's count (masked); zeroed so those reads see valid indices
-    const size_t nslots = (size_t)cap * (h->p.max_neighbors + dfsph::kNbrSlack);
-    CHK(dalloc(h->nbr, nslots));
-    HIPCHK(hipMemsetAsync(h->nbr, 0, sizeof(uint32_t) * nslots, h->s));
-    CHK(dalloc(h->nbr16, nslots));
-    HIPCHK(hipMemsetAsync(h->nbr16, 0, sizeof(uint16_t) * nslots, h->s));
-    // staged sets: one per block of the XCD-padded grid
+    // blocked-interleaved lists: 16-byte groups, with room for the last batch
+    // of 8 a sweep reads; zeroed so unread slots hold valid indices
+    const uint32_t mx = h->p.max_neighbors;
+    const size_t n16 = (size_t)cap * dfsph::list_groups16(mx) * dfsph::kGroup16;
+    const size_t n32 = (size_t)cap * dfsph::list_groups32(mx) * dfsph::kGroup32;
+    CHK(dalloc(h->nbr16, n16));
+    CHK(dalloc(h->nbrf, n32));
+    CHK(dalloc(h->nbrb, n32));
+    HIPCHK(hipMemsetAsync(h->nbr16, 0, sizeof(uint16_t) * n16, h->s));
+    HIPCHK(hipMemsetAsync(h->nbrf, 0, sizeof(uint32_t) * n32, h->s));
+    HIPCHK(hipMemsetAsync(h->nbrb, 0, sizeof(uint32_t) * n32, h->s));
+    // neighbor sets: one per block of the XCD-padded grid
     const uint32_t nblocks = dfsph::grid_blocks_xcd(cap);
     CHK(dalloc(h->stage_n, nblocks));
-    CHK(dalloc(h->stg, (size_t)nblocks * dfsph::kStageMax));
+    CHK(dalloc(h->stg, (size_t)nblocks * dfsph::kStageCap));
     CHK(ensure_tmp(h, std::max(cap, h->nb)));
     CHK(ensure_partials(h, cap));
     h->cap = cap;
@@ -563,6 +590,13 @@ int alloc_fluid(Handle* h, uint32_t cap)
         HIPCHK(hipMemsetAsync(h->kvtot[b], 0, sizeof(float) * cap, h->s));
     }
     HIPCHK(hipMemsetAsync(h->cnt, 0, sizeof(uint32_t) * cap, h->s));
+    // per-step fields start at zero: a restricted-out particle's stored
+    // rho_adv enters the error sums before it was ever computed
+    HIPCHK(hipMemsetAsync(h->rho, 0, sizeof(float) * cap, h->s));
+    HIPCHK(hipMemsetAsync(h->alpha, 0, sizeof(float) * cap, h->s));
+    HIPCHK(hipMemsetAsync(h->radv, 0, sizeof(float) * cap, h->s));
+    HIPCHK(hipMemsetAsync(h->kk, 0, sizeof(float) * cap, h->s));
+    HIPCHK(hipMemsetAsync(h->acc, 0, sizeof(float4) * cap, h->s));
     return DFSPH_OK;
 }
 
@@ -578,10 +612,12 @@ int write_fluid_range(Handle* h, uint32_t at, const float* xyz, const float* vel
         P[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], mass ? mass[i] : m0);
         V[i] = vel ? make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0f) : make_float4(0, 0, 0, 0);
         I[i] = id ? id[i] : at + i;
+        h->next_id = std::max(h->next_id, I[i] + 1u);
     }
     HIPCHK(hipMemcpyAsync(h->pos[h->pc] + at, P.data(), sizeof(float4) * m, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipMemcpyAsync(h->vel[h->vc] + at, V.data(), sizeof(float4) * m, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipMemcpyAsync(h->id[h->pc] + at, I.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, h->s));
+    dfsph::launch_fill_u32(h->s, h->tag[h->pc] + at, dfsph::kTagUntagged, m);
     HIPCHK(hipMemcpyAsync(h->kap[h->pc] + at, Z.data(), sizeof(float) * m, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipMemcpyAsync(h->kapv[h->pc] + at, Z.data(), sizeof(float) * m, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipMemcpyAsync(h->ktot[h->pc] + at, Z.data(), sizeof(float) * m, hipMemcpyHostToDevice, h->s));
@@ -631,6 +667,7 @@ void dfsph_default_params(dfsph_params* p)
     p->fluid_capacity = 0;
     p->warm_start_mode = DFSPH_WARMSTART_REFERENCE;
     p->check_interval = 4;
+    p->list_mode = DFSPH_LISTS_AUTO;
 }
 
 const char* dfsph_last_error(void) { return g_err.c_str(); }
@@ -642,6 +679,7 @@ int dfsph_create(const dfsph_params* p, void** out)
         return fail(DFSPH_ERR_ARG, "particle_radius, density0 and dt must be positive");
     if (p->max_neighbors == 0 || p->max_neighbors > 255) return fail(DFSPH_ERR_ARG, "max_neighbors must be in [1, 255]");
     if (p->warm_start_mode < 0 || p->warm_start_mode > 2) return fail(DFSPH_ERR_ARG, "bad warm_start_mode");
+    if (p->list_mode < 0 || p->list_mode > 2) return fail(DFSPH_ERR_ARG, "bad list_mode");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(DFSPH_ERR_HIP, "no HIP device");
     Handle* h = new Handle();
@@ -679,6 +717,8 @@ int dfsph_create(const dfsph_params* p, void** out)
         c.gz = (float)h->p.gravity[2];
         c.viscosity = (float)h->p.viscosity;
         c.max_nbr = h->p.max_neighbors;
+        c.stage_lds = h->p.list_mode == DFSPH_LISTS_AUTO ? dfsph::kStageMax : 0u;
+        c.stage_cap = h->p.list_mode == DFSPH_LISTS_GLOBAL ? 0u : dfsph::kStageCap;
         if (hipStreamSynchronize(h->s) != hipSuccess) { rc = fail(DFSPH_ERR_HIP, "sync"); break; }
     } while (0);
     if (rc != DFSPH_OK) { delete h; return rc; }
@@ -831,7 +871,8 @@ int dfsph_update_pos(void* hv)
     Handle* h = H(hv);
     CHK(flush_vel(h));
     const int pr = pbeg(h, DFSPH_KERNEL_UPDATE_POS);
-    dfsph::launch_update_pos(h->s, h->tm, h->n, h->vel[h->vc], h->pos[h->pc], h->bmin, h->bmax, h->ctl);
+    dfsph::launch_update_pos(h->s, h->tm, h->n, h->vel[h->vc], h->pos[h->pc], h->bmin, h->bmax, h->ctl,
+                             restriction(h));
     pend(h, pr);
     HIPCHK(hipGetLastError());
     h->lists_valid = false;
@@ -897,7 +938,7 @@ int dfsph_step(void* hv, dfsph_step_stats* st)
     Handle* h = H(hv);
     if (!h->grid_ready) return fail(DFSPH_ERR_STATE, "no boundary set uploaded");
     dfsph_step_stats loc{};
-    CHK(neighbor_search(h, true));
+    CHK(neighbor_search(h, h->rmode != 2));    // mode 2 needs the active prefix in place
     if (h->p.enable_divergence) CHK(divergence_solve(h, h->p.max_iter_v, h->p.max_error_v, &loc.iters_v, &loc.err_v));
     else CHK(density_only(h));
     CHK(external_forces(h));
@@ -965,6 +1006,7 @@ int dfsph_download(void* hv, int field, void* dst, uint32_t count)
     case DFSPH_FIELD_KAPPA: return get_raw(h->kap[h->pc], 4ull * n);
     case DFSPH_FIELD_KAPPAV: return get_raw(h->kapv[h->pc], 4ull * n);
     case DFSPH_FIELD_ID: return get_raw(h->id[h->pc], 4ull * n);
+    case DFSPH_FIELD_TAG: return get_raw(h->tag[h->pc], 4ull * n);
     case DFSPH_FIELD_NBR_FLUID:
     case DFSPH_FIELD_NBR_BOUNDARY: {
         std::vector<uint32_t> c(n);
@@ -1004,6 +1046,7 @@ int dfsph_upload_field(void* hv, int field, const void* src, uint32_t count)
     }
     if (field == DFSPH_FIELD_KAPPA) { HIPCHK(hipMemcpy(h->kap[h->pc], src, 4ull * n, hipMemcpyHostToDevice)); return DFSPH_OK; }
     if (field == DFSPH_FIELD_KAPPAV) { HIPCHK(hipMemcpy(h->kapv[h->pc], src, 4ull * n, hipMemcpyHostToDevice)); return DFSPH_OK; }
+    if (field == DFSPH_FIELD_TAG) { HIPCHK(hipMemcpy(h->tag[h->pc], src, 4ull * n, hipMemcpyHostToDevice)); return DFSPH_OK; }
     return fail(DFSPH_ERR_ARG, "field cannot be uploaded");
 }
 
@@ -1070,6 +1113,268 @@ int dfsph_get_info(void* hv, dfsph_info* info)
 }
 
 int dfsph_synchronize(void* hv) { return sync(H(hv)); }
+
+// ---- dynamic window: restriction and tags ----
+int dfsph_set_restriction(void* hv, int mode, uint32_t count_active, uint32_t count_active_neighbors,
+                          int64_t true_particle_count)
+{
+    Handle* h = H(hv);
+    if (mode < 0 || mode > 2) return fail(DFSPH_ERR_ARG, "restriction mode must be 0, 1 or 2");
+    if (mode == 2 && (uint64_t)count_active + count_active_neighbors > h->n)
+        return fail(DFSPH_ERR_ARG, "restriction prefix beyond the active particles");
+    CHK(flush_vel(h));
+    h->rmode = mode;
+    h->r_active = count_active;
+    h->r_active_nbrs = count_active_neighbors;
+    h->true_count = true_particle_count;
+    h->lists_valid = false;             // lists are built for the restricted set only
+    return DFSPH_OK;
+}
+
+int dfsph_count_tagged(void* hv, uint32_t tag, uint32_t* count)
+{
+    Handle* h = H(hv);
+    if (!count) return fail(DFSPH_ERR_ARG, "null count");
+    CHK(flush_vel(h));
+    dfsph::launch_fill_u32(h->s, &h->ctl->pad, 0u, 1);
+    dfsph::launch_count_tag(h->s, h->tag[h->pc], h->n, tag, &h->ctl->pad);
+    HIPCHK(hipMemcpyAsync(&h->hctl->pad, &h->ctl->pad, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s));
+    CHK(sync(h));
+    *count = h->hctl->pad;
+    return DFSPH_OK;
+}
+
+int dfsph_tag_neighbors(void* hv, uint32_t tag_in, uint32_t tag_out, int untagged_only, uint32_t* changed)
+{
+    Handle* h = H(hv);
+    if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale: call dfsph_neighbor_search");
+    CHK(flush_vel(h));
+    const uint32_t n = h->n;
+    if (n == 0) { if (changed) *changed = 0; return DFSPH_OK; }
+    // snapshot of the tags in the spare buffer, then mark into the live array
+    uint32_t* snap = h->tag[h->pc ^ 1];
+    HIPCHK(hipMemcpyAsync(snap, h->tag[h->pc], 4ull * n, hipMemcpyDeviceToDevice, h->s));
+    dfsph::launch_tag_neighbors(h->s, h->c, n, lists(h), snap, h->tag[h->pc], tag_in, tag_out, untagged_only,
+                                dfsph::grid_blocks_xcd(n));
+    dfsph::launch_fill_u32(h->s, &h->ctl->pad, 0u, 1);
+    dfsph::launch_count_changed(h->s, snap, h->tag[h->pc], n, &h->ctl->pad);
+    HIPCHK(hipMemcpyAsync(&h->hctl->pad, &h->ctl->pad, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s));
+    CHK(sync(h));
+    if (changed) *changed = h->hctl->pad;
+    return DFSPH_OK;
+}
+
+int dfsph_order_active_first(void* hv, uint32_t* count_active, uint32_t* count_active_neighbors)
+{
+    Handle* h = H(hv);
+    CHK(flush_vel(h));
+    const uint32_t n = h->n;
+    // f0 / r0 in keep / rank, f1 / r1 in the sort scratch (key / slot)
+    dfsph::launch_tag_classes(h->s, h->tag[h->pc], n, h->keep, h->key);
+    dfsph::launch_exclusive_scan(h->s, h->keep, h->rank, n + 1, h->scan_tmp);
+    dfsph::launch_exclusive_scan(h->s, h->key, h->slot, n + 1, h->scan_tmp);
+    const int o = h->pc ^ 1, vo = h->vc ^ 1;
+    dfsph::launch_partition3(h->s, n, h->keep, h->rank, h->key, h->slot, fluid_ptrs(h, h->pc, h->vc),
+                             fluid_ptrs(h, o, vo));
+    uint32_t c2[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&h->hctl->pad, h->rank + n, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s));
+    CHK(sync(h));
+    c2[0] = h->hctl->pad;
+    HIPCHK(hipMemcpyAsync(&h->hctl->pad, h->slot + n, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s));
+    CHK(sync(h));
+    c2[1] = h->hctl->pad;
+    h->pc = o;
+    h->vc = vo;
+    h->lists_valid = false;
+    h->acc_valid = false;
+    h->pred_valid = false;
+    if (count_active) *count_active = c2[0];
+    if (count_active_neighbors) *count_active_neighbors = c2[1];
+    return DFSPH_OK;
+}
+
+int dfsph_remove_tagged(void* hv, uint32_t tag, uint32_t* removed)
+{
+    Handle* h = H(hv);
+    CHK(flush_vel(h));
+    dfsph::launch_keep_not_tag(h->s, h->tag[h->pc], h->n, tag, h->keep);
+    return compact(h, removed);
+}
+
+// ---- open boundary ----
+static dfsph::Surface to_surface(const dfsph_surface& a)
+{
+    dfsph::Surface S{};
+    S.type = a.type;
+    S.reversed = a.reversed;
+    for (int k = 0; k < 3; ++k) {
+        S.o[k] = a.o[k]; S.n[k] = a.n[k]; S.center[k] = a.center[k]; S.half[k] = a.half[k]; S.dir[k] = a.direction[k];
+    }
+    S.radius = a.radius;
+    S.radius_internal = a.radius_internal;
+    S.points = a.points;
+    return S;
+}
+
+static int check_surface(const dfsph_surface* a)
+{
+    if (!a) return fail(DFSPH_ERR_ARG, "null surface");
+    const int t = a->type;
+    if (t != DFSPH_SURFACE_PLANE && t != DFSPH_SURFACE_CUBOID && t != DFSPH_SURFACE_CYLINDER &&
+        t != DFSPH_SURFACE_SPHERE && t != DFSPH_SURFACE_STAR)
+        return fail(DFSPH_ERR_ARG, "unknown surface type (meshes are not supported)");
+    if (t == DFSPH_SURFACE_STAR && a->points == 0) return fail(DFSPH_ERR_ARG, "star surface needs points > 0");
+    return DFSPH_OK;
+}
+
+int dfsph_surface_inside(void* hv, const dfsph_surface* s, const float* xyz, uint32_t n, uint8_t* inside)
+{
+    Handle* h = H(hv);
+    CHK(check_surface(s));
+    if (n == 0) return DFSPH_OK;
+    if (!xyz || !inside) return fail(DFSPH_ERR_ARG, "null points");
+    std::vector<float4> P(n);
+    for (uint32_t i = 0; i < n; ++i) P[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.0f);
+    float4* d = nullptr;
+    uint32_t* k = nullptr;
+    CHK(dalloc(d, n));
+    int rc = dalloc(k, n + 1);
+    std::vector<uint32_t> K(n);
+    if (rc == DFSPH_OK) {
+        if (hipMemcpyAsync(d, P.data(), sizeof(float4) * n, hipMemcpyHostToDevice, h->s) != hipSuccess)
+            rc = fail(DFSPH_ERR_HIP, "hipMemcpy");
+        if (rc == DFSPH_OK) {
+            dfsph::launch_surface_keep(h->s, d, n, to_surface(*s), 1, k);
+            if (hipMemcpyAsync(K.data(), k, 4ull * n, hipMemcpyDeviceToHost, h->s) != hipSuccess)
+                rc = fail(DFSPH_ERR_HIP, "hipMemcpy");
+        }
+        if (rc == DFSPH_OK) rc = sync(h);
+    }
+    dfree(d);
+    dfree(k);
+    if (rc != DFSPH_OK) return rc;
+    for (uint32_t i = 0; i < n; ++i) inside[i] = (uint8_t)K[i];
+    return DFSPH_OK;
+}
+
+int dfsph_open_boundary_init(void* hv, const dfsph_surface* boundary, const dfsph_surface* interior,
+                             const dfsph_surface* fluid_surface, const float* inflow_xyz, uint32_t n, uint32_t* kept)
+{
+    Handle* h = H(hv);
+    CHK(check_surface(boundary));
+    CHK(check_surface(interior));
+    CHK(check_surface(fluid_surface));
+    if (n && !inflow_xyz) return fail(DFSPH_ERR_ARG, "null inflow positions");
+    h->ob_boundary = to_surface(*boundary);
+    h->ob_interior = to_surface(*interior);
+    h->ob_surface = to_surface(*fluid_surface);
+    for (int a = 0; a < 3; ++a) h->ob_disp[a] = 0.0;
+    h->ob_n = 0;
+    dfree(h->ob_cand);
+    if (n) {
+        std::vector<float4> P(n);
+        for (uint32_t i = 0; i < n; ++i) P[i] = make_float4(inflow_xyz[3 * i], inflow_xyz[3 * i + 1], inflow_xyz[3 * i + 2], 0.0f);
+        float4* raw = nullptr;
+        uint32_t *k = nullptr, *r = nullptr, *tmp = nullptr;
+        CHK(dalloc(raw, n));
+        CHK(dalloc(h->ob_cand, n));
+        CHK(dalloc(k, n + 1));
+        CHK(dalloc(r, n + 1));
+        CHK(dalloc(tmp, dfsph::scan_temp_words(n + 1)));
+        HIPCHK(hipMemcpyAsync(raw, P.data(), sizeof(float4) * n, hipMemcpyHostToDevice, h->s));
+        // keep the positions outside the interior (the reference tags those
+        // outside the reversed interior, OpenBoundariesSimple.cu:300-318) and
+        // inside the fluid surface (:320-326), in their order
+        dfsph::Surface rev = h->ob_interior;
+        rev.reversed = !rev.reversed;
+        dfsph::launch_surface_keep(h->s, raw, n, rev, 1, k);
+        dfsph::launch_surface_keep_and(h->s, raw, n, h->ob_surface, 1, k);
+        dfsph::launch_exclusive_scan(h->s, k, r, n + 1, tmp);
+        dfsph::launch_compact_f4(h->s, n, k, r, raw, h->ob_cand);
+        HIPCHK(hipMemcpyAsync(&h->hctl->pad, r + n, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s));
+        CHK(sync(h));
+        h->ob_n = h->hctl->pad;
+        dfree(raw);
+        dfree(k);
+        dfree(r);
+        dfree(tmp);
+    }
+    h->ob_ready = true;
+    if (kept) *kept = h->ob_n;
+    return DFSPH_OK;
+}
+
+int dfsph_open_boundary_apply(void* hv, const double* window_displacement, float allowed_new_distance,
+                              float allowed_new_density, int use_inflow, int use_outflow, uint32_t* added,
+                              uint32_t* removed)
+{
+    Handle* h = H(hv);
+    if (!h->ob_ready) return fail(DFSPH_ERR_STATE, "dfsph_open_boundary_init was not called");
+    if (added) *added = 0;
+    if (removed) *removed = 0;
+    CHK(flush_vel(h));
+    // follow the dynamic window (OpenBoundariesSimple.cu:553-584)
+    if (window_displacement) {
+        double d[3], sq = 0.0;
+        for (int a = 0; a < 3; ++a) { d[a] = window_displacement[a] - h->ob_disp[a]; sq += d[a] * d[a]; }
+        if (sq > 1e-6) {
+            for (dfsph::Surface* S : {&h->ob_boundary, &h->ob_interior, &h->ob_surface})
+                for (int a = 0; a < 3; ++a) { S->o[a] += (float)d[a]; S->center[a] += (float)d[a]; }
+            dfsph::launch_apply_delta(h->s, h->ob_cand, h->ob_n, (float)d[0], (float)d[1], (float)d[2]);
+            for (int a = 0; a < 3; ++a) h->ob_disp[a] = window_displacement[a];
+        }
+    }
+    if (use_inflow) {
+        if (!(allowed_new_distance > 0.0f) || !(allowed_new_density > 0.0f))
+            return fail(DFSPH_ERR_ARG, "inflow needs a positive distance and density");
+        const uint32_t n = h->n, nc = h->ob_n;
+        if (nc && n) {
+            // cell table of the current positions without reordering
+            CHK(counting_sort(h, h->pos[h->pc], n, h->cstart));
+            uint32_t *k = nullptr, *r = nullptr, *tmp = nullptr;
+            float4* vnew = nullptr;
+            CHK(dalloc(k, nc + 1));
+            CHK(dalloc(r, nc + 1));
+            CHK(dalloc(tmp, dfsph::scan_temp_words(nc + 1)));
+            CHK(dalloc(vnew, nc));
+            float mass = 0.0f;
+            float4 p0;
+            HIPCHK(hipMemcpyAsync(&p0, h->pos[h->pc], sizeof(float4), hipMemcpyDeviceToHost, h->s));
+            CHK(sync(h));
+            mass = p0.w;
+            dfsph::launch_inflow(h->s, h->c, h->ob_cand, nc, h->pos[h->pc], h->vel[h->vc], h->cstart, h->perm,
+                                 h->bpos, h->bcstart, h->ob_boundary, allowed_new_distance, allowed_new_density,
+                                 mass, k, vnew);
+            dfsph::launch_exclusive_scan(h->s, k, r, nc + 1, tmp);
+            HIPCHK(hipMemcpyAsync(&h->hctl->pad, r + nc, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s));
+            CHK(sync(h));
+            const uint32_t m = h->hctl->pad;
+            int rc = DFSPH_OK;
+            if (n + m > h->cap) rc = fail(DFSPH_ERR_CAPACITY, "inflow beyond the fluid capacity");
+            if (rc == DFSPH_OK && m) {
+                dfsph::launch_append_inflow(h->s, h->ob_cand, nc, k, r, vnew, n, mass, h->next_id,
+                                            fluid_ptrs(h, h->pc, h->vc));
+                h->n = n + m;
+                h->next_id += m;
+                rc = sync(h);
+            }
+            dfree(k);
+            dfree(r);
+            dfree(tmp);
+            dfree(vnew);
+            if (rc != DFSPH_OK) return rc;
+            if (added) *added = m;
+            h->lists_valid = false;
+            h->acc_valid = false;
+            h->pred_valid = false;
+        }
+    }
+    if (use_outflow && h->n) {
+        dfsph::launch_outflow_keep(h->s, h->pos[h->pc], h->n, h->ob_interior, h->ob_surface, h->keep);
+        CHK(compact(h, removed));
+    }
+    return DFSPH_OK;
+}
 
 int dfsph_profile_enable(void* hv, uint32_t mask)
 {

@@ -4,13 +4,16 @@
 //   pos   float4 (x, y, z, mass)        fluid; boundary: (x, y, z, psi)
 //   vel   float4 (vx, vy, vz, 0)
 //   rho, alpha, radv, k, kappa, kappaV  float
-//   nbr16 uint16 [max_neighbors + slack][capacity]  fluid lists of staged
-//         blocks: indices into the block's staged set (interleaved: slot k of
-//         particle i at k*capacity + i, so a wavefront reads 64 consecutive words)
-//   nbr   uint32 [max_neighbors + slack][capacity]  boundary lists (from slot 0
-//         in staged blocks) and the fluid-then-boundary lists of unstaged blocks
+//   Neighbor lists are "blocked-interleaved": entry k of particle i lies in
+//   group k / G at [(k / G) * capacity + i] (16 B per group), entry k % G
+//   inside it, so a wavefront reads 64 consecutive 16-byte groups and a batch
+//   of 8 neighbors is one or two vector loads per lane.
+//   nbr16 uint16 groups of 8: fluid lists of blocks with a staged set
+//         (indices into the block's set)
+//   nbrf  uint32 groups of 4: fluid lists of blocks without one (global ids)
+//   nbrb  uint32 groups of 4: boundary lists (boundary ids)
 //   cnt   uint32  fluid count | boundary count << 16
-//   stg   uint32 [blocks][kStageMax]  global ids of each block's staged set
+//   stg   uint32 [blocks][kStageCap]  global ids of each block's staged set
 //
 // Cells are ordered in 8x8x8-cell tiles (tiles x-fastest, then z, then y),
 // Morton order inside a tile, so 256 consecutive particles occupy a compact
@@ -43,6 +46,8 @@ struct SimConst {
     uint32_t ncell;                // cell keys: tiles0 tiles1 tiles2 512
     uint32_t cap;                  // neighbor list stride (fluid capacity)
     uint32_t max_nbr;
+    uint32_t stage_lds;            // largest neighbor set staged in LDS (<= kStageMax)
+    uint32_t stage_cap;            // largest neighbor set with 16-bit lists (<= kStageCap)
 };
 
 // Time-step scalars live on the device so a whole step can be queued
@@ -83,24 +88,29 @@ struct DevCtl {
     unsigned long long nbr_fluid_unstaged;  // of which in 32-bit (unstaged) lists
 };
 
-// ---- LDS-staged neighbor sets ----
-// A sweep block (256 consecutive particles) loads the positions / velocities
-// of every fluid particle in the cells adjacent to its own cells into LDS once
-// (coalesced), and its fluid lists hold 16-bit indices into that set.  Blocks
-// whose set exceeds kStageMax (sparse splashes, the unsorted search) keep
-// 32-bit global lists.
-constexpr uint32_t kStageMax = 1600;       // staged particles per block
+// ---- staged neighbor sets ----
+// A sweep block (256 consecutive particles) owns the set of fluid particles in
+// the cells adjacent to its particles' cells, in cell-key order.  Up to
+// kStageMax of them are loaded into LDS once per sweep (coalesced) and the
+// block's fluid lists are 16-bit indices into the set; sets up to kStageCap
+// keep the 16-bit lists and read the particles through the set's global ids;
+// larger ones (sparse splashes, more than kAdjMax / 27 own cells, the
+// unsorted search) use 32-bit global lists (stage_n = kUnstaged).
+constexpr uint32_t kStageMax = 1600;       // LDS-staged particles per block
+constexpr uint32_t kStageCap = 4096;       // largest set with 16-bit lists
 constexpr uint32_t kUnstaged = 0xffffffffu;
+constexpr uint32_t kSetNoLds = 0x80000000u;  // stage_n flag: 16-bit lists, set not in LDS
 constexpr uint32_t kAdjMax = 2048;         // 27 x unique own cells per block
 constexpr uint32_t kKeyInvalid = 0xffffffffu;
+constexpr uint32_t kGroup16 = 8;           // entries per 16-byte group
+constexpr uint32_t kGroup32 = 4;
 
-struct NbrLists {
-    const uint32_t* nbr;           // 32-bit lists (see the layout above)
-    const uint16_t* nbr16;         // 16-bit staged fluid lists
-    const uint32_t* cnt;
-    const uint32_t* stage_n;       // per block: staged count or kUnstaged
-    const uint32_t* stg;           // per block: global ids of the staged set
-};
+// list groups allocated per particle for a given max_neighbors: every batch
+// of 8 that a sweep reads stays inside
+__host__ __device__ inline uint32_t list_groups16(uint32_t max_nbr) { return (max_nbr + 8) / kGroup16; }
+__host__ __device__ inline uint32_t list_groups32(uint32_t max_nbr) { return (max_nbr + 8) / kGroup32 + 1; }
+
+
 
 // The persistent per-particle fields that move with a particle when the
 // set is reordered or compacted.  kappa / kappaV are the reference's
@@ -115,6 +125,63 @@ struct FluidPtrs {
     float* ktot;
     float* kvtot;
     uint32_t* id;
+    uint32_t* tag;                 // dynamic-window / open-boundary tag
+};
+
+// Particle tags of the dynamic window and the open boundary, with the
+// reference's values (cuda_compilation_center.cu:2-11).
+constexpr uint32_t kTagActive = 10000000u;
+constexpr uint32_t kTagActiveNeighbors = 20000000u;
+constexpr uint32_t kTagUntagged = 40000000u;
+constexpr uint32_t kTagRemoval = 310000000u;
+
+// A BufferFluidSurface (BufferFluidSurface.cu:421-470) by value: the C ABI's
+// dfsph_surface with the same fields.
+enum SurfaceType : int { kSurfPlane = 0, kSurfCuboid = 1, kSurfCylinder = 2, kSurfSphere = 4, kSurfStar = 5 };
+struct Surface {
+    int type;
+    int reversed;
+    float o[3], n[3];              // plane: point and normal (inside: (p - o).n >= 0)
+    float center[3], half[3];      // cuboid half lengths; cylinder / star: half[1] = half height
+    float radius, radius_internal; // cylinder, sphere; star: outer / inner radius
+    uint32_t points;               // star
+    float dir[3];                  // star: direction of the first point (unit, in x-z)
+};
+
+// Restriction of the solver to part of the fluid (DFSPHCData::restriction_mode,
+// count_active, count_active_neighbors, DFSPH_c_arrays_structure.h:410-418):
+// mode 1 by tag, mode 2 by index prefix.  "Compute" kernels (velocity and
+// position updates) run on the active particles only; "source" kernels
+// (density, factor, divergence / density source, neighbor search) also on
+// their active neighbors (DFSPH_core_cuda.cu:225-231, 500-506, 2545-2552).
+struct Restrict {
+    int mode;
+    uint32_t n_active, n_source;   // mode 2: prefixes [0, n_active), [0, n_source)
+    const uint32_t* tag;           // mode 1
+};
+
+__device__ __forceinline__ bool is_compute(const Restrict& r, uint32_t i)
+{
+    if (r.mode == 1) return r.tag[i] == kTagActive;
+    if (r.mode == 2) return i < r.n_active;
+    return true;
+}
+
+__device__ __forceinline__ bool is_source(const Restrict& r, uint32_t i)
+{
+    if (r.mode == 1) { const uint32_t t = r.tag[i]; return t == kTagActive || t == kTagActiveNeighbors; }
+    if (r.mode == 2) return i < r.n_source;
+    return true;
+}
+
+struct NbrLists {
+    const uint16_t* nbr16;         // 16-bit fluid lists (blocks with a staged set)
+    const uint32_t* nbrf;          // 32-bit fluid lists (unstaged blocks)
+    const uint32_t* nbrb;          // 32-bit boundary lists
+    const uint32_t* cnt;
+    const uint32_t* stage_n;       // per block: set size or kUnstaged
+    const uint32_t* stg;           // per block: global ids of the set (kStageCap each)
+    Restrict rs;                   // solver restriction (mode 0: none)
 };
 
 // Blocks are dispatched round robin over the 8 XCDs, each with its own L2.
@@ -234,71 +301,85 @@ __device__ __forceinline__ void for_each_adjacent_cell(const SimConst& c, int cx
     }
 }
 
-// Neighbor loop with memory-level parallelism: U list entries are loaded at
-// once (U coalesced loads of the interleaved list), then the U neighbors'
-// gathers can all be in flight before the first one is used.  Without the
-// batching every neighbor costs two dependent memory round trips (list entry,
-// then the gather).  The list loads are unconditional: the last batch reads up
-// to U-1 slots past the particle's entries, which the list buffer's
-// kNbrSlack extra slots keep in bounds; those entries are replaced by `pad`
-// (a valid particle index) and passed with valid == false, so the callee
-// zeroes their contributions.  Callees mask with keep() on values already
-// computed, so the body has no branch the compiler could sink a gather into.
-#ifndef DFSPH_NBR_UNROLL
-#define DFSPH_NBR_UNROLL 8
-#endif
-constexpr int kNbrSlack = 8;       // extra list slots past max_neighbors
-static_assert(DFSPH_NBR_UNROLL >= 1 && DFSPH_NBR_UNROLL <= kNbrSlack + 1, "unroll exceeds the list slack");
-
-template <int U = DFSPH_NBR_UNROLL, class F>
-__device__ __forceinline__ void for_each_nbr(const uint32_t* __restrict__ p, uint32_t stride, uint32_t count,
-                                             uint32_t pad, F&& f)
+// Neighbor loops with memory-level parallelism: a batch of 8 list entries is
+// one 16-byte load (16-bit lists) or two (32-bit lists) per lane, then the 8
+// neighbors' gathers can all be in flight before the first one is used.  The
+// last batch reads the rest of its group(s), which the list allocation keeps
+// in bounds; those entries are replaced by `pad` (a valid index) and passed
+// with valid == false, so the callee zeroes their contributions.  Callees
+// mask with keep() on values already computed, so the body has no branch the
+// compiler could sink a gather into.
+template <class F>
+__device__ __forceinline__ void for_each_nbr16(const uint16_t* __restrict__ lists, uint32_t cap, uint32_t i,
+                                               uint32_t count, F&& f)
 {
-    for (uint32_t k = 0; k < count; k += U) {
-        uint32_t j[U];
-        bool v[U];
+    const uint4* p = reinterpret_cast<const uint4*>(lists) + i;
+    for (uint32_t k = 0; k < count; k += 8) {
+        const uint4 q = p[(size_t)(k >> 3) * cap];
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+        uint32_t j[8];
+        bool v[8];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t e = p[(size_t)(k + u) * stride];
-            v[u] = k + u < count;
-            j[u] = v[u] ? e : pad;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) f(j[u], v[u]);
-    }
-}
-
-// The same over a 16-bit staged list; invalid entries read staged slot 0.
-template <int U = DFSPH_NBR_UNROLL, class F>
-__device__ __forceinline__ void for_each_nbr16(const uint16_t* __restrict__ p, uint32_t stride, uint32_t count,
-                                               F&& f)
-{
-    for (uint32_t k = 0; k < count; k += U) {
-        uint32_t j[U];
-        bool v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t e = p[(size_t)(k + u) * stride];
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t e = (w[u >> 1] >> ((u & 1) * 16)) & 0xffffu;
             v[u] = k + u < count;
             j[u] = v[u] ? e : 0u;
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) f(j[u], v[u]);
+        for (int u = 0; u < 8; ++u) f(j[u], v[u]);
     }
 }
 
-// Loads a block's staged set: st(k, global id) for k < count, then a block
-// barrier (reached by every thread).  Returns the staged count or kUnstaged.
+template <class F>
+__device__ __forceinline__ void for_each_nbr32(const uint32_t* __restrict__ lists, uint32_t cap, uint32_t i,
+                                               uint32_t count, uint32_t pad, F&& f)
+{
+    const uint4* p = reinterpret_cast<const uint4*>(lists) + i;
+    for (uint32_t k = 0; k < count; k += 8) {
+        const uint4 a = p[(size_t)(k >> 2) * cap];
+        const uint4 b = p[(size_t)((k >> 2) + 1) * cap];
+        const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        uint32_t j[8];
+        bool v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            v[u] = k + u < count;
+            j[u] = v[u] ? w[u] : pad;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) f(j[u], v[u]);
+    }
+}
+
+// Loads an LDS-staged set: st(k, global id) for k < count, then a block
+// barrier (reached by every thread).  Returns stage_n[b].
 template <class St>
 __device__ __forceinline__ uint32_t stage_block(const NbrLists& L, uint32_t b, St&& st)
 {
     const uint32_t sn = L.stage_n[b];
-    if (sn != kUnstaged) {
-        const uint32_t* g = L.stg + (size_t)b * kStageMax;
+    if (sn <= kStageMax) {
+        const uint32_t* g = L.stg + (size_t)b * kStageCap;
         for (uint32_t k = threadIdx.x; k < sn; k += blockDim.x) st(k, g[k]);
     }
     __syncthreads();
     return sn;
+}
+
+// The fluid neighbors of particle i (block b, sn = stage_n[b]) in whichever
+// form its block's lists have: from_lds(staged index, valid) for an
+// LDS-staged set, otherwise from_global(global id, valid).
+template <class FL, class FG>
+__device__ __forceinline__ void fluid_nbrs(const NbrLists& L, uint32_t cap, uint32_t b, uint32_t i, uint32_t nf,
+                                           uint32_t sn, FL&& from_lds, FG&& from_global)
+{
+    if (sn <= kStageMax) {
+        for_each_nbr16(L.nbr16, cap, i, nf, from_lds);
+    } else if (sn != kUnstaged) {
+        const uint32_t* g = L.stg + (size_t)b * kStageCap;
+        for_each_nbr16(L.nbr16, cap, i, nf, [&](uint32_t l, bool v) { from_global(g[l], v); });
+    } else {
+        for_each_nbr32(L.nbrf, cap, i, nf, i, from_global);
+    }
 }
 
 // x where c holds, else 0 (a select of an already computed value)
@@ -368,14 +449,15 @@ __device__ __forceinline__ uint32_t block_excl_scan_u(uint32_t v, uint32_t* excl
 }
 
 // Ascending bitonic sort of a[0, P) in LDS, P a power of two (kBlock threads).
-__device__ __forceinline__ void bitonic_sort_lds(uint32_t* a, uint32_t P)
+template <class T>
+__device__ __forceinline__ void bitonic_sort_lds(T* a, uint32_t P)
 {
     for (uint32_t k = 2; k <= P; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
             for (uint32_t t = threadIdx.x; t < P; t += kBlock) {
                 const uint32_t x = t ^ j;
                 if (x > t) {
-                    const uint32_t va = a[t], vb = a[x];
+                    const T va = a[t], vb = a[x];
                     const bool up = (t & k) == 0;
                     if ((va > vb) == up) { a[t] = vb; a[x] = va; }
                 }

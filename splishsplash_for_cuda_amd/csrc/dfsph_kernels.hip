@@ -245,6 +245,7 @@ __device__ __forceinline__ void copy_fluid(const FluidPtrs& a, uint32_t from, co
     b.ktot[to] = a.ktot[from];
     b.kvtot[to] = a.kvtot[from];
     b.id[to] = a.id[from];
+    b.tag[to] = a.tag[from];
 }
 
 __global__ void k_gather_fluid(uint32_t n, const uint32_t* perm, FluidPtrs in, FluidPtrs out)
@@ -261,35 +262,89 @@ __global__ void k_gather_f4(uint32_t n, const uint32_t* perm, const float4* in, 
     out[k] = in[perm[k]];
 }
 
+// Appends list entries to a particle's blocked-interleaved list: entries
+// collect in registers and each full 16-byte group is one store.
+struct ListWriter16 {
+    uint4* out;                    // group 0 of this particle
+    uint32_t cap;
+    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    __device__ void push(uint32_t n, uint32_t v)      // v = entry n
+    {
+        const uint32_t p = n & 7u, x = v << ((p & 1u) * 16u), wi = p >> 1;
+        w0 |= wi == 0 ? x : 0u;
+        w1 |= wi == 1 ? x : 0u;
+        w2 |= wi == 2 ? x : 0u;
+        w3 |= wi == 3 ? x : 0u;
+        if (p == 7u) flush(n);
+    }
+    __device__ void flush(uint32_t n)                 // the group holding entry n
+    {
+        out[(size_t)(n >> 3) * cap] = make_uint4(w0, w1, w2, w3);
+        w0 = w1 = w2 = w3 = 0;
+    }
+    __device__ void finish(uint32_t n)                // n entries pushed in total
+    {
+        if (n & 7u) flush(n);
+    }
+};
+
+struct ListWriter32 {
+    uint4* out;
+    uint32_t cap;
+    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    __device__ void push(uint32_t n, uint32_t v)
+    {
+        const uint32_t p = n & 3u;
+        w0 = p == 0 ? v : w0;
+        w1 = p == 1 ? v : w1;
+        w2 = p == 2 ? v : w2;
+        w3 = p == 3 ? v : w3;
+        if (p == 3u) flush(n);
+    }
+    __device__ void flush(uint32_t n) { out[(size_t)(n >> 2) * cap] = make_uint4(w0, w1, w2, w3); }
+    __device__ void finish(uint32_t n)
+    {
+        if (n & 3u) flush(n);
+    }
+};
+
 // DFSPH_neighborsSearch_kernel<true> (DFSPH_core_cuda.cu:2219-2531,
 // DFSPH_macro_cuda.h:136-155): every fluid particle within h (strict, FP32) of
 // particle i except i itself, over the 27 cells around its cell, then the
-// boundary particles the same way; interleaved storage
-// (DFSPH_macro_cuda.h:210-214), at most max_neighbors in total (fluid first).
+// boundary particles the same way, at most max_neighbors in total (fluid
+// first).
 //
-// Each block also builds its staged set: the sorted unique keys of the cells
-// adjacent to its particles' cells (a bitonic sort in LDS), their particle
-// ranges in the cell table and the prefix offsets.  If the set fits
-// kStageMax, the block's positions are staged in LDS, the candidate tests read
-// them from there and the fluid lists are 16-bit indices into the set;
-// otherwise the block keeps 32-bit global lists (stage_n[b] = kUnstaged).
-// INDIRECT: the fluid is not reordered and the table indexes it through perm
-// (the need_sort == false path); such blocks are never staged.
+// Each block also builds its neighbor set.  The keys of the cells adjacent
+// to its own cells (own cell first, 27 slots per own cell) are sorted in LDS
+// as (key << 11 | slot); the unique keys give the set's cell ranges and
+// prefix offsets, and the slot map gives every own cell its 27 ranges without
+// a search.  Sets up to kStageMax have their positions staged in LDS for the
+// candidate tests; sets up to kStageCap store 16-bit lists; larger ones and
+// the INDIRECT search (fluid not reordered, the table indexes it through
+// perm: the need_sort == false path) store 32-bit global lists.
 template <bool INDIRECT>
 __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* __restrict__ pos,
                                                       const uint32_t* __restrict__ cs_f,
                                                       const uint32_t* __restrict__ perm,
                                                       const float4* __restrict__ bpos,
                                                       const uint32_t* __restrict__ cs_b, uint32_t n,
-                                                      uint32_t* __restrict__ nbr, uint16_t* __restrict__ nbr16,
-                                                      uint32_t* __restrict__ cnt, uint32_t* __restrict__ stage_n,
-                                                      uint32_t* __restrict__ stg, DevCtl* ctl)
+                                                      uint16_t* __restrict__ nbr16, uint32_t* __restrict__ nbrf,
+                                                      uint32_t* __restrict__ nbrb, uint32_t* __restrict__ cnt,
+                                                      uint32_t* __restrict__ stage_n, uint32_t* __restrict__ stg,
+                                                      Restrict rs, DevCtl* ctl)
 {
-    __shared__ uint32_t s_key[kAdjMax];        // adjacent cell keys, then the unique ones
-    __shared__ uint32_t s_off[kAdjMax + 1];    // staged offset of each unique cell
-    __shared__ uint32_t s_src[kAdjMax];        // first table slot of each unique cell
-    __shared__ float4 s_pos[kStageMax];
+    // the sort buffer (kAdjMax x 8 B) and the staged positions share one area
+    __shared__ __align__(16) unsigned char s_raw[kStageMax * sizeof(float4)];
+    __shared__ uint32_t s_off[kAdjMax + 1];    // own keys, then set offsets of the unique cells
+    __shared__ uint32_t s_src[kAdjMax];        // unique keys, then their first table slot
+    __shared__ uint16_t s_map[kAdjMax];        // slot (own cell x 27 + m) -> unique cell
     __shared__ uint32_t s_w[kBlock / 64];
+    static_assert(kAdjMax * sizeof(uint64_t) <= kStageMax * sizeof(float4), "sort buffer exceeds the stage");
+    static_assert(kStageCap <= 65536 && kAdjMax <= 2048, "16-bit indices / 11-bit slots");
+    uint64_t* s_sort = reinterpret_cast<uint64_t*>(s_raw);
+    float4* s_pos = reinterpret_cast<float4*>(s_raw);
+    constexpr uint16_t kNoCell = 0xffffu;
+
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
     const bool act = i < n;
@@ -303,121 +358,133 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
         cz = cell_coord(c, p.z, 2);
         key = cell_key(c, cx, cy, cz);
     }
-    bool staged = false;
-    uint32_t U = 0, S = 0;
+    uint32_t S = kUnstaged, U = 0, uo = 0;
     if (!INDIRECT) {
-        // unique own cells: keys are non-decreasing along the block
-        s_key[threadIdx.x] = key;
+        // unique own cells (keys are non-decreasing along the block)
+        s_off[threadIdx.x] = key;
         __syncthreads();
-        const bool first = act && (threadIdx.x == 0 || s_key[threadIdx.x - 1] != key);
-        uint32_t uown = 0;
-        const uint32_t nown = block_excl_scan_u(first ? 1u : 0u, &uown, s_w);
+        const bool first = act && (threadIdx.x == 0 || s_off[threadIdx.x - 1] != key);
+        uint32_t ex = 0;
+        const uint32_t nown = block_excl_scan_u(first ? 1u : 0u, &ex, s_w);
+        uo = ex + (first ? 1u : 0u) - 1u;          // own cell index (valid for active threads)
         if (nown * 27u <= kAdjMax) {
             const uint32_t m = nown * 27u;
             if (first) {
-                uint32_t* o = s_key + uown * 27u;
+                const uint32_t s0 = uo * 27u;
                 uint32_t q = 0;
-                for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) { o[q++] = nk; });
-                for (; q < 27u; ++q) o[q] = kKeyInvalid;
+                s_sort[s0] = ((uint64_t)key << 11) | s0;   // own cell first
+                ++q;
+                for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) {
+                    if (nk != key) { s_sort[s0 + q] = ((uint64_t)nk << 11) | (s0 + q); ++q; }
+                });
+                for (; q < 27u; ++q) s_sort[s0 + q] = ((uint64_t)kKeyInvalid << 11) | (s0 + q);
             }
             uint32_t P = 64;
             while (P < m) P <<= 1;
-            for (uint32_t k = m + threadIdx.x; k < P; k += kBlock) s_key[k] = kKeyInvalid;
+            for (uint32_t k = m + threadIdx.x; k < P; k += kBlock) s_sort[k] = ~0ull;
             __syncthreads();
-            bitonic_sort_lds(s_key, P);
-            // unique keys, compacted in place (reads, barrier, writes)
+            bitonic_sort_lds(s_sort, P);
+            // unique keys and the slot map: element k belongs to the run that
+            // began at the last unique start at or before k
             const uint32_t E = P > kBlock ? P / kBlock : 1u;
             const uint32_t k0 = threadIdx.x * E;
-            uint32_t loc[kAdjMax / kBlock];
+            uint64_t v[kAdjMax / kBlock];
             uint32_t nloc = 0;
 #pragma unroll
             for (uint32_t e = 0; e < kAdjMax / kBlock; ++e) {
-                loc[e] = kKeyInvalid;
                 const uint32_t k = k0 + e;
+                v[e] = ~0ull;
                 if (e < E && k < P) {
-                    const uint32_t v = s_key[k];
-                    const uint32_t prev = k > 0 ? s_key[k - 1] : kKeyInvalid;
-                    if (v != kKeyInvalid && v != prev) { loc[e] = v; ++nloc; }
+                    v[e] = s_sort[k];
+                    const uint32_t kk = (uint32_t)(v[e] >> 11);
+                    const uint32_t prev = k > 0 ? (uint32_t)(s_sort[k - 1] >> 11) : kKeyInvalid;
+                    if (kk != kKeyInvalid && kk != prev) ++nloc;
                 }
             }
             uint32_t base = 0;
             U = block_excl_scan_u(nloc, &base, s_w);
+            uint32_t cur = base - 1u;
 #pragma unroll
-            for (uint32_t e = 0; e < kAdjMax / kBlock; ++e)
-                if (loc[e] != kKeyInvalid) s_key[base++] = loc[e];
+            for (uint32_t e = 0; e < kAdjMax / kBlock; ++e) {
+                const uint32_t k = k0 + e;
+                if (e < E && k < P) {
+                    const uint32_t kk = (uint32_t)(v[e] >> 11), slot = (uint32_t)(v[e] & 2047u);
+                    const uint32_t prev = k > 0 ? (uint32_t)(s_sort[k - 1] >> 11) : kKeyInvalid;
+                    if (kk != kKeyInvalid && kk != prev) { cur = base++; s_src[cur] = kk; }
+                    if (slot < m) s_map[slot] = kk != kKeyInvalid ? (uint16_t)cur : kNoCell;
+                }
+            }
             __syncthreads();
-            // particle counts of the unique cells -> staged offsets
+            // particle counts of the unique cells -> prefix offsets
             const uint32_t E2 = (U + kBlock - 1) / kBlock;
             const uint32_t u0 = threadIdx.x * E2;
             uint32_t sum = 0;
             for (uint32_t e = 0; e < E2; ++e) {
                 const uint32_t u = u0 + e;
                 if (u < U) {
-                    const uint32_t a = cs_f[s_key[u]], z = cs_f[s_key[u] + 1];
+                    const uint32_t kk = s_src[u];
+                    const uint32_t a = cs_f[kk], z = cs_f[kk + 1];
                     s_src[u] = a;
+                    s_off[u] = z - a;
                     sum += z - a;
                 }
             }
             uint32_t off = 0;
-            S = block_excl_scan_u(sum, &off, s_w);
+            const uint32_t tot = block_excl_scan_u(sum, &off, s_w);
             for (uint32_t e = 0; e < E2; ++e) {
                 const uint32_t u = u0 + e;
                 if (u < U) {
+                    const uint32_t cntu = s_off[u];
                     s_off[u] = off;
-                    off += cs_f[s_key[u] + 1] - s_src[u];
+                    off += cntu;
                 }
             }
-            if (threadIdx.x == 0) s_off[U] = S;
-            staged = S <= kStageMax;
+            if (threadIdx.x == 0) s_off[U] = tot;
+            S = tot <= c.stage_cap ? tot : kUnstaged;
             __syncthreads();
         }
     }
-    if (threadIdx.x == 0) stage_n[b] = staged ? S : kUnstaged;
-    if (staged) {
-        uint32_t* g_out = stg + (size_t)b * kStageMax;
+    const bool in_lds = S <= c.stage_lds;
+    if (threadIdx.x == 0) stage_n[b] = in_lds || S == kUnstaged ? S : (S | kSetNoLds);
+    if (S != kUnstaged) {
+        uint32_t* g_out = stg + (size_t)b * kStageCap;
         for (uint32_t k = threadIdx.x; k < S; k += kBlock) {
-            uint32_t lo = 0, hi = U;            // last u with s_off[u] <= k
+            uint32_t lo = 0, hi = U;            // the unique cell holding set index k
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
                 if (s_off[mid] <= k) lo = mid; else hi = mid;
             }
             const uint32_t g = s_src[lo] + (k - s_off[lo]);
-            s_pos[k] = pos[g];
+            if (in_lds) s_pos[k] = pos[g];
             g_out[k] = g;
         }
         __syncthreads();
     }
     uint32_t total = 0, trunc = 0, stored_f = 0, stored_b = 0, stored_u32 = 0;
-    if (act) {
-        const uint32_t maxn = c.max_nbr, stride = c.cap;
+    if (act && !is_source(rs, i)) cnt[i] = 0;           // restricted out: no lists
+    if (act && is_source(rs, i)) {
+        const uint32_t maxn = c.max_nbr, cap = c.cap;
         uint32_t nf = 0, nb = 0;
-        if (staged) {
-            auto find = [&](uint32_t nk) -> uint32_t {
-                uint32_t lo = 0, hi = U;         // lower bound of nk
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (s_key[mid] < nk) lo = mid + 1; else hi = mid;
-                }
-                return lo;
-            };
-            const uint32_t io = find(key);     // the own cell is always in the set
-            const uint32_t self = io < U ? s_off[io] + (i - s_src[io]) : kKeyInvalid;
-            uint16_t* out = nbr16 + i;
-            for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) {
-                const uint32_t u = find(nk);
-                if (u >= U || s_key[u] != nk) return;
-                const uint32_t e = s_off[u + 1];
+        if (S != kUnstaged) {
+            ListWriter16 w{reinterpret_cast<uint4*>(nbr16) + i, cap};
+            const uint16_t* map = s_map + uo * 27u;
+            const uint32_t self = s_off[map[0]] + (i - s_src[map[0]]);
+            for (uint32_t q = 0; q < 27u; ++q) {
+                const uint32_t u = map[q];
+                if (u == kNoCell) continue;
+                const uint32_t e = s_off[u + 1], src = s_src[u] - s_off[u];
                 for (uint32_t l = s_off[u]; l < e; ++l) {
                     if (l == self) continue;
-                    const float4 q = s_pos[l];
-                    if (dist2(p.x - q.x, p.y - q.y, p.z - q.z) < c.hk2) {
-                        if (nf < maxn) out[(size_t)nf * stride] = (uint16_t)l;
+                    const float4 qp = in_lds ? s_pos[l] : pos[src + l];
+                    if (dist2(p.x - qp.x, p.y - qp.y, p.z - qp.z) < c.hk2) {
+                        if (nf < maxn) w.push(nf, l);
                         ++nf;
                     }
                 }
-            });
+            }
+            w.finish(nf < maxn ? nf : maxn);
         } else {
-            uint32_t* out = nbr + i;
+            ListWriter32 w{reinterpret_cast<uint4*>(nbrf) + i, cap};
             for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) {
                 const uint32_t s0 = cs_f[nk], e = cs_f[nk + 1];
                 for (uint32_t q = s0; q < e; ++q) {
@@ -425,31 +492,33 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
                     if (j == i) continue;
                     const float4 pj = pos[j];
                     if (dist2(p.x - pj.x, p.y - pj.y, p.z - pj.z) < c.hk2) {
-                        if (nf < maxn) out[(size_t)nf * stride] = j;
+                        if (nf < maxn) w.push(nf, j);
                         ++nf;
                     }
                 }
             });
+            w.finish(nf < maxn ? nf : maxn);
         }
-        uint32_t* bout = nbr + i + (size_t)(staged ? 0u : nf) * stride;
+        const uint32_t sf = nf < maxn ? nf : maxn;
+        ListWriter32 wb{reinterpret_cast<uint4*>(nbrb) + i, cap};
         for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) {
             const uint32_t s0 = cs_b[nk], e = cs_b[nk + 1];
             for (uint32_t j = s0; j < e; ++j) {
                 const float4 pj = bpos[j];
                 if (dist2(p.x - pj.x, p.y - pj.y, p.z - pj.z) < c.hk2) {
-                    if (nf + nb < maxn) bout[(size_t)nb * stride] = j;
+                    if (sf + nb < maxn) wb.push(nb, j);
                     ++nb;
                 }
             }
         });
-        total = nf + nb;
-        const uint32_t sf = nf < maxn ? nf : maxn;
         const uint32_t sb = nb < maxn - sf ? nb : maxn - sf;
+        wb.finish(sb);
+        total = nf + nb;
         trunc = total > maxn ? 1u : 0u;
         cnt[i] = sf | (sb << 16);
         stored_f = sf;
         stored_b = sb;
-        stored_u32 = staged ? 0u : sf;
+        stored_u32 = S == kUnstaged ? sf : 0u;
     }
     const uint32_t wmax = wave_max_u(total);
     const uint32_t wtr = wave_sum_u(trunc);
@@ -463,11 +532,11 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
         if (wsb) atomicAdd(&ctl->nbr_boundary, (unsigned long long)wsb);
         if (wsu) atomicAdd(&ctl->nbr_fluid_unstaged, (unsigned long long)wsu);
     }
-    if (threadIdx.x == 0 && !staged && n > b * kBlock) atomicAdd(&ctl->unstaged_blocks, 1u);
+    if (threadIdx.x == 0 && !in_lds && n > b * kBlock) atomicAdd(&ctl->unstaged_blocks, 1u);
 }
 
 // Every list of the last search as 32-bit global ids, fluid then boundary
-// (the format of the reference's interleaved storage), for readback.
+// (the reference's plain interleaved layout, slot k at k * cap + i), for readback.
 __global__ __launch_bounds__(kBlock) void k_expand_lists(uint32_t n, uint32_t cap, NbrLists L, uint32_t* out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -476,13 +545,18 @@ __global__ __launch_bounds__(kBlock) void k_expand_lists(uint32_t n, uint32_t ca
     const uint32_t cn = L.cnt[i];
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
     const uint32_t sn = L.stage_n[b];
-    if (sn != kUnstaged) {
-        const uint32_t* g = L.stg + (size_t)b * kStageMax;
-        for (uint32_t k = 0; k < nf; ++k) out[(size_t)k * cap + i] = g[L.nbr16[(size_t)k * cap + i]];
-        for (uint32_t k = 0; k < nb; ++k) out[(size_t)(nf + k) * cap + i] = L.nbr[(size_t)k * cap + i];
-    } else {
-        for (uint32_t k = 0; k < nf + nb; ++k) out[(size_t)k * cap + i] = L.nbr[(size_t)k * cap + i];
+    for (uint32_t k = 0; k < nf; ++k) {
+        uint32_t j;
+        if (sn != kUnstaged) {
+            const uint32_t l = L.nbr16[(((size_t)(k >> 3) * cap + i) << 3) + (k & 7u)];
+            j = L.stg[(size_t)b * kStageCap + l];
+        } else {
+            j = L.nbrf[(((size_t)(k >> 2) * cap + i) << 2) + (k & 3u)];
+        }
+        out[(size_t)k * cap + i] = j;
     }
+    for (uint32_t k = 0; k < nb; ++k)
+        out[(size_t)(nf + k) * cap + i] = L.nbrb[(((size_t)(k >> 2) * cap + i) << 2) + (k & 3u)];
 }
 
 // FluidModel::computeBoundaryPsi (FluidModel.cpp:233-258) on the device:
@@ -582,13 +656,14 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
         sV[k] = make_float4(v.x, v.y, v.z, ws_of(g));
     });
     if (i >= n) return;
-    const float4 pi = pos[i];
     const float4 vi = vel_i[i];
+    const float ki = ws_of(i);
+    if (kvtot) kvtot[i] = ki;
+    if (!is_source(L.rs, i)) { vel_o[i] = vi; return; }
+    const float4 pi = pos[i];
     const uint32_t cn = L.cnt[i];
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
     const uint32_t stride = c.cap;
-    const float ki = ws_of(i);
-    if (kvtot) kvtot[i] = ki;
     float dens = pi.w * c.W0, sg2 = 0.0f, dr = 0.0f;
     float gx = 0.0f, gy = 0.0f, gz = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
     auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
@@ -607,17 +682,13 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
             ax += s * px; ay += s * py; az += s * pz;
         }
     };
-    if (sn != kUnstaged) {
-        for_each_nbr16(L.nbr16 + i, stride, nf, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); });
-    } else {
-        for_each_nbr(L.nbr + i, stride, nf, i, [&](uint32_t j, bool valid) {
-            const float4 v = vel_i[j];
-            fluid(pos[j], make_float4(v.x, v.y, v.z, ws_of(j)), valid);
-        });
-    }
+    fluid_nbrs(L, stride, b, i, nf, sn, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); },
+               [&](uint32_t j, bool valid) {
+                   const float4 v = vel_i[j];
+                   fluid(pos[j], make_float4(v.x, v.y, v.z, ws_of(j)), valid);
+               });
     float bx = 0.0f, by = 0.0f, bz = 0.0f;
-    const uint32_t* bl = L.nbr + i + (size_t)(sn != kUnstaged ? 0u : nf) * stride;
-    for_each_nbr(bl, stride, nb, 0u, [&](uint32_t j, bool valid) {
+    for_each_nbr32(L.nbrb, stride, i, nb, 0u, [&](uint32_t j, bool valid) {
         const float4 pj = bpos[j];
         const float m = keep(valid, pj.w);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -637,7 +708,7 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
     alpha[i] = a;
     radv[i] = r;
     float4 vo = vi;
-    if (WS != 2 && r > 0.0f) {
+    if (WS != 2 && r > 0.0f && is_compute(L.rs, i)) {
         if (fabsf(ki) > kEps) { ax += ki * bx; ay += ki * by; az += ki * bz; }
         vo.x += h * ax; vo.y += h * ay; vo.z += h * az;
     }
@@ -666,7 +737,8 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) { sP[k] = pos[g]; sV[k] = vel[g]; });
     float r = 0.0f;
-    if (i < n) {
+    if (i < n && !is_source(L.rs, i)) r = radv[i];      // restricted out: stored value in the error sum
+    if (i < n && is_source(L.rs, i)) {
         const float4 pi = pos[i];
         const float4 vi = vel[i];
         const uint32_t cn = L.cnt[i];
@@ -677,12 +749,9 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
             const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         };
-        if (sn != kUnstaged)
-            for_each_nbr16(L.nbr16 + i, c.cap, nf, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); });
-        else
-            for_each_nbr(L.nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) { fluid(pos[j], vel[j], valid); });
-        const uint32_t* bl = L.nbr + i + (size_t)(sn != kUnstaged ? 0u : nf) * c.cap;
-        for_each_nbr(bl, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
+        fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); },
+                   [&](uint32_t j, bool valid) { fluid(pos[j], vel[j], valid); });
+        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j, bool valid) {
             const float4 pj = bpos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
@@ -715,7 +784,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t,
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) { sP[k] = pos[g]; sK[k] = kk[g]; });
-    if (i >= n) return;
+    if (i >= n || !is_compute(L.rs, i)) return;
     const float ki = kk[i];
     if (DIV) kappaV[i] += ki;
     const float4 pi = pos[i];
@@ -729,13 +798,10 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t,
         const float s = keep(valid && fabsf(ks) > kEps, ks * pj.w * g);
         ax += s * dx; ay += s * dy; az += s * dz;
     };
-    if (sn != kUnstaged)
-        for_each_nbr16(L.nbr16 + i, c.cap, nf, [&](uint32_t l, bool valid) { fluid(sP[l], sK[l], valid); });
-    else
-        for_each_nbr(L.nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) { fluid(pos[j], kk[j], valid); });
+    fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t l, bool valid) { fluid(sP[l], sK[l], valid); },
+               [&](uint32_t j, bool valid) { fluid(pos[j], kk[j], valid); });
     if (fabsf(ki) > kEps) {
-        const uint32_t* bl = L.nbr + i + (size_t)(sn != kUnstaged ? 0u : nf) * c.cap;
-        for_each_nbr(bl, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
+        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j, bool valid) {
             const float4 pj = bpos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float s = keep(valid, ki * pj.w * kernel_grad_coef(c, dist2(dx, dy, dz)));
@@ -769,7 +835,15 @@ __global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, u
         sV[k] = make_float4(v.x, v.y, v.z, p.w / rho[g]);
     });
     float m2 = 0.0f;
-    if (i < n) {
+    if (i < n && !is_compute(L.rs, i)) {
+        // restricted out: its stored acceleration still enters the CFL bound
+        // (the reference's maximum runs over all particles, DFSPH_core_cuda.cu:3340-3355)
+        const float4 vi = vel[i], a = acc[i];
+        const float h = t->h;
+        const float wx = vi.x + a.x * h, wy = vi.y + a.y * h, wz = vi.z + a.z * h;
+        m2 = wx * wx + wy * wy + wz * wz;
+    }
+    if (i < n && is_compute(L.rs, i)) {
         const float coef = t->inv_h * c.viscosity;
         const float4 pi = pos[i];
         const float4 vi = vel[i];
@@ -781,15 +855,12 @@ __global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, u
             const float s = keep(valid, coef * vj.w * w);
             ax -= s * (vi.x - vj.x); ay -= s * (vi.y - vj.y); az -= s * (vi.z - vj.z);
         };
-        if (sn != kUnstaged) {
-            for_each_nbr16(L.nbr16 + i, c.cap, nf, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); });
-        } else {
-            for_each_nbr(L.nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) {
-                const float4 p = pos[j];
-                const float4 v = vel[j];
-                fluid(p, make_float4(v.x, v.y, v.z, p.w / rho[j]), valid);
-            });
-        }
+        fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); },
+                   [&](uint32_t j, bool valid) {
+                       const float4 p = pos[j];
+                       const float4 v = vel[j];
+                       fluid(p, make_float4(v.x, v.y, v.z, p.w / rho[j]), valid);
+                   });
         const float4 a = make_float4(c.gx + ax, c.gy + ay, c.gz + az, 0.0f);
         acc[i] = a;
         const float h = t->h;
@@ -825,10 +896,10 @@ __global__ __launch_bounds__(kBlock) void k_cfl_finalize(DevCtl* ctl, const floa
 
 // DFSPH_update_vel_kernel (DFSPH_core_cuda.cu:3047-3070): v += h a and the
 // pressure warm-start rescale kappa = max(kappa h^2 / h_f^2, -0.5).
-__global__ void k_update_vel(const DevTime* t, uint32_t n, const float4* acc, float4* vel, float* kappa)
+__global__ void k_update_vel(const DevTime* t, uint32_t n, const float4* acc, float4* vel, float* kappa, Restrict rs)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || !is_compute(rs, i)) return;
     const float h = t->h;
     const float4 a = acc[i];
     float4 v = vel[i];
@@ -865,7 +936,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) { sP[k] = pos[g]; sK[k] = ws_of(g); });
-    if (i >= n) return;
+    if (i >= n || !is_compute(L.rs, i)) return;
     const float ki = ws_of(i);
     if (ktot) ktot[i] = ki;
     const float4 pi = pos[i];
@@ -879,13 +950,10 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
         const float s = keep(valid && fabsf(ks) > kEps, ks * pj.w * g);
         ax += s * dx; ay += s * dy; az += s * dz;
     };
-    if (sn != kUnstaged)
-        for_each_nbr16(L.nbr16 + i, c.cap, nf, [&](uint32_t l, bool valid) { fluid(sP[l], sK[l], valid); });
-    else
-        for_each_nbr(L.nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) { fluid(pos[j], ws_of(j), valid); });
+    fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t l, bool valid) { fluid(sP[l], sK[l], valid); },
+               [&](uint32_t j, bool valid) { fluid(pos[j], ws_of(j), valid); });
     if (fabsf(ki) > kEps) {
-        const uint32_t* bl = L.nbr + i + (size_t)(sn != kUnstaged ? 0u : nf) * c.cap;
-        for_each_nbr(bl, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
+        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j, bool valid) {
             const float4 pj = bpos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float s = keep(valid, ki * pj.w * kernel_grad_coef(c, dist2(dx, dy, dz)));
@@ -926,7 +994,8 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) { sP[k] = pos[g]; sV[k] = vel[g]; });
     float r = 0.0f;
-    if (i < n) {
+    if (i < n && !is_source(L.rs, i)) r = radv[i];      // restricted out: stored value in the error sum
+    if (i < n && is_source(L.rs, i)) {
         const float hf = t->h_future;
         const float4 pi = pos[i];
         const float4 vi = vel[i];
@@ -938,12 +1007,9 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
             const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         };
-        if (sn != kUnstaged)
-            for_each_nbr16(L.nbr16 + i, c.cap, nf, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); });
-        else
-            for_each_nbr(L.nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) { fluid(pos[j], vel[j], valid); });
-        const uint32_t* bl = L.nbr + i + (size_t)(sn != kUnstaged ? 0u : nf) * c.cap;
-        for_each_nbr(bl, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
+        fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); },
+                   [&](uint32_t j, bool valid) { fluid(pos[j], vel[j], valid); });
+        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j, bool valid) {
             const float4 pj = bpos[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
@@ -967,7 +1033,8 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
 // escape count of tag_particles_outside_bounding_box_kernel
 // (SPH_other_systems_cuda.cu:1244-1281): outside if any coordinate leaves the box.
 __global__ void k_update_pos(const DevTime* t, uint32_t n, const float4* vel, float4* pos,
-                             float bx0, float by0, float bz0, float bx1, float by1, float bz1, DevCtl* ctl)
+                             float bx0, float by0, float bz0, float bx1, float by1, float bz1, DevCtl* ctl,
+                             Restrict rs)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t out = 0;
@@ -975,8 +1042,10 @@ __global__ void k_update_pos(const DevTime* t, uint32_t n, const float4* vel, fl
         const float h = t->h;
         const float4 v = vel[i];
         float4 x = pos[i];
-        x.x += h * v.x; x.y += h * v.y; x.z += h * v.z;
-        pos[i] = x;
+        if (is_compute(rs, i)) {
+            x.x += h * v.x; x.y += h * v.y; x.z += h * v.z;
+            pos[i] = x;
+        }
         out = (x.x > bx1 || x.y > by1 || x.z > bz1 || x.x < bx0 || x.y < by0 || x.z < bz0) ? 1u : 0u;
     }
     const uint32_t s = wave_sum_u(out);
@@ -1060,13 +1129,10 @@ __global__ __launch_bounds__(kBlock) void k_features(SimConst c, const DevTime* 
         ox -= m * dx; oy -= m * dy; oz -= m * dz;
         od += m * __builtin_sqrtf(r2);
     };
-    if (sn != kUnstaged)
-        for_each_nbr16(L.nbr16 + i, c.cap, nf, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); });
-    else
-        for_each_nbr(L.nbr + i, c.cap, nf, i, [&](uint32_t j, bool valid) { fluid(pos[j], vel[j], valid); });
+    fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); },
+               [&](uint32_t j, bool valid) { fluid(pos[j], vel[j], valid); });
     float dbd = 0.0f;
-    const uint32_t* bl = L.nbr + i + (size_t)(sn != kUnstaged ? 0u : nf) * c.cap;
-    for_each_nbr(bl, c.cap, nb, 0u, [&](uint32_t j, bool valid) {
+    for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j, bool valid) {
         const float4 pj = bpos[j];
         const float m = keep(valid, 1.0f);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -1213,16 +1279,17 @@ void launch_gather_f4(hipStream_t s, uint32_t n, const uint32_t* perm, const flo
 }
 
 void launch_neighbors(hipStream_t s, const SimConst& c, bool indirect, const float4* pos, const uint32_t* cs_f,
-                      const uint32_t* perm, const float4* bpos, const uint32_t* cs_b, uint32_t n,
-                      uint32_t* nbr, uint16_t* nbr16, uint32_t* cnt, uint32_t* stage_n, uint32_t* stg, DevCtl* ctl)
+                      const uint32_t* perm, const float4* bpos, const uint32_t* cs_b, uint32_t n, uint16_t* nbr16,
+                      uint32_t* nbrf, uint32_t* nbrb, uint32_t* cnt, uint32_t* stage_n, uint32_t* stg,
+                      const Restrict& rs, DevCtl* ctl)
 {
     if (!n) return;
     if (indirect)
-        k_neighbors<true><<<nblk_xcd(n), kBlock, 0, s>>>(c, pos, cs_f, perm, bpos, cs_b, n, nbr, nbr16, cnt, stage_n,
-                                                         stg, ctl);
+        k_neighbors<true><<<nblk_xcd(n), kBlock, 0, s>>>(c, pos, cs_f, perm, bpos, cs_b, n, nbr16, nbrf, nbrb, cnt,
+                                                         stage_n, stg, rs, ctl);
     else
-        k_neighbors<false><<<nblk_xcd(n), kBlock, 0, s>>>(c, pos, cs_f, perm, bpos, cs_b, n, nbr, nbr16, cnt, stage_n,
-                                                          stg, ctl);
+        k_neighbors<false><<<nblk_xcd(n), kBlock, 0, s>>>(c, pos, cs_f, perm, bpos, cs_b, n, nbr16, nbrf, nbrb, cnt,
+                                                          stage_n, stg, rs, ctl);
 }
 
 void launch_expand_lists(hipStream_t s, uint32_t n, uint32_t cap, const NbrLists& L, uint32_t* out)
@@ -1290,9 +1357,10 @@ void launch_xsph(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n,
     k_cfl_finalize<<<1, kBlock, 0, s>>>(ctl, partial, g);
 }
 
-void launch_update_vel(hipStream_t s, const DevTime* t, uint32_t n, const float4* acc, float4* vel, float* kappa)
+void launch_update_vel(hipStream_t s, const DevTime* t, uint32_t n, const float4* acc, float4* vel, float* kappa,
+                       const Restrict& rs)
 {
-    if (n) k_update_vel<<<nblk(n), kBlock, 0, s>>>(t, n, acc, vel, kappa);
+    if (n) k_update_vel<<<nblk(n), kBlock, 0, s>>>(t, n, acc, vel, kappa, rs);
 }
 
 void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c, const DevTime* t, uint32_t n,
@@ -1329,10 +1397,11 @@ void launch_density_source(hipStream_t s, bool init, const SimConst& c, const De
 }
 
 void launch_update_pos(hipStream_t s, const DevTime* t, uint32_t n, const float4* vel, float4* pos,
-                       const float* bmin, const float* bmax, DevCtl* ctl)
+                       const float* bmin, const float* bmax, DevCtl* ctl, const Restrict& rs)
 {
     if (n)
-        k_update_pos<<<nblk(n), kBlock, 0, s>>>(t, n, vel, pos, bmin[0], bmin[1], bmin[2], bmax[0], bmax[1], bmax[2], ctl);
+        k_update_pos<<<nblk(n), kBlock, 0, s>>>(t, n, vel, pos, bmin[0], bmin[1], bmin[2], bmax[0], bmax[1], bmax[2], ctl,
+                                                rs);
 }
 
 void launch_keep_inside(hipStream_t s, uint32_t n, const float4* pos, const float* bmin, const float* bmax,

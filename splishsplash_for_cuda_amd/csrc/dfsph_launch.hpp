@@ -34,8 +34,8 @@ void launch_gather_fluid(hipStream_t s, uint32_t n, const uint32_t* perm, const 
 void launch_gather_f4(hipStream_t s, uint32_t n, const uint32_t* perm, const float4* in, float4* out);
 void launch_neighbors(hipStream_t s, const SimConst& c, bool indirect, const float4* pos, const uint32_t* cs_f,
                       const uint32_t* perm, const float4* bpos, const uint32_t* cs_b, uint32_t n,
-                      uint32_t* nbr, uint16_t* nbr16, uint32_t* cnt, uint32_t* stage_n, uint32_t* stg,
-                      DevCtl* ctl);
+                      uint16_t* nbr16, uint32_t* nbrf, uint32_t* nbrb, uint32_t* cnt, uint32_t* stage_n,
+                      uint32_t* stg, const Restrict& rs, DevCtl* ctl);
 void launch_expand_lists(hipStream_t s, uint32_t n, uint32_t cap, const NbrLists& L, uint32_t* out);
 void launch_boundary_psi(hipStream_t s, const SimConst& c, const float4* bpos, const uint32_t* cs_b,
                          uint32_t nb, float* psi);
@@ -54,7 +54,8 @@ void launch_jacobi(hipStream_t s, bool div, const SimConst& c, const DevTime* t,
 void launch_xsph(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
                  const NbrLists& L, const float4* vel, const float* rho, float4* acc,
                  DevCtl* ctl, float* partial);
-void launch_update_vel(hipStream_t s, const DevTime* t, uint32_t n, const float4* acc, float4* vel, float* kappa);
+void launch_update_vel(hipStream_t s, const DevTime* t, uint32_t n, const float4* acc, float4* vel, float* kappa,
+                       const Restrict& rs);
 void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c, const DevTime* t, uint32_t n,
                         const float4* pos, const float4* bpos, const NbrLists& L,
                         const float4* acc, float4* vel, const float* kws, int rescale, float* ktot);
@@ -63,7 +64,7 @@ void launch_density_source(hipStream_t s, bool init, const SimConst& c, const De
                            const float4* vel, const float* rho, float* alpha, float* radv, float* kk,
                            float* kappa, SolveCtl* ctl, double* partial);
 void launch_update_pos(hipStream_t s, const DevTime* t, uint32_t n, const float4* vel, float4* pos,
-                       const float* bmin, const float* bmax, DevCtl* ctl);
+                       const float* bmin, const float* bmax, DevCtl* ctl, const Restrict& rs);
 void launch_keep_inside(hipStream_t s, uint32_t n, const float4* pos, const float* bmin, const float* bmax,
                         uint32_t* keep);
 void launch_keep_from_u8(hipStream_t s, uint32_t n, const uint8_t* k8, uint32_t* keep);
@@ -75,5 +76,29 @@ void launch_features(hipStream_t s, const SimConst& c, const DevTime* t, uint32_
                      const float* ktot, const float* kvtot, float diameter, float* feat);
 void launch_harvest(hipStream_t s, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap, const float* feat,
                     const float* kvtot, const float* ktot, float* ds_x, float* ds_t);
+
+// open boundary / dynamic window (dfsph_window.hip)
+void launch_surface_keep(hipStream_t s, const float4* p, uint32_t n, const Surface& S, int want_inside, uint32_t* keep);
+void launch_surface_keep_and(hipStream_t s, const float4* p, uint32_t n, const Surface& S, int want_inside,
+                             uint32_t* keep);
+void launch_compact_f4(hipStream_t s, uint32_t n, const uint32_t* keep, const uint32_t* rank, const float4* in,
+                       float4* out);
+void launch_outflow_keep(hipStream_t s, const float4* p, uint32_t n, const Surface& interior, const Surface& surface,
+                         uint32_t* keep);
+void launch_inflow(hipStream_t s, const SimConst& c, const float4* cand, uint32_t nc, const float4* pos,
+                   const float4* vel, const uint32_t* cs_f, const uint32_t* perm, const float4* bpos,
+                   const uint32_t* cs_b, const Surface& boundary, float min_dist, float max_density, float mass,
+                   uint32_t* keep, float4* vel_out);
+void launch_append_inflow(hipStream_t s, const float4* cand, uint32_t nc, const uint32_t* keep, const uint32_t* rank,
+                          const float4* vnew, uint32_t n, float mass, uint32_t first_id, const FluidPtrs& f);
+void launch_apply_delta(hipStream_t s, float4* p, uint32_t n, float dx, float dy, float dz);
+void launch_count_tag(hipStream_t s, const uint32_t* tag, uint32_t n, uint32_t t, uint32_t* count);
+void launch_tag_neighbors(hipStream_t s, const SimConst& c, uint32_t n, const NbrLists& L, const uint32_t* src,
+                          uint32_t* dst, uint32_t tag_in, uint32_t tag_out, int untagged_only, uint32_t grid);
+void launch_count_changed(hipStream_t s, const uint32_t* a, const uint32_t* b, uint32_t n, uint32_t* changed);
+void launch_tag_classes(hipStream_t s, const uint32_t* tag, uint32_t n, uint32_t* f0, uint32_t* f1);
+void launch_partition3(hipStream_t s, uint32_t n, const uint32_t* f0, const uint32_t* r0, const uint32_t* f1,
+                       const uint32_t* r1, const FluidPtrs& in, const FluidPtrs& out);
+void launch_keep_not_tag(hipStream_t s, const uint32_t* tag, uint32_t n, uint32_t t, uint32_t* keep);
 
 }  // namespace dfsph

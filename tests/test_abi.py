@@ -56,3 +56,32 @@ def test_scene_sizes():
     assert s.n_fluid == 16000 and s.n_boundary == 18002
     j = s.fluid - (scene.fluid_block((0.05,) * 3, (20, 40, 20), 0.05))
     assert abs(j).max() <= 0.01 * 0.025 * 1.001      # jitter U(-0.01 r, 0.01 r), float32 positions
+
+
+def test_ctypes_struct_layouts_match_the_header(tmp_path):
+    """sizeof / offsetof of the ABI structs, from the C compiler, equal the
+    ctypes mirrors (a field added on one side only fails here)."""
+    import shutil
+    import subprocess
+    from splishsplash_for_cuda_amd import _native
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"dfsph_params": _native.DFSPHParams, "dfsph_info": _native.DFSPHInfo,
+               "dfsph_step_stats": _native.DFSPHStepStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dfsph_hip.h"', "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (name, name))
+        for f in cls._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (name, f[0], name, f[0]))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                  text=True).stdout.splitlines())
+    for name, cls in structs.items():
+        assert int(got[name]) == ctypes.sizeof(cls), name
+        for f in cls._fields_:
+            assert int(got["%s.%s" % (name, f[0])]) == getattr(cls, f[0]).offset, (name, f[0])

@@ -52,6 +52,53 @@ def test_neighbor_lists_bit_exact_config1(oracle_mod, sort):
     assert inf.max_neighbors_seen == mx
 
 
+@pytest.mark.parametrize("list_mode", [S.LISTS_AUTO, S.LISTS_NO_LDS, S.LISTS_GLOBAL])
+def test_neighbor_lists_bit_exact_every_list_form(oracle_mod, list_mode):
+    """The same sets from the LDS-staged, the 16-bit unstaged and the 32-bit
+    list forms (a dam break after a few steps: dense and sparse blocks)."""
+    sc = scene.seeded_dam_break(5, 6000)
+    s = _solver(sc, dt=0.002, list_mode=list_mode)
+    for _ in range(30):
+        s.step()
+    s.neighbor_search()
+    pos = s.download(S.FIELD_POS).astype(np.float32)
+    ids = s.download(S.FIELD_ID)
+    inf = s.info()
+    if list_mode == S.LISTS_AUTO:
+        assert inf.unstaged_blocks < (inf.n_fluid + 255) // 256
+    else:
+        assert inf.unstaged_blocks == (inf.n_fluid + 255) // 256
+    if list_mode == S.LISTS_GLOBAL:
+        assert inf.neighbors_fluid_unstaged == inf.neighbors_fluid_total
+    else:
+        assert inf.neighbors_fluid_unstaged == 0
+    hk = np.float32(4 * sc.particle_radius)
+    nf, nb, L, _ = oracle_mod.neighbors_fp32(pos, sc.boundary, hk, brute=True)
+    want = {int(ids[i]): v for i, v in oracle_neighbor_sets(nf, nb, L).items()}
+    want = {k: (frozenset(int(ids[j]) for j in f), b) for k, (f, b) in want.items()}
+    got = gpu_neighbor_sets(s, boundary_index_map(s, sc.boundary))
+    assert got == want
+
+
+def test_list_forms_give_the_same_trajectory():
+    """LDS-staged and unstaged 16-bit lists read the same neighbors in the same
+    order: bit-identical; the 32-bit lists visit cells in another order: equal
+    up to FP32 summation order."""
+    sc = scene.seeded_dam_break(5, 6000)
+    out = {}
+    for mode in (S.LISTS_AUTO, S.LISTS_NO_LDS, S.LISTS_GLOBAL):
+        s = _solver(sc, dt=0.002, list_mode=mode)
+        its = [(st.iters_v, st.iters) for st in (s.step() for _ in range(20))]
+        out[mode] = (s.state_by_id(), its)
+    a, b, g = out[S.LISTS_AUTO], out[S.LISTS_NO_LDS], out[S.LISTS_GLOBAL]
+    for f in ("pos", "vel", "density", "kappa", "kappaV"):
+        assert np.array_equal(a[0][f], b[0][f]), f
+    assert a[1] == b[1]
+    assert max_rel(g[0]["density"], a[0]["density"]) < 1e-5
+    vs = np.abs(a[0]["vel"]).max() + 9.81 * 0.002
+    assert np.abs(g[0]["vel"] - a[0]["vel"]).max() < 1e-3 * vs
+
+
 def test_neighbor_lists_bit_exact_random_cloud(oracle_mod):
     """Ragged densities, duplicates and pairs at exactly h."""
     rng = np.random.default_rng(21)

@@ -1,0 +1,272 @@
+"""Parity of the open boundary and the dynamic-window restriction (HIP path
+through the C ABI) with the CPU oracle.
+
+Tolerances:
+  surface inside tests      exact, except points within 1e-5 of a border
+  inflow acceptance         exact set, except candidates within 1e-4 of the
+                            distance or density threshold
+  inflow velocity           1e-5 of max |v|
+  outflow removal           exact set by id
+  tags / active-first order exact
+  restricted steps          density 2e-6 rho0, velocity 5e-5 (max |v| + g h),
+                            position 1e-6 r, iteration counts equal, frozen
+                            particles bit-identical
+"""
+import numpy as np
+import pytest
+
+import splishsplash_for_cuda_amd as S
+from splishsplash_for_cuda_amd import scene
+from gpu_util import max_rel
+
+pytestmark = pytest.mark.gpu
+
+TAG_ACTIVE, TAG_NBR, TAG_UNTAGGED = S.TAG_ACTIVE, S.TAG_ACTIVE_NEIGHBORS, S.TAG_UNTAGGED
+
+
+def _solver(sc, psi=None, **kw):
+    p = S.default_params(particle_radius=sc.particle_radius, **kw)
+    s = S.Solver(p)
+    s.upload_boundary(sc.boundary, psi)
+    s.upload_fluid(sc.fluid)
+    return s
+
+
+def _pair(om, kind, **kw):
+    return S.surface(kind, **kw), om.surface(kind, **kw)
+
+
+def test_surface_inside_matches_oracle(oracle_mod):
+    om = oracle_mod
+    rng = np.random.default_rng(5)
+    pts = rng.uniform(-2.5, 2.5, size=(20000, 3)).astype(np.float32)
+    shapes = [
+        ("plane", dict(o=(0.1, -0.2, 0.3), n=(0.6, 0.8, 0.0))),
+        ("cuboid", dict(center=(0.2, 0.1, -0.3), half=(1.0, 0.7, 1.5))),
+        ("cylinder", dict(center=(0.0, 0.5, 0.0), half=(0.0, 1.2, 0.0), radius=1.5)),
+        ("sphere", dict(center=(0.3, -0.2, 0.1), radius=1.7)),
+        ("star", dict(center=(0.0, 0.0, 0.0), half=(0.0, 2.0, 0.0), radius=1.75, radius_internal=1.0, points=5,
+                      direction=(0.0, 0.0, 1.0))),
+        ("star", dict(center=(0.1, 0.0, -0.2), half=(0.0, 1.0, 0.0), radius=2.2, radius_internal=0.9, points=7,
+                      direction=(0.6, 0.0, 0.8))),
+    ]
+    s = _solver(scene.small_dam_break())
+    for kind, kw in shapes:
+        for rev in (False, True):
+            gs, os_ = _pair(om, kind, reversed=rev, **kw)
+            got = s.surface_inside(gs, pts)
+            want, margin = om.surface_inside(os_, pts.astype(np.float64))
+            ok = margin > 1e-5
+            assert ok.mean() > 0.99
+            assert np.array_equal(got[ok], want[ok]), (kind, rev)
+            assert 0.05 < want.mean() < 0.95
+
+
+def _basin():
+    """A cylindrical basin (r = 1.5 m boundary), fluid up to y = 0.5: the open
+    boundary of OpenBoundariesSimple's cylinder configuration at a small size."""
+    r = 0.04
+    d = 2 * r
+    R = 0.6
+    # boundary: floor disk and the cylinder wall
+    g = np.arange(-R, R + 1e-9, d)
+    fx, fz = np.meshgrid(g, g, indexing="ij")
+    floor = np.stack([fx.ravel(), np.zeros(fx.size), fz.ravel()], 1)
+    floor = floor[np.hypot(floor[:, 0], floor[:, 2]) <= R + d]
+    th = np.arange(0, 2 * np.pi, d / (R + d))
+    hs = np.arange(d, 1.0, d)
+    T, H = np.meshgrid(th, hs, indexing="ij")
+    wall = np.stack([(R + d) * np.cos(T).ravel(), H.ravel(), (R + d) * np.sin(T).ravel()], 1)
+    b = np.concatenate([floor, wall]).astype(np.float32)
+    # fluid: inside radius R - 3r, up to y = 0.5, jittered lattice
+    rng = np.random.default_rng(2)
+    fl = np.stack(np.meshgrid(g, np.arange(d, 0.5, d), g, indexing="ij"), -1).reshape(-1, 3)
+    fl = fl[np.hypot(fl[:, 0], fl[:, 2]) < R - 3 * r]
+    fl = fl + rng.uniform(-0.01 * r, 0.01 * r, fl.shape)
+    # inflow candidates: a ring of positions at radii R - 2r .. R - 0.5 r
+    rr = np.array([R - 2 * r, R - r])
+    tt = np.arange(0, 2 * np.pi, d / R)
+    yy = np.arange(d, 0.7, d)
+    RR, TT, YY = np.meshgrid(rr, tt, yy, indexing="ij")
+    cand = np.stack([(RR * np.cos(TT)).ravel(), YY.ravel(), (RR * np.sin(TT)).ravel()], 1).astype(np.float32)
+    return scene.Scene("basin", r, fl.astype(np.float32), b), cand, R
+
+
+def test_open_boundary_inflow_and_outflow_match_oracle(oracle_mod):
+    om = oracle_mod
+    sc, cand, R = _basin()
+    r = sc.particle_radius
+    s = _solver(sc, dt=0.002)
+    for _ in range(5):
+        s.step()
+    kw_b = dict(center=(0, 0, 0), half=(0, 10, 0), radius=R)
+    kw_i = dict(center=(0, 0, 0), half=(0, 10, 0), radius=R - 3 * r)
+    kw_s = dict(o=(0, 0.5, 0), n=(0, -1, 0))
+    gb, ob = _pair(om, "cylinder", **kw_b)
+    gi, oi = _pair(om, "cylinder", **kw_i)
+    gsu, osu = _pair(om, "plane", **kw_s)
+    kept = s.open_boundary_init(gb, gi, gsu, cand)
+    # candidates kept: outside the interior and below the fluid surface
+    c64 = cand.astype(np.float64)
+    keep_o = ~om.surface_inside(oi, c64)[0] & om.surface_inside(osu, c64)[0]
+    assert kept == int(keep_o.sum()) and kept > 0
+    kc = c64[keep_o]
+
+    # the oracle sees the device state before the call
+    st0 = s.state_by_id()
+    o = om.Oracle(particle_radius=r)
+    bpos = s.download(S.FIELD_BOUNDARY_POS).astype(np.float64)
+    o.set_boundary(bpos, s.download(S.FIELD_BOUNDARY_PSI).astype(np.float64))
+    o.set_fluid(st0["pos"].astype(np.float64), vel=st0["vel"].astype(np.float64),
+                mass=st0["mass"].astype(np.float64), ids=st0["id"])
+    m0 = float(s.download(S.FIELD_MASS)[0])
+    acc, vel, dmin, dens = o.inflow(ob, kc, 1.75 * r, 700.0, m0)
+    n0 = s.info().n_fluid
+    added, removed = s.open_boundary_apply(1.75 * r, 700.0, use_inflow=True, use_outflow=False)
+    sharp = (np.abs(dmin - 1.75 * r) > 1e-4 * r) & (np.abs(dens - 700.0) > 1e-4 * 700.0)
+    ids = s.download(S.FIELD_ID)
+    pos = s.download(S.FIELD_POS)
+    v = s.download(S.FIELD_VEL)
+    assert s.info().n_fluid == n0 + added and removed == 0
+    new = np.argsort(ids)[n0:]                         # fresh ids after every old one
+    got_pos = {tuple(p) for p in pos[new].astype(np.float32)}
+    want_idx = np.nonzero(acc & sharp)[0]
+    want_pos = {tuple(p) for p in kc[want_idx].astype(np.float32)}
+    assert want_pos <= got_pos
+    assert len(got_pos - want_pos) <= int((~sharp).sum())
+    assert added > 0
+    # velocities of the accepted candidates
+    key = {tuple(p): i for i, p in zip(new, pos[new].astype(np.float32))}
+    vs = np.abs(vel).max() + 1e-6
+    for c in want_idx:
+        i = key[tuple(kc[c].astype(np.float32))]
+        assert np.abs(v[i] - vel[c]).max() < 1e-5 * vs
+    # outflow on the new state, with a narrower interior and a lower surface
+    gi2, oi2 = _pair(om, "cylinder", center=(0, 0, 0), half=(0, 10, 0), radius=R - 6 * r)
+    gs2, os2 = _pair(om, "plane", o=(0, 0.3, 0), n=(0, -1, 0))
+    s.open_boundary_init(gb, gi2, gs2, cand)
+    st1 = s.state_by_id()
+    o2 = om.Oracle(particle_radius=r)
+    o2.set_boundary(bpos)
+    o2.set_fluid(st1["pos"].astype(np.float64), ids=st1["id"])
+    keep = o2.outflow_keep(oi2, os2)
+    _, removed = s.open_boundary_apply(1.75 * r, 700.0, use_inflow=False, use_outflow=True)
+    assert removed == int((~keep).sum()) and removed > 0
+    assert set(s.download(S.FIELD_ID).tolist()) == set(st1["id"][keep].tolist())
+
+
+def test_window_displacement_moves_surfaces_and_inflow(oracle_mod):
+    """Moving the window by d is the same as building everything shifted by d."""
+    sc, cand, R = _basin()
+    r = sc.particle_radius
+    res = []
+    for shift in (np.zeros(3), np.array([0.0, 0.125, 0.0])):     # 0.125: exact in binary
+        s = _solver(sc, dt=0.002)
+        for _ in range(3):
+            s.step()
+        c0 = tuple(-shift)
+        gb = S.surface("cylinder", center=c0, half=(0, 10, 0), radius=R)
+        gi = S.surface("cylinder", center=c0, half=(0, 10, 0), radius=R - 6 * r)
+        gsu = S.surface("plane", o=tuple(np.array([0, 0.3, 0]) - shift), n=(0, -1, 0))
+        s.open_boundary_init(gb, gi, gsu, cand - shift.astype(np.float32))
+        a, rm = s.open_boundary_apply(1.75 * r, 700.0, window_displacement=tuple(shift))
+        res.append((a, rm, np.sort(s.download(S.FIELD_POS)[:, 1])))
+    assert res[0][0] == res[1][0] and res[0][1] == res[1][1]
+    assert res[0][1] > 0
+    assert np.abs(res[0][2] - res[1][2]).max() < 1e-5
+
+
+def _column(n=(8, 16, 8), r=0.025):
+    d = 2 * r
+    x = np.stack(np.meshgrid(*[np.arange(k) for k in n], indexing="ij"), -1).reshape(-1, 3) * d + d
+    x = x + np.random.default_rng(4).uniform(-0.01 * r, 0.01 * r, x.shape)
+    tank = (n[0] * d + d, n[1] * d * 1.6 + d, n[2] * d + d)
+    return scene.Scene("column", r, x.astype(np.float32), scene.box_surface((0, 0, 0), tank, d).astype(np.float32))
+
+
+def test_tag_neighbors_and_active_first_order(oracle_mod):
+    om = oracle_mod
+    sc = _column()
+    s = _solver(sc, dt=0.002)
+    s.neighbor_search()
+    pos = s.download(S.FIELD_POS)
+    n = len(pos)
+    tags = np.full(n, TAG_UNTAGGED, np.uint32)
+    act = pos[:, 1] < np.median(pos[:, 1]) - 0.1
+    tags[act] = TAG_ACTIVE
+    s.upload_field(S.FIELD_TAG, tags)
+    changed = s.tag_neighbors(TAG_ACTIVE, TAG_NBR, untagged_only=True)
+    got = s.download(S.FIELD_TAG)
+    # expected: untagged fluid neighbors of the active set (FP32 predicate)
+    nf, nb, L, _ = om.neighbors_fp32(pos, sc.boundary, np.float32(4 * sc.particle_radius), max_nbr=90)
+    want = tags.copy()
+    for i in np.nonzero(act)[0]:
+        for j in L[i, :nf[i]]:
+            if tags[j] == TAG_UNTAGGED:
+                want[j] = TAG_NBR
+    assert np.array_equal(got, want)
+    assert changed == int((want != tags).sum()) > 0
+    assert s.count_tagged(TAG_ACTIVE) == int(act.sum())
+    ids0 = s.download(S.FIELD_ID)
+    ca, can = s.order_active_first()
+    assert (ca, can) == (int((want == TAG_ACTIVE).sum()), int((want == TAG_NBR).sum()))
+    t1, ids1 = s.download(S.FIELD_TAG), s.download(S.FIELD_ID)
+    assert np.all(t1[:ca] == TAG_ACTIVE) and np.all(t1[ca:ca + can] == TAG_NBR)
+    for cls in (TAG_ACTIVE, TAG_NBR, TAG_UNTAGGED):       # stable inside every class
+        assert np.array_equal(ids1[t1 == cls], ids0[want == cls])
+    assert s.remove_tagged(TAG_NBR) == can
+    assert s.info().n_fluid == n - can
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_restricted_steps_match_oracle(oracle_mod, mode):
+    om = oracle_mod
+    sc = _column()
+    r, dt = sc.particle_radius, 0.002
+    o0 = om.Oracle(particle_radius=r)
+    o0.set_boundary(sc.boundary.astype(np.float64))
+    psi = o0.boundary_psi(sc.n_boundary)
+    s = _solver(sc, psi=psi.astype(np.float32), dt=dt)
+    s.neighbor_search()
+    pos = s.download(S.FIELD_POS)
+    n = len(pos)
+    tags = np.full(n, TAG_UNTAGGED, np.uint32)
+    tags[pos[:, 1] < np.median(pos[:, 1])] = TAG_ACTIVE
+    s.upload_field(S.FIELD_TAG, tags)
+    s.tag_neighbors(TAG_ACTIVE, TAG_NBR)
+    if mode == 2:
+        ca, can = s.order_active_first()
+        s.set_restriction(2, ca, can, ca + can)
+    else:
+        s.set_restriction(1, 0, 0, -1)
+    st = s.state_by_id()
+    # the oracle in the device's current order (mode 2 depends on it)
+    ids = s.download(S.FIELD_ID)
+    order = np.argsort(np.argsort(ids))
+    o = om.Oracle(particle_radius=r, dt=dt, fp32_predicate=1)
+    o.set_boundary(sc.boundary.astype(np.float64), psi)
+    o.set_fluid(st["pos"][order].astype(np.float64), ids=ids)
+    t_dev = s.download(S.FIELD_TAG)
+    if mode == 2:
+        o.set_restriction(2, ca, can, ca + can)
+    else:
+        o.set_restriction(1, tags=t_dev)
+    frozen = st["pos"].copy()
+    for k in range(3):
+        sg = s.step()
+        so = o.step()
+        assert (sg.iters_v, sg.iters) == (so.iters_v, so.iters), k
+    gs = s.state_by_id()
+    os_ = {f: o.get(fld) for f, fld in (("pos", om.F_POS), ("vel", om.F_VEL), ("density", om.F_DENSITY))}
+    oid = o.ids()
+    perm = np.argsort(oid)
+    for f in os_:
+        os_[f] = os_[f][perm]
+    vscale = np.abs(os_["vel"]).max() + 9.81 * dt
+    assert max_rel(gs["vel"], os_["vel"], vscale) < 5e-5
+    assert max_rel(gs["pos"], os_["pos"], r) < 1e-6
+    src = np.isin(gs["id"], ids[np.isin(t_dev, (TAG_ACTIVE, TAG_NBR))])
+    assert max_rel(gs["density"][src], os_["density"][src], 1000.0) < 2e-6
+    # everything outside the active set is frozen
+    inactive = ~np.isin(gs["id"], ids[t_dev == TAG_ACTIVE])
+    assert np.array_equal(gs["pos"][inactive], frozen[inactive])

@@ -320,3 +320,133 @@ def test_mlp_silu_forward_small_case(oracle_mod):
     # the other hidden units have zero pre-activation -> silu(0) = 0, except
     # W2/W3 rows with zero inputs: b terms only
     assert y[0, 1] == pytest.approx(0.25, rel=1e-14)
+
+
+# ---------------------------------------------------------------- open boundary / dynamic window (A19, A20)
+
+def test_surfaces_closed_forms(oracle_mod):
+    om = oracle_mod
+    S = om.surface
+    cases = [
+        (S("sphere", center=(1, 2, 3), radius=0.5), [(1, 2, 3.4), (1.2, 2.2, 3)], [(1, 2, 3.6), (1.4, 2.4, 3)]),
+        (S("plane", o=(0, 1, 0), n=(0, 1, 0)), [(5, 1.0, 0), (0, 1.01, 9)], [(0, 0.99, 0)]),
+        (S("cuboid", center=(0, 0, 0), half=(1, 2, 3)), [(0.9, -1.9, 2.9)], [(1.1, 0, 0), (0, 2.1, 0), (0, 0, -3.1)]),
+        (S("cylinder", center=(0, 0, 0), half=(0, 1, 0), radius=0.5), [(0.4, 0.9, 0), (0, -0.99, 0.45)],
+         [(0.4, 1.1, 0), (0.6, 0, 0), (0.3, 0, 0.45)]),
+    ]
+    for surf, inside, outside in cases:
+        for pts, want in ((inside, True), (outside, False)):
+            got, _ = om.surface_inside(surf, pts)
+            assert got.tolist() == [want] * len(pts)
+            surf.reversed = 1
+            got, _ = om.surface_inside(surf, pts)
+            assert got.tolist() == [not want] * len(pts)
+            surf.reversed = 0
+
+
+def test_star_surface_vertices_and_area(oracle_mod):
+    """The linear-sided star: its border passes through the outer vertices
+    (angle 0 from the first point's direction) and the inner ones (half a
+    period), and the enclosed area is that of the star polygon,
+    P r_e r_i sin(pi / P) (Monte Carlo, 200k points)."""
+    om = oracle_mod
+    P, re, ri = 5, 2.0, 1.0
+    st = om.surface("star", center=(0, 0, 0), half=(0, 1, 0), radius=re, radius_internal=ri, points=P,
+                    direction=(0, 0, 1))
+    a0 = 2 * math.pi / P
+
+    def at(angle, rad):        # angle from +z towards +x
+        return (rad * math.sin(angle), 0.0, rad * math.cos(angle))
+
+    ins, _ = om.surface_inside(st, [at(0, 0.99 * re), at(a0 / 2, 0.99 * ri), at(a0, 0.99 * re), at(0.3, 0.5)])
+    assert ins.all()
+    out, _ = om.surface_inside(st, [at(0, 1.01 * re), at(a0 / 2, 1.01 * ri), at(a0 / 2, 1.5), at(a0 / 2, 1.9)])
+    assert not out.any()
+    rng = np.random.default_rng(3)
+    p = np.zeros((200_000, 3))
+    p[:, 0] = rng.uniform(-re, re, len(p))
+    p[:, 2] = rng.uniform(-re, re, len(p))
+    ins, _ = om.surface_inside(st, p)
+    area = ins.mean() * (2 * re) ** 2
+    assert area == pytest.approx(P * re * ri * math.sin(a0 / 2), rel=0.01)
+
+
+def test_inflow_rules_by_hand(oracle_mod):
+    """inflow_with_predefined_positions: rejected near existing fluid; with one
+    fluid and one boundary particle in range the velocity is the fluid's
+    times m W_f / (m W_f + psi W_b); rejected when the velocity points
+    against the boundary normal (towards the sphere's outside)."""
+    om = oracle_mod
+    r = 0.025
+    hk = 4 * r
+    o = om.Oracle(particle_radius=r)
+    o.set_boundary(np.array([[0.0, 0.0, 1.0 + 0.06]]), psi=np.array([0.3]))
+    o.set_fluid(np.array([[0.0, 0.0, 1.0 - 0.05]]), vel=np.array([[0.0, 0.0, -2.0]]))
+    m = o.get(om.F_MASS)[0]
+    ball = om.surface("sphere", center=(0, 0, 0), radius=1.2)       # normal: towards the centre
+    cand = np.array([[0.0, 0.0, 1.0], [0.0, 0.0, 1.0 - 0.05 + 0.02], [5.0, 5.0, 5.0]])
+    acc, vel, dmin, dens = o.inflow(ball, cand, 1.75 * r, 700.0, m)
+    assert dmin[0] == pytest.approx(0.05) and dmin[1] == pytest.approx(0.02) and dmin[2] == 100.0
+    wf = om.kernel_W(0.05, hk) * m
+    wb = om.kernel_W(0.06, hk) * 0.3
+    assert vel[0] == pytest.approx([0, 0, -2.0 * wf / (wf + wb)])
+    assert dens[0] == pytest.approx(om.kernel_W(0.0, hk) * m + wf + wb)
+    assert acc.tolist() == [True, False, True]          # 0: inward velocity, 1: too close, 2: empty space
+    o.set(om.F_VEL, np.array([[0.0, 0.0, 2.0]]))       # outward: against the normal
+    acc, _, _, _ = o.inflow(ball, cand[:1], 1.75 * r, 700.0, m)
+    assert not acc[0]
+    acc, _, _, _ = o.inflow(ball, cand[2:], 1.75 * r, 0.5 * om.kernel_W(0.0, hk) * m, m)
+    assert not acc[0]                                   # density cap below the self contribution
+
+
+def test_outflow_removes_outside_both_shapes(oracle_mod):
+    om = oracle_mod
+    o = om.Oracle(particle_radius=0.025)
+    o.set_boundary(np.array([[9.0, 9.0, 9.0]]))
+    x = np.array([[0.0, 0.5, 0.0], [0.0, 2.0, 0.0], [1.4, 0.5, 0.0], [1.4, 2.0, 0.0]])
+    o.set_fluid(x)
+    interior = om.surface("cylinder", center=(0, 0, 0), half=(0, 10, 0), radius=1.0)
+    surf = om.surface("plane", o=(0, 1.0, 0), n=(0, -1, 0))          # inside below y = 1
+    assert o.outflow_keep(interior, surf).tolist() == [True, True, True, False]
+
+
+def _restricted(om, x, b, mode, **kw):
+    o = om.Oracle(particle_radius=0.025, dt=0.002)
+    o.set_boundary(b)
+    o.set_fluid(x)
+    o.set_restriction(mode, **kw)
+    return o
+
+
+def test_restriction_full_set_equals_unrestricted(oracle_mod):
+    om = oracle_mod
+    x, b = _column()
+    n = len(x)
+    ref = _restricted(om, x, b, 0)
+    a = _restricted(om, x, b, 2, count_active=n)
+    t = _restricted(om, x, b, 1, tags=np.full(n, 10000000, np.uint32))
+    for _ in range(3):
+        ref.step(); a.step(); t.step()
+    for o in (a, t):
+        for f in (om.F_POS, om.F_VEL, om.F_DENSITY, om.F_KAPPA, om.F_KAPPAV):
+            assert np.array_equal(o.get(f), ref.get(f))
+
+
+def test_restriction_freezes_the_inactive(oracle_mod):
+    """Mode 2 with the lower half active: the particles beyond the source
+    prefix keep position, velocity and density; the active-neighbor band
+    gets densities but no velocity; errors average over true_particle_count."""
+    om = oracle_mod
+    x, b = _column()
+    order = np.argsort(x[:, 1], kind="stable")
+    x = x[order]
+    n = len(x)
+    ca, can = n // 2, 72
+    o = _restricted(om, x, b, 2, count_active=ca, count_active_neighbors=can, true_particle_count=ca + can)
+    o.step()
+    x0, v0 = x.copy(), np.zeros_like(x)
+    xs, vs, rho = o.get(om.F_POS), o.get(om.F_VEL), o.get(om.F_DENSITY)
+    assert np.array_equal(xs[ca:], x0[ca:]) and np.array_equal(vs[ca:], v0[ca:])
+    assert np.all(rho[ca + can:] == 0.0)            # never computed
+    assert np.all(rho[ca:ca + can] > 0.0)           # source band: densities
+    assert np.abs(vs[:ca]).max() > 0.0              # the active part moved
