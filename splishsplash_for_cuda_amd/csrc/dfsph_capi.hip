@@ -711,6 +711,10 @@ int dfsph_create(const dfsph_params* p, void** out)
         c.lh2 = (float)(lc / (hk * hk));
         c.lh = (float)(lc / hk);
         c.W0 = (float)kc;
+        c.hk2_4 = (float)(0.25 * hk * hk);
+        c.gin_a = (float)(3.0 * lc / (hk * hk * hk));
+        c.gin_b = (float)(-2.0 * lc / (hk * hk));
+        c.gout = (float)(-lc / (hk * hk * hk));
         c.density0 = (float)h->p.density0;
         c.gx = (float)h->p.gravity[0];
         c.gy = (float)h->p.gravity[1];

@@ -36,6 +36,9 @@ struct SimConst {
     float lh2;                     // (48 / (pi h_k^3)) / h_k^2
     float lh;                      // (48 / (pi h_k^3)) / h_k
     float W0;                      // W(0)
+    float hk2_4;                   // (h_k / 2)^2: the branch point q = 1/2
+    float gin_a, gin_b;            // gradient, q <= 1/2: gin_a r + gin_b = l (3q - 2) / h^2
+    float gout;                    // gradient, q > 1/2: gout (h_k - r)^2 / r = -l (1-q)^2 / (r h)
     float density0;
     float gx, gy, gz;
     float viscosity;
@@ -107,8 +110,8 @@ constexpr uint32_t kGroup32 = 4;
 
 // list groups allocated per particle for a given max_neighbors: every batch
 // of 8 that a sweep reads stays inside
-__host__ __device__ inline uint32_t list_groups16(uint32_t max_nbr) { return (max_nbr + 8) / kGroup16; }
-__host__ __device__ inline uint32_t list_groups32(uint32_t max_nbr) { return (max_nbr + 8) / kGroup32 + 1; }
+__host__ __device__ inline uint32_t list_groups16(uint32_t max_nbr) { return max_nbr / kGroup16 + 2; }
+__host__ __device__ inline uint32_t list_groups32(uint32_t max_nbr) { return max_nbr / kGroup32 + 4; }
 
 
 
@@ -254,6 +257,53 @@ __device__ __forceinline__ float kernel_W(const SimConst& c, float r2)
     return kernel_pair(c, r2).W;
 }
 
+// The same for a pair taken from a neighbor list, where r2 < h^2 holds by
+// construction of the list (the same FP32 test on the same positions), so
+// the support test is not needed, and with the constants folded.  ok is
+// false for coincident particles (|r| <= 1e-6), whose gradient the reference
+// zeroes; the caller folds it into its one final select.
+struct PairL {
+    float W, g;
+    bool ok;
+};
+
+__device__ __forceinline__ PairL grad_listed(const SimConst& c, float r2)
+{
+    const float r2c = fmaxf(r2, 1.0e-24f);
+    const float ir = __builtin_amdgcn_rsqf(r2c);
+    const float r = r2c * ir;
+    const float d = c.hk - r;
+    PairL o;
+    o.W = 0.0f;
+    o.g = (r2 <= c.hk2_4) ? __builtin_fmaf(c.gin_a, r, c.gin_b) : (c.gout * ir) * (d * d);
+    o.ok = r2 > 1.0e-12f;
+    return o;
+}
+
+__device__ __forceinline__ PairL pair_listed(const SimConst& c, float r2)
+{
+    const float r2c = fmaxf(r2, 1.0e-24f);
+    const float ir = __builtin_amdgcn_rsqf(r2c);
+    const float r = r2c * ir;
+    const float d = c.hk - r;
+    const float q = r * c.inv_hk;
+    const float f = 1.0f - q;
+    const bool in = r2 <= c.hk2_4;
+    PairL o;
+    o.W = in ? c.kcoef * __builtin_fmaf(6.0f * q * q, q - 1.0f, 1.0f) : 2.0f * c.kcoef * f * f * f;
+    o.g = in ? __builtin_fmaf(c.gin_a, r, c.gin_b) : (c.gout * ir) * (d * d);
+    o.ok = r2 > 1.0e-12f;
+    return o;
+}
+
+__device__ __forceinline__ float W_listed(const SimConst& c, float r2)
+{
+    const float r2c = fmaxf(r2, 1.0e-24f);
+    const float q = r2c * __builtin_amdgcn_rsqf(r2c) * c.inv_hk;
+    const float f = 1.0f - q;
+    return r2 <= c.hk2_4 ? c.kcoef * __builtin_fmaf(6.0f * q * q, q - 1.0f, 1.0f) : 2.0f * c.kcoef * f * f * f;
+}
+
 __device__ __forceinline__ int cell_coord(const SimConst& c, float v, int a)
 {
     int k = (int)floorf((v - c.org[a]) * c.inv_cell);
@@ -309,46 +359,98 @@ __device__ __forceinline__ void for_each_adjacent_cell(const SimConst& c, int cx
 // with valid == false, so the callee zeroes their contributions.  Callees
 // mask with keep() on values already computed, so the body has no branch the
 // compiler could sink a gather into.
+// Each batch first issues the loads of all 8 neighbors (load(entry) returns
+// the neighbor's data), then runs body(data, valid) on them, so the LDS or
+// L2 latency of one neighbor overlaps the others'; the next batch's list
+// group is loaded while the current batch computes.
+template <class Load, class Body>
+__device__ __forceinline__ void for_each_nbr16(const uint16_t* __restrict__ lists, uint32_t cap, uint32_t i,
+                                               uint32_t count, Load&& load, Body&& body)
+{
+    using T = decltype(load(0u));
+    const uint4* p = reinterpret_cast<const uint4*>(lists) + i;
+    uint4 q = p[0];                             // every particle has at least one group
+    uint32_t k = 0;
+    for (; k + 8 <= count; k += 8) {            // full batches: no masking
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+        q = p[(size_t)((k >> 3) + 1) * cap];    // next group (allocated up to max_neighbors / 8)
+        T it[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) it[u] = load((w[u >> 1] >> ((u & 1) * 16)) & 0xffffu);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) body(it[u], true);
+    }
+    if (k < count) {                            // the last, partial batch
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+        T it[8];
+        bool v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            v[u] = k + u < count;
+            it[u] = load(v[u] ? (w[u >> 1] >> ((u & 1) * 16)) & 0xffffu : 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) body(it[u], v[u]);
+    }
+}
+
+template <class Load, class Body>
+__device__ __forceinline__ void for_each_nbr32(const uint32_t* __restrict__ lists, uint32_t cap, uint32_t i,
+                                               uint32_t count, uint32_t pad, Load&& load, Body&& body)
+{
+    using T = decltype(load(0u));
+    const uint4* p = reinterpret_cast<const uint4*>(lists) + i;
+    uint4 a = p[0], b = p[cap];                 // every particle has at least two groups
+    uint32_t k = 0;
+    for (; k + 8 <= count; k += 8) {
+        const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        a = p[(size_t)((k >> 2) + 2) * cap];    // next two groups (allocated up to max_neighbors / 4 + 2)
+        b = p[(size_t)((k >> 2) + 3) * cap];
+        T it[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) it[u] = load(w[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) body(it[u], true);
+    }
+    if (k < count) {
+        const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        T it[8];
+        bool v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            v[u] = k + u < count;
+            it[u] = load(v[u] ? w[u] : pad);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) body(it[u], v[u]);
+    }
+}
+
+// the plain forms: f(entry, valid)
 template <class F>
 __device__ __forceinline__ void for_each_nbr16(const uint16_t* __restrict__ lists, uint32_t cap, uint32_t i,
                                                uint32_t count, F&& f)
 {
-    const uint4* p = reinterpret_cast<const uint4*>(lists) + i;
-    for (uint32_t k = 0; k < count; k += 8) {
-        const uint4 q = p[(size_t)(k >> 3) * cap];
-        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-        uint32_t j[8];
-        bool v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const uint32_t e = (w[u >> 1] >> ((u & 1) * 16)) & 0xffffu;
-            v[u] = k + u < count;
-            j[u] = v[u] ? e : 0u;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) f(j[u], v[u]);
-    }
+    for_each_nbr16(lists, cap, i, count, [](uint32_t e) { return e; }, f);
 }
 
 template <class F>
 __device__ __forceinline__ void for_each_nbr32(const uint32_t* __restrict__ lists, uint32_t cap, uint32_t i,
                                                uint32_t count, uint32_t pad, F&& f)
 {
-    const uint4* p = reinterpret_cast<const uint4*>(lists) + i;
-    for (uint32_t k = 0; k < count; k += 8) {
-        const uint4 a = p[(size_t)(k >> 2) * cap];
-        const uint4 b = p[(size_t)((k >> 2) + 1) * cap];
-        const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        uint32_t j[8];
-        bool v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            v[u] = k + u < count;
-            j[u] = v[u] ? w[u] : pad;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) f(j[u], v[u]);
-    }
+    for_each_nbr32(lists, cap, i, count, pad, [](uint32_t e) { return e; }, f);
+}
+
+// Entries of the 16-bit lists are LDS byte offsets of float4 slots (16 x the
+// index in the block's set), so a staged read needs no address arithmetic.
+__device__ __forceinline__ const float4& lds_f4(const float4* base, uint32_t off)
+{
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + off);
+}
+
+__device__ __forceinline__ float lds_f1(const float* base, uint32_t off)   // base: one float per slot
+{
+    return base[off >> 4];
 }
 
 // Loads an LDS-staged set: st(k, global id) for k < count, then a block
@@ -366,19 +468,20 @@ __device__ __forceinline__ uint32_t stage_block(const NbrLists& L, uint32_t b, S
 }
 
 // The fluid neighbors of particle i (block b, sn = stage_n[b]) in whichever
-// form its block's lists have: from_lds(staged index, valid) for an
-// LDS-staged set, otherwise from_global(global id, valid).
-template <class FL, class FG>
+// form its block's lists have: lds(LDS byte offset) loads a neighbor of an
+// LDS-staged set, glob(global id) one from global memory; body(data, valid)
+// consumes it.
+template <class FL, class FG, class Body>
 __device__ __forceinline__ void fluid_nbrs(const NbrLists& L, uint32_t cap, uint32_t b, uint32_t i, uint32_t nf,
-                                           uint32_t sn, FL&& from_lds, FG&& from_global)
+                                           uint32_t sn, FL&& lds, FG&& glob, Body&& body)
 {
     if (sn <= kStageMax) {
-        for_each_nbr16(L.nbr16, cap, i, nf, from_lds);
+        for_each_nbr16(L.nbr16, cap, i, nf, lds, body);
     } else if (sn != kUnstaged) {
         const uint32_t* g = L.stg + (size_t)b * kStageCap;
-        for_each_nbr16(L.nbr16, cap, i, nf, [&](uint32_t l, bool v) { from_global(g[l], v); });
+        for_each_nbr16(L.nbr16, cap, i, nf, [&](uint32_t e) { return glob(g[e >> 4]); }, body);
     } else {
-        for_each_nbr32(L.nbrf, cap, i, nf, i, from_global);
+        for_each_nbr32(L.nbrf, cap, i, nf, i, glob, body);
     }
 }
 

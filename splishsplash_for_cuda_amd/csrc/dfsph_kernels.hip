@@ -340,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
     __shared__ uint16_t s_map[kAdjMax];        // slot (own cell x 27 + m) -> unique cell
     __shared__ uint32_t s_w[kBlock / 64];
     static_assert(kAdjMax * sizeof(uint64_t) <= kStageMax * sizeof(float4), "sort buffer exceeds the stage");
-    static_assert(kStageCap <= 65536 && kAdjMax <= 2048, "16-bit indices / 11-bit slots");
+    static_assert((kStageCap - 1) * 16 <= 0xffffu && kAdjMax <= 2048, "16-bit byte offsets / 11-bit slots");
     uint64_t* s_sort = reinterpret_cast<uint64_t*>(s_raw);
     float4* s_pos = reinterpret_cast<float4*>(s_raw);
     constexpr uint16_t kNoCell = 0xffffu;
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
                     if (l == self) continue;
                     const float4 qp = in_lds ? s_pos[l] : pos[src + l];
                     if (dist2(p.x - qp.x, p.y - qp.y, p.z - qp.z) < c.hk2) {
-                        if (nf < maxn) w.push(nf, l);
+                        if (nf < maxn) w.push(nf, l << 4);          // LDS byte offset
                         ++nf;
                     }
                 }
@@ -548,7 +548,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_lists(uint32_t n, uint32_t ca
     for (uint32_t k = 0; k < nf; ++k) {
         uint32_t j;
         if (sn != kUnstaged) {
-            const uint32_t l = L.nbr16[(((size_t)(k >> 3) * cap + i) << 3) + (k & 7u)];
+            const uint32_t l = L.nbr16[(((size_t)(k >> 3) * cap + i) << 3) + (k & 7u)] >> 4;
             j = L.stg[(size_t)b * kStageCap + l];
         } else {
             j = L.nbrf[(((size_t)(k >> 2) * cap + i) << 2) + (k & 3u)];
@@ -588,6 +588,15 @@ __global__ void k_set_w(float4* p, const float* w, uint32_t n)
 // ------------------------------------------------------------------------
 // solver sweeps
 // ------------------------------------------------------------------------
+
+// neighbor records loaded in the first phase of a batch
+struct PV {
+    float4 p, v;                   // position + mass, velocity (+ a per-kernel scalar in w)
+};
+struct PK {
+    float4 p;                      // position + mass
+    float k;                       // k_j of the Jacobi sweep / the warm start
+};
 
 // Reducing sweeps write one FP64 partial per block with a plain store; the
 // separate one-block kernel below sums them.  (A "last block finishes"
@@ -669,9 +678,9 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
     auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
         const float m = keep(valid, pj.w);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const PairK kp = kernel_pair(c, dist2(dx, dy, dz));
+        const PairL kp = pair_listed(c, dist2(dx, dy, dz));
         dens += m * kp.W;
-        const float mg = m * kp.g;
+        const float mg = keep(kp.ok, m * kp.g);
         const float px = mg * dx, py = mg * dy, pz = mg * dz;
         sg2 += px * px + py * py + pz * pz;
         gx += px; gy += py; gz += pz;
@@ -682,19 +691,20 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
             ax += s * px; ay += s * py; az += s * pz;
         }
     };
-    fluid_nbrs(L, stride, b, i, nf, sn, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); },
-               [&](uint32_t j, bool valid) {
+    fluid_nbrs(L, stride, b, i, nf, sn, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+               [&](uint32_t j) {
                    const float4 v = vel_i[j];
-                   fluid(pos[j], make_float4(v.x, v.y, v.z, ws_of(j)), valid);
-               });
+                   return PV{pos[j], make_float4(v.x, v.y, v.z, ws_of(j))};
+               },
+               [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
     float bx = 0.0f, by = 0.0f, bz = 0.0f;
-    for_each_nbr32(L.nbrb, stride, i, nb, 0u, [&](uint32_t j, bool valid) {
-        const float4 pj = bpos[j];
+    for_each_nbr32(L.nbrb, stride, i, nb, 0u, [&](uint32_t j) { return bpos[j]; },
+        [&](const float4& pj, bool valid) {
         const float m = keep(valid, pj.w);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const PairK kp = kernel_pair(c, dist2(dx, dy, dz));
+        const PairL kp = pair_listed(c, dist2(dx, dy, dz));
         dens += m * kp.W;
-        const float mg = m * kp.g;
+        const float mg = keep(kp.ok, m * kp.g);
         const float px = mg * dx, py = mg * dy, pz = mg * dz;
         sg2 += px * px + py * py + pz * pz;
         gx += px; gy += py; gz += pz;
@@ -746,15 +756,18 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
         float dr = 0.0f;
         auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
+            const PairL kp = grad_listed(c, dist2(dx, dy, dz));
+            const float mg = keep(valid && kp.ok, pj.w * kp.g);
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         };
-        fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); },
-                   [&](uint32_t j, bool valid) { fluid(pos[j], vel[j], valid); });
-        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j, bool valid) {
-            const float4 pj = bpos[j];
+        fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+                   [&](uint32_t j) { return PV{pos[j], vel[j]}; },
+                   [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
+        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j) { return bpos[j]; },
+            [&](const float4& pj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
+            const PairL kp = grad_listed(c, dist2(dx, dy, dz));
+            const float mg = keep(valid && kp.ok, pj.w * kp.g);
             dr += mg * (vi.x * dx + vi.y * dy + vi.z * dz);
         });
         r = (nf + nb < 20u) ? 0.0f : fmaxf(dr, 0.0f);
@@ -794,17 +807,19 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t,
     auto fluid = [&](const float4& pj, float kj, bool valid) {
         const float ks = ki + kj;
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float g = kernel_grad_coef(c, dist2(dx, dy, dz));
-        const float s = keep(valid && fabsf(ks) > kEps, ks * pj.w * g);
+        const PairL kp = grad_listed(c, dist2(dx, dy, dz));
+        const float s = keep(valid && kp.ok && fabsf(ks) > kEps, ks * pj.w * kp.g);
         ax += s * dx; ay += s * dy; az += s * dz;
     };
-    fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t l, bool valid) { fluid(sP[l], sK[l], valid); },
-               [&](uint32_t j, bool valid) { fluid(pos[j], kk[j], valid); });
+    fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t e) { return PK{lds_f4(sP, e), lds_f1(sK, e)}; },
+               [&](uint32_t j) { return PK{pos[j], kk[j]}; },
+               [&](const PK& d, bool valid) { fluid(d.p, d.k, valid); });
     if (fabsf(ki) > kEps) {
-        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j, bool valid) {
-            const float4 pj = bpos[j];
+        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j) { return bpos[j]; },
+            [&](const float4& pj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float s = keep(valid, ki * pj.w * kernel_grad_coef(c, dist2(dx, dy, dz)));
+            const PairL kp = grad_listed(c, dist2(dx, dy, dz));
+            const float s = keep(valid && kp.ok, ki * pj.w * kp.g);
             ax += s * dx; ay += s * dy; az += s * dz;
         });
     }
@@ -851,16 +866,17 @@ __global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, u
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float w = kernel_W(c, dist2(dx, dy, dz));
+            const float w = W_listed(c, dist2(dx, dy, dz));
             const float s = keep(valid, coef * vj.w * w);
             ax -= s * (vi.x - vj.x); ay -= s * (vi.y - vj.y); az -= s * (vi.z - vj.z);
         };
-        fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); },
-                   [&](uint32_t j, bool valid) {
+        fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+                   [&](uint32_t j) {
                        const float4 p = pos[j];
                        const float4 v = vel[j];
-                       fluid(p, make_float4(v.x, v.y, v.z, p.w / rho[j]), valid);
-                   });
+                       return PV{p, make_float4(v.x, v.y, v.z, p.w / rho[j])};
+                   },
+                   [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
         const float4 a = make_float4(c.gx + ax, c.gy + ay, c.gz + az, 0.0f);
         acc[i] = a;
         const float h = t->h;
@@ -946,17 +962,19 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
     auto fluid = [&](const float4& pj, float kj, bool valid) {
         const float ks = ki + kj;
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float g = kernel_grad_coef(c, dist2(dx, dy, dz));
-        const float s = keep(valid && fabsf(ks) > kEps, ks * pj.w * g);
+        const PairL kp = grad_listed(c, dist2(dx, dy, dz));
+        const float s = keep(valid && kp.ok && fabsf(ks) > kEps, ks * pj.w * kp.g);
         ax += s * dx; ay += s * dy; az += s * dz;
     };
-    fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t l, bool valid) { fluid(sP[l], sK[l], valid); },
-               [&](uint32_t j, bool valid) { fluid(pos[j], ws_of(j), valid); });
+    fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t e) { return PK{lds_f4(sP, e), lds_f1(sK, e)}; },
+               [&](uint32_t j) { return PK{pos[j], ws_of(j)}; },
+               [&](const PK& d, bool valid) { fluid(d.p, d.k, valid); });
     if (fabsf(ki) > kEps) {
-        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j, bool valid) {
-            const float4 pj = bpos[j];
+        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j) { return bpos[j]; },
+            [&](const float4& pj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float s = keep(valid, ki * pj.w * kernel_grad_coef(c, dist2(dx, dy, dz)));
+            const PairL kp = grad_listed(c, dist2(dx, dy, dz));
+            const float s = keep(valid && kp.ok, ki * pj.w * kp.g);
             ax += s * dx; ay += s * dy; az += s * dz;
         });
     }
@@ -1004,15 +1022,18 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
         float dr = 0.0f;
         auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
+            const PairL kp = grad_listed(c, dist2(dx, dy, dz));
+            const float mg = keep(valid && kp.ok, pj.w * kp.g);
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         };
-        fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); },
-                   [&](uint32_t j, bool valid) { fluid(pos[j], vel[j], valid); });
-        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j, bool valid) {
-            const float4 pj = bpos[j];
+        fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+                   [&](uint32_t j) { return PV{pos[j], vel[j]}; },
+                   [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
+        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j) { return bpos[j]; },
+            [&](const float4& pj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float mg = keep(valid, pj.w) * kernel_grad_coef(c, dist2(dx, dy, dz));
+            const PairL kp = grad_listed(c, dist2(dx, dy, dz));
+            const float mg = keep(valid && kp.ok, pj.w * kp.g);
             dr += mg * (vi.x * dx + vi.y * dy + vi.z * dz);
         });
         float a = alpha[i];
@@ -1122,25 +1143,26 @@ __global__ __launch_bounds__(kBlock) void k_features(SimConst c, const DevTime* 
         const float m = keep(valid, 1.0f);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2(dx, dy, dz);
-        const PairK kp = kernel_pair(c, r2);
+        const PairL kp = pair_listed(c, r2);
         const float mj = m * pj.w;
         dfl += mj * kp.W;
-        dr += mj * kp.g * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+        dr += keep(kp.ok, mj * kp.g) * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         ox -= m * dx; oy -= m * dy; oz -= m * dz;
         od += m * __builtin_sqrtf(r2);
     };
-    fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t l, bool valid) { fluid(sP[l], sV[l], valid); },
-               [&](uint32_t j, bool valid) { fluid(pos[j], vel[j], valid); });
+    fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+               [&](uint32_t j) { return PV{pos[j], vel[j]}; },
+               [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
     float dbd = 0.0f;
-    for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j, bool valid) {
-        const float4 pj = bpos[j];
+    for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j) { return bpos[j]; },
+        [&](const float4& pj, bool valid) {
         const float m = keep(valid, 1.0f);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2(dx, dy, dz);
-        const PairK kp = kernel_pair(c, r2);
+        const PairL kp = pair_listed(c, r2);
         const float mj = m * pj.w;
         dbd += mj * kp.W;
-        dr += mj * kp.g * (vi.x * dx + vi.y * dy + vi.z * dz);
+        dr += keep(kp.ok, mj * kp.g) * (vi.x * dx + vi.y * dy + vi.z * dz);
         ox -= m * dx; oy -= m * dy; oz -= m * dz;
         od += m * __builtin_sqrtf(r2);
     });

@@ -257,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void k_tag_neighbors(SimConst c, uint32_t n
     };
     if (sn != kUnstaged) {
         const uint32_t* g = L.stg + (size_t)b * kStageCap;
-        for_each_nbr16(L.nbr16, c.cap, i, nf, [&](uint32_t l, bool v) { mark(g[l], v); });
+        for_each_nbr16(L.nbr16, c.cap, i, nf, [&](uint32_t e, bool v) { mark(g[e >> 4], v); });
     } else {
         for_each_nbr32(L.nbrf, c.cap, i, nf, i, mark);
     }

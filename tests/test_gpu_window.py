@@ -9,8 +9,8 @@ Tolerances:
   outflow removal           exact set by id
   tags / active-first order exact
   restricted steps          density 2e-6 rho0, velocity 5e-5 (max |v| + g h),
-                            position 1e-6 r, iteration counts equal, frozen
-                            particles bit-identical
+                            position 1e-5 r (FP32 positions), iteration
+                            counts equal, frozen particles bit-identical
 """
 import numpy as np
 import pytest
@@ -96,7 +96,7 @@ def test_open_boundary_inflow_and_outflow_match_oracle(oracle_mod):
     om = oracle_mod
     sc, cand, R = _basin()
     r = sc.particle_radius
-    s = _solver(sc, dt=0.002)
+    s = _solver(sc, dt=0.002, fluid_capacity=sc.n_fluid + len(cand) + 64)
     for _ in range(5):
         s.step()
     kw_b = dict(center=(0, 0, 0), half=(0, 10, 0), radius=R)
@@ -161,7 +161,7 @@ def test_window_displacement_moves_surfaces_and_inflow(oracle_mod):
     r = sc.particle_radius
     res = []
     for shift in (np.zeros(3), np.array([0.0, 0.125, 0.0])):     # 0.125: exact in binary
-        s = _solver(sc, dt=0.002)
+        s = _solver(sc, dt=0.002, fluid_capacity=sc.n_fluid + len(cand) + 64)
         for _ in range(3):
             s.step()
         c0 = tuple(-shift)
@@ -264,7 +264,7 @@ def test_restricted_steps_match_oracle(oracle_mod, mode):
         os_[f] = os_[f][perm]
     vscale = np.abs(os_["vel"]).max() + 9.81 * dt
     assert max_rel(gs["vel"], os_["vel"], vscale) < 5e-5
-    assert max_rel(gs["pos"], os_["pos"], r) < 1e-6
+    assert max_rel(gs["pos"], os_["pos"], r) < 1e-5      # FP32 rounding of x += h v over 3 steps
     src = np.isin(gs["id"], ids[np.isin(t_dev, (TAG_ACTIVE, TAG_NBR))])
     assert max_rel(gs["density"][src], os_["density"][src], 1000.0) < 2e-6
     # everything outside the active set is frozen
