@@ -93,7 +93,7 @@ struct DevCtl {
 
 // ---- staged neighbor sets ----
 // A sweep block (256 consecutive particles) owns the set of fluid particles in
-// the cells adjacent to its particles' cells, in cell-key order.  Up to
+// the cells adjacent to its particles' cells, cell by cell.  Up to
 // kStageMax of them are loaded into LDS once per sweep (coalesced) and the
 // block's fluid lists are 16-bit indices into the set; sets up to kStageCap
 // keep the 16-bit lists and read the particles through the set's global ids;
@@ -549,25 +549,6 @@ __device__ __forceinline__ uint32_t block_excl_scan_u(uint32_t v, uint32_t* excl
     *excl = off + incl - v;
     __syncthreads();
     return tot;
-}
-
-// Ascending bitonic sort of a[0, P) in LDS, P a power of two (kBlock threads).
-template <class T>
-__device__ __forceinline__ void bitonic_sort_lds(T* a, uint32_t P)
-{
-    for (uint32_t k = 2; k <= P; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t t = threadIdx.x; t < P; t += kBlock) {
-                const uint32_t x = t ^ j;
-                if (x > t) {
-                    const T va = a[t], vb = a[x];
-                    const bool up = (t & k) == 0;
-                    if ((va > vb) == up) { a[t] = vb; a[x] = va; }
-                }
-            }
-            __syncthreads();
-        }
-    }
 }
 
 // Block sum in a fixed order (wave tree, then waves 0..3): deterministic.

@@ -263,28 +263,30 @@ __global__ void k_gather_f4(uint32_t n, const uint32_t* perm, const float4* in, 
 }
 
 // Appends list entries to a particle's blocked-interleaved list: entries
-// collect in registers and each full 16-byte group is one store.
+// shift into a 16-byte register group from the top (a funnel shift per word)
+// and each full group is one store.
 struct ListWriter16 {
     uint4* out;                    // group 0 of this particle
     uint32_t cap;
     uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
-    __device__ void push(uint32_t n, uint32_t v)      // v = entry n
+    __device__ void shift_in(uint32_t v)
     {
-        const uint32_t p = n & 7u, x = v << ((p & 1u) * 16u), wi = p >> 1;
-        w0 |= wi == 0 ? x : 0u;
-        w1 |= wi == 1 ? x : 0u;
-        w2 |= wi == 2 ? x : 0u;
-        w3 |= wi == 3 ? x : 0u;
-        if (p == 7u) flush(n);
+        w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+        w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+        w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+        w3 = __builtin_amdgcn_alignbit(v, w3, 16);
     }
-    __device__ void flush(uint32_t n)                 // the group holding entry n
+    __device__ void push(uint32_t n, uint32_t v)      // v = entry n (16 bits)
     {
-        out[(size_t)(n >> 3) * cap] = make_uint4(w0, w1, w2, w3);
-        w0 = w1 = w2 = w3 = 0;
+        shift_in(v);
+        if ((n & 7u) == 7u) out[(size_t)(n >> 3) * cap] = make_uint4(w0, w1, w2, w3);
     }
     __device__ void finish(uint32_t n)                // n entries pushed in total
     {
-        if (n & 7u) flush(n);
+        if (n & 7u) {
+            for (uint32_t k = n & 7u; k < 8u; ++k) shift_in(0u);
+            out[(size_t)(n >> 3) * cap] = make_uint4(w0, w1, w2, w3);
+        }
     }
 };
 
@@ -292,36 +294,50 @@ struct ListWriter32 {
     uint4* out;
     uint32_t cap;
     uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    __device__ void shift_in(uint32_t v)
+    {
+        w0 = w1;
+        w1 = w2;
+        w2 = w3;
+        w3 = v;
+    }
     __device__ void push(uint32_t n, uint32_t v)
     {
-        const uint32_t p = n & 3u;
-        w0 = p == 0 ? v : w0;
-        w1 = p == 1 ? v : w1;
-        w2 = p == 2 ? v : w2;
-        w3 = p == 3 ? v : w3;
-        if (p == 3u) flush(n);
+        shift_in(v);
+        if ((n & 3u) == 3u) out[(size_t)(n >> 2) * cap] = make_uint4(w0, w1, w2, w3);
     }
-    __device__ void flush(uint32_t n) { out[(size_t)(n >> 2) * cap] = make_uint4(w0, w1, w2, w3); }
     __device__ void finish(uint32_t n)
     {
-        if (n & 3u) flush(n);
+        if (n & 3u) {
+            for (uint32_t k = n & 3u; k < 4u; ++k) shift_in(0u);
+            out[(size_t)(n >> 2) * cap] = make_uint4(w0, w1, w2, w3);
+        }
     }
 };
+
+// Open-addressing table of the cell keys adjacent to a block's cells.
+constexpr uint32_t kHash = 2048;           // slots (>= kAdjMax: an insertion always finds one)
+constexpr uint32_t kSetCells = 512;        // unique cells of a tabled neighbor set (LDS: 4 blocks per CU)
+__device__ __forceinline__ uint32_t hash_slot(uint32_t key) { return (key * 0x9E3779B1u) >> 21; }
 
 // DFSPH_neighborsSearch_kernel<true> (DFSPH_core_cuda.cu:2219-2531,
 // DFSPH_macro_cuda.h:136-155): every fluid particle within h (strict, FP32) of
 // particle i except i itself, over the 27 cells around its cell, then the
 // boundary particles the same way, at most max_neighbors in total (fluid
-// first).
+// first).  Cells are visited in for_each_adjacent_cell's order and each
+// cell's particles in index order, in every list form.
 //
-// Each block also builds its neighbor set.  The keys of the cells adjacent
-// to its own cells (own cell first, 27 slots per own cell) are sorted in LDS
-// as (key << 11 | slot); the unique keys give the set's cell ranges and
-// prefix offsets, and the slot map gives every own cell its 27 ranges without
-// a search.  Sets up to kStageMax have their positions staged in LDS for the
-// candidate tests; sets up to kStageCap store 16-bit lists; larger ones and
-// the INDIRECT search (fluid not reordered, the table indexes it through
-// perm: the need_sort == false path) store 32-bit global lists.
+// Each block also builds its neighbor set.  The keys of the 27 cells around
+// each of its own cells go into an LDS hash table (slot = own cell x 27 + m);
+// the occupied table slots, compacted in slot order, are the set's unique
+// cells with their fluid ranges, set offsets and boundary ranges, and the
+// slot map gives every own cell its 27 unique cells.  The set order depends
+// on the order of the insertions, but no sum runs over the set: lists name
+// cells in the fixed order above, so results do not depend on it.  Sets up to
+// kStageMax have their positions staged in LDS for the candidate tests; sets
+// up to kStageCap store 16-bit lists; larger ones and the INDIRECT search
+// (fluid not reordered, the grid indexes it through perm: the need_sort ==
+// false path) store 32-bit global lists.
 template <bool INDIRECT>
 __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* __restrict__ pos,
                                                       const uint32_t* __restrict__ cs_f,
@@ -333,15 +349,21 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
                                                       uint32_t* __restrict__ stage_n, uint32_t* __restrict__ stg,
                                                       Restrict rs, DevCtl* ctl)
 {
-    // the sort buffer (kAdjMax x 8 B) and the staged positions share one area
+    // the hash table and own-cell coordinates share one area with the staged positions
     __shared__ __align__(16) unsigned char s_raw[kStageMax * sizeof(float4)];
-    __shared__ uint32_t s_off[kAdjMax + 1];    // own keys, then set offsets of the unique cells
-    __shared__ uint32_t s_src[kAdjMax];        // unique keys, then their first table slot
+    __shared__ uint32_t s_off[kSetCells + 1];  // own keys, then set offsets of the unique cells
+    __shared__ uint32_t s_src[kSetCells];      // first fluid index of each unique cell
+    __shared__ uint32_t s_bs[kSetCells];       // boundary range of each unique cell
+    __shared__ uint32_t s_be[kSetCells];
     __shared__ uint16_t s_map[kAdjMax];        // slot (own cell x 27 + m) -> unique cell
     __shared__ uint32_t s_w[kBlock / 64];
-    static_assert(kAdjMax * sizeof(uint64_t) <= kStageMax * sizeof(float4), "sort buffer exceeds the stage");
-    static_assert((kStageCap - 1) * 16 <= 0xffffu && kAdjMax <= 2048, "16-bit byte offsets / 11-bit slots");
-    uint64_t* s_sort = reinterpret_cast<uint64_t*>(s_raw);
+    static_assert(kHash * 4 + 3 * kBlock * 4 <= kStageMax * sizeof(float4), "table exceeds the stage");
+    static_assert(kHash >= kAdjMax && kHash == 2048, "hash_slot yields 11 bits; every insertion finds a slot");
+    static_assert((kStageCap - 1) * 16 <= 0xffffu && kSetCells >= kBlock, "16-bit byte offsets / own keys");
+    uint32_t* s_tab = reinterpret_cast<uint32_t*>(s_raw);
+    int* s_ocx = reinterpret_cast<int*>(s_raw + kHash * 4);
+    int* s_ocy = s_ocx + kBlock;
+    int* s_ocz = s_ocy + kBlock;
     float4* s_pos = reinterpret_cast<float4*>(s_raw);
     constexpr uint16_t kNoCell = 0xffffu;
 
@@ -359,6 +381,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
         key = cell_key(c, cx, cy, cz);
     }
     uint32_t S = kUnstaged, U = 0, uo = 0;
+    bool tabled = false;                        // block uniform: s_map / s_off / s_src / s_b* valid
     if (!INDIRECT) {
         // unique own cells (keys are non-decreasing along the block)
         s_off[threadIdx.x] = key;
@@ -369,79 +392,71 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
         uo = ex + (first ? 1u : 0u) - 1u;          // own cell index (valid for active threads)
         if (nown * 27u <= kAdjMax) {
             const uint32_t m = nown * 27u;
-            if (first) {
-                const uint32_t s0 = uo * 27u;
-                uint32_t q = 0;
-                s_sort[s0] = ((uint64_t)key << 11) | s0;   // own cell first
-                ++q;
-                for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) {
-                    if (nk != key) { s_sort[s0 + q] = ((uint64_t)nk << 11) | (s0 + q); ++q; }
-                });
-                for (; q < 27u; ++q) s_sort[s0 + q] = ((uint64_t)kKeyInvalid << 11) | (s0 + q);
-            }
-            uint32_t P = 64;
-            while (P < m) P <<= 1;
-            for (uint32_t k = m + threadIdx.x; k < P; k += kBlock) s_sort[k] = ~0ull;
+            for (uint32_t k = threadIdx.x; k < kHash; k += kBlock) s_tab[k] = kKeyInvalid;
+            if (first) { s_ocx[uo] = cx; s_ocy[uo] = cy; s_ocz[uo] = cz; }
             __syncthreads();
-            bitonic_sort_lds(s_sort, P);
-            // unique keys and the slot map: element k belongs to the run that
-            // began at the last unique start at or before k
-            const uint32_t E = P > kBlock ? P / kBlock : 1u;
-            const uint32_t k0 = threadIdx.x * E;
-            uint64_t v[kAdjMax / kBlock];
-            uint32_t nloc = 0;
+            for (uint32_t sl = threadIdx.x; sl < m; sl += kBlock) {
+                const uint32_t o = sl / 27u, q = sl - o * 27u;
+                const int y = s_ocy[o] + (int)(q / 9u) - 1;
+                const int z = s_ocz[o] + (int)((q / 3u) % 3u) - 1;
+                const int x = s_ocx[o] + (int)(q % 3u) - 1;
+                uint32_t h = kNoCell;
+                if (x >= 0 && x < c.dim[0] && y >= 0 && y < c.dim[1] && z >= 0 && z < c.dim[2]) {
+                    const uint32_t nk = cell_key(c, x, y, z);
+                    h = hash_slot(nk);
+                    for (;;) {
+                        const uint32_t old = atomicCAS(s_tab + h, kKeyInvalid, nk);
+                        if (old == kKeyInvalid || old == nk) break;
+                        h = (h + 1u) & (kHash - 1u);
+                    }
+                }
+                s_map[sl] = (uint16_t)h;
+            }
+            __syncthreads();
+            // compaction: thread t owns table slots [8t, 8t + 8)
+            constexpr uint32_t E = kHash / kBlock;
+            uint32_t tk[E], fa[E], fc[E];
+            uint32_t occ = 0, fsum = 0;
 #pragma unroll
-            for (uint32_t e = 0; e < kAdjMax / kBlock; ++e) {
-                const uint32_t k = k0 + e;
-                v[e] = ~0ull;
-                if (e < E && k < P) {
-                    v[e] = s_sort[k];
-                    const uint32_t kk = (uint32_t)(v[e] >> 11);
-                    const uint32_t prev = k > 0 ? (uint32_t)(s_sort[k - 1] >> 11) : kKeyInvalid;
-                    if (kk != kKeyInvalid && kk != prev) ++nloc;
+            for (uint32_t e = 0; e < E; ++e) {
+                tk[e] = s_tab[threadIdx.x * E + e];
+                fa[e] = 0;
+                fc[e] = 0;
+                if (tk[e] != kKeyInvalid) {
+                    fa[e] = cs_f[tk[e]];
+                    fc[e] = cs_f[tk[e] + 1] - fa[e];
+                    ++occ;
+                    fsum += fc[e];
                 }
             }
-            uint32_t base = 0;
-            U = block_excl_scan_u(nloc, &base, s_w);
-            uint32_t cur = base - 1u;
+            uint32_t ubase = 0, obase = 0;
+            U = block_excl_scan_u(occ, &ubase, s_w);
+            const uint32_t tot = block_excl_scan_u(fsum, &obase, s_w);
+            tabled = U <= kSetCells;
+            if (tabled) {
 #pragma unroll
-            for (uint32_t e = 0; e < kAdjMax / kBlock; ++e) {
-                const uint32_t k = k0 + e;
-                if (e < E && k < P) {
-                    const uint32_t kk = (uint32_t)(v[e] >> 11), slot = (uint32_t)(v[e] & 2047u);
-                    const uint32_t prev = k > 0 ? (uint32_t)(s_sort[k - 1] >> 11) : kKeyInvalid;
-                    if (kk != kKeyInvalid && kk != prev) { cur = base++; s_src[cur] = kk; }
-                    if (slot < m) s_map[slot] = kk != kKeyInvalid ? (uint16_t)cur : kNoCell;
+                for (uint32_t e = 0; e < E; ++e) {
+                    if (tk[e] != kKeyInvalid) {
+                        const uint32_t u = ubase++;
+                        s_src[u] = fa[e];
+                        s_off[u] = obase;
+                        obase += fc[e];
+                        s_bs[u] = cs_b[tk[e]];
+                        s_be[u] = cs_b[tk[e] + 1];
+                        s_tab[threadIdx.x * E + e] = u;
+                    }
                 }
+                if (threadIdx.x == 0) s_off[U] = tot;
+                S = tot <= c.stage_cap ? tot : kUnstaged;
             }
             __syncthreads();
-            // particle counts of the unique cells -> prefix offsets
-            const uint32_t E2 = (U + kBlock - 1) / kBlock;
-            const uint32_t u0 = threadIdx.x * E2;
-            uint32_t sum = 0;
-            for (uint32_t e = 0; e < E2; ++e) {
-                const uint32_t u = u0 + e;
-                if (u < U) {
-                    const uint32_t kk = s_src[u];
-                    const uint32_t a = cs_f[kk], z = cs_f[kk + 1];
-                    s_src[u] = a;
-                    s_off[u] = z - a;
-                    sum += z - a;
+            if (tabled) {
+                for (uint32_t sl = threadIdx.x; sl < m; sl += kBlock) {
+                    const uint32_t h = s_map[sl];
+                    s_map[sl] = h == kNoCell ? kNoCell : (uint16_t)s_tab[h];
                 }
+                __syncthreads();
             }
-            uint32_t off = 0;
-            const uint32_t tot = block_excl_scan_u(sum, &off, s_w);
-            for (uint32_t e = 0; e < E2; ++e) {
-                const uint32_t u = u0 + e;
-                if (u < U) {
-                    const uint32_t cntu = s_off[u];
-                    s_off[u] = off;
-                    off += cntu;
-                }
-            }
-            if (threadIdx.x == 0) s_off[U] = tot;
-            S = tot <= c.stage_cap ? tot : kUnstaged;
-            __syncthreads();
         }
     }
     const bool in_lds = S <= c.stage_lds;
@@ -464,11 +479,11 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
     if (act && !is_source(rs, i)) cnt[i] = 0;           // restricted out: no lists
     if (act && is_source(rs, i)) {
         const uint32_t maxn = c.max_nbr, cap = c.cap;
+        const uint16_t* map = s_map + uo * 27u;
         uint32_t nf = 0, nb = 0;
         if (S != kUnstaged) {
             ListWriter16 w{reinterpret_cast<uint4*>(nbr16) + i, cap};
-            const uint16_t* map = s_map + uo * 27u;
-            const uint32_t self = s_off[map[0]] + (i - s_src[map[0]]);
+            const uint32_t self = s_off[map[13]] + (i - s_src[map[13]]);
             for (uint32_t q = 0; q < 27u; ++q) {
                 const uint32_t u = map[q];
                 if (u == kNoCell) continue;
@@ -485,8 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
             w.finish(nf < maxn ? nf : maxn);
         } else {
             ListWriter32 w{reinterpret_cast<uint4*>(nbrf) + i, cap};
-            for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) {
-                const uint32_t s0 = cs_f[nk], e = cs_f[nk + 1];
+            auto scan_range = [&](uint32_t s0, uint32_t e) {
                 for (uint32_t q = s0; q < e; ++q) {
                     const uint32_t j = INDIRECT ? perm[q] : q;
                     if (j == i) continue;
@@ -496,13 +510,20 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
                         ++nf;
                     }
                 }
-            });
+            };
+            if (tabled) {
+                for (uint32_t q = 0; q < 27u; ++q) {
+                    const uint32_t u = map[q];
+                    if (u != kNoCell) scan_range(s_src[u], s_src[u] + (s_off[u + 1] - s_off[u]));
+                }
+            } else {
+                for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) { scan_range(cs_f[nk], cs_f[nk + 1]); });
+            }
             w.finish(nf < maxn ? nf : maxn);
         }
         const uint32_t sf = nf < maxn ? nf : maxn;
         ListWriter32 wb{reinterpret_cast<uint4*>(nbrb) + i, cap};
-        for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) {
-            const uint32_t s0 = cs_b[nk], e = cs_b[nk + 1];
+        auto scan_boundary = [&](uint32_t s0, uint32_t e) {
             for (uint32_t j = s0; j < e; ++j) {
                 const float4 pj = bpos[j];
                 if (dist2(p.x - pj.x, p.y - pj.y, p.z - pj.z) < c.hk2) {
@@ -510,7 +531,15 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
                     ++nb;
                 }
             }
-        });
+        };
+        if (tabled) {
+            for (uint32_t q = 0; q < 27u; ++q) {
+                const uint32_t u = map[q];
+                if (u != kNoCell) scan_boundary(s_bs[u], s_be[u]);
+            }
+        } else {
+            for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) { scan_boundary(cs_b[nk], cs_b[nk + 1]); });
+        }
         const uint32_t sb = nb < maxn - sf ? nb : maxn - sf;
         wb.finish(sb);
         total = nf + nb;

@@ -81,26 +81,22 @@ def test_neighbor_lists_bit_exact_every_list_form(oracle_mod, list_mode):
 
 
 def test_list_forms_give_the_same_trajectory():
-    """LDS-staged and unstaged 16-bit lists read the same neighbors in the same
-    order: bit-identical over 20 steps.  The 32-bit lists visit cells in
-    another order, so sums differ in their last bits and, once a solve stops
-    one iteration earlier or later, by up to the solver tolerance: compared
-    after 2 steps, at the FP32 summation-order level."""
+    """LDS-staged, unstaged 16-bit and 32-bit global lists hold the same
+    neighbors in the same order (cells in one fixed order, each cell's
+    particles by index), so every sum runs in the same order: bit-identical
+    trajectories over 20 steps."""
     sc = scene.seeded_dam_break(5, 6000)
     out = {}
-    for mode, steps in ((S.LISTS_AUTO, 20), (S.LISTS_NO_LDS, 20), (S.LISTS_AUTO, 2), (S.LISTS_GLOBAL, 2)):
+    for mode in (S.LISTS_AUTO, S.LISTS_NO_LDS, S.LISTS_GLOBAL):
         s = _solver(sc, dt=0.002, list_mode=mode)
-        its = [(st.iters_v, st.iters) for st in (s.step() for _ in range(steps))]
-        out[(mode, steps)] = (s.state_by_id(), its)
-    a, b = out[(S.LISTS_AUTO, 20)], out[(S.LISTS_NO_LDS, 20)]
-    for f in ("pos", "vel", "density", "kappa", "kappaV"):
-        assert np.array_equal(a[0][f], b[0][f]), f
-    assert a[1] == b[1]
-    a2, g = out[(S.LISTS_AUTO, 2)], out[(S.LISTS_GLOBAL, 2)]
-    assert a2[1] == g[1]
-    assert max_rel(g[0]["density"], a2[0]["density"], 1000.0) < 2e-6
-    vs = np.abs(a2[0]["vel"]).max() + 9.81 * 0.002
-    assert np.abs(g[0]["vel"] - a2[0]["vel"]).max() < 5e-5 * vs
+        its = [(st.iters_v, st.iters) for st in (s.step() for _ in range(20))]
+        out[mode] = (s.state_by_id(), its)
+    a = out[S.LISTS_AUTO]
+    for mode in (S.LISTS_NO_LDS, S.LISTS_GLOBAL):
+        b = out[mode]
+        for f in ("pos", "vel", "density", "kappa", "kappaV"):
+            assert np.array_equal(a[0][f], b[0][f]), (mode, f)
+        assert a[1] == b[1], mode
 
 
 def test_neighbor_lists_bit_exact_random_cloud(oracle_mod):
