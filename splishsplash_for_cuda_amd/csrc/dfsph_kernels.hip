@@ -197,15 +197,35 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* out, uint32_t n, 
 // DFSPH_computeGridIdx_kernel (DFSPH_core_cuda.cu:1833-1848): cell id and the
 // particle's arrival slot in its cell.  Particles outside the grid are
 // clamped into the border cells.
-__global__ void k_cell_hist(SimConst c, const float4* pos, uint32_t n, uint32_t* key,
-                            uint32_t* slot, uint32_t* hist)
+// Consecutive particles mostly share a cell (the previous step left them in
+// cell order), so each run of equal keys in a wave takes its slots with one
+// atomic: the run's first lane adds the run length and the others offset from
+// the returned base.
+__global__ __launch_bounds__(kBlock) void k_cell_hist(SimConst c, const float4* pos, uint32_t n, uint32_t* key,
+                                                      uint32_t* slot, uint32_t* hist)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 p = pos[i];
-    const uint32_t k = cell_id(c, p.x, p.y, p.z);
-    key[i] = k;
-    slot[i] = atomicAdd(hist + k, 1u);
+    const bool act = i < n;
+    uint32_t k = kKeyInvalid;
+    if (act) {
+        const float4 p = pos[i];
+        k = cell_id(c, p.x, p.y, p.z);
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t prev = __shfl_up(k, 1, 64);
+    const bool head = lane == 0u || prev != k;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);   // lanes 0..lane
+    const uint32_t h = 63u - (uint32_t)__clzll((long long)(heads & upto));  // my run's first lane
+    const uint64_t after = heads & ~upto;
+    const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
+    uint32_t base = 0;
+    if (act && head) base = atomicAdd(hist + k, next - lane);
+    base = __shfl(base, (int)h, 64);
+    if (act) {
+        key[i] = k;
+        slot[i] = base + (lane - h);
+    }
 }
 
 // DFSPH_CountingSortIds_kernel (DFSPH_core_cuda.cu:1851-1858)
@@ -315,29 +335,47 @@ struct ListWriter32 {
     }
 };
 
-// Open-addressing table of the cell keys adjacent to a block's cells.
-constexpr uint32_t kHash = 2048;           // slots (>= kAdjMax: an insertion always finds one)
-constexpr uint32_t kSetCells = 512;        // unique cells of a tabled neighbor set (LDS: 4 blocks per CU)
-__device__ __forceinline__ uint32_t hash_slot(uint32_t key) { return (key * 0x9E3779B1u) >> 21; }
+// Neighbor-search tables.  A block's set cells are the grid cells within one
+// cell of its own cells; they are indexed through a dense table over the
+// bounding box of those cells, x fastest, then z, then y, so the set lists
+// them in (y, z, x) order and the three cells x-1..x+1 of any row (y, z) are
+// consecutive: one contiguous range of the set per row.
+constexpr uint32_t kNbrStage = 2048;       // set positions staged in LDS by the search
+constexpr uint32_t kDense = 8192;          // bounding-box cells of a tabled set
+constexpr uint32_t kSetCells = 384;        // set cells of a tabled set (LDS: 4 blocks per CU)
+constexpr uint32_t kNoRow = 0xffffffffu;
+constexpr int kIntMax = 0x7fffffff;
+
+__device__ __forceinline__ int wave_min_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
 
 // DFSPH_neighborsSearch_kernel<true> (DFSPH_core_cuda.cu:2219-2531,
 // DFSPH_macro_cuda.h:136-155): every fluid particle within h (strict, FP32) of
 // particle i except i itself, over the 27 cells around its cell, then the
 // boundary particles the same way, at most max_neighbors in total (fluid
-// first).  Cells are visited in for_each_adjacent_cell's order and each
-// cell's particles in index order, in every list form.
+// first).  Cells are visited in for_each_adjacent_cell's order (y, then z,
+// then x) and each cell's particles in index order, in every list form.
 //
-// Each block also builds its neighbor set.  The keys of the 27 cells around
-// each of its own cells go into an LDS hash table (slot = own cell x 27 + m);
-// the occupied table slots, compacted in slot order, are the set's unique
-// cells with their fluid ranges, set offsets and boundary ranges, and the
-// slot map gives every own cell its 27 unique cells.  The set order depends
-// on the order of the insertions, but no sum runs over the set: lists name
-// cells in the fixed order above, so results do not depend on it.  Sets up to
-// kStageMax have their positions staged in LDS for the candidate tests; sets
-// up to kStageCap store 16-bit lists; larger ones and the INDIRECT search
-// (fluid not reordered, the grid indexes it through perm: the need_sort ==
-// false path) store 32-bit global lists.
+// Each block also builds its neighbor set: the set cells are marked in the
+// dense bounding-box table, numbered by a block scan, and get their fluid
+// ranges, set offsets and boundary ranges.  Every lane then keeps its 9 rows
+// as (first, last) set cells in registers.  Sets up to kNbrStage have their
+// positions staged in LDS for the candidate tests (over the table, which is
+// dead by then); sets up to kStageCap store 16-bit lists; larger ones, blocks
+// whose tables do not fit, and the INDIRECT search (fluid not reordered, the
+// grid indexes it through perm: the need_sort == false path) store 32-bit
+// global lists.
 template <bool INDIRECT>
 __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* __restrict__ pos,
                                                       const uint32_t* __restrict__ cs_f,
@@ -349,128 +387,151 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
                                                       uint32_t* __restrict__ stage_n, uint32_t* __restrict__ stg,
                                                       Restrict rs, DevCtl* ctl)
 {
-    // the hash table and own-cell coordinates share one area with the staged positions
-    __shared__ __align__(16) unsigned char s_raw[kStageMax * sizeof(float4)];
-    __shared__ uint32_t s_off[kSetCells + 1];  // own keys, then set offsets of the unique cells
-    __shared__ uint32_t s_src[kSetCells];      // first fluid index of each unique cell
-    __shared__ uint32_t s_bs[kSetCells];       // boundary range of each unique cell
+    // the dense table shares one area with the staged positions
+    __shared__ __align__(16) unsigned char s_raw[kNbrStage * sizeof(float4)];
+    __shared__ uint32_t s_off[kSetCells + 1];  // set offsets of the set cells
+    __shared__ uint32_t s_src[kSetCells];      // first fluid index of each set cell
+    __shared__ uint32_t s_bs[kSetCells];       // boundary range of each set cell
     __shared__ uint32_t s_be[kSetCells];
-    __shared__ uint16_t s_map[kAdjMax];        // slot (own cell x 27 + m) -> unique cell
-    __shared__ uint32_t s_w[kBlock / 64];
-    static_assert(kHash * 4 + 3 * kBlock * 4 <= kStageMax * sizeof(float4), "table exceeds the stage");
-    static_assert(kHash >= kAdjMax && kHash == 2048, "hash_slot yields 11 bits; every insertion finds a slot");
-    static_assert((kStageCap - 1) * 16 <= 0xffffu && kSetCells >= kBlock, "16-bit byte offsets / own keys");
-    uint32_t* s_tab = reinterpret_cast<uint32_t*>(s_raw);
-    int* s_ocx = reinterpret_cast<int*>(s_raw + kHash * 4);
-    int* s_ocy = s_ocx + kBlock;
-    int* s_ocz = s_ocy + kBlock;
+    __shared__ int s_w[6 * (kBlock / 64)];
+    static_assert(kDense * sizeof(uint16_t) <= sizeof(s_raw), "dense table exceeds the stage");
+    static_assert((kStageCap - 1) * 16 <= 0xffffu && kSetCells < 0xffffu, "16-bit byte offsets / cell ids");
+    uint16_t* s_dense = reinterpret_cast<uint16_t*>(s_raw);
     float4* s_pos = reinterpret_cast<float4*>(s_raw);
-    constexpr uint16_t kNoCell = 0xffffu;
+    uint32_t* s_wu = reinterpret_cast<uint32_t*>(s_w);
 
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
     const bool act = i < n;
     float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     int cx = 0, cy = 0, cz = 0;
-    uint32_t key = kKeyInvalid;
     if (act) {
         p = pos[i];
         cx = cell_coord(c, p.x, 0);
         cy = cell_coord(c, p.y, 1);
         cz = cell_coord(c, p.z, 2);
-        key = cell_key(c, cx, cy, cz);
     }
-    uint32_t S = kUnstaged, U = 0, uo = 0;
-    bool tabled = false;                        // block uniform: s_map / s_off / s_src / s_b* valid
-    if (!INDIRECT) {
-        // unique own cells (keys are non-decreasing along the block)
-        s_off[threadIdx.x] = key;
+    uint32_t S = kUnstaged, U = 0, uown = 0;
+    uint32_t row[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) row[r] = kNoRow;
+    bool tabled = false;                        // block uniform: row / s_off / s_src / s_b* valid
+    if (!INDIRECT && n <= b * kBlock) S = 0;    // a block past the end: empty set
+    if (!INDIRECT && n > b * kBlock) {
+        // bounding box of the set cells, clamped to the grid
+        const int w = threadIdx.x >> 6;
+        const int lo_x = wave_min_i(act ? cx : kIntMax), hi_x = wave_max_i(act ? cx : -1);
+        const int lo_y = wave_min_i(act ? cy : kIntMax), hi_y = wave_max_i(act ? cy : -1);
+        const int lo_z = wave_min_i(act ? cz : kIntMax), hi_z = wave_max_i(act ? cz : -1);
+        if ((threadIdx.x & 63) == 0) {
+            s_w[w] = lo_x; s_w[4 + w] = lo_y; s_w[8 + w] = lo_z;
+            s_w[12 + w] = hi_x; s_w[16 + w] = hi_y; s_w[20 + w] = hi_z;
+        }
         __syncthreads();
-        const bool first = act && (threadIdx.x == 0 || s_off[threadIdx.x - 1] != key);
-        uint32_t ex = 0;
-        const uint32_t nown = block_excl_scan_u(first ? 1u : 0u, &ex, s_w);
-        uo = ex + (first ? 1u : 0u) - 1u;          // own cell index (valid for active threads)
-        if (nown * 27u <= kAdjMax) {
-            const uint32_t m = nown * 27u;
-            for (uint32_t k = threadIdx.x; k < kHash; k += kBlock) s_tab[k] = kKeyInvalid;
-            if (first) { s_ocx[uo] = cx; s_ocy[uo] = cy; s_ocz[uo] = cz; }
+        int lo[3] = {kIntMax, kIntMax, kIntMax}, hi[3] = {-1, -1, -1};
+#pragma unroll
+        for (int k = 0; k < kBlock / 64; ++k) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = min(lo[a], s_w[4 * a + k]);
+                hi[a] = max(hi[a], s_w[12 + 4 * a + k]);
+            }
+        }
+        __syncthreads();                        // s_w is reused by the scans
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = max(lo[a] - 1, 0);
+            hi[a] = min(hi[a] + 1, c.dim[a] - 1);
+        }
+        const uint32_t W = (uint32_t)(hi[0] - lo[0] + 1), D = (uint32_t)(hi[2] - lo[2] + 1);
+        const uint32_t V = W * D * (uint32_t)(hi[1] - lo[1] + 1);
+        if (V <= kDense) {
+            for (uint32_t k = threadIdx.x; k < V; k += kBlock) s_dense[k] = 0;
             __syncthreads();
-            for (uint32_t sl = threadIdx.x; sl < m; sl += kBlock) {
-                const uint32_t o = sl / 27u, q = sl - o * 27u;
-                const int y = s_ocy[o] + (int)(q / 9u) - 1;
-                const int z = s_ocz[o] + (int)((q / 3u) % 3u) - 1;
-                const int x = s_ocx[o] + (int)(q % 3u) - 1;
-                uint32_t h = kNoCell;
-                if (x >= 0 && x < c.dim[0] && y >= 0 && y < c.dim[1] && z >= 0 && z < c.dim[2]) {
-                    const uint32_t nk = cell_key(c, x, y, z);
-                    h = hash_slot(nk);
-                    for (;;) {
-                        const uint32_t old = atomicCAS(s_tab + h, kKeyInvalid, nk);
-                        if (old == kKeyInvalid || old == nk) break;
-                        h = (h + 1u) & (kHash - 1u);
+            const int xl = max(cx - 1, 0), xh = min(cx + 1, c.dim[0] - 1);
+            const uint32_t key = act ? cell_key(c, cx, cy, cz) : kKeyInvalid;
+            const bool first = act && ((threadIdx.x & 63) == 0 || __shfl_up(key, 1, 64) != key);
+            if (first) {                        // mark the 27 cells (one lane per run of a cell in a wave)
+                for (int dy = -1; dy <= 1; ++dy) {
+                    const int y = cy + dy;
+                    if (y < 0 || y >= c.dim[1]) continue;
+                    for (int dz = -1; dz <= 1; ++dz) {
+                        const int z = cz + dz;
+                        if (z < 0 || z >= c.dim[2]) continue;
+                        const uint32_t base = W * ((uint32_t)(z - lo[2]) + D * (uint32_t)(y - lo[1])) - lo[0];
+                        for (int x = xl; x <= xh; ++x) s_dense[base + x] = 1;
                     }
                 }
-                s_map[sl] = (uint16_t)h;
             }
             __syncthreads();
-            // compaction: thread t owns table slots [8t, 8t + 8)
-            constexpr uint32_t E = kHash / kBlock;
-            uint32_t tk[E], fa[E], fc[E];
-            uint32_t occ = 0, fsum = 0;
-#pragma unroll
-            for (uint32_t e = 0; e < E; ++e) {
-                tk[e] = s_tab[threadIdx.x * E + e];
-                fa[e] = 0;
-                fc[e] = 0;
-                if (tk[e] != kKeyInvalid) {
-                    fa[e] = cs_f[tk[e]];
-                    fc[e] = cs_f[tk[e] + 1] - fa[e];
-                    ++occ;
-                    fsum += fc[e];
-                }
-            }
-            uint32_t ubase = 0, obase = 0;
-            U = block_excl_scan_u(occ, &ubase, s_w);
-            const uint32_t tot = block_excl_scan_u(fsum, &obase, s_w);
+            // number the marked cells in table order; thread t owns entries [tE, tE + E)
+            const uint32_t E = (V + kBlock - 1) / kBlock;
+            const uint32_t k0 = threadIdx.x * E, k1 = min(k0 + E, V);
+            uint32_t occ = 0;
+            for (uint32_t k = k0; k < k1; ++k) occ += s_dense[k];
+            uint32_t ubase = 0;
+            U = block_excl_scan_u(occ, &ubase, s_wu);
             tabled = U <= kSetCells;
             if (tabled) {
-#pragma unroll
-                for (uint32_t e = 0; e < E; ++e) {
-                    if (tk[e] != kKeyInvalid) {
+                for (uint32_t k = k0; k < k1; ++k) {
+                    if (s_dense[k]) {
                         const uint32_t u = ubase++;
-                        s_src[u] = fa[e];
-                        s_off[u] = obase;
-                        obase += fc[e];
-                        s_bs[u] = cs_b[tk[e]];
-                        s_be[u] = cs_b[tk[e] + 1];
-                        s_tab[threadIdx.x * E + e] = u;
+                        const uint32_t t = k / W;
+                        const uint32_t key = cell_key(c, lo[0] + (int)(k - t * W), lo[1] + (int)(t / D),
+                                                      lo[2] + (int)(t % D));
+                        const uint32_t a = cs_f[key];
+                        s_src[u] = a;
+                        s_off[u] = cs_f[key + 1] - a;
+                        s_bs[u] = cs_b[key];
+                        s_be[u] = cs_b[key + 1];
+                        s_dense[k] = (uint16_t)u;
                     }
+                }
+            }
+            __syncthreads();
+            if (tabled) {
+                // fluid counts of the set cells -> set offsets
+                const uint32_t E2 = (U + kBlock - 1) / kBlock;
+                const uint32_t u0 = threadIdx.x * E2, u1 = min(u0 + E2, U);
+                uint32_t sum = 0;
+                for (uint32_t u = u0; u < u1; ++u) sum += s_off[u];
+                uint32_t off = 0;
+                const uint32_t tot = block_excl_scan_u(sum, &off, s_wu);
+                for (uint32_t u = u0; u < u1; ++u) {
+                    const uint32_t cu = s_off[u];
+                    s_off[u] = off;
+                    off += cu;
                 }
                 if (threadIdx.x == 0) s_off[U] = tot;
                 S = tot <= c.stage_cap ? tot : kUnstaged;
-            }
-            __syncthreads();
-            if (tabled) {
-                for (uint32_t sl = threadIdx.x; sl < m; sl += kBlock) {
-                    const uint32_t h = s_map[sl];
-                    s_map[sl] = h == kNoCell ? kNoCell : (uint16_t)s_tab[h];
+                // each lane's rows (dy, dz) as (first, last) set cell of x-1..x+1
+                if (act) {
+                    uown = s_dense[W * ((uint32_t)(cz - lo[2]) + D * (uint32_t)(cy - lo[1])) + (uint32_t)(cx - lo[0])];
+#pragma unroll
+                    for (int r = 0; r < 9; ++r) {
+                        const int y = cy + r / 3 - 1, z = cz + r % 3 - 1;
+                        if (y >= 0 && y < c.dim[1] && z >= 0 && z < c.dim[2]) {
+                            const uint32_t base = W * ((uint32_t)(z - lo[2]) + D * (uint32_t)(y - lo[1])) - lo[0];
+                            row[r] = (uint32_t)s_dense[base + xl] | ((uint32_t)s_dense[base + xh] << 16);
+                        }
+                    }
                 }
-                __syncthreads();
+                __syncthreads();                // the table area is staged over next
             }
         }
     }
-    const bool in_lds = S <= c.stage_lds;
-    if (threadIdx.x == 0) stage_n[b] = in_lds || S == kUnstaged ? S : (S | kSetNoLds);
+    const bool lds_search = S <= kNbrStage && c.stage_lds > 0u;
+    if (threadIdx.x == 0) stage_n[b] = S == kUnstaged || S <= c.stage_lds ? S : (S | kSetNoLds);
     if (S != kUnstaged) {
         uint32_t* g_out = stg + (size_t)b * kStageCap;
         for (uint32_t k = threadIdx.x; k < S; k += kBlock) {
-            uint32_t lo = 0, hi = U;            // the unique cell holding set index k
+            uint32_t lo = 0, hi = U;            // the set cell holding set index k
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
                 if (s_off[mid] <= k) lo = mid; else hi = mid;
             }
             const uint32_t g = s_src[lo] + (k - s_off[lo]);
-            if (in_lds) s_pos[k] = pos[g];
+            if (lds_search) s_pos[k] = pos[g];
             g_out[k] = g;
         }
         __syncthreads();
@@ -479,21 +540,47 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
     if (act && !is_source(rs, i)) cnt[i] = 0;           // restricted out: no lists
     if (act && is_source(rs, i)) {
         const uint32_t maxn = c.max_nbr, cap = c.cap;
-        const uint16_t* map = s_map + uo * 27u;
+        const float hk2 = c.hk2;
         uint32_t nf = 0, nb = 0;
         if (S != kUnstaged) {
             ListWriter16 w{reinterpret_cast<uint4*>(nbr16) + i, cap};
-            const uint32_t self = s_off[map[13]] + (i - s_src[map[13]]);
-            for (uint32_t q = 0; q < 27u; ++q) {
-                const uint32_t u = map[q];
-                if (u == kNoCell) continue;
-                const uint32_t e = s_off[u + 1], src = s_src[u] - s_off[u];
-                for (uint32_t l = s_off[u]; l < e; ++l) {
-                    if (l == self) continue;
-                    const float4 qp = in_lds ? s_pos[l] : pos[src + l];
-                    if (dist2(p.x - qp.x, p.y - qp.y, p.z - qp.z) < c.hk2) {
-                        if (nf < maxn) w.push(nf, l << 4);          // LDS byte offset
-                        ++nf;
+            const uint32_t self = s_off[uown] + (i - s_src[uown]);
+            if (lds_search) {
+#pragma unroll
+                for (int r = 0; r < 9; ++r) {
+                    if (row[r] == kNoRow) continue;
+                    const uint32_t e = s_off[(row[r] >> 16) + 1u];
+                    uint32_t l = s_off[row[r] & 0xffffu];
+                    for (; l + 2u <= e; l += 2u) {               // two reads in flight
+                        const float4 qa = s_pos[l], qb = s_pos[l + 1u];
+                        const bool ia = dist2(p.x - qa.x, p.y - qa.y, p.z - qa.z) < hk2 && l != self;
+                        const bool ib = dist2(p.x - qb.x, p.y - qb.y, p.z - qb.z) < hk2 && l + 1u != self;
+                        if (ia && nf < maxn) w.push(nf, l << 4);  // LDS byte offset
+                        nf += ia ? 1u : 0u;
+                        if (ib && nf < maxn) w.push(nf, (l + 1u) << 4);
+                        nf += ib ? 1u : 0u;
+                    }
+                    if (l < e) {
+                        const float4 qa = s_pos[l];
+                        const bool ia = dist2(p.x - qa.x, p.y - qa.y, p.z - qa.z) < hk2 && l != self;
+                        if (ia && nf < maxn) w.push(nf, l << 4);
+                        nf += ia ? 1u : 0u;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 9; ++r) {
+                    if (row[r] == kNoRow) continue;
+                    for (uint32_t u = row[r] & 0xffffu; u <= (row[r] >> 16); ++u) {
+                        const uint32_t e = s_off[u + 1u], src = s_src[u] - s_off[u];
+                        for (uint32_t l = s_off[u]; l < e; ++l) {
+                            const float4 qp = pos[src + l];
+                            const bool in = dist2(p.x - qp.x, p.y - qp.y, p.z - qp.z) < hk2 && l != self;
+                            if (in) {
+                                if (nf < maxn) w.push(nf, l << 4);
+                                ++nf;
+                            }
+                        }
                     }
                 }
             }
@@ -503,18 +590,19 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
             auto scan_range = [&](uint32_t s0, uint32_t e) {
                 for (uint32_t q = s0; q < e; ++q) {
                     const uint32_t j = INDIRECT ? perm[q] : q;
-                    if (j == i) continue;
                     const float4 pj = pos[j];
-                    if (dist2(p.x - pj.x, p.y - pj.y, p.z - pj.z) < c.hk2) {
+                    if (dist2(p.x - pj.x, p.y - pj.y, p.z - pj.z) < hk2 && j != i) {
                         if (nf < maxn) w.push(nf, j);
                         ++nf;
                     }
                 }
             };
             if (tabled) {
-                for (uint32_t q = 0; q < 27u; ++q) {
-                    const uint32_t u = map[q];
-                    if (u != kNoCell) scan_range(s_src[u], s_src[u] + (s_off[u + 1] - s_off[u]));
+#pragma unroll
+                for (int r = 0; r < 9; ++r) {
+                    if (row[r] == kNoRow) continue;
+                    for (uint32_t u = row[r] & 0xffffu; u <= (row[r] >> 16); ++u)
+                        scan_range(s_src[u], s_src[u] + (s_off[u + 1u] - s_off[u]));
                 }
             } else {
                 for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) { scan_range(cs_f[nk], cs_f[nk + 1]); });
@@ -526,16 +614,17 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
         auto scan_boundary = [&](uint32_t s0, uint32_t e) {
             for (uint32_t j = s0; j < e; ++j) {
                 const float4 pj = bpos[j];
-                if (dist2(p.x - pj.x, p.y - pj.y, p.z - pj.z) < c.hk2) {
+                if (dist2(p.x - pj.x, p.y - pj.y, p.z - pj.z) < hk2) {
                     if (sf + nb < maxn) wb.push(nb, j);
                     ++nb;
                 }
             }
         };
         if (tabled) {
-            for (uint32_t q = 0; q < 27u; ++q) {
-                const uint32_t u = map[q];
-                if (u != kNoCell) scan_boundary(s_bs[u], s_be[u]);
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                if (row[r] == kNoRow) continue;
+                for (uint32_t u = row[r] & 0xffffu; u <= (row[r] >> 16); ++u) scan_boundary(s_bs[u], s_be[u]);
             }
         } else {
             for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) { scan_boundary(cs_b[nk], cs_b[nk + 1]); });
@@ -561,7 +650,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
         if (wsb) atomicAdd(&ctl->nbr_boundary, (unsigned long long)wsb);
         if (wsu) atomicAdd(&ctl->nbr_fluid_unstaged, (unsigned long long)wsu);
     }
-    if (threadIdx.x == 0 && !in_lds && n > b * kBlock) atomicAdd(&ctl->unstaged_blocks, 1u);
+    if (threadIdx.x == 0 && S > c.stage_lds && n > b * kBlock) atomicAdd(&ctl->unstaged_blocks, 1u);
 }
 
 // Every list of the last search as 32-bit global ids, fluid then boundary
