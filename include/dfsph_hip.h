@@ -158,9 +158,20 @@ int dfsph_divergence_solve(void* handle, uint32_t max_iter, double max_error,
 /* cuda_externalForces (DFSPH_core_cuda.h:55, DFSPH_core_cuda.cu:1613-1627): a = g + XSPH */
 int dfsph_external_forces(void* handle);
 
-/* cuda_CFL (DFSPH_core_cuda.h:77, DFSPH_core_cuda.cu:3321-3399); returns the new h
- * and applies it with DFSPHCData::updateTimeStep. */
+/* cuda_CFL (DFSPH_core_cuda.h:77, DFSPH_core_cuda.cu:3321-3399); applies the new h
+ * with DFSPHCData::updateTimeStep and returns it in dt_out (a host round trip).
+ * dt_out == NULL: no round trip, the value stays on the device (dfsph_get_time). */
 int dfsph_cfl(void* handle, double min_dt, double cfl_factor, double max_dt, double* dt_out);
+
+/* DFSPHCData::h, h_future and the simulated time (DFSPH_c_arrays_structure.h:
+ * 248-252) as the device holds them.  Right after dfsph_divergence_solve or
+ * dfsph_pressure_solve this reads the state their last loop record carried
+ * (no synchronisation); otherwise it waits for the stream. */
+int dfsph_get_time(void* handle, double* h, double* h_future, double* time);
+
+/* DFSPHCData::getFluidParticlesCount (DFSPH_c_arrays_structure.h:441): the
+ * active fluid count, a host-side value (no synchronisation). */
+int dfsph_fluid_count(void* handle, uint32_t* n);
 
 /* DFSPHCData::updateTimeStep (DFSPH_c_arrays_structure.cpp:1100-1106) */
 int dfsph_update_time_step(void* handle, double h_future);
@@ -187,7 +198,9 @@ int dfsph_set_time(void* handle, double h, double h_past, double time);
 /* DFSPHCData::checkParticlesPositions -> check_particles_positions_cuda
  * (DFSPH_c_arrays_structure.h:486-492, SPH_other_systems_cuda.cu:1283-1330):
  * mode 0 counts, mode 2 removes the particles outside the boundary bounding box
- * (order of the others kept). */
+ * (order of the others kept).  Called right after dfsph_step_end it first reads
+ * the count dfsph_update_pos made, from the step end's record, and returns at
+ * once when no particle left the box. */
 int dfsph_check_particles_positions(void* handle, int mode, uint32_t* count);
 
 /* remove_tagged_particles (SPH_other_systems_cuda.cu:1606-1641): remove the

@@ -201,6 +201,8 @@ HIP_SIGNATURES = {
     "dfsph_divergence_solve": (_I, [_P, _U32, _D, _U32P, C.POINTER(_D)]),
     "dfsph_external_forces": (_I, [_P]),
     "dfsph_cfl": (_I, [_P, _D, _D, _D, C.POINTER(_D)]),
+    "dfsph_get_time": (_I, [_P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)]),
+    "dfsph_fluid_count": (_I, [_P, C.POINTER(C.c_uint32)]),
     "dfsph_update_time_step": (_I, [_P, _D]),
     "dfsph_update_vel": (_I, [_P]),
     "dfsph_pressure_solve": (_I, [_P, _U32, _D, _U32P, C.POINTER(_D)]),
@@ -390,13 +392,27 @@ class Solver:
     def external_forces(self):
         _check(self.lib.dfsph_external_forces(self.h), "dfsph_external_forces")
 
-    def cfl(self, min_dt=None, cfl_factor=None, max_dt=None):
+    def cfl(self, min_dt=None, cfl_factor=None, max_dt=None, wait=True):
+        """CFL step size, applied on the device; returned when wait (a host
+        round trip), else left there for get_time()."""
         out = C.c_double()
         p = self.params
         _check(self.lib.dfsph_cfl(self.h, p.dt_min if min_dt is None else min_dt,
                                   p.cfl_factor if cfl_factor is None else cfl_factor,
-                                  p.dt_max if max_dt is None else max_dt, C.byref(out)), "dfsph_cfl")
-        return out.value
+                                  p.dt_max if max_dt is None else max_dt, C.byref(out) if wait else None),
+               "dfsph_cfl")
+        return out.value if wait else None
+
+    def get_time(self):
+        """(h, h_future, time) as the device holds them."""
+        a, b, t = C.c_double(), C.c_double(), C.c_double()
+        _check(self.lib.dfsph_get_time(self.h, C.byref(a), C.byref(b), C.byref(t)), "dfsph_get_time")
+        return a.value, b.value, t.value
+
+    def fluid_count(self):
+        n = C.c_uint32()
+        _check(self.lib.dfsph_fluid_count(self.h, C.byref(n)), "dfsph_fluid_count")
+        return n.value
 
     def update_time_step(self, h_future):
         _check(self.lib.dfsph_update_time_step(self.h, h_future), "dfsph_update_time_step")

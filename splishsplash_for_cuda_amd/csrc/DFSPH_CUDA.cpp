@@ -90,9 +90,9 @@ void DFSPHCData::zeroFluidVelocities()
 
 int DFSPHCData::getFluidParticlesCount() const
 {
-    dfsph_info info;
-    check(dfsph_get_info(m_handle, &info), "dfsph_get_info");
-    return (int)info.n_fluid;
+    uint32_t n = 0;
+    check(dfsph_fluid_count(m_handle, &n), "dfsph_fluid_count");
+    return (int)n;
 }
 
 int DFSPHCData::getFluidParticlesCountMax() const
@@ -166,14 +166,15 @@ void DFSPHCUDA::step()
     }
     check(dfsph_external_forces(h), "dfsph_external_forces");
     if (m_data.params.use_cfl) {
-        double hf = 0.0;
-        check(dfsph_cfl(h, m_data.params.dt_min, m_data.params.cfl_factor, m_data.params.dt_max, &hf), "dfsph_cfl");
-        m_data.h_future = hf;
+        // the new step size stays on the device; the host copy is read after
+        // the pressure solve from the state its loop already brought back
+        check(dfsph_cfl(h, m_data.params.dt_min, m_data.params.cfl_factor, m_data.params.dt_max, nullptr), "dfsph_cfl");
     } else {
         m_data.updateTimeStep(m_data.h);
     }
     check(dfsph_update_vel(h), "dfsph_update_vel");
     check(dfsph_pressure_solve(h, m_maxIterations, m_maxError, &m_iterations, &m_densityError), "dfsph_pressure_solve");
+    if (m_data.params.use_cfl) check(dfsph_get_time(h, nullptr, &m_data.h_future, nullptr), "dfsph_get_time");
     check(dfsph_update_pos(h), "dfsph_update_pos");
     m_data.onSimulationStepEnd();
     m_removedLastStep = 0;

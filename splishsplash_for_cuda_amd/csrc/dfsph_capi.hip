@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -104,7 +105,16 @@ struct Handle {
     uint32_t partial_n = 0;
     DevCtl* hctl = nullptr;           // pinned
     DevTime* htm = nullptr;           // pinned
-    SolveCtl* hslot = nullptr;        // pinned, 2 slots
+    // loop state written by the device into pinned host memory (hrec; drec is
+    // its device address), numbered by seq; last = the copy of the last one seen
+    dfsph::HostRecord* hrec = nullptr;
+    dfsph::HostRecord* drec = nullptr;
+    uint32_t seq = 0;
+    dfsph::HostRecord last{};
+    // ABI calls on this handle (counted by H()); the op of the last record
+    // wait and of the last step end, for the no-synchronisation fast paths
+    uint64_t ops = 0, rec_op = 0, end_op = 0, pos_op = 0;
+    uint32_t end_seq = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
 
     bool pend_vel = false;            // deferred cuda_update_vel (fused into the pressure warm start)
@@ -159,10 +169,14 @@ void pdrop(Handle* h, int idx)
     if (idx >= 0) h->prof_pending[idx].kind = -1;
 }
 
-// after a stream synchronisation: turn the recorded pairs into durations
-void prof_collect(Handle* h)
+// Turn the recorded pairs into durations: all of them after a stream
+// synchronisation, else only the leading pairs that have completed.
+void prof_collect(Handle* h, bool synced = true)
 {
-    for (auto& r : h->prof_pending) {
+    size_t k = 0;
+    for (; k < h->prof_pending.size(); ++k) {
+        auto& r = h->prof_pending[k];
+        if (!synced && hipEventQuery(r.b) != hipSuccess) break;
         if (r.kind >= 0) {
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
@@ -173,7 +187,28 @@ void prof_collect(Handle* h)
         h->prof_pool.push_back(r.a);
         h->prof_pool.push_back(r.b);
     }
-    h->prof_pending.clear();
+    h->prof_pending.erase(h->prof_pending.begin(), h->prof_pending.begin() + (std::ptrdiff_t)k);
+}
+
+// Wait until the device has published record `seq`, then keep a copy.  The
+// stream is queried now and then so that a failed launch ends the wait.
+int wait_record(Handle* h, uint32_t seq)
+{
+    for (uint32_t k = 1;; ++k) {
+        if (__atomic_load_n(&h->hrec->seq, __ATOMIC_ACQUIRE) == seq) break;
+        if ((k & 1023u) == 0u) {
+            const hipError_t e = hipStreamQuery(h->s);
+            if (e == hipSuccess) {
+                if (__atomic_load_n(&h->hrec->seq, __ATOMIC_ACQUIRE) == seq) break;
+                return fail(DFSPH_ERR_STATE, "the device did not publish the loop state");
+            }
+            if (e != hipErrorNotReady) return fail(DFSPH_ERR_HIP, std::string("stream: ") + hipGetErrorString(e));
+        }
+        __builtin_ia32_pause();
+    }
+    std::memcpy(&h->last, (const void*)h->hrec, sizeof(dfsph::HostRecord));
+    h->rec_op = h->ops;
+    return DFSPH_OK;
 }
 
 template <class T>
@@ -197,7 +232,7 @@ Handle::~Handle()
     dfree(ctl); dfree(tm); dfree(partial); dfree(partial_f);
     if (hctl) (void)hipHostFree(hctl);
     if (htm) (void)hipHostFree(htm);
-    if (hslot) (void)hipHostFree(hslot);
+    if (hrec) (void)hipHostFree(hrec);
     if (mlp) dfsph::mlp_destroy(mlp);
     for (auto& r : prof_pending) {
         (void)hipEventDestroy(r.a);
@@ -216,7 +251,12 @@ int dalloc(T*& p, size_t count)
     return DFSPH_OK;
 }
 
-Handle* H(void* h) { return static_cast<Handle*>(h); }
+Handle* H(void* hv)
+{
+    Handle* h = static_cast<Handle*>(hv);
+    ++h->ops;
+    return h;
+}
 
 dfsph::Restrict restriction(const Handle* h)
 {
@@ -358,20 +398,22 @@ int jacobi_loop(Handle* h, bool div, uint32_t max_iter, double max_error, uint32
     };
     auto loop_end = [&]() {
         const int pr = pbeg(h, div ? DFSPH_KERNEL_DIV_LOOP_END : DFSPH_KERNEL_DENS_LOOP_END);
+        const dfsph::Publish pub{h->ctl, h->tm, h->drec, ++h->seq};
         if (div)
             dfsph::launch_div_source(h->s, false, h->c, h->n, h->pos[h->pc], h->bpos, lists(h), h->vel[h->vc],
-                                     h->alpha, h->radv, h->kk, h->kapv[h->pc], sc, h->partial);
+                                     h->alpha, h->radv, h->kk, h->kapv[h->pc], sc, h->partial, pub);
         else
             dfsph::launch_density_source(h->s, false, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h),
                                          h->vel[h->vc], h->rho, h->alpha, h->radv, h->kk, h->kap[h->pc], sc,
-                                         h->partial);
+                                         h->partial, pub);
         pend(h, pr);
     };
     // The minimum iterations are always run.  After that, the velocity sweep
-    // of the next iteration is queued before the host reads the loop state of
-    // the current one, so the readback hides behind it; the source sweep is
-    // queued only once the loop is known to continue.  The iteration count is
-    // exact, and at most one velocity sweep per solve exits at once.
+    // of the next iteration is queued before the host waits for the loop
+    // state of the current one (published by the finalize kernel into host
+    // memory), so the wait hides behind it; the source sweep is queued only
+    // once the loop is known to continue.  The iteration count is exact, and
+    // at most one velocity sweep per solve exits at once.
     // with min_iter == 0 the reference loop does not start (its error starts at 0)
     const uint32_t first = std::min(min_iter, max_iter);
     if (first == 0) {
@@ -383,13 +425,11 @@ int jacobi_loop(Handle* h, bool div, uint32_t max_iter, double max_error, uint32
     uint32_t queued = first;
     SolveCtl res{};
     for (;;) {
-        HIPCHK(hipMemcpyAsync(h->hslot, sc, sizeof(SolveCtl), hipMemcpyDeviceToHost, h->s));
-        HIPCHK(hipEventRecord(h->ev[0], h->s));
         const bool spec = queued < max_iter;
         const int pr = spec ? jacobi() : -1;
         HIPCHK(hipGetLastError());
-        HIPCHK(hipEventSynchronize(h->ev[0]));
-        res = h->hslot[0];
+        CHK(wait_record(h, h->seq));
+        res = h->last.sc;
         if (res.done) pdrop(h, pr);   // that sweep exited at once
         if (res.done || !spec) break;
         loop_end();
@@ -692,10 +732,13 @@ int dfsph_create(const dfsph_params* p, void** out)
         if ((rc = dalloc(h->tm, 1)) != DFSPH_OK) break;
         if (hipHostMalloc((void**)&h->hctl, sizeof(DevCtl)) != hipSuccess ||
             hipHostMalloc((void**)&h->htm, sizeof(DevTime)) != hipSuccess ||
-            hipHostMalloc((void**)&h->hslot, 2 * sizeof(SolveCtl)) != hipSuccess) {
+            hipHostMalloc((void**)&h->hrec, sizeof(dfsph::HostRecord), hipHostMallocCoherent | hipHostMallocMapped) !=
+                hipSuccess ||
+            hipHostGetDevicePointer((void**)&h->drec, h->hrec, 0) != hipSuccess) {
             rc = fail(DFSPH_ERR_HIP, "pinned allocation");
             break;
         }
+        std::memset(h->hrec, 0, sizeof(dfsph::HostRecord));
         for (int b = 0; b < 2; ++b)
             if (hipEventCreateWithFlags(&h->ev[b], hipEventDisableTiming) != hipSuccess) rc = fail(DFSPH_ERR_HIP, "event");
         if (rc) break;
@@ -841,8 +884,34 @@ int dfsph_cfl(void* hv, double min_dt, double cfl_factor, double max_dt, double*
     if (!h->acc_valid) CHK(external_forces(h));
     dfsph::launch_time_update(h->s, h->tm, 0.0, h->ctl, 1, cfl_factor, 2.0 * h->p.particle_radius, min_dt, max_dt, 0.0);
     HIPCHK(hipGetLastError());
+    if (!dt_out) return DFSPH_OK;       // stays on the device (dfsph_get_time reads it later)
     CHK(read_ctl(h));
-    if (dt_out) *dt_out = h->htm->hfd;
+    *dt_out = h->htm->hfd;
+    return DFSPH_OK;
+}
+
+int dfsph_get_time(void* hv, double* h_now, double* h_future, double* time)
+{
+    Handle* h = H(hv);
+    const dfsph::DevTime* t = nullptr;
+    if (h->rec_op + 1 == h->ops && h->seq == h->last.seq) {
+        t = &h->last.tm;                // the record the previous call waited for is the latest state
+    } else {
+        CHK(flush_vel(h));
+        CHK(read_ctl(h));
+        t = h->htm;
+    }
+    if (h_now) *h_now = t->hd;
+    if (h_future) *h_future = t->hfd;
+    if (time) *time = t->time;
+    return DFSPH_OK;
+}
+
+int dfsph_fluid_count(void* hv, uint32_t* n)
+{
+    Handle* h = H(hv);
+    if (!n) return fail(DFSPH_ERR_ARG, "null count");
+    *n = h->n;
     return DFSPH_OK;
 }
 
@@ -881,14 +950,17 @@ int dfsph_update_pos(void* hv)
     HIPCHK(hipGetLastError());
     h->lists_valid = false;
     h->acc_valid = false;
+    h->pos_op = h->ops;
     return DFSPH_OK;
 }
 
 int dfsph_step_end(void* hv)
 {
     Handle* h = H(hv);
-    dfsph::launch_time_step_end(h->s, h->tm);
+    dfsph::launch_time_step_end(h->s, h->tm, dfsph::Publish{h->ctl, h->tm, h->drec, ++h->seq});
     HIPCHK(hipGetLastError());
+    h->end_op = h->ops;
+    h->end_seq = h->seq;
     return DFSPH_OK;
 }
 
@@ -906,6 +978,17 @@ int dfsph_check_particles_positions(void* hv, int mode, uint32_t* count)
 {
     Handle* h = H(hv);
     if (mode != 0 && mode != 2) return fail(DFSPH_ERR_ARG, "mode must be 0 (count) or 2 (remove)");
+    if (h->pos_op + 1 == h->end_op && h->end_op + 1 == h->ops && h->end_seq == h->seq) {
+        // right after dfsph_update_pos, dfsph_step_end: the step end's record
+        // holds the update's count of particles outside the bounding box (the
+        // same test on the same positions; an upper bound if the counter was
+        // not cleared since an earlier update)
+        CHK(wait_record(h, h->end_seq));
+        if (h->last.escaped == 0) {
+            if (count) *count = 0;
+            return DFSPH_OK;
+        }
+    }
     return check_positions(h, mode, count);
 }
 
@@ -951,18 +1034,42 @@ int dfsph_step(void* hv, dfsph_step_stats* st)
     dfsph::launch_time_update(h->s, h->tm, 0.0, h->ctl, h->p.use_cfl ? 1 : 2, h->p.cfl_factor,
                               2.0 * h->p.particle_radius, h->p.dt_min, h->p.dt_max, 0.0);
     h->pend_vel = true;
+    const uint32_t seq0 = h->seq;
     CHK(pressure_solve(h, h->p.max_iter, h->p.max_error, &loc.iters, &loc.err));
+    const bool published = h->seq != seq0;     // h->last is this pressure solve's record
     CHK(dfsph_update_pos(h));
     CHK(dfsph_step_end(h));
-    CHK(read_ctl(h));
-    loc.dt = h->htm->hpd;
-    loc.time = h->htm->time;
-    loc.max_neighbors_seen = h->hctl->max_seen;
-    loc.overflow = h->hctl->overflow;
-    loc.neighbors_fluid_total = h->hctl->nbr_fluid;
-    loc.neighbors_boundary_total = h->hctl->nbr_boundary;
-    prof_collect(h);
-    if (h->p.check_positions_mode == 2 && h->hctl->escaped) {
+    uint32_t escaped = 0;
+    if (h->p.check_positions_mode == 2) {
+        // the step end's record: time after the step and the escaped count
+        CHK(wait_record(h, h->end_seq));
+        loc.dt = h->last.tm.hpd;
+        loc.time = h->last.tm.time;
+        escaped = h->last.escaped;
+    } else if (published) {
+        // the pressure solve's last record holds this step's statistics: no
+        // wait, the next step queues behind this one (k_time_step_end:
+        // h_past = h, time += h_future)
+        loc.dt = h->last.tm.hd;
+        loc.time = h->last.tm.time + h->last.tm.hfd;
+    }
+    if (h->p.check_positions_mode == 2 || published) {
+        loc.max_neighbors_seen = h->last.max_seen;
+        loc.overflow = h->last.overflow;
+        loc.neighbors_fluid_total = h->last.nbr_fluid;
+        loc.neighbors_boundary_total = h->last.nbr_boundary;
+        prof_collect(h, false);
+    } else {
+        CHK(read_ctl(h));
+        loc.dt = h->htm->hpd;
+        loc.time = h->htm->time;
+        loc.max_neighbors_seen = h->hctl->max_seen;
+        loc.overflow = h->hctl->overflow;
+        loc.neighbors_fluid_total = h->hctl->nbr_fluid;
+        loc.neighbors_boundary_total = h->hctl->nbr_boundary;
+        prof_collect(h);
+    }
+    if (h->p.check_positions_mode == 2 && escaped) {
         CHK(check_positions(h, 2, &loc.removed));
     } else if (h->p.check_positions_mode == 0) {
         loc.removed = 0;

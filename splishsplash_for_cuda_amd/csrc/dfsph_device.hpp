@@ -91,6 +91,24 @@ struct DevCtl {
     unsigned long long nbr_fluid_unstaged;  // of which in 32-bit (unstaged) lists
 };
 
+// Loop state published to pinned, mapped host memory by the one-block
+// finalize kernel of each Jacobi iteration: the host waits on seq (written
+// last, after a system-scope fence) instead of a copy and an event.
+struct HostRecord {
+    SolveCtl sc;
+    DevTime tm;
+    uint32_t max_seen, overflow, unstaged_blocks, escaped;
+    unsigned long long nbr_fluid, nbr_boundary, nbr_fluid_unstaged;
+    uint32_t seq, pad[3];
+};
+
+struct Publish {
+    const DevCtl* dc;
+    const DevTime* t;
+    HostRecord* rec;               // device address of the host record (nullptr: none)
+    uint32_t seq;
+};
+
 // ---- staged neighbor sets ----
 // A sweep block (256 consecutive particles) owns the set of fluid particles in
 // the cells adjacent to its particles' cells, cell by cell.  Up to

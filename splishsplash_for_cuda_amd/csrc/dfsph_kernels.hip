@@ -73,6 +73,24 @@ static __device__ void time_fill(DevTime* t, double h, double hf, double hp)
     t->inv_h_future = (float)(1.0 / hf);
 }
 
+// Thread-level: copy the loop and step state into the host record, then
+// release its sequence number at system scope.
+__device__ void publish(const Publish& pub, const SolveCtl* sc)
+{
+    HostRecord* r = pub.rec;
+    r->sc = *sc;
+    r->tm = *pub.t;
+    r->max_seen = pub.dc->max_seen;
+    r->overflow = pub.dc->overflow;
+    r->unstaged_blocks = pub.dc->unstaged_blocks;
+    r->escaped = pub.dc->escaped;
+    r->nbr_fluid = pub.dc->nbr_fluid;
+    r->nbr_boundary = pub.dc->nbr_boundary;
+    r->nbr_fluid_unstaged = pub.dc->nbr_fluid_unstaged;
+    __threadfence_system();
+    __hip_atomic_store(&r->seq, pub.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // DFSPHCData::reset time bookkeeping (DFSPH_c_arrays_structure.cpp:1000-1004)
 __global__ void k_time_reset(DevTime* t, double h)
 {
@@ -122,12 +140,13 @@ __global__ void k_time_set(DevTime* t, double h, double hp, double time)
 
 // DFSPHCData::onSimulationStepEnd (DFSPH_c_arrays_structure.cpp:1108-1118) and the
 // TimeManager advance time += h (DFSPH_CUDA.cpp:449-452, with the new h).
-__global__ void k_time_step_end(DevTime* t)
+__global__ void k_time_step_end(DevTime* t, Publish pub)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const double hp = t->hd, h = t->hfd;
         time_fill(t, h, h, hp);
         t->time += h;
+        if (pub.rec) publish(pub, &pub.dc->dens);
     }
 }
 
@@ -729,22 +748,25 @@ __device__ __forceinline__ void store_partial(double* partial, double bsum)
 // Sums the block partials in a fixed order (deterministic), forms
 // err = sum / N and evaluates the loop condition of cuda_divergenceSolve /
 // cuda_pressureSolve: continue while ((err > eta) || (it < min)) && (it < max).
-__global__ __launch_bounds__(kBlock) void k_solve_finalize(SolveCtl* ctl, const double* partial, uint32_t nparts)
+__global__ __launch_bounds__(kBlock) void k_solve_finalize(SolveCtl* ctl, const double* partial, uint32_t nparts,
+                                                          Publish pub)
 {
     __shared__ double lds4[kBlock / 64];
-    if (ctl->done) return;
-    double acc = 0.0;
-    for (uint32_t k = threadIdx.x; k < nparts; k += blockDim.x) acc += partial[k];
-    const double tot = block_sum(acc, lds4);
-    if (threadIdx.x == 0) {
-        const double err = ctl->n_div > 0.0 ? tot / ctl->n_div : 0.0;
-        const uint32_t it = ctl->iters + 1;
-        ctl->sum = tot;
-        ctl->err = err;
-        ctl->iters = it;
-        const bool cont = ((err > ctl->eta) || (it < ctl->min_iter)) && (it < ctl->max_iter);
-        ctl->done = cont ? 0u : 1u;
+    if (!ctl->done) {
+        double acc = 0.0;
+        for (uint32_t k = threadIdx.x; k < nparts; k += blockDim.x) acc += partial[k];
+        const double tot = block_sum(acc, lds4);
+        if (threadIdx.x == 0) {
+            const double err = ctl->n_div > 0.0 ? tot / ctl->n_div : 0.0;
+            const uint32_t it = ctl->iters + 1;
+            ctl->sum = tot;
+            ctl->err = err;
+            ctl->iters = it;
+            const bool cont = ((err > ctl->eta) || (it < ctl->min_iter)) && (it < ctl->max_iter);
+            ctl->done = cont ? 0u : 1u;
+        }
     }
+    if (threadIdx.x == 0 && pub.rec) publish(pub, ctl);
 }
 
 // Fused density + factor + divergence source, then the divergence warm start:
@@ -1384,7 +1406,7 @@ void launch_time_update(hipStream_t s, DevTime* t, double hf, const DevCtl* ctl,
     k_time_update<<<1, 64, 0, s>>>(t, hf, ctl, mode, cfl_factor, diameter, dt_min, dt_max, seed);
 }
 
-void launch_time_step_end(hipStream_t s, DevTime* t) { k_time_step_end<<<1, 64, 0, s>>>(t); }
+void launch_time_step_end(hipStream_t s, DevTime* t, const Publish& pub) { k_time_step_end<<<1, 64, 0, s>>>(t, pub); }
 
 void launch_time_set(hipStream_t s, DevTime* t, double h, double hp, double time)
 {
@@ -1465,14 +1487,15 @@ void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime*
 
 void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
                        const float4* bpos, const NbrLists& L, const float4* vel,
-                       const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial)
+                       const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial,
+                       const Publish& pub)
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
     if (init) {
         k_div_source<true><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial);
     } else {
         k_div_source<false><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial);
-        k_solve_finalize<<<1, kBlock, 0, s>>>(ctl, partial, g);
+        k_solve_finalize<<<1, kBlock, 0, s>>>(ctl, partial, g, pub);
     }
 }
 
@@ -1525,14 +1548,14 @@ void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c
 void launch_density_source(hipStream_t s, bool init, const SimConst& c, const DevTime* t, uint32_t n,
                            const float4* pos, const float4* bpos, const NbrLists& L,
                            const float4* vel, const float* rho, float* alpha, float* radv, float* kk,
-                           float* kappa, SolveCtl* ctl, double* partial)
+                           float* kappa, SolveCtl* ctl, double* partial, const Publish& pub)
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
     if (init) {
         k_density_source<true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl, partial);
     } else {
         k_density_source<false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl, partial);
-        k_solve_finalize<<<1, kBlock, 0, s>>>(ctl, partial, g);
+        k_solve_finalize<<<1, kBlock, 0, s>>>(ctl, partial, g, pub);
     }
 }
 

@@ -22,7 +22,7 @@ void launch_nbr_ctl_init(hipStream_t s, DevCtl* c);
 void launch_time_reset(hipStream_t s, DevTime* t, double h);
 void launch_time_update(hipStream_t s, DevTime* t, double hf, const DevCtl* ctl, int mode,
                         double cfl_factor, double diameter, double dt_min, double dt_max, double seed);
-void launch_time_step_end(hipStream_t s, DevTime* t);
+void launch_time_step_end(hipStream_t s, DevTime* t, const Publish& pub = Publish{});
 void launch_time_set(hipStream_t s, DevTime* t, double h, double hp, double time);
 
 void launch_cell_hist(hipStream_t s, const SimConst& c, const float4* pos, uint32_t n, uint32_t* key,
@@ -47,7 +47,8 @@ void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime*
                         float* radv, float* kvtot);
 void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
                        const float4* bpos, const NbrLists& L, const float4* vel,
-                       const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial);
+                       const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial,
+                       const Publish& pub = Publish{});
 void launch_jacobi(hipStream_t s, bool div, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
                    const float4* bpos, const NbrLists& L, const float* kk, float* kappaV,
                    float4* vel, const SolveCtl* ctl);
@@ -62,7 +63,7 @@ void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c
 void launch_density_source(hipStream_t s, bool init, const SimConst& c, const DevTime* t, uint32_t n,
                            const float4* pos, const float4* bpos, const NbrLists& L,
                            const float4* vel, const float* rho, float* alpha, float* radv, float* kk,
-                           float* kappa, SolveCtl* ctl, double* partial);
+                           float* kappa, SolveCtl* ctl, double* partial, const Publish& pub = Publish{});
 void launch_update_pos(hipStream_t s, const DevTime* t, uint32_t n, const float4* vel, float4* pos,
                        const float* bmin, const float* bmax, DevCtl* ctl, const Restrict& rs);
 void launch_keep_inside(hipStream_t s, uint32_t n, const float4* pos, const float* bmin, const float* bmax,

@@ -275,18 +275,56 @@ def test_full_step_entry_point_matches_phase_sequence():
         assert np.array_equal(sa[k], sb[k]), k
 
 
-def test_host_class_step_matches_abi_step():
-    """SPH::DFSPHCUDA::step() (C++ host over the C ABI) == dfsph_step."""
+@pytest.mark.parametrize("use_cfl", [0, 1])
+def test_host_class_step_matches_abi_step(use_cfl):
+    """SPH::DFSPHCUDA::step() (C++ host over the C ABI) == dfsph_step, and
+    the step statistics read from the device's loop records (no stream
+    synchronisation) equal the synchronised device state."""
     sc = scene.small_dam_break()
-    p = S.default_params(particle_radius=sc.particle_radius, dt=0.002)
+    p = S.default_params(particle_radius=sc.particle_radius, dt=0.002, use_cfl=use_cfl)
     host = S.HostSolver(p, sc.boundary, sc.fluid)
-    abi = _solver(sc, dt=0.002)
+    abi = S.Solver(p)
+    abi.upload_boundary(sc.boundary)
+    abi.upload_fluid(sc.fluid)
     for _ in range(5):
         st = host.step()
         sa = abi.step()
         assert (st.iters_v, st.iters) == (sa.iters_v, sa.iters)
+        assert (st.h, st.time) == (sa.dt, sa.time)
+        assert st.n_fluid == sa.n_fluid == sc.n_fluid
+        inf = abi.info()
+        assert (sa.dt, sa.time) == (inf.h_past, inf.time)
+        assert sa.max_neighbors_seen == inf.max_neighbors_seen
+        assert sa.neighbors_fluid_total == inf.neighbors_fluid_total
     x1, x2 = host.view.state_by_id(), abi.state_by_id()
     assert np.array_equal(x1["pos"], x2["pos"]) and np.array_equal(x1["vel"], x2["vel"])
+    host.close()
+
+
+def test_particle_leaving_the_box_in_a_step_is_removed():
+    """A particle that update_pos carries out of the bounding box is removed
+    at the end of the step (the escaped count of the step end's record), in
+    the host class and in dfsph_step alike."""
+    sc = scene.small_dam_break()
+    # no pressure solves: nothing but XSPH damps the planted velocity
+    p = S.default_params(particle_radius=sc.particle_radius, dt=0.002, use_cfl=0, enable_divergence=0, max_iter=0)
+    host = S.HostSolver(p, sc.boundary, sc.fluid)
+    abi = S.Solver(p)
+    abi.upload_boundary(sc.boundary)
+    abi.upload_fluid(sc.fluid)
+    for s in (host.view, abi):
+        s.step() if s is abi else host.step()
+        ids = s.download(S.FIELD_ID)
+        vel = s.download(S.FIELD_VEL).copy()
+        vel[int(np.nonzero(ids == 7)[0][0])] = (0.0, 5000.0, 0.0)
+        s.upload_field(S.FIELD_VEL, vel)
+    st, sa = host.step(), abi.step()
+    assert st.removed == sa.removed == 1
+    assert st.n_fluid == sa.n_fluid == sc.n_fluid - 1
+    for s in (host.view, abi):
+        assert 7 not in set(s.download(S.FIELD_ID).tolist())
+    st, sa = host.step(), abi.step()
+    assert st.removed == sa.removed == 0
     host.close()
 
 
