@@ -471,15 +471,32 @@ __device__ __forceinline__ float lds_f1(const float* base, uint32_t off)   // ba
     return base[off >> 4];
 }
 
-// Loads an LDS-staged set: st(k, global id) for k < count, then a block
-// barrier (reached by every thread).  Returns stage_n[b].
-template <class St>
-__device__ __forceinline__ uint32_t stage_block(const NbrLists& L, uint32_t b, St&& st)
+// Loads an LDS-staged set: wr(k, ld(global id)) for k < count, then a block
+// barrier (reached by every thread).  Returns stage_n[b].  All of a thread's
+// index loads are issued first, then all its record loads (ld takes any valid
+// id: the lanes past the end load the set's last record), then the LDS
+// writes, so the loads of one thread are in flight together.
+template <class Ld, class Wr>
+__device__ __forceinline__ uint32_t stage_block(const NbrLists& L, uint32_t b, Ld&& ld, Wr&& wr)
 {
+    using T = decltype(ld(0u));
+    constexpr uint32_t U = (kStageMax + kBlock - 1) / kBlock;
     const uint32_t sn = L.stage_n[b];
-    if (sn <= kStageMax) {
+    if (sn <= kStageMax && sn > 0u) {
         const uint32_t* g = L.stg + (size_t)b * kStageCap;
-        for (uint32_t k = threadIdx.x; k < sn; k += blockDim.x) st(k, g[k]);
+        uint32_t gi[U];
+        T d[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (u * kBlock < sn) gi[u] = g[min(threadIdx.x + u * kBlock, sn - 1u)];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (u * kBlock < sn) d[u] = ld(gi[u]);
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t k = threadIdx.x + u * kBlock;
+            if (k < sn) wr(k, d[u]);
+        }
     }
     __syncthreads();
     return sn;

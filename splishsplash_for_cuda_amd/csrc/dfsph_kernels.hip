@@ -799,11 +799,10 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
     };
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) {
-        sP[k] = pos[g];
+    const uint32_t sn = stage_block(L, b, [&](uint32_t g) {
         const float4 v = vel_i[g];
-        sV[k] = make_float4(v.x, v.y, v.z, ws_of(g));
-    });
+        return PV{pos[g], make_float4(v.x, v.y, v.z, ws_of(g))};
+    }, [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
     if (i >= n) return;
     const float4 vi = vel_i[i];
     const float ki = ws_of(i);
@@ -885,7 +884,8 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
     if (!INIT && ctl->done) return;
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) { sP[k] = pos[g]; sV[k] = vel[g]; });
+    const uint32_t sn = stage_block(L, b, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
+                                    [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
     float r = 0.0f;
     if (i < n && !is_source(L.rs, i)) r = radv[i];      // restricted out: stored value in the error sum
     if (i < n && is_source(L.rs, i)) {
@@ -936,7 +936,8 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t,
     if (ctl->done) return;
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) { sP[k] = pos[g]; sK[k] = kk[g]; });
+    const uint32_t sn = stage_block(L, b, [&](uint32_t g) { return PK{pos[g], kk[g]}; },
+                                    [&](uint32_t k, const PK& d) { sP[k] = d.p; sK[k] = d.k; });
     if (i >= n || !is_compute(L.rs, i)) return;
     const float ki = kk[i];
     if (DIV) kappaV[i] += ki;
@@ -983,12 +984,11 @@ __global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, u
     __shared__ float4 sV[kStageMax];           // (v, m / rho) of the staged set
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) {
+    const uint32_t sn = stage_block(L, b, [&](uint32_t g) {
         const float4 p = pos[g];
         const float4 v = vel[g];
-        sP[k] = p;
-        sV[k] = make_float4(v.x, v.y, v.z, p.w / rho[g]);
-    });
+        return PV{p, make_float4(v.x, v.y, v.z, p.w / rho[g])};
+    }, [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
     float m2 = 0.0f;
     if (i < n && !is_compute(L.rs, i)) {
         // restricted out: its stored acceleration still enters the CFL bound
@@ -1091,7 +1091,8 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
     };
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) { sP[k] = pos[g]; sK[k] = ws_of(g); });
+    const uint32_t sn = stage_block(L, b, [&](uint32_t g) { return PK{pos[g], ws_of(g)}; },
+                                    [&](uint32_t k, const PK& d) { sP[k] = d.p; sK[k] = d.k; });
     if (i >= n || !is_compute(L.rs, i)) return;
     const float ki = ws_of(i);
     if (ktot) ktot[i] = ki;
@@ -1150,7 +1151,8 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
     if (!INIT && ctl->done) return;
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) { sP[k] = pos[g]; sV[k] = vel[g]; });
+    const uint32_t sn = stage_block(L, b, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
+                                    [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
     float r = 0.0f;
     if (i < n && !is_source(L.rs, i)) r = radv[i];      // restricted out: stored value in the error sum
     if (i < n && is_source(L.rs, i)) {
@@ -1271,7 +1273,8 @@ __global__ __launch_bounds__(kBlock) void k_features(SimConst c, const DevTime* 
     __shared__ float4 sV[kStageMax];
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t sn = stage_block(L, b, [&](uint32_t k, uint32_t g) { sP[k] = pos[g]; sV[k] = vel[g]; });
+    const uint32_t sn = stage_block(L, b, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
+                                    [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
     if (i >= n) return;
     const float4 pi = pos[i];
     const float4 vi = vel[i];
