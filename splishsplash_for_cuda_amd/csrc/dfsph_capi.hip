@@ -95,6 +95,7 @@ struct Handle {
     // boundary (static, cell-sorted once)
     float4* bpos = nullptr;
     uint32_t* bcstart = nullptr;
+    uint32_t* tile_rank = nullptr;    // SimConst::tile_rank
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
 
     // device-side control
@@ -228,7 +229,7 @@ Handle::~Handle()
     }
     dfree(rho); dfree(alpha); dfree(radv); dfree(kk); dfree(acc); dfree(cnt); dfree(nbrf); dfree(nbrb); dfree(nbr16); dfree(stage_n); dfree(stg); dfree(ob_cand);
     dfree(key); dfree(slot); dfree(perm); dfree(hist); dfree(cstart); dfree(keep); dfree(rank);
-    dfree(keep8); dfree(scan_tmp); dfree(bpos); dfree(bcstart);
+    dfree(keep8); dfree(scan_tmp); dfree(bpos); dfree(bcstart); dfree(tile_rank);
     dfree(ctl); dfree(tm); dfree(partial); dfree(partial_f);
     if (hctl) (void)hipHostFree(hctl);
     if (htm) (void)hipHostFree(htm);
@@ -398,15 +399,15 @@ int jacobi_loop(Handle* h, bool div, uint32_t max_iter, double max_error, uint32
     };
     auto loop_end = [&]() {
         const int pr = pbeg(h, div ? DFSPH_KERNEL_DIV_LOOP_END : DFSPH_KERNEL_DENS_LOOP_END);
-        const dfsph::Publish pub{h->ctl, h->tm, h->drec, ++h->seq};
         if (div)
             dfsph::launch_div_source(h->s, false, h->c, h->n, h->pos[h->pc], h->bpos, lists(h), h->vel[h->vc],
-                                     h->alpha, h->radv, h->kk, h->kapv[h->pc], sc, h->partial, pub);
+                                     h->alpha, h->radv, h->kk, h->kapv[h->pc], sc, h->partial);
         else
             dfsph::launch_density_source(h->s, false, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h),
                                          h->vel[h->vc], h->rho, h->alpha, h->radv, h->kk, h->kap[h->pc], sc,
-                                         h->partial, pub);
-        pend(h, pr);
+                                         h->partial);
+        pend(h, pr);                    // the timed region is the sweep alone
+        dfsph::launch_solve_finalize(h->s, h->n, sc, h->partial, dfsph::Publish{h->ctl, h->tm, h->drec, ++h->seq});
     };
     // The minimum iterations are always run.  After that, the velocity sweep
     // of the next iteration is queued before the host waits for the loop
@@ -806,6 +807,29 @@ int dfsph_upload_boundary(void* hv, const float* xyz, const float* psi, uint32_t
     if (ncell >= (1ull << 31)) return fail(DFSPH_ERR_ARG, "domain too large for the cell grid");
     h->c.inv_cell = (float)(1.0 / cell);
     h->c.ncell = (uint32_t)ncell;
+    {
+        // tiles in Morton order of their coordinates (bits of x, y, z interleaved)
+        const uint32_t tx = (uint32_t)h->c.tiles[0], ty = (uint32_t)h->c.tiles[1], tz = (uint32_t)h->c.tiles[2];
+        const uint32_t nt = tx * ty * tz;
+        auto spread = [](uint64_t v) {
+            uint64_t r = 0;
+            for (int bit = 0; bit < 21; ++bit) r |= ((v >> bit) & 1ull) << (3 * bit);
+            return r;
+        };
+        std::vector<std::pair<uint64_t, uint32_t>> order(nt);
+        for (uint32_t y = 0; y < ty; ++y)
+            for (uint32_t z = 0; z < tz; ++z)
+                for (uint32_t x = 0; x < tx; ++x) {
+                    const uint32_t t = x + tx * (z + tz * y);
+                    order[t] = {spread(x) | (spread(y) << 1) | (spread(z) << 2), t};
+                }
+        std::sort(order.begin(), order.end());
+        std::vector<uint32_t> rank(nt);
+        for (uint32_t k = 0; k < nt; ++k) rank[order[k].second] = k;
+        CHK(dalloc(h->tile_rank, nt));
+        HIPCHK(hipMemcpy(h->tile_rank, rank.data(), sizeof(uint32_t) * nt, hipMemcpyHostToDevice));
+        h->c.tile_rank = h->tile_rank;
+    }
     h->nb = n;
     CHK(dalloc(h->hist, ncell + 1));
     CHK(dalloc(h->cstart, ncell + 1));

@@ -46,6 +46,7 @@ struct SimConst {
     float inv_cell;                // 1 / cell edge (cell edge = 1.001 h_k)
     int dim[3];                    // cells per axis
     int tiles[3];                  // 8-cell tiles per axis (dim rounded up / 8)
+    const uint32_t* tile_rank;     // tile (x + tiles0 (z + tiles2 y)) -> its place in Morton order
     uint32_t ncell;                // cell keys: tiles0 tiles1 tiles2 512
     uint32_t cap;                  // neighbor list stride (fluid capacity)
     uint32_t max_nbr;
@@ -336,13 +337,16 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v)
     return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4);
 }
 
-// Cell key: tile index (x fastest, then z, then y) * 512 + Morton index in the tile.
+// Cell key: the tile's place in a Morton order of the tiles * 512 + the Morton
+// index of the cell in its tile.  Particles sorted by key fill blocks that
+// are compact in 3D at both scales, so the blocks that share a neighbor set
+// region run close together in time and on the same XCD (shared L2).
 __device__ __forceinline__ uint32_t cell_key(const SimConst& c, int x, int y, int z)
 {
     const uint32_t tile = (uint32_t)(x >> 3) +
                           (uint32_t)c.tiles[0] * ((uint32_t)(z >> 3) + (uint32_t)c.tiles[2] * (uint32_t)(y >> 3));
     const uint32_t local = spread3((uint32_t)x & 7u) | (spread3((uint32_t)y & 7u) << 1) | (spread3((uint32_t)z & 7u) << 2);
-    return (tile << 9) | local;
+    return (c.tile_rank[tile] << 9) | local;
 }
 
 __device__ __forceinline__ uint32_t cell_id(const SimConst& c, float x, float y, float z)
@@ -383,15 +387,16 @@ __device__ __forceinline__ void for_each_adjacent_cell(const SimConst& c, int cx
 // group is loaded while the current batch computes.
 template <class Load, class Body>
 __device__ __forceinline__ void for_each_nbr16(const uint16_t* __restrict__ lists, uint32_t cap, uint32_t i,
-                                               uint32_t count, Load&& load, Body&& body)
+                                               uint32_t count, uint4 q, uint4 qn, Load&& load, Body&& body)
 {
+    // q, qn: the particle's groups 0 and 1 (loaded by the caller)
     using T = decltype(load(0u));
     const uint4* p = reinterpret_cast<const uint4*>(lists) + i;
-    uint4 q = p[0];                             // every particle has at least one group
     uint32_t k = 0;
     for (; k + 8 <= count; k += 8) {            // full batches: no masking
         const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-        q = p[(size_t)((k >> 3) + 1) * cap];    // next group (allocated up to max_neighbors / 8)
+        q = qn;
+        if (k + 16u < count) qn = p[(size_t)((k >> 3) + 2) * cap];   // two groups ahead, when a batch reads it
         T it[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) it[u] = load((w[u >> 1] >> ((u & 1) * 16)) & 0xffffu);
@@ -413,17 +418,27 @@ __device__ __forceinline__ void for_each_nbr16(const uint16_t* __restrict__ list
 }
 
 template <class Load, class Body>
-__device__ __forceinline__ void for_each_nbr32(const uint32_t* __restrict__ lists, uint32_t cap, uint32_t i,
-                                               uint32_t count, uint32_t pad, Load&& load, Body&& body)
+__device__ __forceinline__ void for_each_nbr16(const uint16_t* __restrict__ lists, uint32_t cap, uint32_t i,
+                                               uint32_t count, Load&& load, Body&& body)
 {
+    const uint4* p = reinterpret_cast<const uint4*>(lists) + i;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for_each_nbr16(lists, cap, i, count, count > 0u ? p[0] : z, count > 8u ? p[cap] : z, load, body);
+}
+
+template <class Load, class Body>
+__device__ __forceinline__ void for_each_nbr32(const uint32_t* __restrict__ lists, uint32_t cap, uint32_t i,
+                                               uint32_t count, uint32_t pad, uint4 a, uint4 b, Load&& load,
+                                               Body&& body)
+{
+    // a, b: the particle's groups 0 and 1 (loaded by the caller)
     using T = decltype(load(0u));
     const uint4* p = reinterpret_cast<const uint4*>(lists) + i;
-    uint4 a = p[0], b = p[cap];                 // every particle has at least two groups
     uint32_t k = 0;
     for (; k + 8 <= count; k += 8) {
         const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        a = p[(size_t)((k >> 2) + 2) * cap];    // next two groups (allocated up to max_neighbors / 4 + 2)
-        b = p[(size_t)((k >> 2) + 3) * cap];
+        if (k + 8u < count) a = p[(size_t)((k >> 2) + 2) * cap];    // the next two groups, when read
+        if (k + 12u < count) b = p[(size_t)((k >> 2) + 3) * cap];
         T it[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) it[u] = load(w[u]);
@@ -442,6 +457,15 @@ __device__ __forceinline__ void for_each_nbr32(const uint32_t* __restrict__ list
 #pragma unroll
         for (int u = 0; u < 8; ++u) body(it[u], v[u]);
     }
+}
+
+template <class Load, class Body>
+__device__ __forceinline__ void for_each_nbr32(const uint32_t* __restrict__ lists, uint32_t cap, uint32_t i,
+                                               uint32_t count, uint32_t pad, Load&& load, Body&& body)
+{
+    const uint4* p = reinterpret_cast<const uint4*>(lists) + i;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for_each_nbr32(lists, cap, i, count, pad, count > 0u ? p[0] : z, count > 4u ? p[cap] : z, load, body);
 }
 
 // the plain forms: f(entry, valid)
@@ -477,11 +501,10 @@ __device__ __forceinline__ float lds_f1(const float* base, uint32_t off)   // ba
 // id: the lanes past the end load the set's last record), then the LDS
 // writes, so the loads of one thread are in flight together.
 template <class Ld, class Wr>
-__device__ __forceinline__ uint32_t stage_block(const NbrLists& L, uint32_t b, Ld&& ld, Wr&& wr)
+__device__ __forceinline__ uint32_t stage_block(const NbrLists& L, uint32_t b, uint32_t sn, Ld&& ld, Wr&& wr)
 {
     using T = decltype(ld(0u));
     constexpr uint32_t U = (kStageMax + kBlock - 1) / kBlock;
-    const uint32_t sn = L.stage_n[b];
     if (sn <= kStageMax && sn > 0u) {
         const uint32_t* g = L.stg + (size_t)b * kStageCap;
         uint32_t gi[U];
@@ -502,6 +525,31 @@ __device__ __forceinline__ uint32_t stage_block(const NbrLists& L, uint32_t b, L
     return sn;
 }
 
+template <class Ld, class Wr>
+__device__ __forceinline__ uint32_t stage_block(const NbrLists& L, uint32_t b, Ld&& ld, Wr&& wr)
+{
+    return stage_block(L, b, L.stage_n[b], ld, wr);
+}
+
+// The list counts and first two groups of particle i's fluid list (16-bit
+// when its block's set has them, else 32-bit) and boundary list, loaded
+// before the set is staged so that their latency hides behind it.
+struct Head {
+    uint32_t cnt;
+    uint4 f0, f1, b0, b1;
+};
+
+__device__ __forceinline__ Head list_head(const NbrLists& L, uint32_t sn, uint32_t cap, uint32_t i)
+{
+    const uint4* f = (sn != kUnstaged ? reinterpret_cast<const uint4*>(L.nbr16) : reinterpret_cast<const uint4*>(L.nbrf)) + i;
+    const uint4* bb = reinterpret_cast<const uint4*>(L.nbrb) + i;
+    const uint32_t cn = L.cnt[i];
+    const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
+    const uint32_t f1 = sn != kUnstaged ? 8u : 4u;   // entries in group 0
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    return Head{cn, nf > 0u ? f[0] : z, nf > f1 ? f[cap] : z, nb > 0u ? bb[0] : z, nb > 4u ? bb[cap] : z};
+}
+
 // The fluid neighbors of particle i (block b, sn = stage_n[b]) in whichever
 // form its block's lists have: lds(LDS byte offset) loads a neighbor of an
 // LDS-staged set, glob(global id) one from global memory; body(data, valid)
@@ -517,6 +565,22 @@ __device__ __forceinline__ void fluid_nbrs(const NbrLists& L, uint32_t cap, uint
         for_each_nbr16(L.nbr16, cap, i, nf, [&](uint32_t e) { return glob(g[e >> 4]); }, body);
     } else {
         for_each_nbr32(L.nbrf, cap, i, nf, i, glob, body);
+    }
+}
+
+// The same with the list head loaded beforehand (hd = list_head(L, sn, cap, i)).
+template <class FL, class FG, class Body>
+__device__ __forceinline__ void fluid_nbrs(const NbrLists& L, uint32_t cap, uint32_t b, uint32_t i, uint32_t sn,
+                                           const Head& hd, FL&& lds, FG&& glob, Body&& body)
+{
+    const uint32_t nf = hd.cnt & 0xffffu;
+    if (sn <= kStageMax) {
+        for_each_nbr16(L.nbr16, cap, i, nf, hd.f0, hd.f1, lds, body);
+    } else if (sn != kUnstaged) {
+        const uint32_t* g = L.stg + (size_t)b * kStageCap;
+        for_each_nbr16(L.nbr16, cap, i, nf, hd.f0, hd.f1, [&](uint32_t e) { return glob(g[e >> 4]); }, body);
+    } else {
+        for_each_nbr32(L.nbrf, cap, i, nf, i, hd.f0, hd.f1, glob, body);
     }
 }
 

@@ -792,24 +792,29 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
     __shared__ float4 sP[kStageMax];
     __shared__ float4 sV[kStageMax];           // (v, warm-start k) of the staged set
     const float ratio = t->ratio, h = t->h, inv_h = t->inv_h;
-    auto ws_of = [&](uint32_t j) -> float {
-        if (WS == 0) return fmaxf(kws[j] * ratio / 2.0f, -0.5f);
-        if (WS == 1) return fmaxf(kws[j] * inv_h, -0.5f);
+    auto ws_val = [&](float raw) -> float {
+        if (WS == 0) return fmaxf(raw * ratio / 2.0f, -0.5f);
+        if (WS == 1) return fmaxf(raw * inv_h, -0.5f);
         return 0.0f;
     };
+    auto ws_of = [&](uint32_t j) -> float { return WS == 2 ? 0.0f : ws_val(kws[j]); };
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t sn = stage_block(L, b, [&](uint32_t g) {
+    const uint32_t ic = i < n ? i : 0u;         // own loads before staging, so they overlap it
+    const uint32_t sn0 = L.stage_n[b];
+    const Head hd = list_head(L, sn0, c.cap, ic);
+    const float4 vi = vel_i[ic];
+    const float4 pi = pos[ic];
+    const float kraw = WS == 2 ? 0.0f : kws[ic];
+    const uint32_t sn = stage_block(L, b, sn0, [&](uint32_t g) {
         const float4 v = vel_i[g];
         return PV{pos[g], make_float4(v.x, v.y, v.z, ws_of(g))};
     }, [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
     if (i >= n) return;
-    const float4 vi = vel_i[i];
-    const float ki = ws_of(i);
+    const float ki = ws_val(kraw);
     if (kvtot) kvtot[i] = ki;
     if (!is_source(L.rs, i)) { vel_o[i] = vi; return; }
-    const float4 pi = pos[i];
-    const uint32_t cn = L.cnt[i];
+    const uint32_t cn = hd.cnt;
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
     const uint32_t stride = c.cap;
     float dens = pi.w * c.W0, sg2 = 0.0f, dr = 0.0f;
@@ -830,14 +835,15 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
             ax += s * px; ay += s * py; az += s * pz;
         }
     };
-    fluid_nbrs(L, stride, b, i, nf, sn, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+    fluid_nbrs(L, stride, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
                [&](uint32_t j) {
                    const float4 v = vel_i[j];
                    return PV{pos[j], make_float4(v.x, v.y, v.z, ws_of(j))};
                },
                [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
+    (void)nf;
     float bx = 0.0f, by = 0.0f, bz = 0.0f;
-    for_each_nbr32(L.nbrb, stride, i, nb, 0u, [&](uint32_t j) { return bpos[j]; },
+    for_each_nbr32(L.nbrb, stride, i, nb, 0u, hd.b0, hd.b1, [&](uint32_t j) { return bpos[j]; },
         [&](const float4& pj, bool valid) {
         const float m = keep(valid, pj.w);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -884,14 +890,18 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
     if (!INIT && ctl->done) return;
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t sn = stage_block(L, b, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
+    const uint32_t ic = i < n ? i : 0u;         // own loads before staging, so they overlap it
+    const uint32_t sn0 = L.stage_n[b];
+    const Head hd = list_head(L, sn0, c.cap, ic);
+    const float4 pi = pos[ic];
+    const float4 vi = vel[ic];
+    const float ai = alpha[ic];
+    const uint32_t sn = stage_block(L, b, sn0, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
                                     [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
     float r = 0.0f;
     if (i < n && !is_source(L.rs, i)) r = radv[i];      // restricted out: stored value in the error sum
     if (i < n && is_source(L.rs, i)) {
-        const float4 pi = pos[i];
-        const float4 vi = vel[i];
-        const uint32_t cn = L.cnt[i];
+        const uint32_t cn = hd.cnt;
         const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
         float dr = 0.0f;
         auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
@@ -900,10 +910,10 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
             const float mg = keep(valid && kp.ok, pj.w * kp.g);
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         };
-        fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+        fluid_nbrs(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
                    [&](uint32_t j) { return PV{pos[j], vel[j]}; },
                    [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
-        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j) { return bpos[j]; },
+        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, hd.b0, hd.b1, [&](uint32_t j) { return bpos[j]; },
             [&](const float4& pj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const PairL kp = grad_listed(c, dist2(dx, dy, dz));
@@ -912,7 +922,7 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
         });
         r = (nf + nb < 20u) ? 0.0f : fmaxf(dr, 0.0f);
         radv[i] = r;
-        kk[i] = r * alpha[i];
+        kk[i] = r * ai;
         if (INIT) kappaV[i] = 0.0f;
     }
     if (!INIT) {
@@ -936,12 +946,16 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t,
     if (ctl->done) return;
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t sn = stage_block(L, b, [&](uint32_t g) { return PK{pos[g], kk[g]}; },
+    const uint32_t ic = i < n ? i : 0u;         // own loads before staging, so they overlap it
+    const uint32_t sn0 = L.stage_n[b];
+    const float ki = kk[ic];
+    const float kv = DIV ? kappaV[ic] : 0.0f;
+    const float4 pi = pos[ic];
+    float4 v = vel[ic];
+    const uint32_t sn = stage_block(L, b, sn0, [&](uint32_t g) { return PK{pos[g], kk[g]}; },
                                     [&](uint32_t k, const PK& d) { sP[k] = d.p; sK[k] = d.k; });
     if (i >= n || !is_compute(L.rs, i)) return;
-    const float ki = kk[i];
-    if (DIV) kappaV[i] += ki;
-    const float4 pi = pos[i];
+    if (DIV) kappaV[i] = kv + ki;
     const uint32_t cn = L.cnt[i];
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
@@ -965,7 +979,6 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t,
         });
     }
     const float step = DIV ? t->h : t->h_future;
-    float4 v = vel[i];
     v.x += step * ax; v.y += step * ay; v.z += step * az;
     vel[i] = v;
 }
@@ -984,7 +997,12 @@ __global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, u
     __shared__ float4 sV[kStageMax];           // (v, m / rho) of the staged set
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t sn = stage_block(L, b, [&](uint32_t g) {
+    const uint32_t ic = i < n ? i : 0u;         // own loads before staging, so they overlap it
+    const uint32_t sn0 = L.stage_n[b];
+    const Head hd = list_head(L, sn0, c.cap, ic);
+    const float4 pi = pos[ic];
+    const float4 vi = vel[ic];
+    const uint32_t sn = stage_block(L, b, sn0, [&](uint32_t g) {
         const float4 p = pos[g];
         const float4 v = vel[g];
         return PV{p, make_float4(v.x, v.y, v.z, p.w / rho[g])};
@@ -993,16 +1011,13 @@ __global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, u
     if (i < n && !is_compute(L.rs, i)) {
         // restricted out: its stored acceleration still enters the CFL bound
         // (the reference's maximum runs over all particles, DFSPH_core_cuda.cu:3340-3355)
-        const float4 vi = vel[i], a = acc[i];
+        const float4 a = acc[i];
         const float h = t->h;
         const float wx = vi.x + a.x * h, wy = vi.y + a.y * h, wz = vi.z + a.z * h;
         m2 = wx * wx + wy * wy + wz * wz;
     }
     if (i < n && is_compute(L.rs, i)) {
         const float coef = t->inv_h * c.viscosity;
-        const float4 pi = pos[i];
-        const float4 vi = vel[i];
-        const uint32_t nf = L.cnt[i] & 0xffffu;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -1010,7 +1025,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, u
             const float s = keep(valid, coef * vj.w * w);
             ax -= s * (vi.x - vj.x); ay -= s * (vi.y - vj.y); az -= s * (vi.z - vj.z);
         };
-        fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+        fluid_nbrs(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
                    [&](uint32_t j) {
                        const float4 p = pos[j];
                        const float4 v = vel[j];
@@ -1091,12 +1106,16 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
     };
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t sn = stage_block(L, b, [&](uint32_t g) { return PK{pos[g], ws_of(g)}; },
+    const uint32_t ic = i < n ? i : 0u;         // own loads before staging, so they overlap it
+    const uint32_t sn0 = L.stage_n[b];
+    const float ki = ws_of(ic);
+    const float4 pi = pos[ic];
+    float4 v = vel[ic];
+    const float4 aown = APPLY_ACC ? acc[ic] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t sn = stage_block(L, b, sn0, [&](uint32_t g) { return PK{pos[g], ws_of(g)}; },
                                     [&](uint32_t k, const PK& d) { sP[k] = d.p; sK[k] = d.k; });
     if (i >= n || !is_compute(L.rs, i)) return;
-    const float ki = ws_of(i);
     if (ktot) ktot[i] = ki;
-    const float4 pi = pos[i];
     const uint32_t cn = L.cnt[i];
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
@@ -1119,10 +1138,9 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
             ax += s * dx; ay += s * dy; az += s * dz;
         });
     }
-    float4 v = vel[i];
     if (APPLY_ACC) {
         const float h = t->h;
-        const float4 a = acc[i];
+        const float4 a = aown;
         v.x += h * a.x; v.y += h * a.y; v.z += h * a.z;
     }
     const float hf = t->h_future;
@@ -1151,15 +1169,20 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
     if (!INIT && ctl->done) return;
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t sn = stage_block(L, b, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
+    const uint32_t ic = i < n ? i : 0u;         // own loads before staging, so they overlap it
+    const uint32_t sn0 = L.stage_n[b];
+    const Head hd = list_head(L, sn0, c.cap, ic);
+    const float4 pi = pos[ic];
+    const float4 vi = vel[ic];
+    const float a0 = alpha[ic], rho_i = rho[ic];
+    const float kap0 = INIT ? 0.0f : kappa[ic];
+    const uint32_t sn = stage_block(L, b, sn0, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
                                     [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
     float r = 0.0f;
     if (i < n && !is_source(L.rs, i)) r = radv[i];      // restricted out: stored value in the error sum
     if (i < n && is_source(L.rs, i)) {
         const float hf = t->h_future;
-        const float4 pi = pos[i];
-        const float4 vi = vel[i];
-        const uint32_t cn = L.cnt[i];
+        const uint32_t cn = hd.cnt;
         const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
         float dr = 0.0f;
         auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
@@ -1168,23 +1191,24 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
             const float mg = keep(valid && kp.ok, pj.w * kp.g);
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         };
-        fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+        fluid_nbrs(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
                    [&](uint32_t j) { return PV{pos[j], vel[j]}; },
                    [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
-        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j) { return bpos[j]; },
+        (void)nf;
+        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, hd.b0, hd.b1, [&](uint32_t j) { return bpos[j]; },
             [&](const float4& pj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const PairL kp = grad_listed(c, dist2(dx, dy, dz));
             const float mg = keep(valid && kp.ok, pj.w * kp.g);
             dr += mg * (vi.x * dx + vi.y * dy + vi.z * dz);
         });
-        float a = alpha[i];
+        float a = a0;
         if (INIT) { a *= t->inv_h_future; alpha[i] = a; }
-        r = fmaxf(rho[i] + hf * dr - c.density0, 0.0f);
+        r = fmaxf(rho_i + hf * dr - c.density0, 0.0f);
         radv[i] = r;
         const float ka = r * a;
         kk[i] = ka;
-        kappa[i] = INIT ? ka : kappa[i] + ka;
+        kappa[i] = INIT ? ka : kap0 + ka;
     }
     if (!INIT) {
         const double bs = block_sum((double)r, lds4);
@@ -1273,12 +1297,15 @@ __global__ __launch_bounds__(kBlock) void k_features(SimConst c, const DevTime* 
     __shared__ float4 sV[kStageMax];
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t sn = stage_block(L, b, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
+    const uint32_t ic = i < n ? i : 0u;         // own loads before staging, so they overlap it
+    const uint32_t sn0 = L.stage_n[b];
+    const Head hd = list_head(L, sn0, c.cap, ic);
+    const float4 pi = pos[ic];
+    const float4 vi = vel[ic];
+    const uint32_t sn = stage_block(L, b, sn0, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
                                     [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
     if (i >= n) return;
-    const float4 pi = pos[i];
-    const float4 vi = vel[i];
-    const uint32_t cn = L.cnt[i];
+    const uint32_t cn = hd.cnt;
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
     float dens = pi.w * c.W0, dfl = 0.0f, dr = 0.0f;
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, od = 0.0f;
@@ -1293,11 +1320,11 @@ __global__ __launch_bounds__(kBlock) void k_features(SimConst c, const DevTime* 
         ox -= m * dx; oy -= m * dy; oz -= m * dz;
         od += m * __builtin_sqrtf(r2);
     };
-    fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+    fluid_nbrs(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
                [&](uint32_t j) { return PV{pos[j], vel[j]}; },
                [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
     float dbd = 0.0f;
-    for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j) { return bpos[j]; },
+    for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, hd.b0, hd.b1, [&](uint32_t j) { return bpos[j]; },
         [&](const float4& pj, bool valid) {
         const float m = keep(valid, 1.0f);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -1490,16 +1517,19 @@ void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime*
 
 void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
                        const float4* bpos, const NbrLists& L, const float4* vel,
-                       const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial,
-                       const Publish& pub)
+                       const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial)
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
     if (init) {
         k_div_source<true><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial);
     } else {
         k_div_source<false><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial);
-        k_solve_finalize<<<1, kBlock, 0, s>>>(ctl, partial, g, pub);
     }
+}
+
+void launch_solve_finalize(hipStream_t s, uint32_t n, SolveCtl* ctl, const double* partial, const Publish& pub)
+{
+    k_solve_finalize<<<1, kBlock, 0, s>>>(ctl, partial, nblk_xcd(n ? n : 1), pub);
 }
 
 void launch_jacobi(hipStream_t s, bool div, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
@@ -1551,14 +1581,13 @@ void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c
 void launch_density_source(hipStream_t s, bool init, const SimConst& c, const DevTime* t, uint32_t n,
                            const float4* pos, const float4* bpos, const NbrLists& L,
                            const float4* vel, const float* rho, float* alpha, float* radv, float* kk,
-                           float* kappa, SolveCtl* ctl, double* partial, const Publish& pub)
+                           float* kappa, SolveCtl* ctl, double* partial)
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
     if (init) {
         k_density_source<true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl, partial);
     } else {
         k_density_source<false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl, partial);
-        k_solve_finalize<<<1, kBlock, 0, s>>>(ctl, partial, g, pub);
     }
 }
 

@@ -47,8 +47,9 @@ void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime*
                         float* radv, float* kvtot);
 void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
                        const float4* bpos, const NbrLists& L, const float4* vel,
-                       const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial,
-                       const Publish& pub = Publish{});
+                       const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial);
+// the one-block error reduction and loop decision after a loop-end sweep of n particles
+void launch_solve_finalize(hipStream_t s, uint32_t n, SolveCtl* ctl, const double* partial, const Publish& pub);
 void launch_jacobi(hipStream_t s, bool div, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
                    const float4* bpos, const NbrLists& L, const float* kk, float* kappaV,
                    float4* vel, const SolveCtl* ctl);
@@ -63,7 +64,7 @@ void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c
 void launch_density_source(hipStream_t s, bool init, const SimConst& c, const DevTime* t, uint32_t n,
                            const float4* pos, const float4* bpos, const NbrLists& L,
                            const float4* vel, const float* rho, float* alpha, float* radv, float* kk,
-                           float* kappa, SolveCtl* ctl, double* partial, const Publish& pub = Publish{});
+                           float* kappa, SolveCtl* ctl, double* partial);
 void launch_update_pos(hipStream_t s, const DevTime* t, uint32_t n, const float4* vel, float4* pos,
                        const float* bmin, const float* bmax, DevCtl* ctl, const Restrict& rs);
 void launch_keep_inside(hipStream_t s, uint32_t n, const float4* pos, const float* bmin, const float* bmax,
