@@ -149,40 +149,114 @@ __device__ void forward_wave(const DevModel& M, const WaveBufs& w, float out[4][
 // ---------------------------------------------------------------- inference
 // x: [n][12] raw features (FP32); output o of row r goes to y[r * ystride + o * ocol]
 // (denormalised when normalise is set).
-__global__ __launch_bounds__(64) void k_mlp_infer(const DevModel* __restrict__ Mg, const float* __restrict__ x,
-                                                   uint32_t n, float* __restrict__ y, int normalise, int ystride,
-                                                   int ocol)
+//
+// Register-resident form: each layer is computed transposed, H_out^T = W H_in^T,
+// with W as the MFMA A operand and the particles as the N dimension.  The
+// 16x16 output tile a lane holds (out features 4 (l/16) + q of 16 particles)
+// is exactly the B operand layout of the next layer's matching K block, so
+// activations never leave registers.  The weights' A fragments are loaded
+// once into registers and each wave runs over 16-particle tiles; biases come
+// from LDS as accumulator seeds.
+constexpr int kInferBlock = 256;
+
+__device__ __forceinline__ bf16x4 frag8(const unsigned short* p)     // 4 consecutive bf16 (8-byte aligned)
 {
-    __shared__ unsigned short X0[64 * kInPad];
-    __shared__ unsigned short A1[64 * kH], A2[64 * kH], A3[64 * kH];
+    const uint2 u = *reinterpret_cast<const uint2*>(p);
+    bf16x4 r;
+    r[0] = (short)(u.x & 0xffffu); r[1] = (short)(u.x >> 16);
+    r[2] = (short)(u.y & 0xffffu); r[3] = (short)(u.y >> 16);
+    return r;
+}
+
+__device__ __forceinline__ bf16x4 act_frag(const f32x4& z)
+{
+    bf16x4 r;
+    r[0] = (short)f2bf(silu(z[0])); r[1] = (short)f2bf(silu(z[1]));
+    r[2] = (short)f2bf(silu(z[2])); r[3] = (short)f2bf(silu(z[3]));
+    return r;
+}
+
+__global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const DevModel* __restrict__ Mg, const float* __restrict__ x,
+                                                           uint32_t n, float* __restrict__ y, int normalise,
+                                                           int ystride, int ocol)
+{
+    __shared__ __align__(16) float sb[3 * kH + kOutPad];
     const DevModel& M = *Mg;
-    const int l = threadIdx.x;
-    const uint32_t base = blockIdx.x * 64u;
-    const uint32_t row = base + l;
-    for (int k = 0; k < kInPad; ++k) {
-        float v = 0.0f;
-        if (k < kIn && row < n) {
-            v = x[(size_t)row * kIn + k];
-            if (normalise) v = (v - M.norm[k]) / M.norm[kIn + k];
-        }
-        X0[l * kInPad + k] = f2bf(v);
-    }
+    for (int k = threadIdx.x; k < 3 * kH + kOutPad; k += blockDim.x)
+        sb[k] = k < kH ? M.b1[k] : k < 2 * kH ? M.b2[k - kH] : k < 3 * kH ? M.b3[k - 2 * kH] : M.b4[k - 3 * kH];
     __syncthreads();
-    WaveBufs w{X0, A1, A2, A3, nullptr, nullptr, nullptr};
-    float out[4][4];
-    forward_wave(M, w, out);
-    const int o = l & 15;
-    if (o < kOut) {
-        for (int mt = 0; mt < 4; ++mt)
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t r = base + mt * 16 + 4 * (l >> 4) + q;
-                if (r < n) {
-                    float v = out[mt][q];
-                    if (normalise) v = v * M.norm[2 * kIn + kOut + o] + M.norm[2 * kIn + o];
-                    y[(size_t)r * ystride + (size_t)o * ocol] = v;
-                }
-            }
+    const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;   // particle column, 4-row group
+    bf16x4 w1[4], w2[4][4], w3[4][4], w4[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        w1[mt] = frag8(M.W1 + (mt * 16 + r) * kInPad + 4 * g);
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) {
+            w2[mt][kt] = frag8(M.W2 + (mt * 16 + r) * kH + kt * 16 + 4 * g);
+            w3[mt][kt] = frag8(M.W3 + (mt * 16 + r) * kH + kt * 16 + 4 * g);
+        }
+        w4[mt] = frag8(M.W4 + r * kH + mt * 16 + 4 * g);      // K block mt of the (padded) output layer
     }
+    float mu[4], isg[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int f = 4 * g + j;
+        mu[j] = (normalise && f < kIn) ? M.norm[f] : 0.0f;
+        isg[j] = (normalise && f < kIn) ? 1.0f / M.norm[kIn + f] : 1.0f;
+    }
+    const uint32_t tiles = (n + 15) / 16;
+    const uint32_t waves = gridDim.x * (blockDim.x / 64);
+    for (uint32_t t = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); t < tiles; t += waves) {
+        const uint32_t p = t * 16 + r;
+        // layer 1 input: features 4g..4g+3 of particle p (12..15 zero)
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (p < n && g < 3) v = *reinterpret_cast<const float4*>(x + (size_t)p * kIn + 4 * g);
+        bf16x4 h[4];
+        {
+            bf16x4 xb;
+            xb[0] = (short)f2bf((v.x - mu[0]) * isg[0]); xb[1] = (short)f2bf((v.y - mu[1]) * isg[1]);
+            xb[2] = (short)f2bf((v.z - mu[2]) * isg[2]); xb[3] = (short)f2bf((v.w - mu[3]) * isg[3]);
+            if (g == 3) xb = bf16x4{0, 0, 0, 0};
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                const float4 b = *reinterpret_cast<const float4*>(sb + mt * 16 + 4 * g);
+                h[mt] = act_frag(mfma(w1[mt], xb, f32x4{b.x, b.y, b.z, b.w}));
+            }
+        }
+#pragma unroll
+        for (int layer = 0; layer < 2; ++layer) {
+            bf16x4 o[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                const float4 b = *reinterpret_cast<const float4*>(sb + (layer + 1) * kH + mt * 16 + 4 * g);
+                f32x4 acc{b.x, b.y, b.z, b.w};
+#pragma unroll
+                for (int kt = 0; kt < 4; ++kt) acc = mfma(layer == 0 ? w2[mt][kt] : w3[mt][kt], h[kt], acc);
+                o[mt] = act_frag(acc);
+            }
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) h[mt] = o[mt];
+        }
+        const float4 b = *reinterpret_cast<const float4*>(sb + 3 * kH + 4 * g);
+        f32x4 acc{b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) acc = mfma(w4[kt], h[kt], acc);
+        if (g == 0 && p < n) {                  // outputs 0 and 1 of particle p
+#pragma unroll
+            for (int o = 0; o < kOut; ++o) {
+                float val = acc[o];
+                if (normalise) val = val * M.norm[2 * kIn + kOut + o] + M.norm[2 * kIn + o];
+                y[(size_t)p * ystride + (size_t)o * ocol] = val;
+            }
+        }
+    }
+}
+
+static uint32_t infer_grid(uint32_t n)
+{
+    const uint32_t tiles = (n + 15) / 16, per_block = kInferBlock / 64;
+    const uint32_t need = (tiles + per_block - 1) / per_block;
+    return std::max(1u, std::min(need, 2048u));
 }
 
 // ---------------------------------------------------------------- training
@@ -510,7 +584,8 @@ int dfsph_mlp_forward(void* handle, const float* x, uint32_t n, float* y, int no
     if (!rc) rc = malloc_dev(dy, (size_t)n * dfsph::kOut);
     if (!rc) {
         (void)hipMemcpyAsync(dx, x, sizeof(float) * n * dfsph::kIn, hipMemcpyHostToDevice, s);
-        dfsph::k_mlp_infer<<<(n + 63) / 64, 64, 0, s>>>(m->model, dx, n, dy, normalise, dfsph::kOut, 1);
+        dfsph::k_mlp_infer<<<dfsph::infer_grid(n), dfsph::kInferBlock, 0, s>>>(m->model, dx, n, dy, normalise,
+                                                                                  dfsph::kOut, 1);
         (void)hipMemcpyAsync(y, dy, sizeof(float) * n * dfsph::kOut, hipMemcpyDeviceToHost, s);
         if (hipStreamSynchronize(s) != hipSuccess) rc = mfail(DFSPH_ERR_HIP, "mlp forward");
     }
@@ -575,7 +650,7 @@ int mlp_ensure_particles(MlpState* m, uint32_t cap)
 int mlp_infer_particles(MlpState* m, hipStream_t s, uint32_t n)
 {
     if (n == 0) return DFSPH_OK;
-    k_mlp_infer<<<(n + 63) / 64, 64, 0, s>>>(m->model, m->feat, n, m->pred, 1, 1, (int)m->cap);
+    k_mlp_infer<<<infer_grid(n), kInferBlock, 0, s>>>(m->model, m->feat, n, m->pred, 1, 1, (int)m->cap);
     MHIP(hipGetLastError());
     return DFSPH_OK;
 }
