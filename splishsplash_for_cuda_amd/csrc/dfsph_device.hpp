@@ -292,9 +292,12 @@ __device__ __forceinline__ PairL grad_listed(const SimConst& c, float r2)
     const float ir = __builtin_amdgcn_rsqf(r2c);
     const float r = r2c * ir;
     const float d = c.hk - r;
+    // both pieces are computed and one is selected: a branch per pair (the
+    // select-to-branch transform) costs more than the other piece
+    const float gi = __builtin_fmaf(c.gin_a, r, c.gin_b), go = (c.gout * ir) * (d * d);
     PairL o;
     o.W = 0.0f;
-    o.g = (r2 <= c.hk2_4) ? __builtin_fmaf(c.gin_a, r, c.gin_b) : (c.gout * ir) * (d * d);
+    o.g = __builtin_unpredictable(r2 <= c.hk2_4) ? gi : go;
     o.ok = r2 > 1.0e-12f;
     return o;
 }
@@ -307,10 +310,12 @@ __device__ __forceinline__ PairL pair_listed(const SimConst& c, float r2)
     const float d = c.hk - r;
     const float q = r * c.inv_hk;
     const float f = 1.0f - q;
-    const bool in = r2 <= c.hk2_4;
+    const bool in = __builtin_unpredictable(r2 <= c.hk2_4);
+    const float wi = c.kcoef * __builtin_fmaf(6.0f * q * q, q - 1.0f, 1.0f), wo = 2.0f * c.kcoef * f * f * f;
+    const float gi = __builtin_fmaf(c.gin_a, r, c.gin_b), go = (c.gout * ir) * (d * d);
     PairL o;
-    o.W = in ? c.kcoef * __builtin_fmaf(6.0f * q * q, q - 1.0f, 1.0f) : 2.0f * c.kcoef * f * f * f;
-    o.g = in ? __builtin_fmaf(c.gin_a, r, c.gin_b) : (c.gout * ir) * (d * d);
+    o.W = in ? wi : wo;
+    o.g = in ? gi : go;
     o.ok = r2 > 1.0e-12f;
     return o;
 }
@@ -320,7 +325,8 @@ __device__ __forceinline__ float W_listed(const SimConst& c, float r2)
     const float r2c = fmaxf(r2, 1.0e-24f);
     const float q = r2c * __builtin_amdgcn_rsqf(r2c) * c.inv_hk;
     const float f = 1.0f - q;
-    return r2 <= c.hk2_4 ? c.kcoef * __builtin_fmaf(6.0f * q * q, q - 1.0f, 1.0f) : 2.0f * c.kcoef * f * f * f;
+    const float wi = c.kcoef * __builtin_fmaf(6.0f * q * q, q - 1.0f, 1.0f), wo = 2.0f * c.kcoef * f * f * f;
+    return __builtin_unpredictable(r2 <= c.hk2_4) ? wi : wo;
 }
 
 __device__ __forceinline__ int cell_coord(const SimConst& c, float v, int a)
