@@ -96,6 +96,7 @@ struct Handle {
     float4* bpos = nullptr;
     uint32_t* bcstart = nullptr;
     uint32_t* tile_rank = nullptr;    // SimConst::tile_rank
+    uint4* nbr_part = nullptr;        // per-block neighbor statistics of the search
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
 
     // device-side control
@@ -229,7 +230,7 @@ Handle::~Handle()
     }
     dfree(rho); dfree(alpha); dfree(radv); dfree(kk); dfree(acc); dfree(cnt); dfree(nbrf); dfree(nbrb); dfree(nbr16); dfree(stage_n); dfree(stg); dfree(ob_cand);
     dfree(key); dfree(slot); dfree(perm); dfree(hist); dfree(cstart); dfree(keep); dfree(rank);
-    dfree(keep8); dfree(scan_tmp); dfree(bpos); dfree(bcstart); dfree(tile_rank);
+    dfree(keep8); dfree(scan_tmp); dfree(bpos); dfree(bcstart); dfree(tile_rank); dfree(nbr_part);
     dfree(ctl); dfree(tm); dfree(partial); dfree(partial_f);
     if (hctl) (void)hipHostFree(hctl);
     if (htm) (void)hipHostFree(htm);
@@ -348,9 +349,12 @@ int neighbor_search(Handle* h, bool sort)
 {
     if (!h->grid_ready) return fail(DFSPH_ERR_STATE, "no boundary set uploaded");
     CHK(flush_vel(h));
-    dfsph::launch_nbr_ctl_init(h->s, h->ctl);
     h->pred_valid = false;
-    if (h->n == 0) { h->lists_valid = true; return DFSPH_OK; }
+    if (h->n == 0) {
+        dfsph::launch_nbr_ctl_init(h->s, h->ctl);     // otherwise the search's finalize writes the statistics
+        h->lists_valid = true;
+        return DFSPH_OK;
+    }
     const int ps = pbeg(h, DFSPH_KERNEL_SORT);
     CHK(counting_sort(h, h->pos[h->pc], h->n, h->cstart));
     if (sort) {
@@ -362,7 +366,8 @@ int neighbor_search(Handle* h, bool sort)
     pend(h, ps);
     const int pn = pbeg(h, DFSPH_KERNEL_NEIGHBORS);
     dfsph::launch_neighbors(h->s, h->c, !sort, h->pos[h->pc], h->cstart, h->perm, h->bpos, h->bcstart, h->n,
-                            h->nbr16, h->nbrf, h->nbrb, h->cnt, h->stage_n, h->stg, restriction(h), h->ctl);
+                            h->nbr16, h->nbrf, h->nbrb, h->cnt, h->stage_n, h->stg, restriction(h), h->nbr_part,
+                            h->ctl);
     pend(h, pn);
     HIPCHK(hipGetLastError());
     h->lists_valid = true;
@@ -618,6 +623,7 @@ int alloc_fluid(Handle* h, uint32_t cap)
     // neighbor sets: one per block of the XCD-padded grid
     const uint32_t nblocks = dfsph::grid_blocks_xcd(cap);
     CHK(dalloc(h->stage_n, nblocks));
+    CHK(dalloc(h->nbr_part, 2 * (size_t)nblocks));
     CHK(dalloc(h->stg, (size_t)nblocks * dfsph::kStageCap));
     CHK(ensure_tmp(h, std::max(cap, h->nb)));
     CHK(ensure_partials(h, cap));

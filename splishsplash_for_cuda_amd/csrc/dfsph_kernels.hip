@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
                                                       uint16_t* __restrict__ nbr16, uint32_t* __restrict__ nbrf,
                                                       uint32_t* __restrict__ nbrb, uint32_t* __restrict__ cnt,
                                                       uint32_t* __restrict__ stage_n, uint32_t* __restrict__ stg,
-                                                      Restrict rs, DevCtl* ctl)
+                                                      Restrict rs, uint4* __restrict__ part)
 {
     // the dense table shares one area with the staged positions
     __shared__ __align__(16) unsigned char s_raw[kNbrStage * sizeof(float4)];
@@ -412,6 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
     __shared__ uint32_t s_src[kSetCells];      // first fluid index of each set cell
     __shared__ uint32_t s_bs[kSetCells];       // boundary range of each set cell
     __shared__ uint32_t s_be[kSetCells];
+    __shared__ uint32_t s_boff[kSetCells + 1]; // boundary set offsets of the set cells
     __shared__ int s_w[6 * (kBlock / 64)];
     static_assert(kDense * sizeof(uint16_t) <= sizeof(s_raw), "dense table exceeds the stage");
     static_assert((kStageCap - 1) * 16 <= 0xffffu && kSetCells < 0xffffu, "16-bit byte offsets / cell ids");
@@ -430,7 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
         cy = cell_coord(c, p.y, 1);
         cz = cell_coord(c, p.z, 2);
     }
-    uint32_t S = kUnstaged, U = 0, uown = 0;
+    uint32_t S = kUnstaged, U = 0, uown = 0, Sb = 0;
     uint32_t row[9];
 #pragma unroll
     for (int r = 0; r < 9; ++r) row[r] = kNoRow;
@@ -523,6 +524,17 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
                 }
                 if (threadIdx.x == 0) s_off[U] = tot;
                 S = tot <= c.stage_cap ? tot : kUnstaged;
+                // boundary counts -> boundary set offsets (the set's boundary
+                // particles follow its fluid ones when both fit in the stage)
+                uint32_t bsum = 0;
+                for (uint32_t u = u0; u < u1; ++u) bsum += s_be[u] - s_bs[u];
+                uint32_t boff = 0;
+                Sb = block_excl_scan_u(bsum, &boff, s_wu);
+                for (uint32_t u = u0; u < u1; ++u) {
+                    s_boff[u] = boff;
+                    boff += s_be[u] - s_bs[u];
+                }
+                if (threadIdx.x == 0) s_boff[U] = Sb;
                 // each lane's rows (dy, dz) as (first, last) set cell of x-1..x+1
                 if (act) {
                     uown = s_dense[W * ((uint32_t)(cz - lo[2]) + D * (uint32_t)(cy - lo[1])) + (uint32_t)(cx - lo[0])];
@@ -540,6 +552,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
         }
     }
     const bool lds_search = S <= kNbrStage && c.stage_lds > 0u;
+    const bool lds_bnd = lds_search && S + Sb <= kNbrStage;   // boundary staged at [S, S + Sb)
     if (threadIdx.x == 0) stage_n[b] = S == kUnstaged || S <= c.stage_lds ? S : (S | kSetNoLds);
     if (S != kUnstaged) {
         uint32_t* g_out = stg + (size_t)b * kStageCap;
@@ -553,6 +566,18 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
             if (lds_search) s_pos[k] = pos[g];
             g_out[k] = g;
         }
+        if (lds_bnd) {
+            for (uint32_t k = threadIdx.x; k < Sb; k += kBlock) {
+                uint32_t lo = 0, hi = U;        // the set cell holding boundary set index k
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_boff[mid] <= k) lo = mid; else hi = mid;
+                }
+                const uint32_t j = s_bs[lo] + (k - s_boff[lo]);
+                const float4 q = bpos[j];
+                s_pos[S + k] = make_float4(q.x, q.y, q.z, __uint_as_float(j));   // w: the global boundary id
+            }
+        }
         __syncthreads();
     }
     uint32_t total = 0, trunc = 0, stored_f = 0, stored_b = 0, stored_u32 = 0;
@@ -565,27 +590,37 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
             ListWriter16 w{reinterpret_cast<uint4*>(nbr16) + i, cap};
             const uint32_t self = s_off[uown] + (i - s_src[uown]);
             if (lds_search) {
+                // Each row range goes in chunks of 32 candidates: a branch-free
+                // pass sets one bit per candidate within h, then the set bits
+                // are appended in order (stored while below max_neighbors).
+                uint32_t ntot = 0;
 #pragma unroll
                 for (int r = 0; r < 9; ++r) {
                     if (row[r] == kNoRow) continue;
                     const uint32_t e = s_off[(row[r] >> 16) + 1u];
-                    uint32_t l = s_off[row[r] & 0xffffu];
-                    for (; l + 2u <= e; l += 2u) {               // two reads in flight
-                        const float4 qa = s_pos[l], qb = s_pos[l + 1u];
-                        const bool ia = dist2(p.x - qa.x, p.y - qa.y, p.z - qa.z) < hk2 && l != self;
-                        const bool ib = dist2(p.x - qb.x, p.y - qb.y, p.z - qb.z) < hk2 && l + 1u != self;
-                        if (ia && nf < maxn) w.push(nf, l << 4);  // LDS byte offset
-                        nf += ia ? 1u : 0u;
-                        if (ib && nf < maxn) w.push(nf, (l + 1u) << 4);
-                        nf += ib ? 1u : 0u;
-                    }
-                    if (l < e) {
-                        const float4 qa = s_pos[l];
-                        const bool ia = dist2(p.x - qa.x, p.y - qa.y, p.z - qa.z) < hk2 && l != self;
-                        if (ia && nf < maxn) w.push(nf, l << 4);
-                        nf += ia ? 1u : 0u;
+                    for (uint32_t l0 = s_off[row[r] & 0xffffu]; l0 < e; l0 += 32u) {
+                        const uint32_t cnt = min(e - l0, 32u);
+                        uint32_t m = 0, t = 0;
+                        for (; t + 2u <= cnt; t += 2u) {
+                            const float4 qa = s_pos[l0 + t], qb = s_pos[l0 + t + 1u];
+                            const uint32_t ia = dist2(p.x - qa.x, p.y - qa.y, p.z - qa.z) < hk2 ? 1u : 0u;
+                            const uint32_t ib = dist2(p.x - qb.x, p.y - qb.y, p.z - qb.z) < hk2 ? 2u : 0u;
+                            m |= (ia | ib) << t;
+                        }
+                        if (t < cnt) {
+                            const float4 qa = s_pos[l0 + t];
+                            m |= (dist2(p.x - qa.x, p.y - qa.y, p.z - qa.z) < hk2 ? 1u : 0u) << t;
+                        }
+                        if (self - l0 < cnt) m &= ~(1u << (self - l0));
+                        ntot += (uint32_t)__popc(m);
+                        while (m != 0u && nf < maxn) {
+                            w.push(nf, (l0 + (uint32_t)__builtin_ctz(m)) << 4);   // LDS byte offset
+                            ++nf;
+                            m &= m - 1u;
+                        }
                     }
                 }
+                nf = ntot;                      // stored: min(nf, max_neighbors), as below
             } else {
 #pragma unroll
                 for (int r = 0; r < 9; ++r) {
@@ -639,7 +674,37 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
                 }
             }
         };
-        if (tabled) {
+        if (lds_bnd) {
+            // the row's boundary particles are one contiguous staged range;
+            // the same chunked bitmask pass as the fluid search
+            uint32_t ntot = 0;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                if (row[r] == kNoRow) continue;
+                const uint32_t e = S + s_boff[(row[r] >> 16) + 1u];
+                for (uint32_t l0 = S + s_boff[row[r] & 0xffffu]; l0 < e; l0 += 32u) {
+                    const uint32_t cntc = min(e - l0, 32u);
+                    uint32_t m = 0, t = 0;
+                    for (; t + 2u <= cntc; t += 2u) {
+                        const float4 qa = s_pos[l0 + t], qb = s_pos[l0 + t + 1u];
+                        const uint32_t ia = dist2(p.x - qa.x, p.y - qa.y, p.z - qa.z) < hk2 ? 1u : 0u;
+                        const uint32_t ib = dist2(p.x - qb.x, p.y - qb.y, p.z - qb.z) < hk2 ? 2u : 0u;
+                        m |= (ia | ib) << t;
+                    }
+                    if (t < cntc) {
+                        const float4 qa = s_pos[l0 + t];
+                        m |= (dist2(p.x - qa.x, p.y - qa.y, p.z - qa.z) < hk2 ? 1u : 0u) << t;
+                    }
+                    ntot += (uint32_t)__popc(m);
+                    while (m != 0u && sf + nb < maxn) {
+                        wb.push(nb, __float_as_uint(s_pos[l0 + (uint32_t)__builtin_ctz(m)].w));
+                        ++nb;
+                        m &= m - 1u;
+                    }
+                }
+            }
+            nb = ntot;
+        } else if (tabled) {
 #pragma unroll
             for (int r = 0; r < 9; ++r) {
                 if (row[r] == kNoRow) continue;
@@ -657,19 +722,56 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
         stored_b = sb;
         stored_u32 = S == kUnstaged ? sf : 0u;
     }
-    const uint32_t wmax = wave_max_u(total);
-    const uint32_t wtr = wave_sum_u(trunc);
-    const uint32_t wsf = wave_sum_u(stored_f);
-    const uint32_t wsb = wave_sum_u(stored_b);
-    const uint32_t wsu = wave_sum_u(stored_u32);
-    if ((threadIdx.x & 63) == 0) {
-        if (wmax) atomicMax(&ctl->max_seen, wmax);
-        if (wtr) atomicAdd(&ctl->overflow, wtr);
-        if (wsf) atomicAdd(&ctl->nbr_fluid, (unsigned long long)wsf);
-        if (wsb) atomicAdd(&ctl->nbr_boundary, (unsigned long long)wsb);
-        if (wsu) atomicAdd(&ctl->nbr_fluid_unstaged, (unsigned long long)wsu);
+    // the block's statistics go to part[2b], part[2b + 1] (k_nbr_finalize
+    // sums them): per-wave atomics on one control line would serialise
+    const uint32_t wv[5] = {wave_max_u(total), wave_sum_u(trunc), wave_sum_u(stored_f), wave_sum_u(stored_b),
+                            wave_sum_u(stored_u32)};
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 5; ++k) s_w[5 * w + k] = (int)wv[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t a[5] = {0u, 0u, 0u, 0u, 0u};
+        for (int v = 0; v < kBlock / 64; ++v)
+            for (int k = 0; k < 5; ++k) {
+                const uint32_t x = (uint32_t)s_w[5 * v + k];
+                a[k] = k == 0 ? max(a[0], x) : a[k] + x;
+            }
+        const uint32_t unstaged = S > c.stage_lds && n > b * kBlock ? 1u : 0u;
+        part[2 * b] = make_uint4(a[0], a[1], a[2], a[3]);
+        part[2 * b + 1] = make_uint4(a[4], unstaged, 0u, 0u);
     }
-    if (threadIdx.x == 0 && S > c.stage_lds && n > b * kBlock) atomicAdd(&ctl->unstaged_blocks, 1u);
+}
+
+// The neighbor statistics of the last search from the block records of
+// k_neighbors (maximum, sums in a fixed order).
+__global__ __launch_bounds__(kBlock) void k_nbr_finalize(const uint4* __restrict__ part, uint32_t nblocks, DevCtl* ctl)
+{
+    __shared__ unsigned long long s[6][kBlock];
+    unsigned long long a[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < nblocks; b += kBlock) {
+        const uint4 p = part[2 * b], q = part[2 * b + 1];
+        a[0] = a[0] > p.x ? a[0] : p.x;
+        a[1] += p.y; a[2] += p.z; a[3] += p.w; a[4] += q.x; a[5] += q.y;
+    }
+    for (int k = 0; k < 6; ++k) s[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (uint32_t h = kBlock / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int k = 0; k < 6; ++k) {
+                const unsigned long long o = s[k][threadIdx.x + h];
+                s[k][threadIdx.x] = k == 0 ? (s[0][threadIdx.x] > o ? s[0][threadIdx.x] : o) : s[k][threadIdx.x] + o;
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ctl->max_seen = (uint32_t)s[0][0];
+        ctl->overflow = (uint32_t)s[1][0];
+        ctl->nbr_fluid = s[2][0];
+        ctl->nbr_boundary = s[3][0];
+        ctl->nbr_fluid_unstaged = s[4][0];
+        ctl->unstaged_blocks = (uint32_t)s[5][0];
+    }
 }
 
 // Every list of the last search as 32-bit global ids, fluid then boundary
@@ -1473,15 +1575,17 @@ void launch_gather_f4(hipStream_t s, uint32_t n, const uint32_t* perm, const flo
 void launch_neighbors(hipStream_t s, const SimConst& c, bool indirect, const float4* pos, const uint32_t* cs_f,
                       const uint32_t* perm, const float4* bpos, const uint32_t* cs_b, uint32_t n, uint16_t* nbr16,
                       uint32_t* nbrf, uint32_t* nbrb, uint32_t* cnt, uint32_t* stage_n, uint32_t* stg,
-                      const Restrict& rs, DevCtl* ctl)
+                      const Restrict& rs, uint4* part, DevCtl* ctl)
 {
     if (!n) return;
+    const uint32_t g = nblk_xcd(n);
     if (indirect)
-        k_neighbors<true><<<nblk_xcd(n), kBlock, 0, s>>>(c, pos, cs_f, perm, bpos, cs_b, n, nbr16, nbrf, nbrb, cnt,
-                                                         stage_n, stg, rs, ctl);
+        k_neighbors<true><<<g, kBlock, 0, s>>>(c, pos, cs_f, perm, bpos, cs_b, n, nbr16, nbrf, nbrb, cnt, stage_n, stg,
+                                               rs, part);
     else
-        k_neighbors<false><<<nblk_xcd(n), kBlock, 0, s>>>(c, pos, cs_f, perm, bpos, cs_b, n, nbr16, nbrf, nbrb, cnt,
-                                                          stage_n, stg, rs, ctl);
+        k_neighbors<false><<<g, kBlock, 0, s>>>(c, pos, cs_f, perm, bpos, cs_b, n, nbr16, nbrf, nbrb, cnt, stage_n, stg,
+                                                rs, part);
+    k_nbr_finalize<<<1, kBlock, 0, s>>>(part, g, ctl);
 }
 
 void launch_expand_lists(hipStream_t s, uint32_t n, uint32_t cap, const NbrLists& L, uint32_t* out)

@@ -35,7 +35,7 @@ void launch_gather_f4(hipStream_t s, uint32_t n, const uint32_t* perm, const flo
 void launch_neighbors(hipStream_t s, const SimConst& c, bool indirect, const float4* pos, const uint32_t* cs_f,
                       const uint32_t* perm, const float4* bpos, const uint32_t* cs_b, uint32_t n,
                       uint16_t* nbr16, uint32_t* nbrf, uint32_t* nbrb, uint32_t* cnt, uint32_t* stage_n,
-                      uint32_t* stg, const Restrict& rs, DevCtl* ctl);
+                      uint32_t* stg, const Restrict& rs, uint4* part, DevCtl* ctl);
 void launch_expand_lists(hipStream_t s, uint32_t n, uint32_t cap, const NbrLists& L, uint32_t* out);
 void launch_boundary_psi(hipStream_t s, const SimConst& c, const float4* bpos, const uint32_t* cs_b,
                          uint32_t nb, float* psi);
