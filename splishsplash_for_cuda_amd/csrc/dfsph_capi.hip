@@ -309,9 +309,8 @@ int ensure_partials(Handle* h, uint32_t n)
 int counting_sort(Handle* h, const float4* pts, uint32_t n, uint32_t* cs)
 {
     const uint32_t nc = h->c.ncell;
-    dfsph::launch_fill_u32(h->s, h->hist, 0u, nc + 1);
     dfsph::launch_cell_hist(h->s, h->c, pts, n, h->key, h->slot, h->hist);
-    dfsph::launch_exclusive_scan(h->s, h->hist, cs, nc + 1, h->scan_tmp);
+    dfsph::launch_exclusive_scan(h->s, h->hist, cs, nc + 1, h->scan_tmp, true);   // leaves hist zeroed
     dfsph::launch_scatter_perm(h->s, h->key, h->slot, cs, n, h->perm);
     dfsph::launch_cell_sort(h->s, cs, nc, h->perm);
     HIPCHK(hipGetLastError());
@@ -378,14 +377,21 @@ int neighbor_search(Handle* h, bool sort)
 // Queue Jacobi iterations in batches; the device decides when the loop ends
 // (kernels of later iterations exit at once).  Two readback slots keep one
 // batch in flight while the previous batch's loop state is inspected.
+// The loop control a solve's first sweep sets (block 0 of that sweep).
+dfsph::CtlInit solve_ctl(Handle* h, bool div, uint32_t max_iter, double max_error)
+{
+    // error average over true_particle_count when set (DFSPH_core_cuda.cu:791-795, 1339-1343)
+    const double n_div = h->true_count >= 0 ? (double)h->true_count : (double)h->n;
+    return dfsph::CtlInit{div ? &h->ctl->div : &h->ctl->dens, h->tm, max_error * 0.01 * h->p.density0, n_div,
+                          div ? 1 : 0, div ? h->p.min_iter_v : h->p.min_iter, max_iter};
+}
+
+// The loop control was set by the solve's first sweep (solve_ctl).
 int jacobi_loop(Handle* h, bool div, uint32_t max_iter, double max_error, uint32_t* iters, double* err)
 {
     SolveCtl* sc = div ? &h->ctl->div : &h->ctl->dens;
     const uint32_t min_iter = div ? h->p.min_iter_v : h->p.min_iter;
-    // error average over true_particle_count when set (DFSPH_core_cuda.cu:791-795, 1339-1343)
-    const double n_div = h->true_count >= 0 ? (double)h->true_count : (double)h->n;
-    dfsph::launch_ctl_init(h->s, sc, max_error * 0.01 * h->p.density0, div ? 1 : 0, h->tm, n_div,
-                           min_iter, max_iter);
+    (void)max_error;
     if (h->n == 0 || max_iter == 0) {
         if (iters) *iters = 0;
         if (err) *err = 0.0;
@@ -484,7 +490,8 @@ int divergence_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* i
     h->vc = vo;
     pr = pbeg(h, DFSPH_KERNEL_DIV_INIT);
     dfsph::launch_div_source(h->s, true, h->c, h->n, h->pos[h->pc], h->bpos, lists(h), h->vel[h->vc],
-                             h->alpha, h->radv, h->kk, h->kapv[h->pc], nullptr, nullptr);
+                             h->alpha, h->radv, h->kk, h->kapv[h->pc], nullptr, nullptr,
+                             solve_ctl(h, true, max_iter, max_error));
     pend(h, pr);
     HIPCHK(hipGetLastError());
     h->acc_valid = false;
@@ -511,7 +518,6 @@ int external_forces(Handle* h)
 {
     CHK(flush_vel(h));
     if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale: call dfsph_neighbor_search");
-    dfsph::launch_step_ctl_init(h->s, h->ctl, h->n);
     const int pr = pbeg(h, DFSPH_KERNEL_XSPH);
     dfsph::launch_xsph(h->s, h->c, h->tm, h->n, h->pos[h->pc], lists(h), h->vel[h->vc], h->rho, h->acc,
                        h->ctl, h->partial_f);
@@ -543,7 +549,8 @@ int pressure_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* ite
     }
     const int pr = pbeg(h, DFSPH_KERNEL_DENS_INIT);
     dfsph::launch_density_source(h->s, true, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h),
-                                 h->vel[h->vc], h->rho, h->alpha, h->radv, h->kk, h->kap[h->pc], nullptr, nullptr);
+                                 h->vel[h->vc], h->rho, h->alpha, h->radv, h->kk, h->kap[h->pc], nullptr, nullptr,
+                                 solve_ctl(h, false, max_iter, max_error));
     pend(h, pr);
     HIPCHK(hipGetLastError());
     h->acc_valid = false;
@@ -838,6 +845,7 @@ int dfsph_upload_boundary(void* hv, const float* xyz, const float* psi, uint32_t
     }
     h->nb = n;
     CHK(dalloc(h->hist, ncell + 1));
+    dfsph::launch_fill_u32(h->s, h->hist, 0u, (uint32_t)ncell + 1);   // each counting sort leaves it zeroed
     CHK(dalloc(h->cstart, ncell + 1));
     CHK(dalloc(h->bcstart, ncell + 1));
     CHK(ensure_scan(h, (uint32_t)ncell + 1));

@@ -103,6 +103,30 @@ struct HostRecord {
     uint32_t seq, pad[3];
 };
 
+// A Jacobi loop's control, set by the first sweep of its solve (block 0)
+// (DFSPH_core_cuda.cu:791-795, 1339-1343: eta, the averaging count, limits).
+struct CtlInit {
+    SolveCtl* sc;                  // nullptr: nothing to set
+    const DevTime* t;
+    double eta_base, n_div;        // eta = eta_base (/ h when divide_by_h)
+    int divide_by_h;
+    uint32_t min_iter, max_iter;
+};
+
+__device__ __forceinline__ void ctl_init(const CtlInit& ci)
+{
+    SolveCtl* s = ci.sc;
+    s->sum = 0.0;
+    s->err = 0.0;
+    s->eta = ci.divide_by_h ? ci.eta_base / ci.t->hd : ci.eta_base;
+    s->n_div = ci.n_div;
+    s->iters = 0;
+    s->done = (ci.max_iter == 0) ? 1u : 0u;
+    s->min_iter = ci.min_iter;
+    s->max_iter = ci.max_iter;
+    s->ticket = 0;
+}
+
 struct Publish {
     const DevCtl* dc;
     const DevTime* t;

@@ -27,32 +27,6 @@ __global__ void k_fill_f32(float* p, float v, uint32_t n)
 // Loop-control init (the host-side locals of cuda_divergenceSolve /
 // cuda_pressureSolve, DFSPH_core_cuda.cu:785-828, 1338-1378).
 // eta = max_error * 0.01 * rho0 (/ h for the divergence solver).
-__global__ void k_ctl_init(SolveCtl* s, double eta_base, int divide_by_h, const DevTime* t,
-                           double n_div, uint32_t min_iter, uint32_t max_iter)
-{
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        s->sum = 0.0;
-        s->err = 0.0;
-        s->eta = divide_by_h ? eta_base / t->hd : eta_base;
-        s->n_div = n_div;
-        s->iters = 0;
-        s->done = (max_iter == 0) ? 1u : 0u;
-        s->min_iter = min_iter;
-        s->max_iter = max_iter;
-        s->ticket = 0;
-    }
-}
-
-__global__ void k_step_ctl_init(DevCtl* c, uint32_t n_active)
-{
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        c->cfl_max = 0.0f;
-        c->cfl_ticket = 0;
-        c->escaped = 0;
-        c->n_active = n_active;
-    }
-}
-
 __global__ void k_nbr_ctl_init(DevCtl* c)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -157,8 +131,8 @@ __global__ void k_time_step_end(DevTime* t, Publish pub)
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
 
-__global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* in, uint32_t* out, uint32_t n,
-                                                        uint32_t* tile_sums)
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(uint32_t* in, uint32_t* out, uint32_t n,
+                                                        uint32_t* tile_sums, int clear_in)
 {
     __shared__ uint32_t tile[kScanTile];
     __shared__ uint32_t wsum[kBlock / 64];
@@ -167,6 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* in, uint3
     for (int k = 0; k < kScanItems; ++k) {
         const uint32_t g = base + k * kBlock + threadIdx.x;
         tile[k * kBlock + threadIdx.x] = g < n ? in[g] : 0u;
+        if (clear_in && g < n) in[g] = 0u;     // a histogram left zeroed for its next use
     }
     __syncthreads();
     uint32_t loc[kScanItems];
@@ -195,6 +170,28 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* in, uint3
         uint32_t tot = 0;
         for (int k = 0; k < kBlock / 64; ++k) tot += wsum[k];
         tile_sums[blockIdx.x] = tot;
+    }
+}
+
+// The tile's offset is the sum of the preceding tile sums, which each block
+// adds up itself (grids of up to kScanDirect tiles): no scan of the sums.
+constexpr uint32_t kScanDirect = 4096;
+__global__ __launch_bounds__(kBlock) void k_scan_add_direct(uint32_t* out, uint32_t n, const uint32_t* tile_sums)
+{
+    __shared__ uint32_t ws[kBlock / 64];
+    uint32_t acc = 0;
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kBlock) acc += tile_sums[t];
+    acc = wave_sum_u(acc);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    uint32_t off = 0;
+#pragma unroll
+    for (int k = 0; k < kBlock / 64; ++k) off += ws[k];
+    const uint32_t base = blockIdx.x * kScanTile;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t g = base + k * kBlock + threadIdx.x;
+        if (g < n) out[g] += off;
     }
 }
 
@@ -984,8 +981,9 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
                                                        const float* __restrict__ alpha,
                                                        float* __restrict__ radv, float* __restrict__ kk,
                                                        float* __restrict__ kappaV, SolveCtl* ctl,
-                                                       double* partial)
+                                                       double* partial, CtlInit ci)
 {
+    if (INIT && ci.sc && blockIdx.x == 0 && threadIdx.x == 0) ctl_init(ci);
     __shared__ double lds4[kBlock / 64];
     __shared__ float4 sP[kStageMax];
     __shared__ float4 sV[kStageMax];
@@ -1164,6 +1162,7 @@ __global__ __launch_bounds__(kBlock) void k_cfl_finalize(DevCtl* ctl, const floa
         float bm = lds4[0];
         for (int k = 1; k < kBlock / 64; ++k) bm = fmaxf(bm, lds4[k]);
         ctl->cfl_max = bm;
+        ctl->escaped = 0;                      // counted afresh by the step's update_pos
     }
 }
 
@@ -1263,8 +1262,9 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
                                                            const float* __restrict__ rho,
                                                            float* __restrict__ alpha, float* __restrict__ radv,
                                                            float* __restrict__ kk, float* __restrict__ kappa,
-                                                           SolveCtl* ctl, double* partial)
+                                                           SolveCtl* ctl, double* partial, CtlInit ci)
 {
+    if (INIT && ci.sc && blockIdx.x == 0 && threadIdx.x == 0) ctl_init(ci);
     __shared__ double lds4[kBlock / 64];
     __shared__ float4 sP[kStageMax];
     __shared__ float4 sV[kStageMax];
@@ -1504,28 +1504,19 @@ size_t scan_temp_words(uint32_t n)
     return words + 4;
 }
 
-void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp)
+void launch_exclusive_scan(hipStream_t s, uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, bool clear_in)
 {
     if (n == 0) return;
     const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
     uint32_t* sums = tmp;
     uint32_t* offs = tmp + tiles + 1;
-    k_scan_tiles<<<tiles, kBlock, 0, s>>>(in, out, n, sums);
-    if (tiles > 1) {
-        launch_exclusive_scan(s, sums, offs, tiles, tmp + 2 * (size_t)tiles + 2);
+    k_scan_tiles<<<tiles, kBlock, 0, s>>>(in, out, n, sums, clear_in ? 1 : 0);
+    if (tiles > 1 && tiles <= kScanDirect) {
+        k_scan_add_direct<<<tiles, kBlock, 0, s>>>(out, n, sums);
+    } else if (tiles > 1) {
+        launch_exclusive_scan(s, sums, offs, tiles, tmp + 2 * (size_t)tiles + 2, false);
         k_scan_add<<<tiles, kBlock, 0, s>>>(out, n, offs);
     }
-}
-
-void launch_ctl_init(hipStream_t s, SolveCtl* ctl, double eta_base, int divide_by_h, const DevTime* t,
-                     double n_div, uint32_t min_iter, uint32_t max_iter)
-{
-    k_ctl_init<<<1, 64, 0, s>>>(ctl, eta_base, divide_by_h, t, n_div, min_iter, max_iter);
-}
-
-void launch_step_ctl_init(hipStream_t s, DevCtl* c, uint32_t n_active)
-{
-    k_step_ctl_init<<<1, 64, 0, s>>>(c, n_active);
 }
 
 void launch_nbr_ctl_init(hipStream_t s, DevCtl* c) { k_nbr_ctl_init<<<1, 64, 0, s>>>(c); }
@@ -1621,13 +1612,15 @@ void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime*
 
 void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
                        const float4* bpos, const NbrLists& L, const float4* vel,
-                       const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial)
+                       const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial,
+                       const CtlInit& ci)
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
     if (init) {
-        k_div_source<true><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial);
+        k_div_source<true><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial, ci);
     } else {
-        k_div_source<false><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial);
+        k_div_source<false><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial,
+                                                 CtlInit{});
     }
 }
 
@@ -1685,13 +1678,15 @@ void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c
 void launch_density_source(hipStream_t s, bool init, const SimConst& c, const DevTime* t, uint32_t n,
                            const float4* pos, const float4* bpos, const NbrLists& L,
                            const float4* vel, const float* rho, float* alpha, float* radv, float* kk,
-                           float* kappa, SolveCtl* ctl, double* partial)
+                           float* kappa, SolveCtl* ctl, double* partial, const CtlInit& ci)
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
     if (init) {
-        k_density_source<true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl, partial);
+        k_density_source<true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl,
+                                                    partial, ci);
     } else {
-        k_density_source<false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl, partial);
+        k_density_source<false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl,
+                                                     partial, CtlInit{});
     }
 }
 

@@ -13,11 +13,9 @@ uint32_t grid_blocks_xcd(uint32_t n);
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint32_t n);
 void launch_fill_f32(hipStream_t s, float* p, float v, uint32_t n);
 size_t scan_temp_words(uint32_t n);
-void launch_exclusive_scan(hipStream_t s, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp);
+void launch_exclusive_scan(hipStream_t s, uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp,
+                           bool clear_in = false);
 
-void launch_ctl_init(hipStream_t s, SolveCtl* ctl, double eta_base, int divide_by_h, const DevTime* t,
-                     double n_div, uint32_t min_iter, uint32_t max_iter);
-void launch_step_ctl_init(hipStream_t s, DevCtl* c, uint32_t n_active);
 void launch_nbr_ctl_init(hipStream_t s, DevCtl* c);
 void launch_time_reset(hipStream_t s, DevTime* t, double h);
 void launch_time_update(hipStream_t s, DevTime* t, double hf, const DevCtl* ctl, int mode,
@@ -47,7 +45,8 @@ void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime*
                         float* radv, float* kvtot);
 void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
                        const float4* bpos, const NbrLists& L, const float4* vel,
-                       const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial);
+                       const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial,
+                       const CtlInit& ci = CtlInit{});
 // the one-block error reduction and loop decision after a loop-end sweep of n particles
 void launch_solve_finalize(hipStream_t s, uint32_t n, SolveCtl* ctl, const double* partial, const Publish& pub);
 void launch_jacobi(hipStream_t s, bool div, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
@@ -64,7 +63,7 @@ void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c
 void launch_density_source(hipStream_t s, bool init, const SimConst& c, const DevTime* t, uint32_t n,
                            const float4* pos, const float4* bpos, const NbrLists& L,
                            const float4* vel, const float* rho, float* alpha, float* radv, float* kk,
-                           float* kappa, SolveCtl* ctl, double* partial);
+                           float* kappa, SolveCtl* ctl, double* partial, const CtlInit& ci = CtlInit{});
 void launch_update_pos(hipStream_t s, const DevTime* t, uint32_t n, const float4* vel, float4* pos,
                        const float* bmin, const float* bmax, DevCtl* ctl, const Restrict& rs);
 void launch_keep_inside(hipStream_t s, uint32_t n, const float4* pos, const float* bmin, const float* bmax,
