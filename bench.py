@@ -37,26 +37,32 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, AMD published
 # lists of LDS-staged blocks, 4 B in 32-bit lists) and the 16 B boundary
 # position + psi are added per launch:
 #   B = N_f * S_k + 2 * sum n_fluid,staged + 4 * (sum n_fluid,unstaged + sum n_boundary) + 16 * N_b
+# Algorithmic bytes per launch, the per-unit figures of the hot-path contract:
+# B_k = N_f S_k + 4 (sum n_f + sum n_b) + 16 N_b -- S_k the compulsory
+# per-particle FP32 state of the reference kernel (mass uniform, not counted),
+# one 32-bit index per stored neighbor, the boundary position + psi once.
+# Fused kernels count the reference kernels they replace.
 SWEEP_STATE_BYTES = {
-    "div_ws_init": 52,    # x12 v12 kappaV4 r; rho alpha rho_adv 12 w; v12 w
-    "div_init": 36,       # x12 v12 alpha4 r; rho_adv4 k4 w  (kappaV := 0 not counted)
-    "div_jacobi": 48,     # x12 k4 r; kappaV8 rw; v24 rw
-    "div_loop_end": 36,   # x12 v12 alpha4 r; rho_adv4 k4 w
-    "xsph": 40,           # x12 v12 rho4 r; acc12 w (fluid neighbors only, no boundary term)
-    "pressure_ws": 52,    # x12 acc12 kappa4 r; v24 rw
-    "dens_init": 48,      # x12 v12 rho4 r; alpha8 rw; rho_adv4 kappa4 k4 w
-    "dens_jacobi": 40,    # x12 k4 r; v24 rw
-    "dens_loop_end": 48,  # x12 v12 rho4 alpha4 r; rho_adv4 k4 w; kappa8 rw
+    "div_ws_init": 44 + 44,   # fused rho/alpha/div pass (x12 v12 r; rho alpha rho_adv 12 w; kappaV8 rw) + divergence warm start (x12 kappaV4 rho_adv4 v24 rw)
+    "div_init": 32,           # x12 v12 r; rho_adv4 kappaV4 w
+    "div_jacobi": 52,         # x12 v24 rw, rho_adv4, alpha4, kappaV8 rw
+    "div_loop_end": 28,       # x12 v12 r; rho_adv4 w
+    "xsph": 40,               # x12 v12 rho4 r; acc12 w (fluid neighbors only, no boundary term)
+    "pressure_ws": 44 + 40,   # update_vel (acc12, v24 rw, kappa8 rw) + pressure warm start (x12 kappa4 v24 rw)
+    "dens_init": 44,          # x12 v12 rho4 r; alpha8 rw; rho_adv4 kappa4 w
+    "dens_jacobi": 44,        # x12 v24 rw, rho_adv4, alpha4
+    "dens_loop_end": 44,      # x12 v12 rho4 alpha4 r; rho_adv4 w; kappa8 rw
 }
+# the reference kernels a fused launch replaces (each reads the neighbor lists once)
+SWEEP_PASSES = {"div_ws_init": 2, "pressure_ws": 1}
 
 
-def kernel_bytes(name, n_f, n_b, sum_nf, sum_nb, sum_nf_unstaged=None):
+def kernel_bytes(name, n_f, n_b, sum_nf, sum_nb):
     s = SWEEP_STATE_BYTES[name]
-    u = sum_nf if sum_nf_unstaged is None else sum_nf_unstaged
-    idx_f = 2 * (sum_nf - u) + 4 * u
+    passes = SWEEP_PASSES.get(name, 1)
     if name == "xsph":
-        return n_f * s + idx_f
-    return n_f * s + idx_f + 4 * sum_nb + 16 * n_b
+        return n_f * s + 4 * sum_nf
+    return n_f * s + passes * (4 * (sum_nf + sum_nb) + 16 * n_b)
 
 
 def make_params(sc, mode, max_neighbors):
@@ -158,7 +164,7 @@ def main():
     sum_nf = (inf0.neighbors_fluid_total + inf1.neighbors_fluid_total) / 2.0
     sum_nb = (inf0.neighbors_boundary_total + inf1.neighbors_boundary_total) / 2.0
     sum_nfu = (inf0.neighbors_fluid_unstaged + inf1.neighbors_fluid_unstaged) / 2.0
-    nbytes = kernel_bytes(dominant, n_f, inf1.n_boundary, sum_nf, sum_nb, sum_nfu)
+    nbytes = kernel_bytes(dominant, n_f, inf1.n_boundary, sum_nf, sum_nb)
     avg_s = ms / 1e3 / max(launches, 1)
     achieved = nbytes / avg_s / 1e9
     traffic = None
@@ -201,6 +207,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
+            "bytes_model": "N_f*S_k + 4*(sum n_f + sum n_b) + 16*N_b, S_k = %d B" % SWEEP_STATE_BYTES[dominant],
             "traffic": traffic,
             "algorithmic_bytes_per_launch": nbytes,
             "avg_launch_us": avg_s * 1e6,
