@@ -20,8 +20,8 @@ extern "C" {
 
 typedef struct dfsph_cuda_stats {
     uint32_t iters_v, iters;
-    double err_v, err;
-    double h, time;
+    double err_v, err;        /* final average source: divergence (rho/s), density (rho) */
+    double h, time;           /* step size used by this step, simulated time after it  */
     uint32_t n_fluid;
     int32_t removed;
 } dfsph_cuda_stats;
@@ -40,6 +40,42 @@ void* dfsph_cuda_handle(void* solver);
 int dfsph_cuda_save(void* solver, const char* path);
 int dfsph_cuda_load(void* solver, const char* path);
 const char* dfsph_cuda_last_error(void);
+
+/* DFSPHCUDA::handleSimulationSave(save_liquid, false, save_boundaries)
+ * (DFSPH_CUDA.cpp:1887-1919): folder/fluid_file.txt and general_data.txt (h),
+ * folder/boundaries_file.txt, as UnifiedParticleSet::write_to_file
+ * (DFSPH_c_arrays_structure.cpp:424-478) writes them. */
+int dfsph_cuda_save_text(void* solver, const char* folder, int save_liquid, int save_boundaries);
+/* DFSPHCUDA::handleSimulationLoad(true, load_liquid_velocities, false, false,
+ * false, false) (DFSPH_CUDA.cpp:1921-1979): replace the fluid with
+ * folder/fluid_file.txt, set h = h_past = h_future from general_data.txt. */
+int dfsph_cuda_load_text(void* solver, const char* folder, int load_liquid_velocities);
+/* a solver whose boundary set is folder/boundaries_file.txt (mass column = psi,
+ * read_boundaries_from_file, DFSPH_c_arrays_structure.cpp:1294-1323) and whose
+ * fluid and h come from the fluid files; NULL on failure (dfsph_cuda_last_error) */
+void* dfsph_cuda_new_from_text(const dfsph_params* p, const char* folder, int load_liquid_velocities);
+
+/* SceneLoader::readScene (Utilities/SceneLoader.cpp:13-300), the fluid of
+ * DemoBase::createFluidBlocks (Demos/Common/DemoBase.cpp:539-600) and the
+ * boundary particles of the RigidBodies (OBJ meshes, paths relative to
+ * base_path; NULL: the scene file's folder).  NULL on failure. */
+void* dfsph_scene_read(const char* path, const char* base_path);
+void dfsph_scene_free(void* scene);
+/* library defaults overridden by the scene's Configuration (DemoBase.cpp:440-455) */
+void dfsph_scene_params(const void* scene, dfsph_params* p);
+uint32_t dfsph_scene_fluid_count(const void* scene);
+void dfsph_scene_fluid(const void* scene, float* xyz, float* vel);   /* 3 n floats each */
+uint32_t dfsph_scene_boundary_count(const void* scene);
+void dfsph_scene_boundary(const void* scene, float* xyz);
+/* the solver of the scene (p == NULL: dfsph_scene_params); NULL on failure */
+void* dfsph_cuda_new_from_scene(const void* scene, const dfsph_params* p);
+
+/* UnifiedParticleSet::write_to_file / load_from_file (DFSPH_c_arrays_structure.cpp:
+ * 424-560) for one particle set; vel == NULL writes zeros.  The reader returns
+ * the particle count (arrays filled only when it is <= capacity), -1 on error. */
+int dfsph_particles_write_text(const char* path, const float* xyz, const float* vel, const float* mass,
+                               uint32_t n, int has_factor_computation, int velocity_impacted_by_fluid_solver);
+int64_t dfsph_particles_read_text(const char* path, float* xyz, float* vel, float* mass, uint32_t capacity);
 
 #ifdef __cplusplus
 }

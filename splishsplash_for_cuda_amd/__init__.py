@@ -11,4 +11,5 @@ from ._native import (  # noqa: F401
     FIELD_BOUNDARY_PSI, WARMSTART_LEARNED, WARMSTART_NONE, WARMSTART_REFERENCE,
     LISTS_AUTO, LISTS_NO_LDS, LISTS_GLOBAL, FIELD_TAG, DFSPHSurface, surface,
     TAG_ACTIVE, TAG_ACTIVE_NEIGHBORS, TAG_UNTAGGED, TAG_REMOVAL,
+    Scene, read_particles_text, write_particles_text,
 )

@@ -251,6 +251,19 @@ HOST_SIGNATURES = {
     "dfsph_cuda_save": (_I, [_P, C.c_char_p]),
     "dfsph_cuda_load": (_I, [_P, C.c_char_p]),
     "dfsph_cuda_last_error": (C.c_char_p, []),
+    "dfsph_cuda_save_text": (_I, [_P, C.c_char_p, _I, _I]),
+    "dfsph_cuda_load_text": (_I, [_P, C.c_char_p, _I]),
+    "dfsph_cuda_new_from_text": (C.c_void_p, [C.POINTER(DFSPHParams), C.c_char_p, _I]),
+    "dfsph_scene_read": (C.c_void_p, [C.c_char_p, C.c_char_p]),
+    "dfsph_scene_free": (None, [_P]),
+    "dfsph_scene_params": (None, [_P, C.POINTER(DFSPHParams)]),
+    "dfsph_scene_fluid_count": (_U32, [_P]),
+    "dfsph_scene_fluid": (None, [_P, _F32P, _F32P]),
+    "dfsph_scene_boundary_count": (_U32, [_P]),
+    "dfsph_scene_boundary": (None, [_P, _F32P]),
+    "dfsph_cuda_new_from_scene": (C.c_void_p, [_P, C.POINTER(DFSPHParams)]),
+    "dfsph_particles_write_text": (_I, [C.c_char_p, _F32P, _F32P, _F32P, _U32, _I, _I]),
+    "dfsph_particles_read_text": (C.c_int64, [C.c_char_p, _F32P, _F32P, _F32P, _U32]),
 }
 
 _hip = None
@@ -639,7 +652,6 @@ class HostSolver:
 
     def __init__(self, params: DFSPHParams, boundary_xyz, fluid_xyz, fluid_vel=None):
         self.lib = load_host()
-        self.params = params
         b = _f32(boundary_xyz).reshape(-1)
         f = _f32(fluid_xyz).reshape(-1)
         v = None if fluid_vel is None else _f32(fluid_vel).reshape(-1)
@@ -648,13 +660,50 @@ class HostSolver:
                                     _ptr(v, C.c_float), f.size // 3, err, 512)
         if not s:
             raise DFSPHError(DFSPH_ERR_STATE, err.value.decode(errors="replace"))
+        self._adopt(s, params)
+
+    def _adopt(self, s, params):
         self.s = s
+        self.params = params
         # a Solver view over the handle owned by the C++ object (not owning)
         self.view = Solver.__new__(Solver)
         self.view.lib = load_hip()
         self.view.params = params
         self.view.h = C.c_void_p(self.lib.dfsph_cuda_handle(s))
         self.view._owns = False
+
+    @classmethod
+    def _wrap(cls, s, params):
+        lib = load_host()
+        if not s:
+            raise DFSPHError(DFSPH_ERR_STATE, lib.dfsph_cuda_last_error().decode(errors="replace"))
+        self = cls.__new__(cls)
+        self.lib = lib
+        self._adopt(s, params)
+        return self
+
+    @classmethod
+    def from_scene(cls, scene: "Scene", params: DFSPHParams = None) -> "HostSolver":
+        """dfsph_cuda_new_from_scene: the scene's fluid blocks and rigid bodies
+        (params None: the scene's Configuration)."""
+        p = scene.params() if params is None else params
+        return cls._wrap(load_host().dfsph_cuda_new_from_scene(scene.h, C.byref(p)), p)
+
+    @classmethod
+    def from_text(cls, params: DFSPHParams, folder: str, load_velocities: bool = True) -> "HostSolver":
+        """dfsph_cuda_new_from_text: boundary, fluid and h from the text files of folder."""
+        s = load_host().dfsph_cuda_new_from_text(C.byref(params), folder.encode(), 1 if load_velocities else 0)
+        return cls._wrap(s, params)
+
+    def save_text(self, folder: str, liquid: bool = True, boundaries: bool = True):
+        rc = self.lib.dfsph_cuda_save_text(self.s, folder.encode(), 1 if liquid else 0, 1 if boundaries else 0)
+        if rc != DFSPH_OK:
+            raise DFSPHError(rc, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
+
+    def load_text(self, folder: str, load_velocities: bool = True):
+        rc = self.lib.dfsph_cuda_load_text(self.s, folder.encode(), 1 if load_velocities else 0)
+        if rc != DFSPH_OK:
+            raise DFSPHError(rc, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
 
     def step(self) -> DFSPHCudaStats:
         st = DFSPHCudaStats()
@@ -687,3 +736,75 @@ class HostSolver:
             self.close()
         except Exception:
             pass
+
+
+class Scene:
+    """A SPlisHSPlasH scene file read by the C++ SceneLoader (dfsph_scene_read)."""
+
+    def __init__(self, path: str, base_path: str = None):
+        self.lib = load_host()
+        self.h = self.lib.dfsph_scene_read(path.encode(), None if base_path is None else base_path.encode())
+        if not self.h:
+            raise DFSPHError(DFSPH_ERR_ARG, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
+
+    def params(self) -> DFSPHParams:
+        p = DFSPHParams()
+        self.lib.dfsph_scene_params(self.h, C.byref(p))
+        return p
+
+    def fluid(self):
+        n = self.lib.dfsph_scene_fluid_count(self.h)
+        x = np.zeros((n, 3), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        self.lib.dfsph_scene_fluid(self.h, _ptr(x, C.c_float), _ptr(v, C.c_float))
+        return x, v
+
+    def boundary(self):
+        n = self.lib.dfsph_scene_boundary_count(self.h)
+        x = np.zeros((n, 3), np.float32)
+        self.lib.dfsph_scene_boundary(self.h, _ptr(x, C.c_float))
+        return x
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dfsph_scene_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def write_particles_text(path: str, xyz, mass, vel=None, has_factor_computation=True,
+                         velocity_impacted_by_fluid_solver=True):
+    """UnifiedParticleSet::write_to_file for one set (dfsph_particles_write_text)."""
+    lib = load_host()
+    x = _f32(xyz).reshape(-1)
+    m = _f32(mass).reshape(-1)
+    v = None if vel is None else _f32(vel).reshape(-1)
+    if x.size != 3 * m.size or (v is not None and v.size != x.size):
+        raise ValueError("xyz, vel must hold 3 floats per mass")
+    rc = lib.dfsph_particles_write_text(path.encode(), _ptr(x, C.c_float), _ptr(v, C.c_float), _ptr(m, C.c_float),
+                                        m.size, int(bool(has_factor_computation)),
+                                        int(bool(velocity_impacted_by_fluid_solver)))
+    if rc != DFSPH_OK:
+        raise DFSPHError(rc, lib.dfsph_cuda_last_error().decode(errors="replace"))
+
+
+def read_particles_text(path: str):
+    """UnifiedParticleSet::load_from_file for one set: (xyz, vel, mass)."""
+    lib = load_host()
+    n = lib.dfsph_particles_read_text(path.encode(), None, None, None, 0)
+    if n < 0:
+        raise DFSPHError(DFSPH_ERR_ARG, lib.dfsph_cuda_last_error().decode(errors="replace"))
+    x = np.zeros((n, 3), np.float32)
+    v = np.zeros((n, 3), np.float32)
+    m = np.zeros(n, np.float32)
+    if n:
+        got = lib.dfsph_particles_read_text(path.encode(), _ptr(x, C.c_float), _ptr(v, C.c_float),
+                                            _ptr(m, C.c_float), n)
+        if got != n:
+            raise DFSPHError(DFSPH_ERR_ARG, lib.dfsph_cuda_last_error().decode(errors="replace"))
+    return x, v, m

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <fstream>
 
+#include "SceneLoader.h"
 #include "dfsph_host.h"
 
 namespace SPH {
@@ -137,6 +138,57 @@ ParticleArrays DFSPHCData::readFluid() const
     return out;
 }
 
+namespace {
+std::string in_folder(const std::string& folder, const char* name)
+{
+    if (folder.empty() || folder.back() == '/') return folder + name;
+    return folder + "/" + name;
+}
+}  // namespace
+
+void DFSPHCData::write_fluid_to_file(const std::string& folder) const
+{
+    const std::vector<float> x = readField(DFSPH_FIELD_POS), v = readField(DFSPH_FIELD_VEL),
+                             m = readField(DFSPH_FIELD_MASS);
+    // header flags of the fluid set (DFSPH_c_arrays_structure.cpp:911)
+    writeParticleText(in_folder(folder, "fluid_file.txt"), x, v, m, 1, 1, 0);
+}
+
+void DFSPHCData::read_fluid_from_file(const std::string& folder, bool load_velocities)
+{
+    ParticleArrays p;
+    readParticleText(in_folder(folder, "fluid_file.txt"), p.pos, p.vel, p.mass);
+    if (!load_velocities) p.vel.clear();
+    // only the fluid set is replaced: the device's time-step history carries over
+    dfsph_info info;
+    check(dfsph_get_info(m_handle, &info), "dfsph_get_info");
+    reset(p);
+    check(dfsph_set_time(m_handle, info.h, info.h_past, info.time), "dfsph_set_time");
+    h = h_future = info.h;
+    h_past = info.h_past;
+    time = info.time;
+}
+
+void DFSPHCData::write_boundaries_to_file(const std::string& folder) const
+{
+    dfsph_info info;
+    check(dfsph_get_info(m_handle, &info), "dfsph_get_info");
+    const uint32_t n = info.n_boundary;
+    std::vector<float> x(3 * (size_t)n), psi(n);
+    check(dfsph_download(m_handle, DFSPH_FIELD_BOUNDARY_POS, x.data(), 3 * n), "dfsph_download");
+    check(dfsph_download(m_handle, DFSPH_FIELD_BOUNDARY_PSI, psi.data(), n), "dfsph_download");
+    // header flags of the boundary set (:893, boundary pressure off)
+    writeParticleText(in_folder(folder, "boundaries_file.txt"), x, {}, psi, 0, 0, 0);
+}
+
+ParticleArrays DFSPHCData::read_boundaries_from_file(const std::string& folder, bool load_velocities)
+{
+    ParticleArrays p;
+    readParticleText(in_folder(folder, "boundaries_file.txt"), p.pos, p.vel, p.mass);
+    if (!load_velocities) p.vel.clear();
+    return p;
+}
+
 // ------------------------------------------------------------------ DFSPHCUDA
 
 DFSPHCUDA::DFSPHCUDA(const dfsph_params& p, const ParticleArrays& boundary, const ParticleArrays& fluid)
@@ -251,6 +303,39 @@ void DFSPHCUDA::handleSimulationLoad(const std::string& path)
     count_steps = (int)hdr[3];
 }
 
+void DFSPHCUDA::handleSimulationSaveText(const std::string& folder, bool save_liquid, bool save_boundaries) const
+{
+    if (save_liquid) {
+        m_data.write_fluid_to_file(folder);
+        dfsph_info info;
+        check(dfsph_get_info(m_data.handle(), &info), "dfsph_get_info");
+        const std::string path = in_folder(folder, "general_data.txt");
+        std::ofstream f(path);
+        if (!f) throw DFSPHError(DFSPH_ERR_ARG, "cannot write " + path);
+        f.precision(17);
+        f << info.h;
+        if (!f) throw DFSPHError(DFSPH_ERR_ARG, "write failed: " + path);
+    }
+    if (save_boundaries) m_data.write_boundaries_to_file(folder);
+}
+
+double DFSPHCUDA::readGeneralData(const std::string& folder)
+{
+    const std::string path = in_folder(folder, "general_data.txt");
+    std::ifstream f(path);
+    double h = 0.0;
+    if (!(f >> h) || !(h > 0.0)) throw DFSPHError(DFSPH_ERR_ARG, "no time step in " + path);
+    return h;
+}
+
+void DFSPHCUDA::handleSimulationLoadText(const std::string& folder, bool load_liquid_velocities)
+{
+    const double h = readGeneralData(folder);
+    m_data.read_fluid_from_file(folder, load_liquid_velocities);
+    check(dfsph_set_time(m_data.handle(), h, h, m_data.time), "dfsph_set_time");
+    m_data.h = m_data.h_future = m_data.h_past = h;
+}
+
 }  // namespace SPH
 
 // ---------------------------------------------------------------- C entry points
@@ -326,6 +411,145 @@ void* dfsph_cuda_handle(void* s) { return S(s)->data().handle(); }
 int dfsph_cuda_save(void* s, const char* path) { return guarded([&] { S(s)->handleSimulationSave(path); }); }
 
 int dfsph_cuda_load(void* s, const char* path) { return guarded([&] { S(s)->handleSimulationLoad(path); }); }
+
+int dfsph_cuda_save_text(void* s, const char* folder, int save_liquid, int save_boundaries)
+{
+    return guarded([&] { S(s)->handleSimulationSaveText(folder, save_liquid != 0, save_boundaries != 0); });
+}
+
+int dfsph_cuda_load_text(void* s, const char* folder, int load_liquid_velocities)
+{
+    return guarded([&] { S(s)->handleSimulationLoadText(folder, load_liquid_velocities != 0); });
+}
+
+void* dfsph_cuda_new_from_text(const dfsph_params* p, const char* folder, int load_liquid_velocities)
+{
+    SPH::DFSPHCUDA* out = nullptr;
+    guarded([&] {
+        if (!p || !folder) throw SPH::DFSPHError(DFSPH_ERR_ARG, "params and folder are required");
+        const SPH::ParticleArrays b = SPH::DFSPHCData::read_boundaries_from_file(folder, false);
+        SPH::ParticleArrays f;
+        SPH::readParticleText(std::string(folder) + "/fluid_file.txt", f.pos, f.vel, f.mass);
+        if (!load_liquid_velocities) f.vel.clear();
+        dfsph_params q = *p;
+        q.dt = SPH::DFSPHCUDA::readGeneralData(folder);
+        out = new SPH::DFSPHCUDA(q, b, f);
+    });
+    return out;
+}
+
+namespace {
+struct SceneHandle {
+    SPH::SceneLoader::Scene scene;
+    std::vector<float> fluid_xyz, fluid_vel, boundary_xyz;
+};
+
+void to_float(const std::vector<SPH::Vec3>& v, std::vector<float>& out)
+{
+    out.resize(3 * v.size());
+    for (size_t i = 0; i < v.size(); ++i)
+        for (int a = 0; a < 3; ++a) out[3 * i + a] = (float)v[i][a];
+}
+}  // namespace
+
+void* dfsph_scene_read(const char* path, const char* base_path)
+{
+    SceneHandle* out = nullptr;
+    guarded([&] {
+        if (!path) throw SPH::DFSPHError(DFSPH_ERR_ARG, "scene path is required");
+        std::string base = base_path ? base_path : "";
+        if (!base_path) {
+            const std::string p(path);
+            const size_t slash = p.find_last_of('/');
+            base = slash == std::string::npos ? "." : p.substr(0, slash);
+        }
+        std::unique_ptr<SceneHandle> sc(new SceneHandle);
+        SPH::SceneLoader::readScene(path, sc->scene);
+        std::vector<SPH::Vec3> x, v, b;
+        SPH::createFluidBlocks(sc->scene, x, v);
+        SPH::sampleRigidBodies(sc->scene, base, b);
+        to_float(x, sc->fluid_xyz);
+        to_float(v, sc->fluid_vel);
+        to_float(b, sc->boundary_xyz);
+        out = sc.release();
+    });
+    return out;
+}
+
+void dfsph_scene_free(void* sc) { delete static_cast<SceneHandle*>(sc); }
+
+void dfsph_scene_params(const void* sc, dfsph_params* p)
+{
+    dfsph_default_params(p);
+    SPH::sceneParams(static_cast<const SceneHandle*>(sc)->scene, *p);
+}
+
+uint32_t dfsph_scene_fluid_count(const void* sc)
+{
+    return (uint32_t)(static_cast<const SceneHandle*>(sc)->fluid_xyz.size() / 3);
+}
+
+void dfsph_scene_fluid(const void* sc, float* xyz, float* vel)
+{
+    const SceneHandle* h = static_cast<const SceneHandle*>(sc);
+    if (xyz) std::copy(h->fluid_xyz.begin(), h->fluid_xyz.end(), xyz);
+    if (vel) std::copy(h->fluid_vel.begin(), h->fluid_vel.end(), vel);
+}
+
+uint32_t dfsph_scene_boundary_count(const void* sc)
+{
+    return (uint32_t)(static_cast<const SceneHandle*>(sc)->boundary_xyz.size() / 3);
+}
+
+void dfsph_scene_boundary(const void* sc, float* xyz)
+{
+    const SceneHandle* h = static_cast<const SceneHandle*>(sc);
+    if (xyz) std::copy(h->boundary_xyz.begin(), h->boundary_xyz.end(), xyz);
+}
+
+void* dfsph_cuda_new_from_scene(const void* sc, const dfsph_params* p)
+{
+    SPH::DFSPHCUDA* out = nullptr;
+    guarded([&] {
+        const SceneHandle* h = static_cast<const SceneHandle*>(sc);
+        if (!h) throw SPH::DFSPHError(DFSPH_ERR_ARG, "scene is required");
+        dfsph_params q;
+        if (p) q = *p; else dfsph_scene_params(sc, &q);
+        SPH::ParticleArrays b, f;
+        b.pos = h->boundary_xyz;
+        f.pos = h->fluid_xyz;
+        f.vel = h->fluid_vel;
+        out = new SPH::DFSPHCUDA(q, b, f);
+    });
+    return out;
+}
+
+int dfsph_particles_write_text(const char* path, const float* xyz, const float* vel, const float* mass, uint32_t n,
+                               int has_factor_computation, int velocity_impacted_by_fluid_solver)
+{
+    return guarded([&] {
+        if (!path || (n && (!xyz || !mass))) throw SPH::DFSPHError(DFSPH_ERR_ARG, "path, xyz and mass are required");
+        std::vector<float> x(xyz, xyz + 3 * (size_t)n), m(mass, mass + n), v;
+        if (vel) v.assign(vel, vel + 3 * (size_t)n);
+        SPH::writeParticleText(path, x, v, m, has_factor_computation != 0, velocity_impacted_by_fluid_solver != 0, 0);
+    });
+}
+
+int64_t dfsph_particles_read_text(const char* path, float* xyz, float* vel, float* mass, uint32_t capacity)
+{
+    int64_t n = -1;
+    guarded([&] {
+        if (!path) throw SPH::DFSPHError(DFSPH_ERR_ARG, "path is required");
+        std::vector<float> x, v, m;
+        SPH::readParticleText(path, x, v, m);
+        n = (int64_t)m.size();
+        if ((uint64_t)n > capacity) return;
+        if (xyz) std::copy(x.begin(), x.end(), xyz);
+        if (vel) std::copy(v.begin(), v.end(), vel);
+        if (mass) std::copy(m.begin(), m.end(), mass);
+    });
+    return n;
+}
 
 const char* dfsph_cuda_last_error(void) { return g_host_err.c_str(); }
 

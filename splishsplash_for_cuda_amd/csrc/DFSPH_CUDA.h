@@ -65,6 +65,17 @@ public:
     ParticleArrays readFluid() const;
     std::vector<float> readField(int field) const;   // device order
 
+    // UnifiedParticleSet text files in `folder` (DFSPH_c_arrays_structure.cpp:
+    // 1196-1323): write_fluid_to_file writes fluid_file.txt, read_fluid_from_file
+    // replaces the fluid with it (velocities zero unless load_velocities),
+    // write_boundaries_to_file writes boundaries_file.txt (mass column = psi).
+    void write_fluid_to_file(const std::string& folder) const;
+    void read_fluid_from_file(const std::string& folder, bool load_velocities);
+    void write_boundaries_to_file(const std::string& folder) const;
+    // read_boundaries_from_file (:1294-1323) as arrays for the constructor:
+    // the boundary set of a solver is static.
+    static ParticleArrays read_boundaries_from_file(const std::string& folder, bool load_velocities = false);
+
     void* handle() const { return m_handle; }
 
     // mirrors of the device time-step state
@@ -101,6 +112,16 @@ public:
     // the step history, so that a resumed run continues bit for bit.
     void handleSimulationSave(const std::string& path) const;
     void handleSimulationLoad(const std::string& path);
+    // The reference's own handleSimulationSave(save_liquid, -, save_boundaries)
+    // / handleSimulationLoad(load_liquid, load_liquid_velocities, ...) on the
+    // text files of `folder`: fluid_file.txt + general_data.txt (h) and
+    // boundaries_file.txt.  Loading sets h = h_past = h_future = the saved h
+    // (updateTimeStep + onSimulationStepEnd twice, DFSPH_CUDA.cpp:1958-1962)
+    // and keeps the simulated time; kappa and kappaV restart at zero.
+    void handleSimulationSaveText(const std::string& folder, bool save_liquid = true, bool save_boundaries = true) const;
+    void handleSimulationLoadText(const std::string& folder, bool load_liquid_velocities = true);
+    // the time step saved in folder/general_data.txt
+    static double readGeneralData(const std::string& folder);
 
     DFSPHCData& data() { return m_data; }
     const DFSPHCData& data() const { return m_data; }
