@@ -330,6 +330,49 @@ int dfsph_open_boundary_apply(void* handle, const double* window_displacement, f
                               float allowed_new_density, int use_inflow, int use_outflow, uint32_t* added,
                               uint32_t* removed);
 
+/* ---- rigid bodies -------------------------------------------------------
+ * The dynamic bodies of DFSPHCData (vector_dynamic_bodies_data,
+ * DFSPH_c_arrays_structure.cpp:897-905) coupled two ways with the fluid:
+ * body particles act as moving boundary particles (their velocity enters the
+ * density sources, ITER_NEIGHBORS_SOLIDS, DFSPH_core_cuda.cu:318-327, 599-603,
+ * 1202-1206) and take the reaction of every pressure update
+ * (DFSPH_core_cuda.cu:467-477, 936-946).  The position update then moves the
+ * bodies: the external PositionBasedDynamics step of the reference's demo is
+ * replaced by a semi-implicit Euler step on the device.  The boundary set must
+ * be uploaded first; bodies stay in the solver until it is destroyed. */
+#define DFSPH_BODY_DYNAMIC 0     /* gravity + fluid force and torque              */
+#define DFSPH_BODY_KINEMATIC 1   /* moved only by dfsph_set_rigid_body             */
+#define DFSPH_BODY_SINE 2        /* translation position + amplitude sin(2 pi t / period) axis */
+typedef struct dfsph_rigid_body {
+    int32_t type;                /* DFSPH_BODY_*                                    */
+    double mass;                 /* kg (DYNAMIC)                                    */
+    double inertia[3];           /* principal moments in the body frame, kg m^2     */
+    double position[3];          /* center of mass (SINE: center at rest), world     */
+    double velocity[3];
+    double rotation[4];          /* unit quaternion x, y, z, w: body frame -> world  */
+    double angular_velocity[3];  /* world frame                                     */
+    double axis[3];              /* SINE: unit direction                            */
+    double amplitude, period;    /* SINE                                            */
+} dfsph_rigid_body;
+/* Add a body from its particles in the body frame (relative to the center of
+ * mass).  psi == NULL: psi of each particle as FluidModel::computeBoundaryPsi
+ * (FluidModel.cpp:233-258) over the walls, the bodies already added and this
+ * one, at their current places.  id: the body's index (0, 1, ...). */
+int dfsph_add_rigid_body(void* handle, const dfsph_rigid_body* body, const float* xyz_body, const float* psi,
+                         uint32_t n, uint32_t* id);
+/* overwrite a body's state and motion (UnifiedParticleSet::updateDynamicBodiesParticles,
+ * DFSPH_c_arrays_structure.cpp:295-303); the particles follow at the next search */
+int dfsph_set_rigid_body(void* handle, uint32_t id, const dfsph_rigid_body* body);
+/* the body's state, and the fluid force and torque (about the center of mass)
+ * of the last step (compute_fluid_impact_on_dynamic_body_cuda,
+ * DFSPH_memory_management_cuda.cu:688-723); force / torque may be NULL */
+int dfsph_get_rigid_body(void* handle, uint32_t id, dfsph_rigid_body* body, double* force, double* torque);
+/* per-particle fluid forces of the last step, in the order the particles were
+ * given (read_rigid_body_force_cuda, DFSPH_memory_management_cuda.cu:676-684):
+ * 3 n floats */
+int dfsph_rigid_body_forces(void* handle, uint32_t id, float* f_xyz, uint32_t count);
+int dfsph_rigid_body_count(void* handle, uint32_t* n);
+
 /* ---- learned warm start -------------------------------------------------
  * No reference counterpart: these entry points add a per-particle MLP
  * (12 -> 64 -> 64 -> 64 -> 2, SiLU; BF16 weights on the matrix cores, FP32

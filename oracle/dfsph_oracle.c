@@ -42,6 +42,13 @@ struct or_state {
     uint32_t nbn;
     double *xb, *psi;
 
+    /* rigid bodies: their particles follow the nbw wall particles in xb / psi
+     * (vector_dynamic_bodies_data, DFSPH_c_arrays_structure.cpp:897-905) */
+    uint32_t nbw, nbody, ns;
+    struct or_body* body;
+    double *x0, *psi0, *vb, *sF;    /* body frame (3 ns), psi (ns), boundary velocity (3 nbn), reactions (3 ns) */
+    uint32_t* sbody;
+
     uint32_t *nf, *nbc, *nbr;
     uint32_t max_seen, overflow;
 
@@ -55,6 +62,12 @@ struct or_state {
     long long true_count;
     uint32_t* tag;
 };
+
+typedef struct or_body {
+    int type;
+    double mass, I[3], c[3], v[3], q[4], w[3], F[3], T[3], c0[3], axis[3], amp, omega;
+    uint32_t first, count;
+} or_body;
 
 #define OR_TAG_ACTIVE 10000000u            /* cuda_compilation_center.cu:6-9 */
 #define OR_TAG_ACTIVE_NEIGHBORS 20000000u
@@ -386,6 +399,7 @@ void or_destroy(or_state* s)
     if (!s) return;
     free_fluid(s);
     free(s->xb); free(s->psi); free(s->tabW); free(s->tabG);
+    free(s->body); free(s->x0); free(s->psi0); free(s->vb); free(s->sF); free(s->sbody);
     free(s);
 }
 
@@ -437,6 +451,10 @@ int or_set_restriction(or_state* s, int mode, uint32_t count_active, uint32_t co
 int or_set_boundary(or_state* s, uint32_t n, const double* x, const double* psi)
 {
     free(s->xb); free(s->psi);
+    free(s->body); free(s->x0); free(s->psi0); free(s->vb); free(s->sF); free(s->sbody);
+    s->body = NULL; s->x0 = s->psi0 = s->vb = s->sF = NULL; s->sbody = NULL;
+    s->nbody = s->ns = 0;
+    s->nbw = n;
     s->nbn = n;
     s->xb = (double*)malloc(3 * (n ? n : 1) * sizeof(double));
     s->psi = (double*)malloc((n ? n : 1) * sizeof(double));
@@ -528,8 +546,11 @@ void or_set_time_step(or_state* s, double h)
     s->h_ratio_to_past = s->h_ratio_to_past2 = 1.0;
 }
 
+static void place_bodies(or_state* s);
+
 int or_neighbor_search(or_state* s)
 {
+    if (s->ns) place_bodies(s);
     const uint32_t n = s->n, max = s->p.max_neighbors;
     s->max_seen = 0; s->overflow = 0;
     if (s->p.fp32_predicate) {
@@ -615,8 +636,8 @@ int or_get_neighbors(const or_state* s, uint32_t* nf, uint32_t* nb, uint32_t* li
 static inline void sub3(const double* a, const double* b, double* o) { o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2]; }
 static inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
-/* Sum m_j (v_i - v_j) . gradW_ij over fluid and boundary (static
- * boundary: v_j = 0).  computeDensityChange / computeDensityAdv
+/* Sum m_j (v_i - v_j) . gradW_ij over fluid and boundary (walls: v_j = 0;
+ * rigid-body particles move).  computeDensityChange / computeDensityAdv
  * (DFSPH_core_cuda.cu:546-614, 1165-1215). */
 static double density_change_sum(const or_state* s, uint32_t i)
 {
@@ -632,7 +653,9 @@ static double density_change_sum(const or_state* s, uint32_t i)
     for (uint32_t k = 0; k < s->nbc[i]; ++k) {
         const uint32_t j = row[s->nf[i] + k];
         sub3(xi, XB(j), d); kG(s, d, g);
-        acc += s->psi[j] * dot3(vi, g);
+        if (s->ns) sub3(vi, s->vb + 3 * (size_t)j, dv);
+        else { dv[0] = vi[0]; dv[1] = vi[1]; dv[2] = vi[2]; }
+        acc += s->psi[j] * dot3(dv, g);     /* body particles: (v_i - v_j), DFSPH_core_cuda.cu:599-603 */
     }
     return acc;
 }
@@ -707,6 +730,30 @@ static void jacobi_update(or_state* s, uint32_t i, const double* k, double step,
     dv_out[0] = step * dv[0]; dv_out[1] = step * dv[1]; dv_out[2] = step * dv[2];
 }
 
+/* Reactions of one Jacobi update on the body particles: -m_i k_i psi_j gradW_ij
+ * for every particle that was updated (gate: also a positive source, the
+ * divergence warm start) and |k_i| > eps (DFSPH_core_cuda.cu:467-477, 936-946).
+ * Serial, in particle order. */
+static void body_reactions(or_state* s, const double* k, int gate_radv)
+{
+    if (!s->ns) return;
+    double d[3], g[3];
+    for (uint32_t i = 0; i < s->n; ++i) {
+        if (!r_compute(s, i) || (gate_radv && !(s->radv[i] > 0.0))) continue;
+        const double ki = k[i];
+        if (!(fabs(ki) > OR_EPS)) continue;
+        const uint32_t* row = ROW(i);
+        for (uint32_t q = 0; q < s->nbc[i]; ++q) {
+            const uint32_t j = row[s->nf[i] + q];
+            if (j < s->nbw) continue;
+            sub3(XI(i), XB(j), d); kG(s, d, g);
+            const double c = -s->m[i] * ki * s->psi[j];
+            double* f = s->sF + 3 * (size_t)(j - s->nbw);
+            f[0] += c * g[0]; f[1] += c * g[1]; f[2] += c * g[2];
+        }
+    }
+}
+
 static double sum_radv(const or_state* s)
 {
     double acc = 0.0;
@@ -746,6 +793,7 @@ uint32_t or_divergence_solve(or_state* s, double* err_out)
         if (s->radv[i] > 0.0 && r_compute(s, i)) jacobi_update(s, i, s->kappaV, h, dv + 3 * i);
         else dv[3 * i] = dv[3 * i + 1] = dv[3 * i + 2] = 0.0;
     }
+    body_reactions(s, s->kappaV, 1);
     for (uint32_t i = 0; i < 3 * n; ++i) s->v[i] += dv[i];
     /* divergence init: kappaV = 0, source from the warm-started velocities */
 #pragma omp parallel for schedule(static)
@@ -767,6 +815,7 @@ uint32_t or_divergence_solve(or_state* s, double* err_out)
             if (r_compute(s, (uint32_t)il)) jacobi_update(s, (uint32_t)il, k, h, dv + 3 * il);
             else dv[3 * il] = dv[3 * il + 1] = dv[3 * il + 2] = 0.0;
         }
+        body_reactions(s, k, 0);
         for (uint32_t i = 0; i < 3 * n; ++i) s->v[i] += dv[i];
 #pragma omp parallel for schedule(static)
         for (long il = 0; il < (long)n; ++il)
@@ -878,6 +927,7 @@ uint32_t or_pressure_solve(or_state* s, double* err_out)
         if (r_compute(s, (uint32_t)il)) jacobi_update(s, (uint32_t)il, s->kappa, hf, dv + 3 * il);
         else dv[3 * il] = dv[3 * il + 1] = dv[3 * il + 2] = 0.0;
     }
+    body_reactions(s, s->kappa, 0);
     for (uint32_t i = 0; i < 3 * n; ++i) s->v[i] += dv[i];
     /* init: kappa = 0, factor *= 1/h_f, rho_adv (core.cu:1217-1262) */
 #pragma omp parallel for schedule(static)
@@ -902,6 +952,7 @@ uint32_t or_pressure_solve(or_state* s, double* err_out)
             if (r_compute(s, (uint32_t)il)) jacobi_update(s, (uint32_t)il, k, hf, dv + 3 * il);
             else dv[3 * il] = dv[3 * il + 1] = dv[3 * il + 2] = 0.0;
         }
+        body_reactions(s, k, 0);
         for (uint32_t i = 0; i < 3 * n; ++i) s->v[i] += dv[i];
 #pragma omp parallel for schedule(static)
         for (long il = 0; il < (long)n; ++il) {
@@ -920,10 +971,13 @@ uint32_t or_pressure_solve(or_state* s, double* err_out)
 }
 
 /* A16: DFSPH_update_pos_kernel (DFSPH_core_cuda.cu:3127-3214) */
+static void integrate_bodies(or_state* s);
+
 void or_update_pos(or_state* s)
 {
     for (uint32_t i = 0; i < 3 * s->n; ++i)
         if (r_compute(s, i / 3)) s->x[i] += s->h * s->v[i];
+    integrate_bodies(s);
 }
 
 /* DFSPHCData::onSimulationStepEnd (DFSPH_c_arrays_structure.cpp:1108-1118)
@@ -1241,4 +1295,240 @@ double or_mlp_loss_grad(uint32_t n, const double* params, const double* x,
         }
     }
     return loss;
+}
+
+/* ------------------------------------------------------------------ */
+/* Rigid bodies (the dynamic bodies of DFSPHCData): body particles are  */
+/* moving boundary particles; see dfsph_oracle.h                        */
+/* ------------------------------------------------------------------ */
+
+static void quat_mat(const double* q, double R[3][3])
+{
+    const double x = q[0], y = q[1], z = q[2], w = q[3];
+    R[0][0] = 1.0 - 2.0 * (y * y + z * z);
+    R[0][1] = 2.0 * (x * y - z * w);
+    R[0][2] = 2.0 * (x * z + y * w);
+    R[1][0] = 2.0 * (x * y + z * w);
+    R[1][1] = 1.0 - 2.0 * (x * x + z * z);
+    R[1][2] = 2.0 * (y * z - x * w);
+    R[2][0] = 2.0 * (x * z - y * w);
+    R[2][1] = 2.0 * (y * z + x * w);
+    R[2][2] = 1.0 - 2.0 * (x * x + y * y);
+}
+
+/* body particles at the body state: x = R(q) x0 + c, v = w x (x - c) + v_c,
+ * reactions cleared (DFSPH_updateDynamicObjectParticles_kernel,
+ * DFSPH_core_cuda.cu:3407-3420) */
+static void place_bodies(or_state* s)
+{
+    for (uint32_t k = 0; k < s->ns; ++k) {
+        const or_body* b = &s->body[s->sbody[k]];
+        double R[3][3], r[3];
+        quat_mat(b->q, R);
+        const double* l = s->x0 + 3 * (size_t)k;
+        double* x = s->xb + 3 * (size_t)(s->nbw + k);
+        double* v = s->vb + 3 * (size_t)(s->nbw + k);
+        for (int a = 0; a < 3; ++a) {
+            r[a] = R[a][0] * l[0] + R[a][1] * l[1] + R[a][2] * l[2];
+            x[a] = r[a] + b->c[a];
+        }
+        v[0] = b->w[1] * r[2] - b->w[2] * r[1] + b->v[0];
+        v[1] = b->w[2] * r[0] - b->w[0] * r[2] + b->v[1];
+        v[2] = b->w[0] * r[1] - b->w[1] * r[0] + b->v[2];
+        s->psi[s->nbw + k] = s->psi0[k];
+        s->sF[3 * k] = s->sF[3 * k + 1] = s->sF[3 * k + 2] = 0.0;
+    }
+}
+
+/* force and torque about the center of mass (compute_fluid_impact_on_dynamic_body_kernel,
+ * DFSPH_memory_management_cuda.cu:688-705), then one step over h: semi-implicit
+ * Euler for dynamic bodies (Euler's equations in the body frame), the
+ * sinusoidal motion at the time after the step, nothing for kinematic ones */
+static void integrate_bodies(or_state* s)
+{
+    const double h = s->h;
+    for (uint32_t id = 0; id < s->nbody; ++id) {
+        or_body* b = &s->body[id];
+        double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+        for (uint32_t k = b->first; k < b->first + b->count; ++k) {
+            const double* f = s->sF + 3 * (size_t)k;
+            const double* x = s->xb + 3 * (size_t)(s->nbw + k);
+            /* lever arm from the float positions the GPU holds */
+            const double rx = (double)(float)x[0] - b->c[0], ry = (double)(float)x[1] - b->c[1],
+                         rz = (double)(float)x[2] - b->c[2];
+            F[0] += f[0]; F[1] += f[1]; F[2] += f[2];
+            T[0] += ry * f[2] - rz * f[1];
+            T[1] += rz * f[0] - rx * f[2];
+            T[2] += rx * f[1] - ry * f[0];
+        }
+        for (int a = 0; a < 3; ++a) { b->F[a] = F[a]; b->T[a] = T[a]; }
+        if (b->type == 2) {
+            const double tn = s->time + h, sn = sin(b->omega * tn), cs = cos(b->omega * tn);
+            for (int a = 0; a < 3; ++a) {
+                b->c[a] = b->c0[a] + b->amp * sn * b->axis[a];
+                b->v[a] = b->amp * b->omega * cs * b->axis[a];
+                b->w[a] = 0.0;
+            }
+            continue;
+        }
+        if (b->type != 0) continue;
+        for (int a = 0; a < 3; ++a) b->v[a] += h * (F[a] / b->mass + s->p.gravity[a]);
+        double R[3][3], wb[3], Tb[3];
+        quat_mat(b->q, R);
+        for (int a = 0; a < 3; ++a) {
+            wb[a] = R[0][a] * b->w[0] + R[1][a] * b->w[1] + R[2][a] * b->w[2];
+            Tb[a] = R[0][a] * T[0] + R[1][a] * T[1] + R[2][a] * T[2];
+        }
+        const double Lb[3] = {b->I[0] * wb[0], b->I[1] * wb[1], b->I[2] * wb[2]};
+        const double gy[3] = {wb[1] * Lb[2] - wb[2] * Lb[1], wb[2] * Lb[0] - wb[0] * Lb[2], wb[0] * Lb[1] - wb[1] * Lb[0]};
+        for (int a = 0; a < 3; ++a) wb[a] += h * (Tb[a] - gy[a]) / b->I[a];
+        for (int a = 0; a < 3; ++a) b->w[a] = R[a][0] * wb[0] + R[a][1] * wb[1] + R[a][2] * wb[2];
+        for (int a = 0; a < 3; ++a) b->c[a] += h * b->v[a];
+        const double* q = b->q;
+        const double* w = b->w;
+        const double dx = q[3] * w[0] + (w[1] * q[2] - w[2] * q[1]);
+        const double dy = q[3] * w[1] + (w[2] * q[0] - w[0] * q[2]);
+        const double dz = q[3] * w[2] + (w[0] * q[1] - w[1] * q[0]);
+        const double dw = -(w[0] * q[0] + w[1] * q[1] + w[2] * q[2]);
+        double nq[4] = {q[0] + 0.5 * h * dx, q[1] + 0.5 * h * dy, q[2] + 0.5 * h * dz, q[3] + 0.5 * h * dw};
+        const double nn = sqrt(nq[0] * nq[0] + nq[1] * nq[1] + nq[2] * nq[2] + nq[3] * nq[3]);
+        for (int a = 0; a < 4; ++a) b->q[a] = nq[a] / nn;
+    }
+}
+
+static int body_from_desc(const or_body_desc* d, double t_now, or_body* b)
+{
+    if (d->type < 0 || d->type > 2) return -1;
+    const double qn = sqrt(d->rotation[0] * d->rotation[0] + d->rotation[1] * d->rotation[1] +
+                           d->rotation[2] * d->rotation[2] + d->rotation[3] * d->rotation[3]);
+    if (!(qn > 0.0)) return -1;
+    double an = 0.0;
+    if (d->type == 2) {
+        an = sqrt(d->axis[0] * d->axis[0] + d->axis[1] * d->axis[1] + d->axis[2] * d->axis[2]);
+        if (!(an > 0.0) || !(d->period > 0.0)) return -1;
+    }
+    if (d->type == 0 && !(d->mass > 0.0 && d->inertia[0] > 0.0 && d->inertia[1] > 0.0 && d->inertia[2] > 0.0))
+        return -1;
+    memset(b, 0, sizeof *b);
+    b->type = d->type;
+    b->mass = d->mass;
+    for (int a = 0; a < 3; ++a) {
+        b->I[a] = d->inertia[a];
+        b->c[a] = b->c0[a] = d->position[a];
+        b->v[a] = d->velocity[a];
+        b->w[a] = d->angular_velocity[a];
+        b->axis[a] = an > 0.0 ? d->axis[a] / an : 0.0;
+    }
+    for (int a = 0; a < 4; ++a) b->q[a] = d->rotation[a] / qn;
+    if (d->type == 2) {
+        b->amp = d->amplitude;
+        b->omega = 2.0 * M_PI / d->period;
+        const double sn = sin(b->omega * t_now), cs = cos(b->omega * t_now);
+        for (int a = 0; a < 3; ++a) {
+            b->c[a] = b->c0[a] + b->amp * sn * b->axis[a];
+            b->v[a] = b->amp * b->omega * cs * b->axis[a];
+            b->w[a] = 0.0;
+        }
+    }
+    return 0;
+}
+
+int or_add_body(or_state* s, const or_body_desc* d, uint32_t n, const double* xb_body, const double* psi)
+{
+    if (!n || !xb_body) return -1;
+    or_body b;
+    if (body_from_desc(d, s->time, &b)) return -1;
+    b.first = s->ns;
+    b.count = n;
+    const uint32_t ns = s->ns + n, nb = s->nbw + ns, first = s->ns;
+    s->body = (or_body*)realloc(s->body, (s->nbody + 1) * sizeof(or_body));
+    s->body[s->nbody++] = b;
+    s->x0 = (double*)realloc(s->x0, 3 * (size_t)ns * sizeof(double));
+    s->psi0 = (double*)realloc(s->psi0, (size_t)ns * sizeof(double));
+    s->sF = (double*)realloc(s->sF, 3 * (size_t)ns * sizeof(double));
+    s->sbody = (uint32_t*)realloc(s->sbody, (size_t)ns * sizeof(uint32_t));
+    s->xb = (double*)realloc(s->xb, 3 * (size_t)nb * sizeof(double));
+    s->psi = (double*)realloc(s->psi, (size_t)nb * sizeof(double));
+    s->vb = (double*)realloc(s->vb, 3 * (size_t)nb * sizeof(double));
+    if (first == 0) memset(s->vb, 0, 3 * (size_t)s->nbw * sizeof(double));
+    memcpy(s->x0 + 3 * (size_t)first, xb_body, 3 * (size_t)n * sizeof(double));
+    for (uint32_t k = first; k < ns; ++k) {
+        s->sbody[k] = s->nbody - 1;
+        s->psi0[k] = psi ? psi[k - first] : 0.0;
+    }
+    s->ns = ns;
+    s->nbn = nb;
+    place_bodies(s);
+    if (!psi) {
+        /* FluidModel::computeBoundaryPsi (FluidModel.cpp:233-258) over walls and
+         * bodies at their current places, FP32 predicate like the GPU's */
+        const float hk2 = (float)s->hk2;
+        for (uint32_t k = first; k < ns; ++k) {
+            const uint32_t i = s->nbw + k;
+            const float xi[3] = {(float)s->xb[3 * i], (float)s->xb[3 * i + 1], (float)s->xb[3 * i + 2]};
+            double delta = s->W0;
+            for (uint32_t j = 0; j < nb; ++j) {
+                if (j == i) continue;
+                const float xj[3] = {(float)s->xb[3 * j], (float)s->xb[3 * j + 1], (float)s->xb[3 * j + 2]};
+                if (!pred32(xi, xj, hk2)) continue;
+                double dd[3] = {s->xb[3 * i] - s->xb[3 * j], s->xb[3 * i + 1] - s->xb[3 * j + 1],
+                                s->xb[3 * i + 2] - s->xb[3 * j + 2]};
+                delta += kW(s, dd);
+            }
+            s->psi0[k] = s->p.density0 / delta;
+        }
+        place_bodies(s);
+    }
+    return (int)(s->nbody - 1);
+}
+
+int or_set_body(or_state* s, uint32_t id, const or_body_desc* d)
+{
+    if (id >= s->nbody) return -1;
+    or_body b;
+    if (body_from_desc(d, s->time, &b)) return -1;
+    b.first = s->body[id].first;
+    b.count = s->body[id].count;
+    s->body[id] = b;
+    return 0;
+}
+
+int or_get_body(const or_state* s, uint32_t id, or_body_desc* d, double* F, double* T)
+{
+    if (id >= s->nbody) return -1;
+    const or_body* b = &s->body[id];
+    if (d) {
+        memset(d, 0, sizeof *d);
+        d->type = b->type;
+        d->mass = b->mass;
+        for (int a = 0; a < 3; ++a) {
+            d->inertia[a] = b->I[a];
+            d->position[a] = b->c[a];
+            d->velocity[a] = b->v[a];
+            d->angular_velocity[a] = b->w[a];
+            d->axis[a] = b->axis[a];
+        }
+        for (int a = 0; a < 4; ++a) d->rotation[a] = b->q[a];
+        d->amplitude = b->amp;
+        d->period = b->omega > 0.0 ? 2.0 * M_PI / b->omega : 0.0;
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (F) F[a] = b->F[a];
+        if (T) T[a] = b->T[a];
+    }
+    return 0;
+}
+
+int or_body_forces(const or_state* s, uint32_t id, double* f)
+{
+    if (id >= s->nbody) return -1;
+    memcpy(f, s->sF + 3 * (size_t)s->body[id].first, 3 * (size_t)s->body[id].count * sizeof(double));
+    return 0;
+}
+
+int or_body_psi(const or_state* s, uint32_t id, double* psi)
+{
+    if (id >= s->nbody) return -1;
+    memcpy(psi, s->psi0 + s->body[id].first, (size_t)s->body[id].count * sizeof(double));
+    return 0;
 }

@@ -124,6 +124,26 @@ uint32_t or_neighbors_fp32(uint32_t n_f, const float* xf, uint32_t n_b, const fl
 int or_set_restriction(or_state* s, int mode, uint32_t count_active, uint32_t count_active_neighbors,
                        long long true_particle_count, const uint32_t* tags);
 
+/* Rigid bodies (the dynamic bodies of DFSPHCData, vector_dynamic_bodies_data):
+ * their particles are moving boundary particles after the walls; v_j enters
+ * the density sources, every Jacobi update pushes -m_i k_i psi_j gradW_ij onto
+ * body particle j, and or_update_pos sums the reactions into force and torque
+ * and moves the body (type 0 dynamic: semi-implicit Euler under gravity and
+ * the fluid force; 1 kinematic: left as set; 2 sine: position + amplitude
+ * sin(2 pi t / period) axis).  Positions in the body frame are relative to
+ * the center of mass; rotation is a unit quaternion (x, y, z, w).  psi NULL:
+ * computed over the walls and every body at its current place. */
+typedef struct or_body_desc {
+    int type;
+    double mass, inertia[3], position[3], velocity[3], rotation[4], angular_velocity[3], axis[3];
+    double amplitude, period;
+} or_body_desc;
+int or_add_body(or_state* s, const or_body_desc* b, uint32_t n, const double* x_body, const double* psi);
+int or_set_body(or_state* s, uint32_t id, const or_body_desc* b);
+int or_get_body(const or_state* s, uint32_t id, or_body_desc* b, double* force, double* torque);
+int or_body_forces(const or_state* s, uint32_t id, double* f);   /* 3 n, last step */
+int or_body_psi(const or_state* s, uint32_t id, double* psi);
+
 /* A19: BufferFluidSurface and the open boundary */
 typedef struct or_surface {
     int type, reversed;       /* 0 plane, 1 cuboid, 2 cylinder, 4 sphere, 5 star */

@@ -72,6 +72,28 @@ _FP = C.POINTER(C.c_float)
 _UP = C.POINTER(C.c_uint32)
 
 
+class OrBodyDesc(C.Structure):
+    _fields_ = [("type", C.c_int), ("mass", C.c_double), ("inertia", C.c_double * 3), ("position", C.c_double * 3),
+                ("velocity", C.c_double * 3), ("rotation", C.c_double * 4), ("angular_velocity", C.c_double * 3),
+                ("axis", C.c_double * 3), ("amplitude", C.c_double), ("period", C.c_double)]
+
+
+def body_desc(type=0, mass=0.0, inertia=(0, 0, 0), position=(0, 0, 0), velocity=(0, 0, 0), rotation=(0, 0, 0, 1),
+              angular_velocity=(0, 0, 0), axis=(0, 0, 0), amplitude=0.0, period=0.0) -> OrBodyDesc:
+    d = OrBodyDesc()
+    d.type = type
+    d.mass = mass
+    d.inertia = (C.c_double * 3)(*inertia)
+    d.position = (C.c_double * 3)(*position)
+    d.velocity = (C.c_double * 3)(*velocity)
+    d.rotation = (C.c_double * 4)(*rotation)
+    d.angular_velocity = (C.c_double * 3)(*angular_velocity)
+    d.axis = (C.c_double * 3)(*axis)
+    d.amplitude = amplitude
+    d.period = period
+    return d
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -119,6 +141,11 @@ def lib():
                                        C.c_double, C.POINTER(C.c_uint8), _DP, _DP, _DP]),
             "or_outflow": (C.c_uint32, [C.c_void_p, C.POINTER(OrSurface), C.POINTER(OrSurface),
                                         C.POINTER(C.c_uint8)]),
+            "or_add_body": (C.c_int, [C.c_void_p, C.POINTER(OrBodyDesc), C.c_uint32, _DP, _DP]),
+            "or_set_body": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(OrBodyDesc)]),
+            "or_get_body": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(OrBodyDesc), _DP, _DP]),
+            "or_body_forces": (C.c_int, [C.c_void_p, C.c_uint32, _DP]),
+            "or_body_psi": (C.c_int, [C.c_void_p, C.c_uint32, _DP]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -254,6 +281,38 @@ class Oracle:
         st = OrStats()
         self.L.or_step(self.s, C.byref(st))
         return st
+
+    def add_body(self, desc, x_body, psi=None):
+        x = _d(x_body).reshape(-1)
+        p = None if psi is None else _d(psi)
+        r = self.L.or_add_body(self.s, C.byref(desc), x.size // 3, x.ctypes.data_as(_DP),
+                               None if p is None else p.ctypes.data_as(_DP))
+        if r < 0:
+            raise ValueError("invalid body")
+        self._bodies = getattr(self, "_bodies", []) + [x.size // 3]
+        return r
+
+    def set_body(self, i, desc):
+        if self.L.or_set_body(self.s, i, C.byref(desc)) != 0:
+            raise ValueError("invalid body")
+
+    def body(self, i):
+        """(OrBodyDesc state, force[3], torque[3]) of body i after the last step"""
+        d = OrBodyDesc()
+        F = np.zeros(3)
+        T = np.zeros(3)
+        self.L.or_get_body(self.s, i, C.byref(d), F.ctypes.data_as(_DP), T.ctypes.data_as(_DP))
+        return d, F, T
+
+    def body_forces(self, i):
+        out = np.zeros(3 * self._bodies[i])
+        self.L.or_body_forces(self.s, i, out.ctypes.data_as(_DP))
+        return out.reshape(-1, 3)
+
+    def body_psi(self, i):
+        out = np.zeros(self._bodies[i])
+        self.L.or_body_psi(self.s, i, out.ctypes.data_as(_DP))
+        return out
 
     def set_restriction(self, mode, count_active=0, count_active_neighbors=0, true_particle_count=-1, tags=None):
         t = None if tags is None else np.ascontiguousarray(tags, np.uint32)

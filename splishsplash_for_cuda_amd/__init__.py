@@ -12,4 +12,5 @@ from ._native import (  # noqa: F401
     LISTS_AUTO, LISTS_NO_LDS, LISTS_GLOBAL, FIELD_TAG, DFSPHSurface, surface,
     TAG_ACTIVE, TAG_ACTIVE_NEIGHBORS, TAG_UNTAGGED, TAG_REMOVAL,
     Scene, read_particles_text, write_particles_text,
+    DFSPHRigidBody, rigid_body, BODY_DYNAMIC, BODY_KINEMATIC, BODY_SINE,
 )

@@ -133,6 +133,35 @@ class DFSPHCudaStats(C.Structure):
 
 
 # every function declared in include/dfsph_hip.h, with its ctypes signature
+BODY_DYNAMIC = 0
+BODY_KINEMATIC = 1
+BODY_SINE = 2
+
+
+class DFSPHRigidBody(C.Structure):
+    _fields_ = [("type", C.c_int32), ("mass", C.c_double), ("inertia", C.c_double * 3),
+                ("position", C.c_double * 3), ("velocity", C.c_double * 3), ("rotation", C.c_double * 4),
+                ("angular_velocity", C.c_double * 3), ("axis", C.c_double * 3), ("amplitude", C.c_double),
+                ("period", C.c_double)]
+
+
+def rigid_body(type=BODY_DYNAMIC, mass=0.0, inertia=(0, 0, 0), position=(0, 0, 0), velocity=(0, 0, 0),
+               rotation=(0, 0, 0, 1), angular_velocity=(0, 0, 0), axis=(0, 0, 0), amplitude=0.0,
+               period=0.0) -> DFSPHRigidBody:
+    b = DFSPHRigidBody()
+    b.type = type
+    b.mass = mass
+    b.inertia = (C.c_double * 3)(*inertia)
+    b.position = (C.c_double * 3)(*position)
+    b.velocity = (C.c_double * 3)(*velocity)
+    b.rotation = (C.c_double * 4)(*rotation)
+    b.angular_velocity = (C.c_double * 3)(*angular_velocity)
+    b.axis = (C.c_double * 3)(*axis)
+    b.amplitude = amplitude
+    b.period = period
+    return b
+
+
 class DFSPHSurface(C.Structure):
     """dfsph_surface: a BufferFluidSurface (plane, cuboid, cylinder, sphere, star)."""
     _fields_ = [
@@ -190,6 +219,11 @@ HIP_SIGNATURES = {
     "dfsph_open_boundary_init": (_I, [_P, C.POINTER(DFSPHSurface), C.POINTER(DFSPHSurface),
                                       C.POINTER(DFSPHSurface), _F32P, _U32, _U32P]),
     "dfsph_open_boundary_apply": (_I, [_P, C.POINTER(C.c_double), C.c_float, C.c_float, _I, _I, _U32P, _U32P]),
+    "dfsph_add_rigid_body": (_I, [_P, C.POINTER(DFSPHRigidBody), _F32P, _F32P, _U32, _U32P]),
+    "dfsph_set_rigid_body": (_I, [_P, _U32, C.POINTER(DFSPHRigidBody)]),
+    "dfsph_get_rigid_body": (_I, [_P, _U32, C.POINTER(DFSPHRigidBody), C.POINTER(_D), C.POINTER(_D)]),
+    "dfsph_rigid_body_forces": (_I, [_P, _U32, _F32P, _U32]),
+    "dfsph_rigid_body_count": (_I, [_P, _U32P]),
     "dfsph_default_params": (None, [C.POINTER(DFSPHParams)]),
     "dfsph_last_error": (C.c_char_p, []),
     "dfsph_create": (_I, [C.POINTER(DFSPHParams), C.POINTER(C.c_void_p)]),
@@ -511,6 +545,36 @@ class Solver:
                                                   1 if use_inflow else 0, 1 if use_outflow else 0,
                                                   C.byref(a), C.byref(r)), "dfsph_open_boundary_apply")
         return a.value, r.value
+
+    # ---- rigid bodies ----
+    def add_rigid_body(self, body: "DFSPHRigidBody", xyz_body, psi=None) -> int:
+        x = _f32(xyz_body).reshape(-1)
+        p = None if psi is None else _f32(psi)
+        i = C.c_uint32()
+        _check(self.lib.dfsph_add_rigid_body(self.h, C.byref(body), _ptr(x, C.c_float), _ptr(p, C.c_float),
+                                             x.size // 3, C.byref(i)), "dfsph_add_rigid_body")
+        return i.value
+
+    def set_rigid_body(self, i: int, body: "DFSPHRigidBody"):
+        _check(self.lib.dfsph_set_rigid_body(self.h, i, C.byref(body)), "dfsph_set_rigid_body")
+
+    def rigid_body(self, i: int):
+        """(DFSPHRigidBody state, force[3], torque[3]) after the last step"""
+        b = DFSPHRigidBody()
+        F = (C.c_double * 3)()
+        T = (C.c_double * 3)()
+        _check(self.lib.dfsph_get_rigid_body(self.h, i, C.byref(b), F, T), "dfsph_get_rigid_body")
+        return b, np.array(F[:]), np.array(T[:])
+
+    def rigid_body_forces(self, i: int, n: int):
+        out = np.zeros((n, 3), np.float32)
+        _check(self.lib.dfsph_rigid_body_forces(self.h, i, _ptr(out, C.c_float), 3 * n), "dfsph_rigid_body_forces")
+        return out
+
+    def rigid_body_count(self) -> int:
+        n = C.c_uint32()
+        _check(self.lib.dfsph_rigid_body_count(self.h, C.byref(n)), "dfsph_rigid_body_count")
+        return n.value
 
     def step(self, raise_on_overflow=True):
         st = DFSPHStepStats()

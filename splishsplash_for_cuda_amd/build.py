@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["dfsph_kernels.hip", "dfsph_capi.hip", "dfsph_mlp.hip", "dfsph_window.hip"]
+HIP_SOURCES = ["dfsph_kernels.hip", "dfsph_capi.hip", "dfsph_mlp.hip", "dfsph_window.hip", "dfsph_bodies.hip"]
 HOST_SOURCES = ["DFSPH_CUDA.cpp", "SceneLoader.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

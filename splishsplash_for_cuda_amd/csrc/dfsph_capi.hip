@@ -92,9 +92,21 @@ struct Handle {
     uint32_t* scan_tmp = nullptr;
     size_t scan_words = 0;
 
-    // boundary (static, cell-sorted once)
+    // boundary (walls cell-sorted once; with rigid bodies, walls + body
+    // particles re-sorted at every neighbor search)
     float4* bpos = nullptr;
     uint32_t* bcstart = nullptr;
+    // rigid bodies: body particles follow the walls in the boundary set
+    uint32_t nbs = 0, ns = 0;                 // wall particles, body particles
+    std::vector<dfsph::DevBody> bodies;       // host copies (ranges, motion parameters)
+    dfsph::DevBody* dbody = nullptr;          // device state
+    float4* x0 = nullptr;                     // body-frame positions + psi
+    uint32_t* sbody = nullptr;                // body of each body particle
+    float4* braw = nullptr;                   // walls (cell order), then body particles at their places
+    float4* bvraw = nullptr;                  // velocity records of braw
+    float4* bvel = nullptr;                   // velocity records in cell order (bpos order)
+    unsigned long long* sforce = nullptr;     // reactions on the body particles, fixed point
+    float fscale = 0.0f;                      // fixed-point units per newton
     uint32_t* tile_rank = nullptr;    // SimConst::tile_rank
     uint4* nbr_part = nullptr;        // per-block neighbor statistics of the search
     float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
@@ -231,6 +243,7 @@ Handle::~Handle()
     dfree(rho); dfree(alpha); dfree(radv); dfree(kk); dfree(acc); dfree(cnt); dfree(nbrf); dfree(nbrb); dfree(nbr16); dfree(stage_n); dfree(stg); dfree(ob_cand);
     dfree(key); dfree(slot); dfree(perm); dfree(hist); dfree(cstart); dfree(keep); dfree(rank);
     dfree(keep8); dfree(scan_tmp); dfree(bpos); dfree(bcstart); dfree(tile_rank); dfree(nbr_part);
+    dfree(dbody); dfree(x0); dfree(sbody); dfree(braw); dfree(bvraw); dfree(bvel); dfree(sforce);
     dfree(ctl); dfree(tm); dfree(partial); dfree(partial_f);
     if (hctl) (void)hipHostFree(hctl);
     if (htm) (void)hipHostFree(htm);
@@ -267,7 +280,8 @@ dfsph::Restrict restriction(const Handle* h)
 
 dfsph::NbrLists lists(const Handle* h)
 {
-    return dfsph::NbrLists{h->nbr16, h->nbrf, h->nbrb, h->cnt, h->stage_n, h->stg, restriction(h)};
+    const dfsph::Solids sol{h->ns ? h->bvel : nullptr, h->sforce, h->fscale};
+    return dfsph::NbrLists{h->nbr16, h->nbrf, h->nbrb, h->cnt, h->stage_n, h->stg, restriction(h), sol};
 }
 
 int ensure_scan(Handle* h, uint32_t n)
@@ -344,11 +358,25 @@ int read_ctl(Handle* h)
     return sync(h);
 }
 
+// Body particles at their bodies' state, then walls + body particles sorted
+// into the boundary cell table (positions and velocity records in cell order).
+int place_bodies(Handle* h)
+{
+    dfsph::launch_bodies_place(h->s, h->dbody, h->x0, h->sbody, h->ns, h->braw + h->nbs, h->bvraw + h->nbs,
+                               h->sforce);
+    CHK(counting_sort(h, h->braw, h->nb, h->bcstart));
+    dfsph::launch_gather_f4(h->s, h->nb, h->perm, h->braw, h->bpos);
+    dfsph::launch_gather_f4(h->s, h->nb, h->perm, h->bvraw, h->bvel);
+    HIPCHK(hipGetLastError());
+    return DFSPH_OK;
+}
+
 int neighbor_search(Handle* h, bool sort)
 {
     if (!h->grid_ready) return fail(DFSPH_ERR_STATE, "no boundary set uploaded");
     CHK(flush_vel(h));
     h->pred_valid = false;
+    if (h->ns) CHK(place_bodies(h));
     if (h->n == 0) {
         dfsph::launch_nbr_ctl_init(h->s, h->ctl);     // otherwise the search's finalize writes the statistics
         h->lists_valid = true;
@@ -695,6 +723,66 @@ MlpView handle_view(void* hv)
 void handle_set_error(const char* msg) { g_err = msg; }
 }  // namespace dfsph
 
+namespace {
+
+// grow a device array to count elements, keeping its first `keep`
+template <class T>
+int grow(Handle* h, T*& p, size_t keep, size_t count)
+{
+    T* q = nullptr;
+    CHK(dalloc(q, count));
+    if (keep && p) HIPCHK(hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, h->s));
+    CHK(sync(h));
+    dfree(p);
+    p = q;
+    return DFSPH_OK;
+}
+
+// dfsph_rigid_body -> device body (checked; quaternion and axis normalised);
+// a sine body is placed at time t_now
+int body_from_abi(const dfsph_rigid_body* b, double t_now, dfsph::DevBody& d)
+{
+    if (b->type < DFSPH_BODY_DYNAMIC || b->type > DFSPH_BODY_SINE) return fail(DFSPH_ERR_ARG, "unknown body type");
+    const double qn = std::sqrt(b->rotation[0] * b->rotation[0] + b->rotation[1] * b->rotation[1] +
+                                b->rotation[2] * b->rotation[2] + b->rotation[3] * b->rotation[3]);
+    if (!(qn > 0.0)) return fail(DFSPH_ERR_ARG, "the body's rotation quaternion is zero");
+    if (b->type == DFSPH_BODY_DYNAMIC) {
+        if (!(b->mass > 0.0)) return fail(DFSPH_ERR_ARG, "a dynamic body needs a positive mass");
+        for (int a = 0; a < 3; ++a)
+            if (!(b->inertia[a] > 0.0)) return fail(DFSPH_ERR_ARG, "a dynamic body needs positive moments of inertia");
+    }
+    double an = 0.0;
+    if (b->type == DFSPH_BODY_SINE) {
+        an = std::sqrt(b->axis[0] * b->axis[0] + b->axis[1] * b->axis[1] + b->axis[2] * b->axis[2]);
+        if (!(b->period > 0.0) || !(an > 0.0)) return fail(DFSPH_ERR_ARG, "a sine body needs a period and an axis");
+    }
+    d = dfsph::DevBody{};
+    d.type = b->type;
+    d.mass = b->mass;
+    for (int a = 0; a < 3; ++a) {
+        d.c[a] = b->position[a];
+        d.v[a] = b->velocity[a];
+        d.w[a] = b->angular_velocity[a];
+        d.inertia[a] = b->inertia[a];
+        d.c0[a] = b->position[a];
+        d.axis[a] = an > 0.0 ? b->axis[a] / an : 0.0;
+    }
+    for (int a = 0; a < 4; ++a) d.q[a] = b->rotation[a] / qn;
+    if (b->type == DFSPH_BODY_SINE) {
+        d.amp = b->amplitude;
+        d.omega = 2.0 * M_PI / b->period;
+        const double sn = std::sin(d.omega * t_now), cs = std::cos(d.omega * t_now);
+        for (int a = 0; a < 3; ++a) {
+            d.c[a] = d.c0[a] + d.amp * sn * d.axis[a];
+            d.v[a] = d.amp * d.omega * cs * d.axis[a];
+            d.w[a] = 0.0;
+        }
+    }
+    return DFSPH_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 void dfsph_default_params(dfsph_params* p)
@@ -873,6 +961,7 @@ int dfsph_upload_boundary(void* hv, const float* xyz, const float* psi, uint32_t
     if (rc == DFSPH_OK) rc = sync(h);
     dfree(raw);
     if (rc != DFSPH_OK) return rc;
+    h->nbs = n;
     h->grid_ready = true;
     h->ncell_alloc = (uint32_t)ncell;
     return DFSPH_OK;
@@ -985,6 +1074,10 @@ int dfsph_update_pos(void* hv)
     dfsph::launch_update_pos(h->s, h->tm, h->n, h->vel[h->vc], h->pos[h->pc], h->bmin, h->bmax, h->ctl,
                              restriction(h));
     pend(h, pr);
+    // the bodies move with the fluid: forces of this step, then their step over h
+    if (!h->bodies.empty())
+        dfsph::launch_bodies_integrate(h->s, h->dbody, (uint32_t)h->bodies.size(), h->braw + h->nbs, h->sforce,
+                                       1.0 / (double)h->fscale, h->tm, h->p.gravity);
     HIPCHK(hipGetLastError());
     h->lists_valid = false;
     h->acc_valid = false;
@@ -1548,6 +1641,143 @@ int dfsph_profile_read(void* hv, uint32_t kernel, uint32_t* launches, double* to
     prof_collect(h);
     if (launches) *launches = h->prof_launches[kernel];
     if (total_ms) *total_ms = h->prof_ms[kernel];
+    return DFSPH_OK;
+}
+
+int dfsph_add_rigid_body(void* hv, const dfsph_rigid_body* body, const float* xyz, const float* psi, uint32_t n,
+                         uint32_t* id)
+{
+    Handle* h = H(hv);
+    if (!body || !xyz || n == 0) return fail(DFSPH_ERR_ARG, "a body needs at least one particle");
+    if (!h->grid_ready) return fail(DFSPH_ERR_STATE, "upload the boundary set first");
+    CHK(flush_vel(h));
+    CHK(read_ctl(h));                           // also waits for the stream
+    dfsph::DevBody d;
+    CHK(body_from_abi(body, h->htm->time, d));
+    const uint32_t bid = (uint32_t)h->bodies.size();
+    if (bid) HIPCHK(hipMemcpy(h->bodies.data(), h->dbody, sizeof(dfsph::DevBody) * bid, hipMemcpyDeviceToHost));
+    d.first = h->ns;
+    d.count = n;
+    const uint32_t first = h->ns, ns = h->ns + n, nb = h->nbs + ns;
+    std::vector<float4> P(n);
+    for (uint32_t i = 0; i < n; ++i) P[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], psi ? psi[i] : 0.0f);
+    std::vector<uint32_t> B(n, bid);
+    CHK(grow(h, h->x0, first, ns));
+    CHK(grow(h, h->sbody, first, ns));
+    HIPCHK(hipMemcpy(h->x0 + first, P.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->sbody + first, B.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    // walls in cell order lead the raw boundary array (bpos holds only walls
+    // before the first body)
+    if (!h->braw) {
+        CHK(dalloc(h->braw, nb));
+        HIPCHK(hipMemcpyAsync(h->braw, h->bpos, sizeof(float4) * h->nbs, hipMemcpyDeviceToDevice, h->s));
+    } else {
+        CHK(grow(h, h->braw, h->nbs, nb));
+    }
+    CHK(dalloc(h->bvraw, nb));
+    dfsph::launch_wall_vel(h->s, h->bvraw, h->nbs);
+    CHK(sync(h));
+    CHK(dalloc(h->bvel, nb));
+    CHK(dalloc(h->bpos, nb));
+    CHK(dalloc(h->sforce, 4ull * ns));
+    h->bodies.push_back(d);
+    CHK(dalloc(h->dbody, h->bodies.size()));
+    HIPCHK(hipMemcpy(h->dbody, h->bodies.data(), sizeof(dfsph::DevBody) * h->bodies.size(), hipMemcpyHostToDevice));
+    h->ns = ns;
+    h->nb = nb;
+    if (h->fscale == 0.0f) {
+        // 2^20 units per weight of one fluid particle (0.8 (2r)^3 rho0 |g|)
+        const double d2 = 2.0 * h->p.particle_radius;
+        const double g = std::sqrt(h->p.gravity[0] * h->p.gravity[0] + h->p.gravity[1] * h->p.gravity[1] +
+                                   h->p.gravity[2] * h->p.gravity[2]);
+        h->fscale = (float)(1048576.0 / (0.8 * d2 * d2 * d2 * h->p.density0 * std::max(g, 1.0)));
+    }
+    CHK(ensure_tmp(h, std::max(nb, h->cap)));
+    CHK(place_bodies(h));
+    if (!psi) {
+        float* tmp = nullptr;
+        CHK(dalloc(tmp, nb));
+        dfsph::launch_boundary_psi(h->s, h->c, h->bpos, h->bcstart, nb, tmp);
+        dfsph::launch_solid_psi_take(h->s, h->bvel, tmp, nb, h->x0, first, ns);
+        int rc = place_bodies(h);
+        if (rc == DFSPH_OK) rc = sync(h);
+        dfree(tmp);
+        CHK(rc);
+    }
+    CHK(sync(h));
+    h->lists_valid = false;
+    if (id) *id = bid;
+    return DFSPH_OK;
+}
+
+int dfsph_set_rigid_body(void* hv, uint32_t id, const dfsph_rigid_body* body)
+{
+    Handle* h = H(hv);
+    if (!body || id >= h->bodies.size()) return fail(DFSPH_ERR_ARG, "no such body");
+    CHK(flush_vel(h));
+    CHK(read_ctl(h));
+    dfsph::DevBody d;
+    CHK(body_from_abi(body, h->htm->time, d));
+    d.first = h->bodies[id].first;
+    d.count = h->bodies[id].count;
+    h->bodies[id] = d;
+    HIPCHK(hipMemcpy(h->dbody + id, &d, sizeof(d), hipMemcpyHostToDevice));
+    h->lists_valid = false;
+    return DFSPH_OK;
+}
+
+int dfsph_get_rigid_body(void* hv, uint32_t id, dfsph_rigid_body* body, double* force, double* torque)
+{
+    Handle* h = H(hv);
+    if (id >= h->bodies.size()) return fail(DFSPH_ERR_ARG, "no such body");
+    CHK(flush_vel(h));
+    CHK(sync(h));
+    dfsph::DevBody d;
+    HIPCHK(hipMemcpy(&d, h->dbody + id, sizeof(d), hipMemcpyDeviceToHost));
+    if (body) {
+        std::memset(body, 0, sizeof(*body));
+        body->type = d.type;
+        body->mass = d.mass;
+        for (int a = 0; a < 3; ++a) {
+            body->inertia[a] = d.inertia[a];
+            body->position[a] = d.c[a];
+            body->velocity[a] = d.v[a];
+            body->angular_velocity[a] = d.w[a];
+            body->axis[a] = d.axis[a];
+        }
+        for (int a = 0; a < 4; ++a) body->rotation[a] = d.q[a];
+        body->amplitude = d.amp;
+        body->period = d.omega > 0.0 ? 2.0 * M_PI / d.omega : 0.0;
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (force) force[a] = d.force[a];
+        if (torque) torque[a] = d.torque[a];
+    }
+    return DFSPH_OK;
+}
+
+int dfsph_rigid_body_forces(void* hv, uint32_t id, float* f, uint32_t count)
+{
+    Handle* h = H(hv);
+    if (id >= h->bodies.size()) return fail(DFSPH_ERR_ARG, "no such body");
+    const dfsph::DevBody& d = h->bodies[id];
+    if (!f || count != 3 * d.count) return fail(DFSPH_ERR_ARG, "count must be 3 x the body's particles");
+    CHK(flush_vel(h));
+    CHK(sync(h));
+    std::vector<unsigned long long> raw(4ull * d.count);
+    HIPCHK(hipMemcpy(raw.data(), h->sforce + 4ull * d.first, sizeof(unsigned long long) * raw.size(),
+                     hipMemcpyDeviceToHost));
+    const double inv = 1.0 / (double)h->fscale;
+    for (uint32_t k = 0; k < d.count; ++k)
+        for (int a = 0; a < 3; ++a) f[3 * k + a] = (float)((double)(long long)raw[4ull * k + a] * inv);
+    return DFSPH_OK;
+}
+
+int dfsph_rigid_body_count(void* hv, uint32_t* n)
+{
+    Handle* h = H(hv);
+    if (!n) return fail(DFSPH_ERR_ARG, "null count");
+    *n = (uint32_t)h->bodies.size();
     return DFSPH_OK;
 }
 

@@ -220,6 +220,22 @@ __device__ __forceinline__ bool is_source(const Restrict& r, uint32_t i)
     return true;
 }
 
+// Rigid-body particles (the dynamic bodies of DFSPHCData,
+// vector_dynamic_bodies_data, DFSPH_c_arrays_structure.cpp:897-905) are
+// moving boundary particles: they sit in the boundary set after the static
+// walls, and while any exist the sweeps read the velocity of every boundary
+// neighbor and push the reaction of the pressure updates onto the body
+// particles (ITER_NEIGHBORS_SOLIDS, DFSPH_core_cuda.cu:467-477, 936-946).
+// bvel[j] = (velocity, solid index as bits; kNoSolid for a wall particle).
+// Reactions are summed in 64-bit fixed point, so integer atomics make the sum
+// independent of their order (fscale units per newton).
+constexpr uint32_t kNoSolid = 0xffffffffu;
+struct Solids {
+    const float4* bvel;            // null: no rigid bodies (walls at rest)
+    unsigned long long* force;     // 4 words per solid particle (x, y, z, unused)
+    float fscale;
+};
+
 struct NbrLists {
     const uint16_t* nbr16;         // 16-bit fluid lists (blocks with a staged set)
     const uint32_t* nbrf;          // 32-bit fluid lists (unstaged blocks)
@@ -228,7 +244,51 @@ struct NbrLists {
     const uint32_t* stage_n;       // per block: set size or kUnstaged
     const uint32_t* stg;           // per block: global ids of the set (kStageCap each)
     Restrict rs;                   // solver restriction (mode 0: none)
+    Solids sol;
 };
+
+// boundary neighbor record of the sweeps: position + psi, velocity + solid index
+struct BN {
+    float4 p, v;
+};
+
+template <bool SOL>
+__device__ __forceinline__ BN load_bnd(const float4* __restrict__ bpos, const Solids& S, uint32_t j)
+{
+    BN r;
+    r.p = bpos[j];
+    r.v = SOL ? S.bvel[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return r;
+}
+
+// The reaction f = -m_i k_i psi_j gradW_ij on body particle j of one velocity
+// update (the reference's atomicAdd on body.F, DFSPH_core_cuda.cu:471-476).
+template <bool SOL>
+__device__ __forceinline__ void add_reaction(const Solids& S, const float4& vj, bool valid, float fx, float fy,
+                                             float fz)
+{
+    if (!SOL || !valid) return;
+    const uint32_t sid = __float_as_uint(vj.w);
+    if (sid == kNoSolid) return;
+    unsigned long long* f = S.force + 4ull * sid;
+    atomicAdd(f + 0, (unsigned long long)__float2ll_rn(fx * S.fscale));
+    atomicAdd(f + 1, (unsigned long long)__float2ll_rn(fy * S.fscale));
+    atomicAdd(f + 2, (unsigned long long)__float2ll_rn(fz * S.fscale));
+}
+
+// One rigid body on the device (state in FP64).  Body particles are stored
+// in the body frame relative to the center of mass c; the world position is
+// R(q) x0 + c and the velocity w x (x - c) + v
+// (DFSPH_updateDynamicObjectParticles_kernel, DFSPH_core_cuda.cu:3407-3420).
+struct DevBody {
+    double c[3], v[3], q[4], w[3];      // center of mass, velocity, orientation (x y z w), angular velocity
+    double mass, inertia[3];            // body-frame principal moments
+    double force[3], torque[3];         // from the fluid during the last step
+    double c0[3], axis[3], amp, omega;  // sinusoidal translation: c = c0 + amp sin(omega t) axis
+    int32_t type, pad;
+    uint32_t first, count;              // particle range in the solid arrays
+};
+constexpr int kBodyDynamic = 0, kBodyKinematic = 1, kBodySine = 2;
 
 // Blocks are dispatched round robin over the 8 XCDs, each with its own L2.
 // Remapping gives every XCD a contiguous range of logical blocks, so

@@ -878,7 +878,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_finalize(SolveCtl* ctl, const 
 // kappaV h (dt-invariant) / h, clamped like the reference (WS == 1), or nothing
 // (WS == 2).  Velocities are read from vel_i and written to vel_o.  The
 // applied k_i starts the particle's total of the step in kvtot (when given).
-template <int WS>
+template <int WS, bool SOL>
 __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTime* t, uint32_t n,
                                                         const float4* __restrict__ pos,
                                                         const float4* __restrict__ bpos, NbrLists L,
@@ -942,8 +942,9 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
                [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
     (void)nf;
     float bx = 0.0f, by = 0.0f, bz = 0.0f;
-    for_each_nbr32(L.nbrb, stride, i, nb, 0u, hd.b0, hd.b1, [&](uint32_t j) { return bpos[j]; },
-        [&](const float4& pj, bool valid) {
+    for_each_nbr32(L.nbrb, stride, i, nb, 0u, hd.b0, hd.b1, [&](uint32_t j) { return load_bnd<SOL>(bpos, L.sol, j); },
+        [&](const BN& bj, bool valid) {
+        const float4& pj = bj.p;
         const float m = keep(valid, pj.w);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const PairL kp = pair_listed(c, dist2(dx, dy, dz));
@@ -952,7 +953,7 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
         const float px = mg * dx, py = mg * dy, pz = mg * dz;
         sg2 += px * px + py * py + pz * pz;
         gx += px; gy += py; gz += pz;
-        dr += vi.x * px + vi.y * py + vi.z * pz;
+        dr += (vi.x - bj.v.x) * px + (vi.y - bj.v.y) * py + (vi.z - bj.v.z) * pz;
         bx += px; by += py; bz += pz;
     });
     sg2 += gx * gx + gy * gy + gz * gz;
@@ -963,7 +964,21 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
     radv[i] = r;
     float4 vo = vi;
     if (WS != 2 && r > 0.0f && is_compute(L.rs, i)) {
-        if (fabsf(ki) > kEps) { ax += ki * bx; ay += ki * by; az += ki * bz; }
+        if (fabsf(ki) > kEps) {
+            ax += ki * bx; ay += ki * by; az += ki * bz;
+            if (SOL) {
+                // reactions on the body particles: a second pass, since the
+                // gate (a positive source) is known only after the first
+                const float mk = -pi.w * ki;
+                for_each_nbr32(L.nbrb, stride, i, nb, 0u, [&](uint32_t j) { return load_bnd<SOL>(bpos, L.sol, j); },
+                    [&](const BN& bj, bool valid) {
+                    const float dx = pi.x - bj.p.x, dy = pi.y - bj.p.y, dz = pi.z - bj.p.z;
+                    const PairL kp = grad_listed(c, dist2(dx, dy, dz));
+                    const float f = keep(valid && kp.ok, mk * bj.p.w * kp.g);
+                    add_reaction<SOL>(L.sol, bj.v, valid, f * dx, f * dy, f * dz);
+                });
+            }
+        }
         vo.x += h * ax; vo.y += h * ay; vo.z += h * az;
     }
     vel_o[i] = vo;
@@ -974,7 +989,7 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
 // kappaV = 0) and DFSPH_divergence_loop_end_kernel (:732-748) + the FP64 error
 // reduction that replaces cub::DeviceReduce::Sum + cudaMemcpy (:774-781).
 // Also stores k_i = rho_adv_i * alpha_i for the next Jacobi sweep.
-template <bool INIT>
+template <bool INIT, bool SOL>
 __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, const float4* __restrict__ pos,
                                                        const float4* __restrict__ bpos, NbrLists L,
                                                        const float4* __restrict__ vel,
@@ -1013,12 +1028,13 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
         fluid_nbrs(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
                    [&](uint32_t j) { return PV{pos[j], vel[j]}; },
                    [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
-        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, hd.b0, hd.b1, [&](uint32_t j) { return bpos[j]; },
-            [&](const float4& pj, bool valid) {
+        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, hd.b0, hd.b1,
+            [&](uint32_t j) { return load_bnd<SOL>(bpos, L.sol, j); }, [&](const BN& bj, bool valid) {
+            const float4& pj = bj.p;
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const PairL kp = grad_listed(c, dist2(dx, dy, dz));
             const float mg = keep(valid && kp.ok, pj.w * kp.g);
-            dr += mg * (vi.x * dx + vi.y * dy + vi.z * dz);
+            dr += mg * ((vi.x - bj.v.x) * dx + (vi.y - bj.v.y) * dy + (vi.z - bj.v.z) * dz);
         });
         r = (nf + nb < 20u) ? 0.0f : fmaxf(dr, 0.0f);
         radv[i] = r;
@@ -1034,7 +1050,7 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
 // Jacobi velocity update: divergenceSolveParticle<false> (DFSPH_core_cuda.cu:400-481,
 // DIV: kappaV += k_i, step h) and pressureSolveParticle<false> (:867-951, step h_future):
 // v_i += step (sum_f [|k_i + k_j| > eps] (k_i + k_j) m_j gradW + [|k_i| > eps] sum_b k_i psi_j gradW)
-template <bool DIV>
+template <bool DIV, bool SOL>
 __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t, uint32_t n,
                                                    const float4* __restrict__ pos,
                                                    const float4* __restrict__ bpos, NbrLists L,
@@ -1070,12 +1086,15 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t,
                [&](uint32_t j) { return PK{pos[j], kk[j]}; },
                [&](const PK& d, bool valid) { fluid(d.p, d.k, valid); });
     if (fabsf(ki) > kEps) {
-        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j) { return bpos[j]; },
-            [&](const float4& pj, bool valid) {
+        const float mi = pi.w;
+        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j) { return load_bnd<SOL>(bpos, L.sol, j); },
+            [&](const BN& bj, bool valid) {
+            const float4& pj = bj.p;
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const PairL kp = grad_listed(c, dist2(dx, dy, dz));
             const float s = keep(valid && kp.ok, ki * pj.w * kp.g);
             ax += s * dx; ay += s * dy; az += s * dz;
+            add_reaction<SOL>(L.sol, bj.v, valid, -mi * s * dx, -mi * s * dy, -mi * s * dz);
         });
     }
     const float step = DIV ? t->h : t->h_future;
@@ -1188,7 +1207,7 @@ __global__ void k_update_vel(const DevTime* t, uint32_t n, const float4* acc, fl
 // or the learned prediction of kappa h_f^2 (dt-invariant) / h_f^2, clamped like
 // the reference (WS == 1).  No gate (unlike the divergence warm start).  The
 // applied k_i starts the particle's total of the step in ktot (when given).
-template <int WS, bool APPLY_ACC>
+template <int WS, bool APPLY_ACC, bool SOL>
 __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTime* t, uint32_t n,
                                                         const float4* __restrict__ pos,
                                                         const float4* __restrict__ bpos, NbrLists L,
@@ -1231,12 +1250,15 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
                [&](uint32_t j) { return PK{pos[j], ws_of(j)}; },
                [&](const PK& d, bool valid) { fluid(d.p, d.k, valid); });
     if (fabsf(ki) > kEps) {
-        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j) { return bpos[j]; },
-            [&](const float4& pj, bool valid) {
+        const float mi = pi.w;
+        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j) { return load_bnd<SOL>(bpos, L.sol, j); },
+            [&](const BN& bj, bool valid) {
+            const float4& pj = bj.p;
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const PairL kp = grad_listed(c, dist2(dx, dy, dz));
             const float s = keep(valid && kp.ok, ki * pj.w * kp.g);
             ax += s * dx; ay += s * dy; az += s * dz;
+            add_reaction<SOL>(L.sol, bj.v, valid, -mi * s * dx, -mi * s * dy, -mi * s * dz);
         });
     }
     if (APPLY_ACC) {
@@ -1254,7 +1276,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
 // INIT: DFSPH_pressure_init_kernel (:1217-1262): alpha *= 1/h_f, kappa = 0 first.
 // Otherwise DFSPH_pressure_loop_end_kernel (:1264-1284) + FP64 error reduction
 // (replaces cub Sum + cudaMemcpy, :1321-1325).
-template <bool INIT>
+template <bool INIT, bool SOL>
 __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const DevTime* t, uint32_t n,
                                                            const float4* __restrict__ pos,
                                                            const float4* __restrict__ bpos, NbrLists L,
@@ -1297,12 +1319,13 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
                    [&](uint32_t j) { return PV{pos[j], vel[j]}; },
                    [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
         (void)nf;
-        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, hd.b0, hd.b1, [&](uint32_t j) { return bpos[j]; },
-            [&](const float4& pj, bool valid) {
+        for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, hd.b0, hd.b1,
+            [&](uint32_t j) { return load_bnd<SOL>(bpos, L.sol, j); }, [&](const BN& bj, bool valid) {
+            const float4& pj = bj.p;
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const PairL kp = grad_listed(c, dist2(dx, dy, dz));
             const float mg = keep(valid && kp.ok, pj.w * kp.g);
-            dr += mg * (vi.x * dx + vi.y * dy + vi.z * dz);
+            dr += mg * ((vi.x - bj.v.x) * dx + (vi.y - bj.v.y) * dy + (vi.z - bj.v.z) * dz);
         });
         float a = a0;
         if (INIT) { a *= t->inv_h_future; alpha[i] = a; }
@@ -1387,6 +1410,7 @@ __global__ void k_finish_totals(const DevTime* t, uint32_t n, int pressure, cons
 //  4 previous total kappa h_f^2    5 previous total kappaV h
 //  6 boundary fraction n_b / n     7-9 mean neighbor offset / h_k
 // 10 mean neighbor distance / h_k 11 fluid share of the density
+template <bool SOL>
 __global__ __launch_bounds__(kBlock) void k_features(SimConst c, const DevTime* t, uint32_t n,
                                                      const float4* __restrict__ pos,
                                                      const float4* __restrict__ bpos, NbrLists L,
@@ -1426,15 +1450,16 @@ __global__ __launch_bounds__(kBlock) void k_features(SimConst c, const DevTime* 
                [&](uint32_t j) { return PV{pos[j], vel[j]}; },
                [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
     float dbd = 0.0f;
-    for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, hd.b0, hd.b1, [&](uint32_t j) { return bpos[j]; },
-        [&](const float4& pj, bool valid) {
+    for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, hd.b0, hd.b1, [&](uint32_t j) { return load_bnd<SOL>(bpos, L.sol, j); },
+        [&](const BN& bj, bool valid) {
+        const float4& pj = bj.p;
         const float m = keep(valid, 1.0f);
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float r2 = dist2(dx, dy, dz);
         const PairL kp = pair_listed(c, r2);
         const float mj = m * pj.w;
         dbd += mj * kp.W;
-        dr += keep(kp.ok, mj * kp.g) * (vi.x * dx + vi.y * dy + vi.z * dz);
+        dr += keep(kp.ok, mj * kp.g) * ((vi.x - bj.v.x) * dx + (vi.y - bj.v.y) * dy + (vi.z - bj.v.z) * dz);
         ox -= m * dx; oy -= m * dy; oz -= m * dz;
         od += m * __builtin_sqrtf(r2);
     });
@@ -1602,12 +1627,12 @@ void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime*
 {
     if (!n) return;
     const uint32_t g = nblk_xcd(n);
-    if (ws == 0)
-        k_div_ws_init<0><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel_i, vel_o, kws, rho, alpha, radv, kvtot);
-    else if (ws == 1)
-        k_div_ws_init<1><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel_i, vel_o, kws, rho, alpha, radv, kvtot);
-    else
-        k_div_ws_init<2><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel_i, vel_o, kws, rho, alpha, radv, kvtot);
+    const bool sol = L.sol.bvel != nullptr;
+#define DWS(W, S) k_div_ws_init<W, S><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel_i, vel_o, kws, rho, alpha, radv, kvtot)
+    if (ws == 0) { if (sol) DWS(0, true); else DWS(0, false); }
+    else if (ws == 1) { if (sol) DWS(1, true); else DWS(1, false); }
+    else { if (sol) DWS(2, true); else DWS(2, false); }
+#undef DWS
 }
 
 void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
@@ -1616,12 +1641,11 @@ void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, 
                        const CtlInit& ci)
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
-    if (init) {
-        k_div_source<true><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial, ci);
-    } else {
-        k_div_source<false><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial,
-                                                 CtlInit{});
-    }
+    const bool sol = L.sol.bvel != nullptr;
+#define DS(I, S, CI) k_div_source<I, S><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial, CI)
+    if (init) { if (sol) DS(true, true, ci); else DS(true, false, ci); }
+    else { if (sol) DS(false, true, CtlInit{}); else DS(false, false, CtlInit{}); }
+#undef DS
 }
 
 void launch_solve_finalize(hipStream_t s, uint32_t n, SolveCtl* ctl, const double* partial, const Publish& pub)
@@ -1635,10 +1659,11 @@ void launch_jacobi(hipStream_t s, bool div, const SimConst& c, const DevTime* t,
 {
     if (!n) return;
     const uint32_t g = nblk_xcd(n);
-    if (div)
-        k_jacobi<true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, kk, kappaV, vel, ctl);
-    else
-        k_jacobi<false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, kk, kappaV, vel, ctl);
+    const bool sol = L.sol.bvel != nullptr;
+#define JA(D, S) k_jacobi<D, S><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, kk, kappaV, vel, ctl)
+    if (div) { if (sol) JA(true, true); else JA(true, false); }
+    else { if (sol) JA(false, true); else JA(false, false); }
+#undef JA
 }
 
 void launch_xsph(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
@@ -1662,17 +1687,16 @@ void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c
 {
     if (!n) return;
     const uint32_t g = nblk_xcd(n);
+    const bool sol = L.sol.bvel != nullptr;
+#define PW(W, A, S) k_pressure_ws<W, A, S><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, acc, vel, kws, rescale, ktot)
     if (ws == 1) {
-        if (apply_acc)
-            k_pressure_ws<1, true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, acc, vel, kws, rescale, ktot);
-        else
-            k_pressure_ws<1, false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, acc, vel, kws, rescale, ktot);
+        if (apply_acc) { if (sol) PW(1, true, true); else PW(1, true, false); }
+        else { if (sol) PW(1, false, true); else PW(1, false, false); }
     } else {
-        if (apply_acc)
-            k_pressure_ws<0, true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, acc, vel, kws, rescale, ktot);
-        else
-            k_pressure_ws<0, false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, acc, vel, kws, rescale, ktot);
+        if (apply_acc) { if (sol) PW(0, true, true); else PW(0, true, false); }
+        else { if (sol) PW(0, false, true); else PW(0, false, false); }
     }
+#undef PW
 }
 
 void launch_density_source(hipStream_t s, bool init, const SimConst& c, const DevTime* t, uint32_t n,
@@ -1681,13 +1705,11 @@ void launch_density_source(hipStream_t s, bool init, const SimConst& c, const De
                            float* kappa, SolveCtl* ctl, double* partial, const CtlInit& ci)
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
-    if (init) {
-        k_density_source<true><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl,
-                                                    partial, ci);
-    } else {
-        k_density_source<false><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl,
-                                                     partial, CtlInit{});
-    }
+    const bool sol = L.sol.bvel != nullptr;
+#define DN(I, S, CI) k_density_source<I, S><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl, partial, CI)
+    if (init) { if (sol) DN(true, true, ci); else DN(true, false, ci); }
+    else { if (sol) DN(false, true, CtlInit{}); else DN(false, false, CtlInit{}); }
+#undef DN
 }
 
 void launch_update_pos(hipStream_t s, const DevTime* t, uint32_t n, const float4* vel, float4* pos,
@@ -1724,7 +1746,11 @@ void launch_features(hipStream_t s, const SimConst& c, const DevTime* t, uint32_
                      const float4* bpos, const NbrLists& L, const float4* vel,
                      const float* ktot, const float* kvtot, float diameter, float* feat)
 {
-    if (n) k_features<<<nblk_xcd(n), kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, ktot, kvtot, diameter, feat);
+    if (!n) return;
+    if (L.sol.bvel)
+        k_features<true><<<nblk_xcd(n), kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, ktot, kvtot, diameter, feat);
+    else
+        k_features<false><<<nblk_xcd(n), kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, ktot, kvtot, diameter, feat);
 }
 
 void launch_harvest(hipStream_t s, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap, const float* feat,

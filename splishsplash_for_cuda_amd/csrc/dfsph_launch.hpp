@@ -39,6 +39,15 @@ void launch_boundary_psi(hipStream_t s, const SimConst& c, const float4* bpos, c
                          uint32_t nb, float* psi);
 void launch_set_w(hipStream_t s, float4* p, const float* w, uint32_t n);
 
+// rigid bodies (dfsph_bodies.hip)
+void launch_bodies_place(hipStream_t s, const DevBody* B, const float4* x0, const uint32_t* sbody, uint32_t ns,
+                         float4* bpos, float4* bvel, unsigned long long* force);
+void launch_bodies_integrate(hipStream_t s, DevBody* B, uint32_t nbody, const float4* spos,
+                             const unsigned long long* force, double inv_fscale, const DevTime* t, const double* g);
+void launch_solid_psi_take(hipStream_t s, const float4* bvel, const float* psi, uint32_t nb, float4* x0,
+                           uint32_t first, uint32_t end);
+void launch_wall_vel(hipStream_t s, float4* bvel, uint32_t n);
+
 void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime* t, uint32_t n,
                         const float4* pos, const float4* bpos, const NbrLists& L,
                         const float4* vel_i, float4* vel_o, const float* kws, float* rho, float* alpha,

@@ -68,7 +68,7 @@ def test_ctypes_struct_layouts_match_the_header(tmp_path):
     if cc is None:
         pytest.skip("no C compiler")
     structs = {"dfsph_params": _native.DFSPHParams, "dfsph_info": _native.DFSPHInfo,
-               "dfsph_step_stats": _native.DFSPHStepStats}
+               "dfsph_step_stats": _native.DFSPHStepStats, "dfsph_rigid_body": _native.DFSPHRigidBody}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dfsph_hip.h"', "int main(void) {"]
     for name, cls in structs.items():
         lines.append('printf("%s %%zu\\n", sizeof(%s));' % (name, name))
