@@ -67,7 +67,13 @@ uint32_t dfsph_scene_fluid_count(const void* scene);
 void dfsph_scene_fluid(const void* scene, float* xyz, float* vel);   /* 3 n floats each */
 uint32_t dfsph_scene_boundary_count(const void* scene);
 void dfsph_scene_boundary(const void* scene, float* xyz);
-/* the solver of the scene (p == NULL: dfsph_scene_params); NULL on failure */
+/* the scene's dynamic rigid bodies ("isDynamic"): mass and principal
+ * moments of the closed mesh at the body's "density", center of mass and
+ * principal axes as position and rotation; returns the particle count and
+ * fills xyz_body (3 n floats, body frame) when not NULL */
+uint32_t dfsph_scene_body_count(const void* scene);
+uint32_t dfsph_scene_body(const void* scene, uint32_t i, dfsph_rigid_body* body, float* xyz_body);
+/* the solver of the scene with its dynamic bodies (p == NULL: dfsph_scene_params); NULL on failure */
 void* dfsph_cuda_new_from_scene(const void* scene, const dfsph_params* p);
 
 /* UnifiedParticleSet::write_to_file / load_from_file (DFSPH_c_arrays_structure.cpp:

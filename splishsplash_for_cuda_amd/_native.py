@@ -296,6 +296,8 @@ HOST_SIGNATURES = {
     "dfsph_scene_boundary_count": (_U32, [_P]),
     "dfsph_scene_boundary": (None, [_P, _F32P]),
     "dfsph_cuda_new_from_scene": (C.c_void_p, [_P, C.POINTER(DFSPHParams)]),
+    "dfsph_scene_body_count": (_U32, [_P]),
+    "dfsph_scene_body": (_U32, [_P, _U32, C.POINTER(DFSPHRigidBody), _F32P]),
     "dfsph_particles_write_text": (_I, [C.c_char_p, _F32P, _F32P, _F32P, _U32, _I, _I]),
     "dfsph_particles_read_text": (C.c_int64, [C.c_char_p, _F32P, _F32P, _F32P, _U32]),
 }
@@ -828,6 +830,17 @@ class Scene:
         x = np.zeros((n, 3), np.float32)
         self.lib.dfsph_scene_boundary(self.h, _ptr(x, C.c_float))
         return x
+
+    def bodies(self):
+        """[(DFSPHRigidBody, particles (n, 3) in the body frame)] of the dynamic rigid bodies"""
+        out = []
+        for i in range(self.lib.dfsph_scene_body_count(self.h)):
+            b = DFSPHRigidBody()
+            n = self.lib.dfsph_scene_body(self.h, i, C.byref(b), None)
+            x = np.zeros((n, 3), np.float32)
+            self.lib.dfsph_scene_body(self.h, i, None, _ptr(x, C.c_float))
+            out.append((b, x))
+        return out
 
     def close(self):
         if getattr(self, "h", None):

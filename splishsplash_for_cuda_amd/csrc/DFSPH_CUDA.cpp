@@ -181,6 +181,23 @@ void DFSPHCData::write_boundaries_to_file(const std::string& folder) const
     writeParticleText(in_folder(folder, "boundaries_file.txt"), x, {}, psi, 0, 0, 0);
 }
 
+uint32_t DFSPHCData::addRigidBody(const dfsph_rigid_body& body, const std::vector<float>& xyz_body,
+                                  const std::vector<float>& psi)
+{
+    uint32_t id = 0;
+    check(dfsph_add_rigid_body(m_handle, &body, xyz_body.data(), psi.empty() ? nullptr : psi.data(),
+                               (uint32_t)(xyz_body.size() / 3), &id),
+          "dfsph_add_rigid_body");
+    return id;
+}
+
+dfsph_rigid_body DFSPHCData::getRigidBody(uint32_t id, double* force, double* torque) const
+{
+    dfsph_rigid_body b;
+    check(dfsph_get_rigid_body(m_handle, id, &b, force, torque), "dfsph_get_rigid_body");
+    return b;
+}
+
 ParticleArrays DFSPHCData::read_boundaries_from_file(const std::string& folder, bool load_velocities)
 {
     ParticleArrays p;
@@ -442,6 +459,8 @@ namespace {
 struct SceneHandle {
     SPH::SceneLoader::Scene scene;
     std::vector<float> fluid_xyz, fluid_vel, boundary_xyz;
+    std::vector<dfsph_rigid_body> bodies;
+    std::vector<std::vector<float>> body_xyz;     // body frame
 };
 
 void to_float(const std::vector<SPH::Vec3>& v, std::vector<float>& out)
@@ -471,6 +490,11 @@ void* dfsph_scene_read(const char* path, const char* base_path)
         to_float(x, sc->fluid_xyz);
         to_float(v, sc->fluid_vel);
         to_float(b, sc->boundary_xyz);
+        for (const SPH::SceneBody& sb : SPH::dynamicRigidBodies(sc->scene, base)) {
+            sc->bodies.push_back(sb.body);
+            sc->body_xyz.emplace_back();
+            to_float(sb.particles, sc->body_xyz.back());
+        }
         out = sc.release();
     });
     return out;
@@ -507,6 +531,20 @@ void dfsph_scene_boundary(const void* sc, float* xyz)
     if (xyz) std::copy(h->boundary_xyz.begin(), h->boundary_xyz.end(), xyz);
 }
 
+uint32_t dfsph_scene_body_count(const void* sc)
+{
+    return (uint32_t)static_cast<const SceneHandle*>(sc)->bodies.size();
+}
+
+uint32_t dfsph_scene_body(const void* sc, uint32_t i, dfsph_rigid_body* body, float* xyz_body)
+{
+    const SceneHandle* h = static_cast<const SceneHandle*>(sc);
+    if (i >= h->bodies.size()) return 0;
+    if (body) *body = h->bodies[i];
+    if (xyz_body) std::copy(h->body_xyz[i].begin(), h->body_xyz[i].end(), xyz_body);
+    return (uint32_t)(h->body_xyz[i].size() / 3);
+}
+
 void* dfsph_cuda_new_from_scene(const void* sc, const dfsph_params* p)
 {
     SPH::DFSPHCUDA* out = nullptr;
@@ -519,7 +557,9 @@ void* dfsph_cuda_new_from_scene(const void* sc, const dfsph_params* p)
         b.pos = h->boundary_xyz;
         f.pos = h->fluid_xyz;
         f.vel = h->fluid_vel;
-        out = new SPH::DFSPHCUDA(q, b, f);
+        std::unique_ptr<SPH::DFSPHCUDA> solver(new SPH::DFSPHCUDA(q, b, f));
+        for (size_t k = 0; k < h->bodies.size(); ++k) solver->data().addRigidBody(h->bodies[k], h->body_xyz[k]);
+        out = solver.release();
     });
     return out;
 }

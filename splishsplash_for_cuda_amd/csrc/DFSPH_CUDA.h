@@ -76,6 +76,14 @@ public:
     // the boundary set of a solver is static.
     static ParticleArrays read_boundaries_from_file(const std::string& folder, bool load_velocities = false);
 
+    // The dynamic bodies (vector_dynamic_bodies_data, DFSPH_c_arrays_structure.cpp:
+    // 897-905): add one from its body-frame particles (psi empty: computed),
+    // read its state and the fluid force / torque of the last step
+    // (compute_fluid_impact_on_dynamic_body_cuda).  step() moves them.
+    uint32_t addRigidBody(const dfsph_rigid_body& body, const std::vector<float>& xyz_body,
+                          const std::vector<float>& psi = {});
+    dfsph_rigid_body getRigidBody(uint32_t id, double* force = nullptr, double* torque = nullptr) const;
+
     void* handle() const { return m_handle; }
 
     // mirrors of the device time-step state

@@ -251,6 +251,9 @@ void SceneLoader::readScene(const std::string& fileName, Scene& scene)
             readVec(m.get("scale"), d.scale);
             readBool(m.get("isDynamic"), d.dynamic);
             readBool(m.get("isWall"), d.isWall);
+            readNum(m.get("density"), d.density);
+            readVec(m.get("velocity"), d.velocity);
+            readVec(m.get("angularVelocity"), d.angularVelocity);
             scene.boundaryModels.push_back(d);
         }
     }
@@ -376,58 +379,214 @@ struct PointSet {
 
 }  // namespace
 
-void sampleRigidBodies(const SceneLoader::Scene& scene, const std::string& base_path, std::vector<Vec3>& pos)
+namespace {
+
+// A rigid body's mesh placed as the demos place it: scaled per axis, then
+// rotated and translated (Demos/StaticBoundaryDemo/main.cpp:421-424).
+Mesh placedMesh(const SceneLoader::BoundaryData& bd, const std::string& base_path)
 {
-    const double d = 2.0 * scene.particleRadius;
-    for (const SceneLoader::BoundaryData& bd : scene.boundaryModels) {
-        if (!bd.samplesFile.empty()) throw std::runtime_error("particle sample files are not read: " + bd.samplesFile);
-        const std::string path = (!bd.meshFile.empty() && bd.meshFile[0] == '/') ? bd.meshFile : base_path + "/" + bd.meshFile;
-        Mesh m = loadObj(path, bd.scale);
-        for (Vec3& p : m.v) {
-            Vec3 q{0, 0, 0};
-            for (int a = 0; a < 3; ++a)
-                q[a] = bd.rotation[a][0] * p[0] + bd.rotation[a][1] * p[1] + bd.rotation[a][2] * p[2] + bd.translation[a];
-            p = q;
-        }
-        PointSet ps{0.25 * scene.particleRadius, {}, pos};
-        // an axis-aligned box (8 corners): regular lattice on its six faces
-        Vec3 lo = m.v[0], hi = m.v[0];
-        for (const Vec3& p : m.v)
-            for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], p[a]); hi[a] = std::max(hi[a], p[a]); }
-        bool box = m.v.size() == 8;
-        const double tol = 1.0e-6 * (1.0 + std::fabs(hi[0] - lo[0]) + std::fabs(hi[1] - lo[1]) + std::fabs(hi[2] - lo[2]));
-        for (const Vec3& p : m.v)
-            for (int a = 0; a < 3; ++a)
-                box = box && (std::fabs(p[a] - lo[a]) < tol || std::fabs(p[a] - hi[a]) < tol);
-        if (box) {
-            int n[3];
-            for (int a = 0; a < 3; ++a) n[a] = std::max(0, (int)std::lround((hi[a] - lo[a]) / d));
-            for (int i = 0; i <= n[0]; ++i)
-                for (int j = 0; j <= n[1]; ++j)
-                    for (int k = 0; k <= n[2]; ++k) {
-                        if (i != 0 && i != n[0] && j != 0 && j != n[1] && k != 0 && k != n[2]) continue;
-                        const int c[3] = {i, j, k};
-                        Vec3 p;
-                        for (int a = 0; a < 3; ++a) p[a] = n[a] ? lo[a] + (hi[a] - lo[a]) * c[a] / n[a] : lo[a];
-                        ps.add(p);
-                    }
-            continue;
-        }
-        // any other mesh: a lattice of spacing <= d on every triangle
-        for (const auto& t : m.f) {
-            const Vec3 &a = m.v[t[0]], &b = m.v[t[1]], &c = m.v[t[2]];
-            Vec3 e1, e2;
-            for (int x = 0; x < 3; ++x) { e1[x] = b[x] - a[x]; e2[x] = c[x] - a[x]; }
-            const int n1 = std::max(1, (int)std::ceil(std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]) / d));
-            const int n2 = std::max(1, (int)std::ceil(std::sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]) / d));
-            for (int i = 0; i <= n1; ++i)
-                for (int j = 0; j <= n2; ++j) {
-                    const double u = (double)i / n1, w = (double)j / n2;
-                    if (u + w > 1.0 + 1e-12) continue;
-                    ps.add(Vec3{a[0] + u * e1[0] + w * e2[0], a[1] + u * e1[1] + w * e2[1], a[2] + u * e1[2] + w * e2[2]});
+    if (!bd.samplesFile.empty()) throw std::runtime_error("particle sample files are not read: " + bd.samplesFile);
+    const std::string path = (!bd.meshFile.empty() && bd.meshFile[0] == '/') ? bd.meshFile : base_path + "/" + bd.meshFile;
+    Mesh m = loadObj(path, bd.scale);
+    for (Vec3& p : m.v) {
+        Vec3 q{0, 0, 0};
+        for (int a = 0; a < 3; ++a)
+            q[a] = bd.rotation[a][0] * p[0] + bd.rotation[a][1] * p[1] + bd.rotation[a][2] * p[2] + bd.translation[a];
+        p = q;
+    }
+    return m;
+}
+
+// Surface particles of a mesh at spacing d: an axis-aligned box (8 corners)
+// gets the regular lattice of its six faces, any other mesh a lattice of
+// spacing <= d on every triangle (coincident points kept once).
+void sampleMesh(const Mesh& m, double r, std::vector<Vec3>& pos)
+{
+    const double d = 2.0 * r;
+    PointSet ps{0.25 * r, {}, pos};
+    Vec3 lo = m.v[0], hi = m.v[0];
+    for (const Vec3& p : m.v)
+        for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], p[a]); hi[a] = std::max(hi[a], p[a]); }
+    bool box = m.v.size() == 8;
+    const double tol = 1.0e-6 * (1.0 + std::fabs(hi[0] - lo[0]) + std::fabs(hi[1] - lo[1]) + std::fabs(hi[2] - lo[2]));
+    for (const Vec3& p : m.v)
+        for (int a = 0; a < 3; ++a)
+            box = box && (std::fabs(p[a] - lo[a]) < tol || std::fabs(p[a] - hi[a]) < tol);
+    if (box) {
+        int n[3];
+        for (int a = 0; a < 3; ++a) n[a] = std::max(0, (int)std::lround((hi[a] - lo[a]) / d));
+        for (int i = 0; i <= n[0]; ++i)
+            for (int j = 0; j <= n[1]; ++j)
+                for (int k = 0; k <= n[2]; ++k) {
+                    if (i != 0 && i != n[0] && j != 0 && j != n[1] && k != 0 && k != n[2]) continue;
+                    const int c[3] = {i, j, k};
+                    Vec3 p;
+                    for (int a = 0; a < 3; ++a) p[a] = n[a] ? lo[a] + (hi[a] - lo[a]) * c[a] / n[a] : lo[a];
+                    ps.add(p);
                 }
+        return;
+    }
+    for (const auto& t : m.f) {
+        const Vec3 &a = m.v[t[0]], &b = m.v[t[1]], &c = m.v[t[2]];
+        Vec3 e1, e2;
+        for (int x = 0; x < 3; ++x) { e1[x] = b[x] - a[x]; e2[x] = c[x] - a[x]; }
+        const int n1 = std::max(1, (int)std::ceil(std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]) / d));
+        const int n2 = std::max(1, (int)std::ceil(std::sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]) / d));
+        for (int i = 0; i <= n1; ++i)
+            for (int j = 0; j <= n2; ++j) {
+                const double u = (double)i / n1, w = (double)j / n2;
+                if (u + w > 1.0 + 1e-12) continue;
+                ps.add(Vec3{a[0] + u * e1[0] + w * e2[0], a[1] + u * e1[1] + w * e2[1], a[2] + u * e1[2] + w * e2[2]});
+            }
+    }
+}
+
+// Eigen decomposition of a symmetric 3x3 matrix by cyclic Jacobi rotations:
+// A = E diag(l) E^T, E's columns the eigenvectors.
+void symEigen(Mat3 A, Vec3& l, Mat3& E)
+{
+    E = Mat3{{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        const double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        const double tr = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+        if (off <= 1e-30 * tr || off == 0.0) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                if (A[p][q] == 0.0) continue;
+                const double th = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < 3; ++k) {       // A <- A J
+                    const double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - s * akq;
+                    A[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; ++k) {       // A <- J^T A
+                    const double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - s * aqk;
+                    A[q][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; ++k) {       // E <- E J
+                    const double ekp = E[k][p], ekq = E[k][q];
+                    E[k][p] = c * ekp - s * ekq;
+                    E[k][q] = s * ekp + c * ekq;
+                }
+            }
+    }
+    for (int a = 0; a < 3; ++a) l[a] = A[a][a];
+}
+
+// unit quaternion (x, y, z, w) of a rotation matrix (Shepperd's method)
+std::array<double, 4> matQuat(const Mat3& R)
+{
+    const double tr = R[0][0] + R[1][1] + R[2][2];
+    std::array<double, 4> q;
+    if (tr > 0.0) {
+        const double s = 2.0 * std::sqrt(tr + 1.0);
+        q = {(R[2][1] - R[1][2]) / s, (R[0][2] - R[2][0]) / s, (R[1][0] - R[0][1]) / s, 0.25 * s};
+    } else if (R[0][0] > R[1][1] && R[0][0] > R[2][2]) {
+        const double s = 2.0 * std::sqrt(1.0 + R[0][0] - R[1][1] - R[2][2]);
+        q = {0.25 * s, (R[0][1] + R[1][0]) / s, (R[0][2] + R[2][0]) / s, (R[2][1] - R[1][2]) / s};
+    } else if (R[1][1] > R[2][2]) {
+        const double s = 2.0 * std::sqrt(1.0 + R[1][1] - R[0][0] - R[2][2]);
+        q = {(R[0][1] + R[1][0]) / s, 0.25 * s, (R[1][2] + R[2][1]) / s, (R[0][2] - R[2][0]) / s};
+    } else {
+        const double s = 2.0 * std::sqrt(1.0 + R[2][2] - R[0][0] - R[1][1]);
+        q = {(R[0][2] + R[2][0]) / s, (R[1][2] + R[2][1]) / s, 0.25 * s, (R[1][0] - R[0][1]) / s};
+    }
+    const double n = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (double& x : q) x /= n;
+    return q;
+}
+
+}  // namespace
+
+void meshMassProperties(const std::vector<Vec3>& v, const std::vector<std::array<uint32_t, 3>>& f, double density,
+                        double& mass, Vec3& com, Vec3& moments, Mat3& axes)
+{
+    // signed tetrahedra (origin, a, b, c): volume a.(b x c) / 6, first moment
+    // V (a + b + c) / 4, second moments V / 20 (sum_k v_k v_k^T + s s^T), s = a + b + c
+    double V = 0.0;
+    Vec3 S{0, 0, 0};
+    Mat3 C{};
+    for (const auto& t : f) {
+        const Vec3 &a = v[t[0]], &b = v[t[1]], &c = v[t[2]];
+        const double vol = (a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) +
+                            a[2] * (b[0] * c[1] - b[1] * c[0])) / 6.0;
+        const Vec3 s{a[0] + b[0] + c[0], a[1] + b[1] + c[1], a[2] + b[2] + c[2]};
+        V += vol;
+        for (int i = 0; i < 3; ++i) {
+            S[i] += vol * s[i] / 4.0;
+            for (int j = 0; j < 3; ++j) C[i][j] += vol / 20.0 * (a[i] * a[j] + b[i] * b[j] + c[i] * c[j] + s[i] * s[j]);
         }
     }
+    if (V < 0.0) {                          // inward-facing triangles
+        V = -V;
+        for (int i = 0; i < 3; ++i) {
+            S[i] = -S[i];
+            for (int j = 0; j < 3; ++j) C[i][j] = -C[i][j];
+        }
+    }
+    if (!(V > 0.0)) throw std::runtime_error("a dynamic body needs a closed mesh");
+    mass = density * V;
+    for (int i = 0; i < 3; ++i) com[i] = S[i] / V;
+    // covariance about the center, then the inertia tensor tr(C) I - C
+    Mat3 Cc;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Cc[i][j] = density * (C[i][j] - V * com[i] * com[j]);
+    const double tr = Cc[0][0] + Cc[1][1] + Cc[2][2];
+    Mat3 I;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) I[i][j] = (i == j ? tr : 0.0) - Cc[i][j];
+    symEigen(I, moments, axes);
+    // a proper rotation (det +1)
+    const double det = axes[0][0] * (axes[1][1] * axes[2][2] - axes[1][2] * axes[2][1]) -
+                       axes[0][1] * (axes[1][0] * axes[2][2] - axes[1][2] * axes[2][0]) +
+                       axes[0][2] * (axes[1][0] * axes[2][1] - axes[1][1] * axes[2][0]);
+    if (det < 0.0)
+        for (int i = 0; i < 3; ++i) axes[i][2] = -axes[i][2];
+}
+
+void sampleRigidBodies(const SceneLoader::Scene& scene, const std::string& base_path, std::vector<Vec3>& pos)
+{
+    for (const SceneLoader::BoundaryData& bd : scene.boundaryModels)
+        if (!bd.dynamic) sampleMesh(placedMesh(bd, base_path), scene.particleRadius, pos);
+}
+
+std::vector<SceneBody> dynamicRigidBodies(const SceneLoader::Scene& scene, const std::string& base_path)
+{
+    std::vector<SceneBody> out;
+    for (const SceneLoader::BoundaryData& bd : scene.boundaryModels) {
+        if (!bd.dynamic) continue;
+        const Mesh m = placedMesh(bd, base_path);
+        SceneBody b;
+        double mass;
+        Vec3 com, I;
+        Mat3 E;
+        meshMassProperties(m.v, m.f, bd.density, mass, com, I, E);
+        std::vector<Vec3> world;
+        sampleMesh(m, scene.particleRadius, world);
+        // body frame: principal axes through the center of mass
+        for (const Vec3& p : world) {
+            const Vec3 d{p[0] - com[0], p[1] - com[1], p[2] - com[2]};
+            b.particles.push_back(Vec3{E[0][0] * d[0] + E[1][0] * d[1] + E[2][0] * d[2],
+                                       E[0][1] * d[0] + E[1][1] * d[1] + E[2][1] * d[2],
+                                       E[0][2] * d[0] + E[1][2] * d[1] + E[2][2] * d[2]});
+        }
+        std::memset(&b.body, 0, sizeof b.body);
+        b.body.type = DFSPH_BODY_DYNAMIC;
+        b.body.mass = mass;
+        const std::array<double, 4> q = matQuat(E);
+        for (int a = 0; a < 3; ++a) {
+            b.body.inertia[a] = I[a];
+            b.body.position[a] = com[a];
+            b.body.velocity[a] = bd.velocity[a];
+            b.body.angular_velocity[a] = bd.angularVelocity[a];
+        }
+        for (int a = 0; a < 4; ++a) b.body.rotation[a] = q[a];
+        out.push_back(std::move(b));
+    }
+    return out;
 }
 
 void sceneParams(const SceneLoader::Scene& scene, dfsph_params& p)

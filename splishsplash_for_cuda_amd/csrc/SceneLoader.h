@@ -3,13 +3,15 @@
 // SceneLoader follows Utilities/SceneLoader.{h,cpp} (readScene: the
 // Configuration, RigidBodies, FluidModels and FluidBlocks sections with the
 // same defaults) and Demos/Common/DemoBase.cpp:539-600 (createFluidBlocks,
-// the three block packings).  Rigid bodies become boundary particles: the
+// the three block packings).  Rigid bodies are sampled as particles: the
 // mesh (OBJ) is scaled, rotated and translated as
 // Demos/StaticBoundaryDemo/main.cpp:421-424 does, then sampled on regular
 // lattices at spacing 2r (an axis-aligned box exactly on its six faces); the
 // reference samples with Poisson disks instead (PoissonDiskSampling.cpp), so
-// the points differ while their spacing matches.  Particle sample files
-// (partio .bgeo) are not read.
+// the points differ while their spacing matches.  Static bodies become wall
+// particles; dynamic ones ("isDynamic") rigid bodies of the solver, with the
+// mass properties of their closed mesh.  Particle sample files (partio
+// .bgeo) are not read.
 //
 // The text particle files are UnifiedParticleSet::write_to_file /
 // load_from_file (SPlisHSPlasH/DFSPH/DFSPH_c_arrays_structure.cpp:424-560):
@@ -54,6 +56,10 @@ public:
         Mat3 rotation{{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
         Vec3 scale{1, 1, 1};
         bool dynamic = false, isWall = false;
+        // dynamic bodies (keys of the demos' rigid-body simulator): density in
+        // kg/m^3, initial velocity and angular velocity
+        double density = 1000.0;
+        Vec3 velocity{0, 0, 0}, angularVelocity{0, 0, 0};
     };
     struct FluidData {
         std::string samplesFile;
@@ -88,9 +94,26 @@ public:
 // DemoBase::createFluidBlocks (Demos/Common/DemoBase.cpp:539-600)
 void createFluidBlocks(const SceneLoader::Scene& scene, std::vector<Vec3>& pos, std::vector<Vec3>& vel);
 
-// Boundary particles of every rigid body (see the file comment); mesh paths
-// are relative to base_path as in the demos.
+// Boundary particles of every static rigid body (see the file comment); mesh
+// paths are relative to base_path as in the demos.
 void sampleRigidBodies(const SceneLoader::Scene& scene, const std::string& base_path, std::vector<Vec3>& pos);
+
+// Dynamic rigid bodies ("isDynamic"): mass, center of mass and inertia of the
+// closed mesh at the body's density, particles sampled as for the static
+// bodies and given in the principal-axis frame through the center of mass
+// (the frame dfsph_add_rigid_body expects), the body's rotation mapping that
+// frame to the world.
+struct SceneBody {
+    dfsph_rigid_body body;
+    std::vector<Vec3> particles;
+};
+std::vector<SceneBody> dynamicRigidBodies(const SceneLoader::Scene& scene, const std::string& base_path);
+
+// Volume integrals of a closed triangle mesh (signed tetrahedra from the
+// origin): mass, center of mass, principal moments of inertia and the
+// principal axes (columns of a rotation matrix).
+void meshMassProperties(const std::vector<Vec3>& v, const std::vector<std::array<uint32_t, 3>>& f, double density,
+                        double& mass, Vec3& com, Vec3& moments, Mat3& axes);
 
 // The scene's Configuration as solver parameters (library defaults elsewhere).
 void sceneParams(const SceneLoader::Scene& scene, dfsph_params& p);

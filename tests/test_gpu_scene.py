@@ -114,3 +114,42 @@ def test_text_files_save_and_load(tmp_path):
     a.close()
     b.close()
     assert sorted(os.listdir(out)) == ["boundaries_file.txt", "fluid_file.txt", "general_data.txt"]
+
+
+CUBE_OBJ = "\n".join(["v 0 0 0", "v 1 0 0", "v 1 1 0", "v 0 1 0", "v 0 0 1", "v 1 0 1", "v 1 1 1", "v 0 1 1",
+                      "f 1 4 3 2", "f 5 6 7 8", "f 1 2 6 5", "f 4 8 7 3", "f 1 5 8 4", "f 2 3 7 6"]) + "\n"
+
+
+def test_scene_with_a_dynamic_body(tmp_path):
+    """A 0.2 m cube (isDynamic, 500 kg/m^3) dropped onto the dam's water
+    falls, is held up by the fluid and carries its force; the solver comes
+    from the scene.  (Bodies meet only the fluid: contacts with walls or
+    other bodies are not modelled.)"""
+    (tmp_path / "tank.obj").write_text(BOX_OBJ)
+    (tmp_path / "cube.obj").write_text(CUBE_OBJ)
+    doc = {"Configuration": {"particleRadius": 0.025, "timeStepSize": 0.002, "cflMethod": 1,
+                             "cflMaxTimeStepSize": 0.004, "maxError": 0.05, "maxErrorV": 0.1},
+           "RigidBodies": [{"geometryFile": "tank.obj", "scale": [1.2, 1.0, 0.6]},
+                           {"geometryFile": "cube.obj", "isDynamic": True, "density": 500,
+                            "scale": [0.2, 0.2, 0.2], "translation": [0.2, 0.55, 0.2]}],
+           "FluidBlocks": [{"denseMode": 0, "start": [0, 0, 0], "end": [0.6, 0.5, 0.6]}]}
+    p = tmp_path / "dam_cube.json"
+    p.write_text(json.dumps(doc))
+    sc = S.Scene(str(p))
+    (b, x), = sc.bodies()
+    assert len(x) == 5 ** 3 - 3 ** 3 and b.mass == pytest.approx(4.0)
+    host = S.HostSolver.from_scene(sc)
+    assert host.view.rigid_body_count() == 1
+    ys, fy = [], []
+    for _ in range(150):
+        st = host.step()
+        assert _within_tolerance(st, sc.params())
+        body, F, T = host.view.rigid_body(0)
+        ys.append(body.position[1])
+        fy.append(F[1])
+    print("cube center y %.3f -> %.3f (min %.3f), last fluid force %s" % (0.65, ys[-1], min(ys), F))
+    assert ys[-1] < 0.65 - 0.05                # it fell onto the water
+    assert min(ys) > 0.15                      # and floats on it
+    assert max(fy) > 10.0                      # the fluid holds it up
+    assert np.isfinite(host.view.download(S.FIELD_POS)).all()
+    host.close()

@@ -43,6 +43,10 @@ f 3 8 7
 f 4 1 5 8
 """
 
+# unit cube with outward-facing quads (mass properties need a consistent winding)
+CUBE_OBJ = "\n".join(["v 0 0 0", "v 1 0 0", "v 1 1 0", "v 0 1 0", "v 0 0 1", "v 1 0 1", "v 1 1 1", "v 0 1 1",
+                       "f 1 4 3 2", "f 5 6 7 8", "f 1 2 6 5", "f 4 8 7 3", "f 1 5 8 4", "f 2 3 7 6"]) + "\n"
+
 TRIANGLE_OBJ = """v 0 0 0
 v 0.5 0 0
 v 0 0 0.3
@@ -53,6 +57,7 @@ f 1 2 3
 def _scene(tmp_path, doc, name="scene.json"):
     (tmp_path / "box.obj").write_text(BOX_OBJ)
     (tmp_path / "tri.obj").write_text(TRIANGLE_OBJ)
+    (tmp_path / "cube.obj").write_text(CUBE_OBJ)
     p = tmp_path / name
     p.write_text(json.dumps(doc) if not isinstance(doc, str) else doc)
     return Scene(str(p))
@@ -216,3 +221,45 @@ def test_particle_text_reference_layout(tmp_path):
     empty.write_text("0 1 1 0\n")
     x, v, m = read_particles_text(str(empty))
     assert x.shape == (0, 3) and m.shape == (0,)
+
+
+def _quat_mat(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def test_dynamic_rigid_body_mass_properties(tmp_path):
+    """A 0.2 x 0.4 x 0.6 box at 500 kg/m^3 turned 30 degrees about z: mass
+    24 kg, principal moments m (b^2 + c^2) / 12 ... = 1.04, 0.80, 0.40 kg m^2,
+    center at the turned box center; its particles, mapped back to the world,
+    lie on the box surface."""
+    ang = math.radians(30.0)
+    rb = {"geometryFile": "cube.obj", "isDynamic": True, "density": 500, "scale": [0.2, 0.4, 0.6],
+          "rotationAxis": [0, 0, 1], "rotationAngle": ang, "translation": [0.5, 0.5, 0.5], "velocity": [1, 0, 0]}
+    tank = {"geometryFile": "box.obj", "scale": [2, 2, 2], "translation": [-0.5, -0.5, -0.5]}
+    sc = _scene(tmp_path, {"RigidBodies": [tank, rb], "FluidBlocks": [_block(0)]})
+    assert len(sc.boundary()) == 41 ** 3 - 39 ** 3        # the tank only
+    bodies = sc.bodies()
+    assert len(bodies) == 1
+    b, x = bodies[0]
+    assert b.type == 0 and b.mass == pytest.approx(24.0, rel=1e-12)
+    assert sorted(b.inertia[:]) == pytest.approx([0.40, 0.80, 1.04], rel=1e-10)
+    Rz = np.array([[math.cos(ang), -math.sin(ang), 0], [math.sin(ang), math.cos(ang), 0], [0, 0, 1]])
+    c = np.array([0.5, 0.5, 0.5]) + Rz @ np.array([0.1, 0.2, 0.3])
+    assert np.allclose(b.position[:], c, atol=1e-12)
+    assert tuple(b.velocity[:]) == (1.0, 0.0, 0.0)
+    R = _quat_mat(b.rotation[:])
+    assert np.allclose(R @ R.T, np.eye(3), atol=1e-12) and np.linalg.det(R) == pytest.approx(1.0)
+    world = x.astype(np.float64) @ R.T + c
+    local = (world - 0.5) @ Rz                              # back into the unturned box frame
+    ext = np.array([0.2, 0.4, 0.6])
+    assert (local > -1e-5).all() and (local < ext + 1e-5).all()
+    on_face = (np.abs(local) < 1e-5) | (np.abs(local - ext) < 1e-5)
+    assert on_face.any(1).all()
+    # the inertia tensor from the particles' frame is diagonal in the body frame
+    # (principal axes): a box's axes are its edges, so each body axis maps to an edge
+    for k in range(3):
+        axis_local = Rz.T @ R[:, k]
+        assert np.isclose(np.abs(axis_local).max(), 1.0, atol=1e-9)
