@@ -65,6 +65,11 @@ def kernel_bytes(name, n_f, n_b, sum_nf, sum_nb):
     return n_f * s + passes * (4 * (sum_nf + sum_nb) + 16 * n_b)
 
 
+def n_f_mean_over(inf0, inf1):
+    """fluid particles per step over the timed region (open boundaries change the count)"""
+    return (inf0.n_fluid + inf1.n_fluid) / 2.0
+
+
 def make_params(sc, mode, max_neighbors):
     return S.default_params(
         particle_radius=sc.particle_radius, dt=0.001, use_cfl=1, cfl_factor=1.0, dt_min=1e-4, dt_max=5e-3,
@@ -72,9 +77,10 @@ def make_params(sc, mode, max_neighbors):
         warm_start_mode=S.WARMSTART_LEARNED if mode == "learned" else S.WARMSTART_REFERENCE)
 
 
-def cpu_baseline(sc, budget_s, max_steps):
+def cpu_baseline(sc, budget_s, max_steps, tank=None):
     """FP64 OpenMP oracle in the reference CPU semantics (TimeStepDFSPH) on the
-    same scene: whole steps until budget_s of CPU time has been used."""
+    same scene: whole steps until budget_s of CPU time has been used (the wave
+    tank: with its two rigid bodies, without the open boundary)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as om
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -83,6 +89,9 @@ def cpu_baseline(sc, budget_s, max_steps):
                   max_neighbors=160, remove_escaped=1)
     o.set_boundary(sc.boundary.astype(np.float64))
     o.set_fluid(sc.fluid.astype(np.float64))
+    if tank is not None:
+        for desc, pts in tank.bodies:
+            o.add_body(om.body_desc(**desc), pts.astype(np.float64))
     steps = 0
     t0 = time.perf_counter()
     while steps < max_steps:
@@ -101,7 +110,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c4"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c4", "c5"],
+                    help="c5: the open-boundary wave tank (piston, absorbing end, floating box)")
     ap.add_argument("--mode", default="plain", choices=["plain", "learned"])
     ap.add_argument("--path", default="host", choices=["host", "abi"],
                     help="host: SPH::DFSPHCUDA::step() (C++); abi: dfsph_step")
@@ -115,11 +125,24 @@ def main():
               "design-only); rerun with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    sc = {"c1": scene.config1, "c2": scene.config2, "c4": scene.config4}[args.config]()
+    tank = None
+    if args.config == "c5":
+        from splishsplash_for_cuda_amd import wave_tank
+        tank = wave_tank.build()
+        sc = scene.Scene("C5_wave_tank", tank.particle_radius, tank.fluid, tank.walls)
+    else:
+        sc = {"c1": scene.config1, "c2": scene.config2, "c4": scene.config4}[args.config]()
     max_nbr = 90 if args.config == "c1" else 128
     p = make_params(sc, args.mode, max_nbr)
     p.min_iter_v, p.min_iter = (int(v) for v in args.min_iters.split(","))
-    if args.path == "host":
+    if tank is not None:
+        if args.path != "host":
+            print("c5 runs through SPH::DFSPHCUDA::step() (open boundaries): use --path host", file=sys.stderr)
+            sys.exit(2)
+        hs = wave_tank.make_solver(tank, p, learned=args.mode == "learned")
+        solver = hs.view
+        step = hs.step
+    elif args.path == "host":
         hs = S.HostSolver(p, sc.boundary, sc.fluid)
         solver = hs.view
         step = hs.step
@@ -128,7 +151,7 @@ def main():
         solver.upload_boundary(sc.boundary)
         solver.upload_fluid(sc.fluid)
         step = solver.step
-    if args.mode == "learned":
+    if args.mode == "learned" and tank is None:
         from splishsplash_for_cuda_amd import learned
         learned.attach(solver)
 
@@ -172,7 +195,7 @@ def main():
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("bytes_per_launch")
 
-    value = sc.n_fluid * args.steps / elapsed
+    value = n_f_mean_over(inf0, inf1) * args.steps / elapsed
     line = {
         "metric": "particle-steps/s at ρ-err≤0.1%; pressure iters/step learned vs plain DFSPH",
         "value": value,
@@ -187,10 +210,16 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "%s: %s DFSPH dam break, %d fluid + %d boundary particles, r=%g m, tolerances 0.1%%, "
-                        "CFL 0.4 dt in [0.1,5] ms, steps %d-%d of the run"
-                        % (args.config.upper(), args.mode, sc.n_fluid, sc.n_boundary, sc.particle_radius,
-                           args.warmup + 5, args.warmup + 5 + args.steps),
+            "workload": ("%s: %s DFSPH dam break, %d fluid + %d boundary particles, r=%g m, tolerances 0.1%%, "
+                         "CFL 0.4 dt in [0.1,5] ms, steps %d-%d of the run"
+                         % (args.config.upper(), args.mode, sc.n_fluid, sc.n_boundary, sc.particle_radius,
+                            args.warmup + 5, args.warmup + 5 + args.steps)) if tank is None else
+                        ("C5: %s DFSPH open-boundary wave tank 10x2x1 m, %d fluid (start) + %d wall particles, "
+                         "r=%g m, sinusoidal piston (0.1 m, 1.5 s) and a floating box (500 kg/m^3) as rigid "
+                         "bodies, absorbing open boundary with per-step inflow / outflow compaction, tolerances "
+                         "0.1%%, CFL dt in [0.1,5] ms, steps %d-%d of the run"
+                         % (args.mode, sc.n_fluid, sc.n_boundary, sc.particle_radius, args.warmup + 5,
+                            args.warmup + 5 + args.steps)),
             "path": "SPH::DFSPHCUDA::step()" if args.path == "host" else "dfsph_step",
             "iters_per_step": {"divergence_mean": float(np.mean(iters_v)), "density_mean": float(np.mean(iters)),
                                "divergence_p95": float(np.percentile(iters_v, 95)),
@@ -216,7 +245,7 @@ def main():
         "kernel_share": {k: round(v[1] / max(sum(x[1] for x in prof.values()), 1e-9), 4) for k, v in prof.items()},
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(sc, args.cpu_budget, 1000)
+        line["cpu_baseline"] = cpu_baseline(sc, args.cpu_budget, 1000, tank)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

@@ -24,6 +24,7 @@ typedef struct dfsph_cuda_stats {
     double h, time;           /* step size used by this step, simulated time after it  */
     uint32_t n_fluid;
     int32_t removed;
+    uint32_t inflow_added, outflow_removed;   /* open boundaries, at the start of the step */
 } dfsph_cuda_stats;
 
 /* new SPH::DFSPHCUDA(params, boundary, fluid); NULL on failure (message in err) */
@@ -34,6 +35,13 @@ void dfsph_cuda_delete(void* solver);
 /* SPH::DFSPHCUDA::step(); returns a DFSPH_* status */
 int dfsph_cuda_step(void* solver, dfsph_cuda_stats* stats);
 int dfsph_cuda_reset(void* solver);
+/* DFSPHCUDA::applyOpenBoundaries (DFSPH_CUDA.cpp:3889-3924): OpenBoundariesSimple::init
+ * now (dfsph_open_boundary_init), then every dfsph_cuda_step starts with
+ * dfsph_open_boundary_apply(allowed_new_distance (0: 1.75 r), allowed_new_density) */
+int dfsph_cuda_set_open_boundaries(void* solver, const dfsph_surface* boundary, const dfsph_surface* interior,
+                                   const dfsph_surface* fluid_surface, const float* inflow_xyz, uint32_t n,
+                                   float allowed_new_distance, float allowed_new_density, int use_inflow,
+                                   int use_outflow);
 void dfsph_cuda_set_divergence(void* solver, int enable);
 /* the libdfsph_hip.so handle owned by the solver (for dfsph_download etc.) */
 void* dfsph_cuda_handle(void* solver);

@@ -66,6 +66,7 @@ struct or_state {
 typedef struct or_body {
     int type;
     double mass, I[3], c[3], v[3], q[4], w[3], F[3], T[3], c0[3], axis[3], amp, omega;
+    double clock;              /* the body's time: the sum of the steps' h */
     uint32_t first, count;
 } or_body;
 
@@ -1343,7 +1344,8 @@ static void place_bodies(or_state* s)
 /* force and torque about the center of mass (compute_fluid_impact_on_dynamic_body_kernel,
  * DFSPH_memory_management_cuda.cu:688-705), then one step over h: semi-implicit
  * Euler for dynamic bodies (Euler's equations in the body frame), the
- * sinusoidal motion at the time after the step, nothing for kinematic ones */
+ * sinusoidal motion at the body's time after the step (its clock sums the
+ * steps' h), nothing for kinematic ones */
 static void integrate_bodies(or_state* s)
 {
     const double h = s->h;
@@ -1362,8 +1364,9 @@ static void integrate_bodies(or_state* s)
             T[2] += rx * f[1] - ry * f[0];
         }
         for (int a = 0; a < 3; ++a) { b->F[a] = F[a]; b->T[a] = T[a]; }
+        b->clock += h;
         if (b->type == 2) {
-            const double tn = s->time + h, sn = sin(b->omega * tn), cs = cos(b->omega * tn);
+            const double tn = b->clock, sn = sin(b->omega * tn), cs = cos(b->omega * tn);
             for (int a = 0; a < 3; ++a) {
                 b->c[a] = b->c0[a] + b->amp * sn * b->axis[a];
                 b->v[a] = b->amp * b->omega * cs * b->axis[a];
@@ -1420,6 +1423,7 @@ static int body_from_desc(const or_body_desc* d, double t_now, or_body* b)
         b->axis[a] = an > 0.0 ? d->axis[a] / an : 0.0;
     }
     for (int a = 0; a < 4; ++a) b->q[a] = d->rotation[a] / qn;
+    b->clock = t_now;
     if (d->type == 2) {
         b->amp = d->amplitude;
         b->omega = 2.0 * M_PI / d->period;

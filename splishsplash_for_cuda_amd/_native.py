@@ -129,6 +129,8 @@ class DFSPHCudaStats(C.Structure):
         ("time", C.c_double),
         ("n_fluid", C.c_uint32),
         ("removed", C.c_int32),
+        ("inflow_added", C.c_uint32),
+        ("outflow_removed", C.c_uint32),
     ]
 
 
@@ -280,6 +282,8 @@ HOST_SIGNATURES = {
     "dfsph_cuda_delete": (None, [_P]),
     "dfsph_cuda_step": (_I, [_P, C.POINTER(DFSPHCudaStats)]),
     "dfsph_cuda_reset": (_I, [_P]),
+    "dfsph_cuda_set_open_boundaries": (_I, [_P, C.POINTER(DFSPHSurface), C.POINTER(DFSPHSurface),
+                                            C.POINTER(DFSPHSurface), _F32P, _U32, C.c_float, C.c_float, _I, _I]),
     "dfsph_cuda_set_divergence": (None, [_P, _I]),
     "dfsph_cuda_handle": (C.c_void_p, [_P]),
     "dfsph_cuda_save": (_I, [_P, C.c_char_p]),
@@ -785,6 +789,16 @@ class HostSolver:
 
     def load(self, path):
         rc = self.lib.dfsph_cuda_load(self.s, path.encode())
+        if rc != DFSPH_OK:
+            raise DFSPHError(rc, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
+
+    def set_open_boundaries(self, boundary, interior, fluid_surface, inflow_xyz, allowed_new_distance=0.0,
+                            allowed_new_density=700.0, use_inflow=True, use_outflow=True):
+        c = _f32(inflow_xyz).reshape(-1)
+        rc = self.lib.dfsph_cuda_set_open_boundaries(self.s, C.byref(boundary), C.byref(interior),
+                                                     C.byref(fluid_surface), _ptr(c, C.c_float), c.size // 3,
+                                                     allowed_new_distance, allowed_new_density,
+                                                     1 if use_inflow else 0, 1 if use_outflow else 0)
         if rc != DFSPH_OK:
             raise DFSPHError(rc, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
 

@@ -221,9 +221,31 @@ DFSPHCUDA::~DFSPHCUDA() = default;
 // with sort, divergence solve, external forces, (CFL,) time-step update,
 // velocity update, density solve, position update, step end, removal of the
 // particles that left the bounding box.
+void DFSPHCUDA::setOpenBoundaries(const OpenBoundaries& ob)
+{
+    uint32_t kept = 0;
+    check(dfsph_open_boundary_init(m_data.handle(), &ob.boundary, &ob.interior, &ob.fluid_surface,
+                                   ob.inflow_xyz.empty() ? nullptr : ob.inflow_xyz.data(),
+                                   (uint32_t)(ob.inflow_xyz.size() / 3), &kept),
+          "dfsph_open_boundary_init");
+    m_ob = ob;
+    m_openBoundaries = true;
+}
+
+void DFSPHCUDA::applyOpenBoundaries()
+{
+    const float dist = m_ob.allowed_new_distance > 0.0f ? m_ob.allowed_new_distance
+                                                        : (float)(1.75 * m_data.params.particle_radius);
+    check(dfsph_open_boundary_apply(m_data.handle(), nullptr, dist, m_ob.allowed_new_density, m_ob.inflow ? 1 : 0,
+                                    m_ob.outflow ? 1 : 0, &m_inflowAdded, &m_outflowRemoved),
+          "dfsph_open_boundary_apply");
+}
+
 void DFSPHCUDA::step()
 {
     void* h = m_data.handle();
+    m_inflowAdded = m_outflowRemoved = 0;
+    if (m_openBoundaries) applyOpenBoundaries();     // DFSPH_CUDA.cpp:132-140
     check(dfsph_neighbor_search(h, 1), "dfsph_neighbor_search");
     if (m_enableDivergenceSolver) {
         check(dfsph_divergence_solve(h, m_maxIterationsV, m_maxErrorV, &m_iterationsV, &m_divergenceError),
@@ -415,11 +437,34 @@ int dfsph_cuda_step(void* s, dfsph_cuda_stats* st)
             st->time = d->data().time;
             st->n_fluid = (uint32_t)d->data().getFluidParticlesCount();
             st->removed = d->getRemovedLastStep();
+            st->inflow_added = d->getInflowAddedLastStep();
+            st->outflow_removed = d->getOutflowRemovedLastStep();
         }
     });
 }
 
 int dfsph_cuda_reset(void* s) { return guarded([&] { S(s)->reset(); }); }
+
+int dfsph_cuda_set_open_boundaries(void* s, const dfsph_surface* boundary, const dfsph_surface* interior,
+                                   const dfsph_surface* fluid_surface, const float* inflow_xyz, uint32_t n,
+                                   float allowed_new_distance, float allowed_new_density, int use_inflow,
+                                   int use_outflow)
+{
+    return guarded([&] {
+        if (!boundary || !interior || !fluid_surface || (n && !inflow_xyz))
+            throw SPH::DFSPHError(DFSPH_ERR_ARG, "surfaces and inflow positions are required");
+        SPH::DFSPHCUDA::OpenBoundaries ob;
+        ob.boundary = *boundary;
+        ob.interior = *interior;
+        ob.fluid_surface = *fluid_surface;
+        if (n) ob.inflow_xyz.assign(inflow_xyz, inflow_xyz + 3 * (size_t)n);
+        ob.allowed_new_distance = allowed_new_distance;
+        ob.allowed_new_density = allowed_new_density;
+        ob.inflow = use_inflow != 0;
+        ob.outflow = use_outflow != 0;
+        S(s)->setOpenBoundaries(ob);
+    });
+}
 
 void dfsph_cuda_set_divergence(void* s, int enable) { S(s)->setEnableDivergenceSolver(enable != 0); }
 

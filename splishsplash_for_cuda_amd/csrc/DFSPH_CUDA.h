@@ -131,6 +131,23 @@ public:
     // the time step saved in folder/general_data.txt
     static double readGeneralData(const std::string& folder);
 
+    // OpenBoundariesSimple as DFSPHCUDA::applyOpenBoundaries drives it
+    // (DFSPH_CUDA.cpp:3889-3924): the surfaces and inflow positions are set
+    // once (OpenBoundariesSimple::init), then every step() starts with the
+    // inflow / outflow pass (applyOpenBoundary with an allowed new distance of
+    // 1.75 r and density 700 unless given).
+    struct OpenBoundaries {
+        dfsph_surface boundary, interior, fluid_surface;
+        std::vector<float> inflow_xyz;      // 3 n
+        float allowed_new_distance = 0.0f;  // 0: 1.75 r
+        float allowed_new_density = 700.0f;
+        bool inflow = true, outflow = true;
+    };
+    void setOpenBoundaries(const OpenBoundaries& ob);
+    void applyOpenBoundaries();
+    uint32_t getInflowAddedLastStep() const { return m_inflowAdded; }
+    uint32_t getOutflowRemovedLastStep() const { return m_outflowRemoved; }
+
     DFSPHCData& data() { return m_data; }
     const DFSPHCData& data() const { return m_data; }
 
@@ -150,6 +167,9 @@ protected:
     int m_removedLastStep = 0;
     bool m_enableDivergenceSolver = true;
     int count_steps = 0;
+    bool m_openBoundaries = false;
+    OpenBoundaries m_ob;
+    uint32_t m_inflowAdded = 0, m_outflowRemoved = 0;
 };
 
 }  // namespace SPH

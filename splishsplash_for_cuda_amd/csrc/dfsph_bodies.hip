@@ -76,7 +76,9 @@ __global__ void k_bodies_place(const DevBody* __restrict__ B, const float4* __re
 //   dynamic    semi-implicit Euler: v += h (F / m + g); w += h I^-1 (T - w x I w)
 //              with I = R diag(I_body) R^T; c += h v; q += h/2 (w, 0) q, normalised
 //   sine       c = c0 + amp sin(omega t) axis, v = amp omega cos(omega t) axis at
-//              the time after the step
+//              the body's time after the step (its clock sums the steps' h; the
+//              solver's time adds each step's successor instead,
+//              DFSPH_CUDA.cpp:441-452)
 //   kinematic  state left to the caller
 __global__ __launch_bounds__(kBlock) void k_bodies_integrate(DevBody* __restrict__ B,
                                                              const float4* __restrict__ spos,
@@ -118,8 +120,9 @@ __global__ __launch_bounds__(kBlock) void k_bodies_integrate(DevBody* __restrict
     const double T[3] = {sT[0][0], sT[1][0], sT[2][0]};
     for (int a = 0; a < 3; ++a) { b.force[a] = F[a]; b.torque[a] = T[a]; }
     const double h = t->hd;
+    b.clock += h;
     if (b.type == kBodySine) {
-        const double tn = t->time + h;
+        const double tn = b.clock;
         const double sn = sin(b.omega * tn), cs = cos(b.omega * tn);
         for (int a = 0; a < 3; ++a) {
             b.c[a] = b.c0[a] + b.amp * sn * b.axis[a];

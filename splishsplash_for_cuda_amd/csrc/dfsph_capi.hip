@@ -74,6 +74,8 @@ struct Handle {
     dfsph::Surface ob_boundary{}, ob_interior{}, ob_surface{};
     float4* ob_cand = nullptr;
     uint32_t ob_n = 0;
+    uint32_t *ob_k = nullptr, *ob_r = nullptr, *ob_tmp = nullptr;   // inflow scratch, sized at init
+    float4* ob_vnew = nullptr;
     double ob_disp[3] = {0.0, 0.0, 0.0};
     uint32_t next_id = 0;             // first id not yet used by any particle
     int pc = 0;                       // current buffer of pos / kappa / kappaV / id
@@ -241,6 +243,7 @@ Handle::~Handle()
         if (ev[b]) (void)hipEventDestroy(ev[b]);
     }
     dfree(rho); dfree(alpha); dfree(radv); dfree(kk); dfree(acc); dfree(cnt); dfree(nbrf); dfree(nbrb); dfree(nbr16); dfree(stage_n); dfree(stg); dfree(ob_cand);
+    dfree(ob_k); dfree(ob_r); dfree(ob_tmp); dfree(ob_vnew);
     dfree(key); dfree(slot); dfree(perm); dfree(hist); dfree(cstart); dfree(keep); dfree(rank);
     dfree(keep8); dfree(scan_tmp); dfree(bpos); dfree(bcstart); dfree(tile_rank); dfree(nbr_part);
     dfree(dbody); dfree(x0); dfree(sbody); dfree(braw); dfree(bvraw); dfree(bvel); dfree(sforce);
@@ -768,6 +771,7 @@ int body_from_abi(const dfsph_rigid_body* b, double t_now, dfsph::DevBody& d)
         d.axis[a] = an > 0.0 ? b->axis[a] / an : 0.0;
     }
     for (int a = 0; a < 4; ++a) d.q[a] = b->rotation[a] / qn;
+    d.clock = t_now;
     if (b->type == DFSPH_BODY_SINE) {
         d.amp = b->amplitude;
         d.omega = 2.0 * M_PI / b->period;
@@ -1541,6 +1545,12 @@ int dfsph_open_boundary_init(void* hv, const dfsph_surface* boundary, const dfsp
         dfree(r);
         dfree(tmp);
     }
+    if (h->ob_n) {
+        CHK(dalloc(h->ob_k, h->ob_n + 1));
+        CHK(dalloc(h->ob_r, h->ob_n + 1));
+        CHK(dalloc(h->ob_tmp, dfsph::scan_temp_words(h->ob_n + 1)));
+        CHK(dalloc(h->ob_vnew, h->ob_n));
+    }
     h->ob_ready = true;
     if (kept) *kept = h->ob_n;
     return DFSPH_OK;
@@ -1573,17 +1583,12 @@ int dfsph_open_boundary_apply(void* hv, const double* window_displacement, float
         if (nc && n) {
             // cell table of the current positions without reordering
             CHK(counting_sort(h, h->pos[h->pc], n, h->cstart));
-            uint32_t *k = nullptr, *r = nullptr, *tmp = nullptr;
-            float4* vnew = nullptr;
-            CHK(dalloc(k, nc + 1));
-            CHK(dalloc(r, nc + 1));
-            CHK(dalloc(tmp, dfsph::scan_temp_words(nc + 1)));
-            CHK(dalloc(vnew, nc));
-            float mass = 0.0f;
+            uint32_t *k = h->ob_k, *r = h->ob_r, *tmp = h->ob_tmp;
+            float4* vnew = h->ob_vnew;
             float4 p0;
             HIPCHK(hipMemcpyAsync(&p0, h->pos[h->pc], sizeof(float4), hipMemcpyDeviceToHost, h->s));
             CHK(sync(h));
-            mass = p0.w;
+            const float mass = p0.w;
             dfsph::launch_inflow(h->s, h->c, h->ob_cand, nc, h->pos[h->pc], h->vel[h->vc], h->cstart, h->perm,
                                  h->bpos, h->bcstart, h->ob_boundary, allowed_new_distance, allowed_new_density,
                                  mass, k, vnew);
@@ -1598,12 +1603,8 @@ int dfsph_open_boundary_apply(void* hv, const double* window_displacement, float
                                             fluid_ptrs(h, h->pc, h->vc));
                 h->n = n + m;
                 h->next_id += m;
-                rc = sync(h);
+                HIPCHK(hipGetLastError());
             }
-            dfree(k);
-            dfree(r);
-            dfree(tmp);
-            dfree(vnew);
             if (rc != DFSPH_OK) return rc;
             if (added) *added = m;
             h->lists_valid = false;

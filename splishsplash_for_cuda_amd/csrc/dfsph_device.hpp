@@ -285,6 +285,7 @@ struct DevBody {
     double mass, inertia[3];            // body-frame principal moments
     double force[3], torque[3];         // from the fluid during the last step
     double c0[3], axis[3], amp, omega;  // sinusoidal translation: c = c0 + amp sin(omega t) axis
+    double clock;                       // the body's time: advanced by the h of every step it takes
     int32_t type, pad;
     uint32_t first, count;              // particle range in the solid arrays
 };
