@@ -142,7 +142,25 @@ struct Publish {
 // keep the 16-bit lists and read the particles through the set's global ids;
 // larger ones (sparse splashes, more than kAdjMax / 27 own cells, the
 // unsorted search) use 32-bit global lists (stage_n = kUnstaged).
-constexpr uint32_t kStageMax = 1600;       // LDS-staged particles per block
+// LDS-staged particles per block.  2048 holds nearly every block's set (C2:
+// mean 1150-1400, p99 1780): a set read through its global ids instead costs
+// about 1.5x per sweep, more than the occupancy the larger stage gives up
+// (position + k sweeps 40 KB: 4 blocks per CU; position + velocity sweeps
+// 64 KB: 2 blocks per CU; measured 1.86 against 2.00 ms per C2 step at 1600).
+#ifndef DFSPH_STAGE_MAX
+#define DFSPH_STAGE_MAX 2048
+#endif
+constexpr uint32_t kStageMax = DFSPH_STAGE_MAX;
+// The sweeps that stage position and velocity (32 B per particle) may stage
+// fewer (a smaller set fits more blocks per CU; larger sets are then read
+// through the set's global ids, with the same lists and sums).
+#ifndef DFSPH_PV_STAGE
+#define DFSPH_PV_STAGE DFSPH_STAGE_MAX
+#endif
+constexpr uint32_t kStageMaxPV = DFSPH_PV_STAGE;
+#ifndef DFSPH_PV_WAVES
+#define DFSPH_PV_WAVES 1
+#endif
 constexpr uint32_t kStageCap = 4096;       // largest set with 16-bit lists
 constexpr uint32_t kUnstaged = 0xffffffffu;
 constexpr uint32_t kSetNoLds = 0x80000000u;  // stage_n flag: 16-bit lists, set not in LDS
@@ -591,12 +609,12 @@ __device__ __forceinline__ float lds_f1(const float* base, uint32_t off)   // ba
 // index loads are issued first, then all its record loads (ld takes any valid
 // id: the lanes past the end load the set's last record), then the LDS
 // writes, so the loads of one thread are in flight together.
-template <class Ld, class Wr>
+template <uint32_t CAP = kStageMax, class Ld, class Wr>
 __device__ __forceinline__ uint32_t stage_block(const NbrLists& L, uint32_t b, uint32_t sn, Ld&& ld, Wr&& wr)
 {
     using T = decltype(ld(0u));
-    constexpr uint32_t U = (kStageMax + kBlock - 1) / kBlock;
-    if (sn <= kStageMax && sn > 0u) {
+    constexpr uint32_t U = (CAP + kBlock - 1) / kBlock;
+    if (sn <= CAP && sn > 0u) {
         const uint32_t* g = L.stg + (size_t)b * kStageCap;
         uint32_t gi[U];
         T d[U];
@@ -645,11 +663,11 @@ __device__ __forceinline__ Head list_head(const NbrLists& L, uint32_t sn, uint32
 // form its block's lists have: lds(LDS byte offset) loads a neighbor of an
 // LDS-staged set, glob(global id) one from global memory; body(data, valid)
 // consumes it.
-template <class FL, class FG, class Body>
+template <uint32_t CAP = kStageMax, class FL, class FG, class Body>
 __device__ __forceinline__ void fluid_nbrs(const NbrLists& L, uint32_t cap, uint32_t b, uint32_t i, uint32_t nf,
                                            uint32_t sn, FL&& lds, FG&& glob, Body&& body)
 {
-    if (sn <= kStageMax) {
+    if (sn <= CAP) {
         for_each_nbr16(L.nbr16, cap, i, nf, lds, body);
     } else if (sn != kUnstaged) {
         const uint32_t* g = L.stg + (size_t)b * kStageCap;
@@ -660,12 +678,12 @@ __device__ __forceinline__ void fluid_nbrs(const NbrLists& L, uint32_t cap, uint
 }
 
 // The same with the list head loaded beforehand (hd = list_head(L, sn, cap, i)).
-template <class FL, class FG, class Body>
+template <uint32_t CAP = kStageMax, class FL, class FG, class Body>
 __device__ __forceinline__ void fluid_nbrs(const NbrLists& L, uint32_t cap, uint32_t b, uint32_t i, uint32_t sn,
                                            const Head& hd, FL&& lds, FG&& glob, Body&& body)
 {
     const uint32_t nf = hd.cnt & 0xffffu;
-    if (sn <= kStageMax) {
+    if (sn <= CAP) {
         for_each_nbr16(L.nbr16, cap, i, nf, hd.f0, hd.f1, lds, body);
     } else if (sn != kUnstaged) {
         const uint32_t* g = L.stg + (size_t)b * kStageCap;

@@ -879,7 +879,7 @@ __global__ __launch_bounds__(kBlock) void k_solve_finalize(SolveCtl* ctl, const 
 // (WS == 2).  Velocities are read from vel_i and written to vel_o.  The
 // applied k_i starts the particle's total of the step in kvtot (when given).
 template <int WS, bool SOL>
-__global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTime* t, uint32_t n,
+__global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_ws_init(SimConst c, const DevTime* t, uint32_t n,
                                                         const float4* __restrict__ pos,
                                                         const float4* __restrict__ bpos, NbrLists L,
                                                         const float4* __restrict__ vel_i,
@@ -888,8 +888,8 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
                                                         float* __restrict__ rho, float* __restrict__ alpha,
                                                         float* __restrict__ radv, float* __restrict__ kvtot)
 {
-    __shared__ float4 sP[kStageMax];
-    __shared__ float4 sV[kStageMax];           // (v, warm-start k) of the staged set
+    __shared__ float4 sP[kStageMaxPV];
+    __shared__ float4 sV[kStageMaxPV];           // (v, warm-start k) of the staged set
     const float ratio = t->ratio, h = t->h, inv_h = t->inv_h;
     auto ws_val = [&](float raw) -> float {
         if (WS == 0) return fmaxf(raw * ratio / 2.0f, -0.5f);
@@ -905,7 +905,7 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
     const float4 vi = vel_i[ic];
     const float4 pi = pos[ic];
     const float kraw = WS == 2 ? 0.0f : kws[ic];
-    const uint32_t sn = stage_block(L, b, sn0, [&](uint32_t g) {
+    const uint32_t sn = stage_block<kStageMaxPV>(L, b, sn0, [&](uint32_t g) {
         const float4 v = vel_i[g];
         return PV{pos[g], make_float4(v.x, v.y, v.z, ws_of(g))};
     }, [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
@@ -934,7 +934,7 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
             ax += s * px; ay += s * py; az += s * pz;
         }
     };
-    fluid_nbrs(L, stride, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+    fluid_nbrs<kStageMaxPV>(L, stride, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
                [&](uint32_t j) {
                    const float4 v = vel_i[j];
                    return PV{pos[j], make_float4(v.x, v.y, v.z, ws_of(j))};
@@ -990,7 +990,7 @@ __global__ __launch_bounds__(kBlock) void k_div_ws_init(SimConst c, const DevTim
 // reduction that replaces cub::DeviceReduce::Sum + cudaMemcpy (:774-781).
 // Also stores k_i = rho_adv_i * alpha_i for the next Jacobi sweep.
 template <bool INIT, bool SOL>
-__global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, const float4* __restrict__ pos,
+__global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_source(SimConst c, uint32_t n, const float4* __restrict__ pos,
                                                        const float4* __restrict__ bpos, NbrLists L,
                                                        const float4* __restrict__ vel,
                                                        const float* __restrict__ alpha,
@@ -1000,8 +1000,8 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
 {
     if (INIT && ci.sc && blockIdx.x == 0 && threadIdx.x == 0) ctl_init(ci);
     __shared__ double lds4[kBlock / 64];
-    __shared__ float4 sP[kStageMax];
-    __shared__ float4 sV[kStageMax];
+    __shared__ float4 sP[kStageMaxPV];
+    __shared__ float4 sV[kStageMaxPV];
     if (!INIT && ctl->done) return;
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
@@ -1011,7 +1011,7 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
     const float4 pi = pos[ic];
     const float4 vi = vel[ic];
     const float ai = alpha[ic];
-    const uint32_t sn = stage_block(L, b, sn0, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
+    const uint32_t sn = stage_block<kStageMaxPV>(L, b, sn0, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
                                     [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
     float r = 0.0f;
     if (i < n && !is_source(L.rs, i)) r = radv[i];      // restricted out: stored value in the error sum
@@ -1025,7 +1025,7 @@ __global__ __launch_bounds__(kBlock) void k_div_source(SimConst c, uint32_t n, c
             const float mg = keep(valid && kp.ok, pj.w * kp.g);
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         };
-        fluid_nbrs(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+        fluid_nbrs<kStageMaxPV>(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
                    [&](uint32_t j) { return PV{pos[j], vel[j]}; },
                    [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
         for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, hd.b0, hd.b1,
@@ -1105,15 +1105,15 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t,
 // DFSPH_viscosityXSPH_kernel (DFSPH_core_cuda.cu:1423-1510):
 // a_i = g - sum_f (1/h) nu (m_j / rho_j) (v_i - v_j) W_ij, fused with the CFL
 // maximum of |v + a h|^2 (DFSPH_CFLVelSquaredNorm_kernel, :3273-3278).
-__global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, uint32_t n,
+__global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_xsph(SimConst c, const DevTime* t, uint32_t n,
                                                  const float4* __restrict__ pos, NbrLists L,
                                                  const float4* __restrict__ vel,
                                                  const float* __restrict__ rho, float4* __restrict__ acc,
                                                  DevCtl* ctl, float* partial)
 {
     __shared__ float lds4[kBlock / 64];
-    __shared__ float4 sP[kStageMax];
-    __shared__ float4 sV[kStageMax];           // (v, m / rho) of the staged set
+    __shared__ float4 sP[kStageMaxPV];
+    __shared__ float4 sV[kStageMaxPV];           // (v, m / rho) of the staged set
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t ic = i < n ? i : 0u;         // own loads before staging, so they overlap it
@@ -1121,7 +1121,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, u
     const Head hd = list_head(L, sn0, c.cap, ic);
     const float4 pi = pos[ic];
     const float4 vi = vel[ic];
-    const uint32_t sn = stage_block(L, b, sn0, [&](uint32_t g) {
+    const uint32_t sn = stage_block<kStageMaxPV>(L, b, sn0, [&](uint32_t g) {
         const float4 p = pos[g];
         const float4 v = vel[g];
         return PV{p, make_float4(v.x, v.y, v.z, p.w / rho[g])};
@@ -1144,7 +1144,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph(SimConst c, const DevTime* t, u
             const float s = keep(valid, coef * vj.w * w);
             ax -= s * (vi.x - vj.x); ay -= s * (vi.y - vj.y); az -= s * (vi.z - vj.z);
         };
-        fluid_nbrs(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+        fluid_nbrs<kStageMaxPV>(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
                    [&](uint32_t j) {
                        const float4 p = pos[j];
                        const float4 v = vel[j];
@@ -1277,7 +1277,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
 // Otherwise DFSPH_pressure_loop_end_kernel (:1264-1284) + FP64 error reduction
 // (replaces cub Sum + cudaMemcpy, :1321-1325).
 template <bool INIT, bool SOL>
-__global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const DevTime* t, uint32_t n,
+__global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_density_source(SimConst c, const DevTime* t, uint32_t n,
                                                            const float4* __restrict__ pos,
                                                            const float4* __restrict__ bpos, NbrLists L,
                                                            const float4* __restrict__ vel,
@@ -1288,8 +1288,8 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
 {
     if (INIT && ci.sc && blockIdx.x == 0 && threadIdx.x == 0) ctl_init(ci);
     __shared__ double lds4[kBlock / 64];
-    __shared__ float4 sP[kStageMax];
-    __shared__ float4 sV[kStageMax];
+    __shared__ float4 sP[kStageMaxPV];
+    __shared__ float4 sV[kStageMaxPV];
     if (!INIT && ctl->done) return;
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
@@ -1300,7 +1300,7 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
     const float4 vi = vel[ic];
     const float a0 = alpha[ic], rho_i = rho[ic];
     const float kap0 = INIT ? 0.0f : kappa[ic];
-    const uint32_t sn = stage_block(L, b, sn0, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
+    const uint32_t sn = stage_block<kStageMaxPV>(L, b, sn0, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
                                     [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
     float r = 0.0f;
     if (i < n && !is_source(L.rs, i)) r = radv[i];      // restricted out: stored value in the error sum
@@ -1315,7 +1315,7 @@ __global__ __launch_bounds__(kBlock) void k_density_source(SimConst c, const Dev
             const float mg = keep(valid && kp.ok, pj.w * kp.g);
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         };
-        fluid_nbrs(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+        fluid_nbrs<kStageMaxPV>(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
                    [&](uint32_t j) { return PV{pos[j], vel[j]}; },
                    [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
         (void)nf;
@@ -1411,7 +1411,7 @@ __global__ void k_finish_totals(const DevTime* t, uint32_t n, int pressure, cons
 //  6 boundary fraction n_b / n     7-9 mean neighbor offset / h_k
 // 10 mean neighbor distance / h_k 11 fluid share of the density
 template <bool SOL>
-__global__ __launch_bounds__(kBlock) void k_features(SimConst c, const DevTime* t, uint32_t n,
+__global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_features(SimConst c, const DevTime* t, uint32_t n,
                                                      const float4* __restrict__ pos,
                                                      const float4* __restrict__ bpos, NbrLists L,
                                                      const float4* __restrict__ vel,
@@ -1419,8 +1419,8 @@ __global__ __launch_bounds__(kBlock) void k_features(SimConst c, const DevTime* 
                                                      const float* __restrict__ kvtot, float diameter,
                                                      float* __restrict__ feat)
 {
-    __shared__ float4 sP[kStageMax];
-    __shared__ float4 sV[kStageMax];
+    __shared__ float4 sP[kStageMaxPV];
+    __shared__ float4 sV[kStageMaxPV];
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t ic = i < n ? i : 0u;         // own loads before staging, so they overlap it
@@ -1428,7 +1428,7 @@ __global__ __launch_bounds__(kBlock) void k_features(SimConst c, const DevTime* 
     const Head hd = list_head(L, sn0, c.cap, ic);
     const float4 pi = pos[ic];
     const float4 vi = vel[ic];
-    const uint32_t sn = stage_block(L, b, sn0, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
+    const uint32_t sn = stage_block<kStageMaxPV>(L, b, sn0, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
                                     [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
     if (i >= n) return;
     const uint32_t cn = hd.cnt;
@@ -1446,7 +1446,7 @@ __global__ __launch_bounds__(kBlock) void k_features(SimConst c, const DevTime* 
         ox -= m * dx; oy -= m * dy; oz -= m * dz;
         od += m * __builtin_sqrtf(r2);
     };
-    fluid_nbrs(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+    fluid_nbrs<kStageMaxPV>(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
                [&](uint32_t j) { return PV{pos[j], vel[j]}; },
                [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
     float dbd = 0.0f;

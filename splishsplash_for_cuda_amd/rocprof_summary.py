@@ -21,17 +21,18 @@ from collections import defaultdict
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
-# bench.py sweep name -> device symbol prefix
+# bench.py sweep name -> device symbol prefix (the last template argument of
+# the sweeps selects the rigid-body variant; both count)
 SWEEPS = {
     "div_ws_init": "void dfsph::k_div_ws_init<",
-    "div_init": "void dfsph::k_div_source<true>",
-    "div_jacobi": "void dfsph::k_jacobi<true>",
-    "div_loop_end": "void dfsph::k_div_source<false>",
+    "div_init": "void dfsph::k_div_source<true,",
+    "div_jacobi": "void dfsph::k_jacobi<true,",
+    "div_loop_end": "void dfsph::k_div_source<false,",
     "xsph": "dfsph::k_xsph(",
     "pressure_ws": "void dfsph::k_pressure_ws<",
-    "dens_init": "void dfsph::k_density_source<true>",
-    "dens_jacobi": "void dfsph::k_jacobi<false>",
-    "dens_loop_end": "void dfsph::k_density_source<false>",
+    "dens_init": "void dfsph::k_density_source<true,",
+    "dens_jacobi": "void dfsph::k_jacobi<false,",
+    "dens_loop_end": "void dfsph::k_density_source<false,",
     "neighbors": "void dfsph::k_neighbors<",
 }
 COPY_BYTES = 256 * 1024 * 1024
