@@ -68,8 +68,9 @@ def test_ctypes_struct_layouts_match_the_header(tmp_path):
     if cc is None:
         pytest.skip("no C compiler")
     structs = {"dfsph_params": _native.DFSPHParams, "dfsph_info": _native.DFSPHInfo,
-               "dfsph_step_stats": _native.DFSPHStepStats, "dfsph_rigid_body": _native.DFSPHRigidBody}
-    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dfsph_hip.h"', "int main(void) {"]
+               "dfsph_step_stats": _native.DFSPHStepStats, "dfsph_rigid_body": _native.DFSPHRigidBody,
+               "dfsph_surface": _native.DFSPHSurface, "dfsph_cuda_stats": _native.DFSPHCudaStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dfsph_host.h"', "int main(void) {"]
     for name, cls in structs.items():
         lines.append('printf("%s %%zu\\n", sizeof(%s));' % (name, name))
         for f in cls._fields_:
@@ -85,3 +86,56 @@ def test_ctypes_struct_layouts_match_the_header(tmp_path):
         assert int(got[name]) == ctypes.sizeof(cls), name
         for f in cls._fields_:
             assert int(got["%s.%s" % (name, f[0])]) == getattr(cls, f[0]).offset, (name, f[0])
+
+
+C_CONSUMER = r"""
+#include <stdio.h>
+#include <stdlib.h>
+#include "dfsph_host.h"
+/* a plain C caller of the host library: scene file -> parameters and
+ * particles, then the text particle format (no GPU call) */
+int main(int argc, char** argv) {
+    void* sc = dfsph_scene_read(argv[1], NULL);
+    if (!sc) { printf("error %s\n", dfsph_cuda_last_error()); return 1; }
+    dfsph_params p;
+    dfsph_scene_params(sc, &p);
+    uint32_t nf = dfsph_scene_fluid_count(sc), nb = dfsph_scene_boundary_count(sc);
+    float* x = (float*)malloc(sizeof(float) * 3 * nf);
+    float* v = (float*)malloc(sizeof(float) * 3 * nf);
+    float* m = (float*)malloc(sizeof(float) * nf);
+    dfsph_scene_fluid(sc, x, v);
+    for (uint32_t i = 0; i < nf; ++i) m[i] = 0.1f;
+    if (dfsph_particles_write_text(argv[2], x, v, m, nf, 1, 1) != 0) return 2;
+    int64_t back = dfsph_particles_read_text(argv[2], NULL, NULL, NULL, 0);
+    printf("%g %u %u %lld %g %g\n", p.particle_radius, nf, nb, (long long)back, x[0], x[3 * nf - 1]);
+    dfsph_scene_free(sc);
+    free(x); free(v); free(m);
+    return 0;
+}
+"""
+
+
+def test_c_program_uses_the_host_library(tmp_path):
+    """dfsph_host.h from plain C, linked against the built libraries."""
+    import shutil
+    import subprocess
+    from splishsplash_for_cuda_amd import _native
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    _native.load_host()
+    pkg = os.path.join(ROOT, "splishsplash_for_cuda_amd")
+    (tmp_path / "c.c").write_text(C_CONSUMER)
+    exe = tmp_path / "c"
+    subprocess.run([cc, "-std=c99", "-Wall", "-I", os.path.join(ROOT, "include"), str(tmp_path / "c.c"),
+                    "-L", pkg, "-ldfsph_host", "-ldfsph_hip", "-Wl,-rpath," + pkg, "-o", str(exe)], check=True)
+    (tmp_path / "box.obj").write_text("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nv 0 0 1\nv 1 0 1\nv 1 1 1\nv 0 1 1\n"
+                                      "f 1 4 3 2\nf 5 6 7 8\nf 1 2 6 5\nf 4 8 7 3\nf 1 5 8 4\nf 2 3 7 6\n")
+    (tmp_path / "s.json").write_text('{"Configuration": {"particleRadius": 0.025}, "RigidBodies": '
+                                     '[{"geometryFile": "box.obj"}], "FluidBlocks": [{"denseMode": 0, '
+                                     '"start": [0, 0, 0], "end": [0.5, 0.5, 0.5]}]}')
+    out = subprocess.run([str(exe), str(tmp_path / "s.json"), str(tmp_path / "f.txt")], check=True,
+                         capture_output=True, text=True).stdout.split()
+    r, nf, nb, back, x0, xl = float(out[0]), int(out[1]), int(out[2]), int(out[3]), float(out[4]), float(out[5])
+    assert r == 0.025 and nf == 9 ** 3 and nb == 21 ** 3 - 19 ** 3 and back == nf
+    assert x0 == pytest.approx(0.05) and xl == pytest.approx(0.05 + 8 * 0.05)
