@@ -148,10 +148,10 @@ def _resync(s, o, om):
         s.upload_field(fs, v32[idx])
 
 
-def _phase_compare(om, sc, dt, steps, report, resync=False):
-    o = _oracle(om, sc, dt=dt)
+def _phase_compare(om, sc, dt, steps, report, resync=False, **kw):
+    o = _oracle(om, sc, dt=dt, **kw)
     psi = o.boundary_psi(sc.n_boundary)
-    s = _solver(sc, psi=psi.astype(np.float32), dt=dt)
+    s = _solver(sc, psi=psi.astype(np.float32), dt=dt, **kw)
     rho0, g = 1000.0, 9.81
     worst = {}
     for k in range(steps):
@@ -199,6 +199,24 @@ def _phase_compare(om, sc, dt, steps, report, resync=False):
             worst[kk] = max(worst.get(kk, 0.0), v)
         report.append((k, itv_g, it_g, e))
     return worst
+
+
+def test_config2_first_steps_match_oracle(oracle_mod):
+    """The 1M-particle dam break (1,048,576 fluid + 600,801 boundary
+    particles), first 20 steps against the FP64 oracle, each step started by
+    both sides from the oracle's state: the one-step tolerances of the C1
+    trajectory hold at full size."""
+    sc = scene.config2()
+    report = []
+    w = _phase_compare(oracle_mod, sc, 0.001, 20, report, resync=True, max_neighbors=128)
+    for row in report[::5]:
+        print(row)
+    print("C2 20-step worst relative errors:", w)
+    assert w["density"] < 2e-6
+    assert w["factor"] < 1e-4
+    assert w["vel_frac"] < 0.01 and w["vel"] < 1e-2
+    # positions: one FP32 ulp of x += h v at |x| up to 6 m is 4.8e-5 r (r = 1 cm)
+    assert w["pos"] < 5e-5
 
 
 def test_one_step_phases_match_oracle(oracle_mod):
