@@ -893,7 +893,9 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_ws_init(SimConst
     const float ratio = t->ratio, h = t->h, inv_h = t->inv_h;
     auto ws_val = [&](float raw) -> float {
         if (WS == 0) return fmaxf(raw * ratio / 2.0f, -0.5f);
-        if (WS == 1) return fmaxf(raw * inv_h, -0.5f);
+        // learned: clamped like the reference below and at 0 above, the range
+        // of the converged total (a sum of rho_adv alpha <= 0 terms)
+        if (WS == 1) return fminf(fmaxf(raw * inv_h, -0.5f), 0.0f);
         return 0.0f;
     };
     auto ws_of = [&](uint32_t j) -> float { return WS == 2 ? 0.0f : ws_val(kws[j]); };
@@ -1222,7 +1224,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
     const float ihf2 = t->inv_h_future * t->inv_h_future;
     auto ws_of = [&](uint32_t j) -> float {
         if (WS == 0) return fmaxf(kws[j] * ratio2, -0.5f);
-        return fmaxf(kws[j] * ihf2, -0.5f);
+        return fminf(fmaxf(kws[j] * ihf2, -0.5f), 0.0f);        // learned: see k_div_ws_init
     };
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
