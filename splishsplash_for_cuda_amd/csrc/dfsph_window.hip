@@ -150,8 +150,11 @@ __global__ __launch_bounds__(kBlock) void k_inflow(SimConst c, const float4* __r
     if (k == nc) { keep[nc] = 0; return; }
     const float4 p = cand[k];
     const int cx = cell_coord(c, p.x, 0), cy = cell_coord(c, p.y, 1), cz = cell_coord(c, p.z, 2);
+    // nearest fluid particle within h; the scan stops once one is closer
+    // than min_dist (the candidate is then rejected whatever the minimum)
     float dmin = 100.0f;
     for_each_adjacent_cell(c, cx, cy, cz, [&](uint32_t nk) {
+        if (dmin <= min_dist) return;
         for (uint32_t q = cs_f[nk]; q < cs_f[nk + 1]; ++q) {
             const float4 pj = pos[perm[q]];
             const float r2 = dist2(p.x - pj.x, p.y - pj.y, p.z - pj.z);
