@@ -377,7 +377,13 @@ int dfsph_rigid_body_count(void* handle, uint32_t* n);
  * No reference counterpart: these entry points add a per-particle MLP
  * (12 -> 64 -> 64 -> 64 -> 2, SiLU; BF16 weights on the matrix cores, FP32
  * accumulate) that predicts the warm starts the reference forms at
- * DFSPH_core_cuda.cu:240 (kappaV) and :3063-3068 (kappa).
+ * DFSPH_core_cuda.cu:240 (kappaV) and :3063-3068 (kappa).  The network's two
+ * outputs are corrections to those reference warm starts in dt-invariant units:
+ * the step's converged totals (kappaV h, kappa h_f^2) less (f5 / 2, f4), where
+ * f4 and f5 are the features that hold the previous step's totals
+ * (kappa h_f^2, kappaV h); the learned warm start adds them back, and its
+ * divergence part asks for no more pressure than the reference's
+ * (max(y0 + f5 / 2, f5 / 2), kappaV <= 0).
  * Parameters use one flat FP32 layout of DFSPH_MLP_NPARAM values:
  *   W1[64x12] b1[64] W2[64x64] b2[64] W3[64x64] b3[64] W4[2x64] b4[2]
  * (row-major, out x in); norm holds in_mean[12] in_std[12] out_mean[2] out_std[2]. */
@@ -395,7 +401,8 @@ int dfsph_mlp_loss_grad(void* handle, const float* x, const float* t, uint32_t n
                         int normalise);
 /* Training set harvest: while capacity is not reached, every step (dfsph_step or
  * the phase calls) appends each stride-th particle's 12 features (computed after
- * the neighbor search) and the step's converged totals (kappaV h, kappa h_f^2).
+ * the neighbor search) and the network's targets: the step's converged totals
+ * (kappaV h, kappa h_f^2) less (f5 / 2, f4).
  * capacity 0 stops harvesting. */
 int dfsph_mlp_harvest(void* handle, uint32_t capacity, uint32_t stride);
 int dfsph_mlp_dataset_size(void* handle, uint32_t* n);
@@ -407,7 +414,9 @@ int dfsph_mlp_fit_normalization(void* handle);
 /* mini-batch Adam (batch a multiple of 64) on the GPU's training kernels */
 int dfsph_mlp_train(void* handle, uint32_t steps, uint32_t batch, float lr, uint32_t seed, float* loss_first,
                     float* loss_last);
-/* features and predictions of the current step (device order, n particles) */
+/* features and predictions of the current step (device order, n particles);
+ * the predictions are the warm starts the solver used: MLP output y plus the
+ * reference warm start, (max(y0 + f5 / 2, f5 / 2), y1 + f4) */
 int dfsph_mlp_step_features(void* handle, float* feat, float* pred_v, float* pred_p, uint32_t n);
 
 #ifdef __cplusplus

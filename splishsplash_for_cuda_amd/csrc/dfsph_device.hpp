@@ -774,4 +774,27 @@ __device__ __forceinline__ double block_sum(double v, double* lds4)
     return t;   // valid in thread 0
 }
 
+// The learned warm start predicts a correction to the reference warm start, in
+// the dt-invariant units (kappaV h, kappa h_f^2).  The reference forms its warm
+// starts from the previous step's totals, which are features 5 (kappaV h) and
+// 4 (kappa h_f^2) of a particle: kappaV h / 2 for the divergence solver
+// (DFSPH_core_cuda.cu:240) and kappa h_f^2 for the density solver (:3063-3068),
+// before the rescale to the new step and the clamp at -0.5 that the sweeps apply.
+constexpr int kFeatKappaP = 4, kFeatKappaV = 5;
+__host__ __device__ __forceinline__ float ws_base(const float* feat_row, int out)
+{
+    return out == 0 ? 0.5f * feat_row[kFeatKappaV] : feat_row[kFeatKappaP];
+}
+
+// The warm start from the network's output y: the reference warm start plus the
+// correction, where the divergence warm start asks for no more pressure than
+// the particle's previous divergence total (kappaV <= 0; at rest that total is
+// near zero, and a stronger start pushes the particles apart, which the
+// divergence error, a sum of compressions only, does not see).
+__host__ __device__ __forceinline__ float ws_learned(const float* feat_row, int out, float y)
+{
+    const float w = y + ws_base(feat_row, out);
+    return out == 0 ? fmaxf(w, ws_base(feat_row, 0)) : w;
+}
+
 }  // namespace dfsph

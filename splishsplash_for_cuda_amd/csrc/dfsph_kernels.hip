@@ -1474,8 +1474,8 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_features(SimConst c,
     f[1] = dr * t->h / c.density0;
     f[2] = nn * (1.0f / 40.0f);
     f[3] = __builtin_sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z) * t->h / diameter;
-    f[4] = ktot[i];
-    f[5] = kvtot[i];
+    f[kFeatKappaP] = ktot[i];
+    f[kFeatKappaV] = kvtot[i];
     f[6] = (float)nb * inv_n;
     f[7] = ox * s;
     f[8] = oy * s;
@@ -1484,7 +1484,9 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_features(SimConst c,
     f[11] = dfl / dens;
 }
 
-// training set: append the features of the step and the step's totals
+// training set: append the features of the step and the step's totals less
+// the reference warm start formed from the features (ws_base), the correction
+// the network learns
 __global__ void k_harvest(uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap, const float* feat,
                           const float* kvtot, const float* ktot, float* ds_x, float* ds_t)
 {
@@ -1492,9 +1494,10 @@ __global__ void k_harvest(uint32_t n, uint32_t stride, uint32_t offset, uint32_t
     const uint32_t i = k * stride;
     if (i >= n || offset + k >= cap) return;
     const size_t o = offset + k;
-    for (int a = 0; a < 12; ++a) ds_x[o * 12 + a] = feat[(size_t)i * 12 + a];
-    ds_t[o * 2 + 0] = kvtot[i];
-    ds_t[o * 2 + 1] = ktot[i];
+    const float* f = feat + (size_t)i * 12;
+    for (int a = 0; a < 12; ++a) ds_x[o * 12 + a] = f[a];
+    ds_t[o * 2 + 0] = kvtot[i] - ws_base(f, 0);
+    ds_t[o * 2 + 1] = ktot[i] - ws_base(f, 1);
 }
 
 // ------------------------------------------------------------------------

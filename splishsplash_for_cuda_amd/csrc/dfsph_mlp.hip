@@ -4,7 +4,10 @@
 //
 // A per-particle MLP 12 -> 64 -> 64 -> 64 -> 2 with SiLU activations predicts
 // the converged (kappaV h, kappa h_f^2) of a step, the dt-invariant units of
-// the reference CPU path (TimeStepDFSPH.cpp:458-459, 659-660).  Weights are
+// the reference CPU path (TimeStepDFSPH.cpp:458-459, 659-660), as a correction
+// to the reference warm start formed from the previous step's totals
+// (ws_base): the network's outputs are the totals less that warm start, and
+// the solver's prediction adds it back.  Weights are
 // BF16 on the matrix cores (v_mfma_f32_16x16x16_bf16, FP32 accumulate);
 // biases, activations between layers and all reductions are FP32.
 //
@@ -148,7 +151,8 @@ __device__ void forward_wave(const DevModel& M, const WaveBufs& w, float out[4][
 
 // ---------------------------------------------------------------- inference
 // x: [n][12] raw features (FP32); output o of row r goes to y[r * ystride + o * ocol]
-// (denormalised when normalise is set).
+// (denormalised when normalise is set; plus the reference warm start of the
+// row's features when residual is set, the solver's prediction).
 //
 // Register-resident form: each layer is computed transposed, H_out^T = W H_in^T,
 // with W as the MFMA A operand and the particles as the N dimension.  The
@@ -192,7 +196,7 @@ __device__ __forceinline__ bf16x4 act_frag(const f32x4& z)
 
 __global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const DevModel* __restrict__ Mg, const float* __restrict__ x,
                                                            uint32_t n, float* __restrict__ y, int normalise,
-                                                           int ystride, int ocol)
+                                                           int ystride, int ocol, int residual)
 {
     __shared__ __align__(16) float sb[3 * kH + kOutPad];
     const DevModel& M = *Mg;
@@ -259,6 +263,7 @@ __global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const DevModel* __res
             for (int o = 0; o < kOut; ++o) {
                 float val = acc[o];
                 if (normalise) val = val * M.norm[2 * kIn + kOut + o] + M.norm[2 * kIn + o];
+                if (residual) val = ws_learned(x + (size_t)p * kIn, o, val);
                 y[(size_t)p * ystride + (size_t)o * ocol] = val;
             }
         }
@@ -598,7 +603,7 @@ int dfsph_mlp_forward(void* handle, const float* x, uint32_t n, float* y, int no
     if (!rc) {
         (void)hipMemcpyAsync(dx, x, sizeof(float) * n * dfsph::kIn, hipMemcpyHostToDevice, s);
         dfsph::k_mlp_infer<<<dfsph::infer_grid(n), dfsph::kInferBlock, 0, s>>>(m->model, dx, n, dy, normalise,
-                                                                                  dfsph::kOut, 1);
+                                                                                  dfsph::kOut, 1, 0);
         (void)hipMemcpyAsync(y, dy, sizeof(float) * n * dfsph::kOut, hipMemcpyDeviceToHost, s);
         if (hipStreamSynchronize(s) != hipSuccess) rc = mfail(DFSPH_ERR_HIP, "mlp forward");
     }
@@ -663,7 +668,7 @@ int mlp_ensure_particles(MlpState* m, uint32_t cap)
 int mlp_infer_particles(MlpState* m, hipStream_t s, uint32_t n)
 {
     if (n == 0) return DFSPH_OK;
-    k_mlp_infer<<<infer_grid(n), kInferBlock, 0, s>>>(m->model, m->feat, n, m->pred, 1, 1, (int)m->cap);
+    k_mlp_infer<<<infer_grid(n), kInferBlock, 0, s>>>(m->model, m->feat, n, m->pred, 1, 1, (int)m->cap, 1);
     MHIP(hipGetLastError());
     return DFSPH_OK;
 }

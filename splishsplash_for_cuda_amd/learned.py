@@ -2,7 +2,13 @@
 
 The network (12 -> 64 -> 64 -> 64 -> 2, SiLU) predicts each particle's
 converged (kappaV h, kappa h_f^2) of a step from features of the step's
-initial state (see k_features in csrc/dfsph_kernels.hip).  Targets come from
+initial state (see k_features in csrc/dfsph_kernels.hip), as a correction to
+the reference warm start: its targets are the converged totals less
+(f5 / 2, f4), the reference's warm starts formed from the previous step's
+totals that features 4 and 5 hold, and the solver adds those back to the
+network's outputs.  A network that outputs zero is therefore the reference
+warm start, and at rest, where the totals repeat from step to step, the
+targets are near zero.  Targets come from
 the plain solver, which keeps iterating to the same tolerances when the
 learned warm start is used: the network only changes where the Jacobi
 iterations start.
@@ -51,7 +57,8 @@ def make_solver(sc, params):
 
 
 def harvest_scene(sc, steps, max_samples, **kw):
-    """Run the plain solver on one scene and collect (features, totals)."""
+    """Run the plain solver on one scene and collect (features, targets): the
+    targets are the converged totals less the reference warm start."""
     s = make_solver(sc, solver_params(sc, **kw))
     stride = max(1, int(np.ceil(sc.n_fluid * steps / max_samples)))
     s.mlp_harvest(max_samples, stride)

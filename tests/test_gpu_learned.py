@@ -1,12 +1,14 @@
 """The learned warm start end to end on the GPU: the committed model cuts the
 divergence iterations on a scene outside its training set while the solver
-still meets the same tolerances, and GPU training lowers the loss.
+still meets the same tolerances, the warm starts a step uses are the network's
+corrections added to the reference warm start, and GPU training lowers the loss.
 
 Training scenes are seeded_dam_break(k, ...) for k < 32; the held-out scene
 here uses k = 77.  Bounds:
-  iterations  learned divergence Jacobi iterations per step <= 0.7 x those of
-              the reference warm start at iteration floors (1, 1); the plain
-              solver measured 5.2 -> 1.0 when the model was trained
+  iterations  learned divergence Jacobi iterations per step <= 0.9 x those of
+              the reference warm start at iteration floors (1, 1); measured
+              5.15 -> 4.01 on the evaluation scene (seed 99, 100k particles)
+              when the model was trained
   tolerance   every step's final errors stay within the 0.1 % targets
               (divergence error as rho_dot h / rho0)
   state       mean |rho_learned - rho_reference| / rho0 after the run within
@@ -41,11 +43,35 @@ def test_trained_model_cuts_divergence_iterations_on_a_new_scene():
         ref["iters_v"], ref["iters"], lrn["iters_v"], lrn["iters"], none["iters_v"], none["iters"]))
     print("density difference to the reference: learned %.3f %%, none %.3f %%" % (
         lrn["rho_diff_pct"], none["rho_diff_pct"]))
-    assert lrn["iters_v"] <= 0.7 * ref["iters_v"]
+    assert lrn["iters_v"] <= 0.9 * ref["iters_v"]
     assert lrn["iters"] <= ref["iters"] + 0.5
     for r in out.values():
         assert r["err_v"] <= TOL * 1.0001 and r["err"] <= TOL * 1.0001
     assert lrn["rho_diff_pct"] <= 3.0 * none["rho_diff_pct"]
+
+
+def test_step_warm_start_is_reference_plus_correction():
+    """The warm starts of a learned step, read back with the step's features,
+    equal the device forward pass of those features plus the reference warm
+    start they hold (f5 / 2 for the divergence solver, f4 for the density
+    solver), the divergence one no stronger than the reference's (both are
+    kappa <= 0 totals).  Same kernel arithmetic on both sides: exact."""
+    from splishsplash_for_cuda_amd._native import WARMSTART_LEARNED
+    sc = scene.seeded_dam_break(5, 20_000)
+    params, norm = learned.load()
+    s = learned.make_solver(sc, learned.solver_params(sc, warm_start=WARMSTART_LEARNED))
+    s.mlp_set_params(params, norm)
+    for _ in range(6):
+        s.step()
+    f, pv, pp = s.mlp_step_features()
+    y = s.mlp_forward(f, normalise=True)
+    s.close()
+    bv = np.float32(0.5) * f[:, 5]
+    bp = f[:, 4]
+    assert (bp < 0).mean() > 0.5 and (bv < 0).mean() > 0.1        # warm starts in play
+    np.testing.assert_array_equal(pv, np.maximum(y[:, 0] + bv, bv))
+    np.testing.assert_array_equal(pp, y[:, 1] + bp)
+    assert (pv > bv).any() and (pv == bv).any()                    # the bound is active somewhere, not everywhere
 
 
 def test_gpu_training_lowers_the_loss():

@@ -445,19 +445,28 @@ def test_cfl_matches_oracle(oracle_mod):
     assert hg == pytest.approx(ho, rel=1e-5)
 
 
-def test_water_at_rest_stays_at_rest():
+@pytest.mark.parametrize("ws", ["reference", "learned"])
+def test_water_at_rest_stays_at_rest(ws):
     """Still water in a closed tank (1 x 0.8 x 1 m, 0.3 m deep, r = 12.5 mm)
-    for 1 s with the reference warm start: the largest speed stays below
-    1 m/s while the lattice start settles (measured 0.5) and below 0.4 m/s
-    over the last half second, and every step meets the tolerances."""
+    for 1 s: the largest speed stays below 1 m/s while the lattice start
+    settles and below 0.4 m/s over the last half second with the reference
+    warm start (measured 0.50 and 0.36), below 2 and 1 m/s with the committed
+    learned model (measured 1.16 and 0.66), and every step meets the
+    tolerances.  The water is outside the learned model's training set (dam
+    breaks)."""
     r = 0.0125
     d = 2 * r
     walls = scene.box_surface((0, 0, 0), (1.0, 0.8, 1.0), d).astype(np.float32)
     fl = scene.fluid_block((d, d, d), (39, 12, 39), d).astype(np.float32)
-    p = S.default_params(particle_radius=r, dt=0.001, use_cfl=1, dt_max=0.002, max_error=0.05, max_error_v=0.1)
+    lw = ws == "learned"
+    p = S.default_params(particle_radius=r, dt=0.001, use_cfl=1, dt_max=0.002, max_error=0.05, max_error_v=0.1,
+                         warm_start_mode=S.WARMSTART_LEARNED if lw else S.WARMSTART_REFERENCE)
     s = S.Solver(p)
     s.upload_boundary(walls)
     s.upload_fluid(fl)
+    if lw:
+        from splishsplash_for_cuda_amd import learned
+        learned.attach(s)
     t, vmax = 0.0, []
     while t < 1.0:
         st = s.step()
@@ -466,5 +475,6 @@ def test_water_at_rest_stays_at_rest():
         vmax.append(float(np.abs(s.download(S.FIELD_VEL)).max()))
     late = vmax[len(vmax) // 2:]
     print("max speed over 1 s: %.3f m/s, over the last 0.5 s %.3f (last %.3f)" % (max(vmax), max(late), vmax[-1]))
-    assert max(vmax) < 1.0 and max(late) < 0.4 and st.n_fluid == len(fl)
+    bound, bound_late = (2.0, 1.0) if lw else (1.0, 0.4)
+    assert max(vmax) < bound and max(late) < bound_late and st.n_fluid == len(fl)
     s.close()

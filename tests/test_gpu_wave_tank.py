@@ -14,9 +14,9 @@ piston stroke 0.1 m at 1 s; linear piston theory (Biesel) gives waves
   * the absorbing end removes crests and refills troughs (both counts > 0)
     and the fluid count stays within 10 % of the start;
   * every step meets the solver tolerances.
-The learned warm start is not run here: the committed model, trained on dam
-breaks, destabilises this tank (a moving wall and waves are outside its
-training set).
+Both warm starts run: the reference one, and the committed learned model,
+trained on dam breaks only (a moving wall, waves and a floating body are
+outside its training set).
 """
 import math
 
@@ -29,7 +29,8 @@ from splishsplash_for_cuda_amd import wave_tank as W
 pytestmark = pytest.mark.gpu
 
 
-def test_small_wave_tank(learned=False):
+@pytest.mark.parametrize("learned", [False, True], ids=["reference", "learned"])
+def test_small_wave_tank(learned):
     A, T = 0.05, 1.0
     t = W.build(r=0.015, length=4.0, height=1.0, width=0.6, depth=0.3, amplitude=A, period=T, absorb=0.5,
                 box_side=0.24)
