@@ -7,19 +7,16 @@ Training scenes are seeded_dam_break(k, ...) for k < 32; the held-out scene
 here uses k = 77.  Bounds:
   iterations  learned divergence Jacobi iterations per step <= 0.9 x those of
               the reference warm start at iteration floors (1, 1); measured
-              5.15 -> 4.01 on the evaluation scene (seed 99, 100k particles)
-              when the model was trained
+              7.60 -> 6.17 here, 5.15 -> 4.01 on the evaluation scene (seed
+              99, 100k particles) when the model was trained
   tolerance   every step's final errors stay within the 0.1 % targets
               (divergence error as rho_dot h / rho0)
   state       mean |rho_learned - rho_reference| / rho0 after the run within
               3 x that of the run without warm start.  Runs to the same
               tolerances differ by up to the tolerance each step, which the
-              free surface of a dam break amplifies over the steps; a solve
-              that starts close to the solution stops just under the tolerance
-              after one iteration, while the others overshoot it, so the
-              learned run drifts further (measured 3.9 % against 2.0 % for no
-              warm start over these 40 steps) while every step meets the
-              targets
+              free surface of a dam break amplifies over the steps (measured
+              1.3 % for the learned run, 1.9 % for no warm start over these
+              40 steps)
   training    loss of the last of 400 Adam steps (batch 2048) < 0.5 x the loss
               of the first (fresh network, targets standardised)
 """
