@@ -105,6 +105,36 @@ def cpu_baseline(sc, budget_s, max_steps, tank=None):
                       "TimeStepDFSPH with OpenMP (%.1f s)" % (steps, sc.n_fluid, t)}
 
 
+def learned_leg(sc, p, tank, steps, warmup):
+    """The same workload through SPH::DFSPHCUDA::step() with the committed
+    learned warm start: ms per step, throughput and iterations per step, run
+    after the plain solver is closed (untimed setup, the same warm-up)."""
+    from splishsplash_for_cuda_amd import learned
+    p.warm_start_mode = S.WARMSTART_LEARNED
+    if tank is not None:
+        from splishsplash_for_cuda_amd import wave_tank
+        hs = wave_tank.make_solver(tank, p, learned=True)
+    else:
+        hs = S.HostSolver(p, sc.boundary, sc.fluid)
+        learned.attach(hs.view)
+    for _ in range(warmup + 5):
+        hs.step()
+    inf0 = hs.view.info()
+    iv, ip = [], []
+    hs.view.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = hs.step()
+        iv.append(st.iters_v)
+        ip.append(st.iters)
+    hs.view.synchronize()
+    elapsed = time.perf_counter() - t0
+    inf1 = hs.view.info()
+    hs.close()
+    return {"ms_per_step": 1e3 * elapsed / steps, "value": n_f_mean_over(inf0, inf1) * steps / elapsed,
+            "iters_per_step": {"divergence_mean": float(np.mean(iv)), "density_mean": float(np.mean(ip))}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +149,8 @@ def main():
                     help="minimum divergence,density Jacobi iterations (the reference GPU solver's are 3,2)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned-leg", action="store_true",
+                    help="plain mode: skip the learned run of the same workload reported beside it")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the solver is single-GPU (multi-GPU slab decomposition is "
@@ -244,13 +276,19 @@ def main():
         },
         "kernel_share": {k: round(v[1] / max(sum(x[1] for x in prof.values()), 1e-9), 4) for k, v in prof.items()},
     }
+    if args.path == "host":
+        hs.close()
+    else:
+        solver.close()
+    if args.mode == "plain" and args.path == "host" and not args.no_learned_leg:
+        lrn = learned_leg(sc, p, tank, args.steps, args.warmup)
+        lrn["speedup_vs_plain"] = line["ms_per_step"] / lrn["ms_per_step"]
+        line["learned_vs_plain"] = lrn
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(sc, args.cpu_budget, 1000, tank)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
-    if args.path == "host":
-        hs.close()
 
 
 if __name__ == "__main__":
