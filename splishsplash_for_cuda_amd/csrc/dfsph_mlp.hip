@@ -160,7 +160,8 @@ __device__ void forward_wave(const DevModel& M, const WaveBufs& w, float out[4][
 // is exactly the B operand layout of the next layer's matching K block, so
 // activations never leave registers.  The weights' A fragments are loaded
 // once into registers and each wave runs over 16-particle tiles; biases come
-// from LDS as accumulator seeds.
+// from LDS as accumulator seeds.  Launch bounds ask for 3 waves per SIMD (168
+// VGPRs, no spills; the default allocation held 2, and 4 spill).
 constexpr int kInferBlock = 256;
 
 __device__ __forceinline__ bf16x4 frag8(const unsigned short* p)     // 4 consecutive bf16 (8-byte aligned)
@@ -194,7 +195,7 @@ __device__ __forceinline__ bf16x4 act_frag(const f32x4& z)
     return pack_bf16(silu_fast(z[0]), silu_fast(z[1]), silu_fast(z[2]), silu_fast(z[3]));
 }
 
-__global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const DevModel* __restrict__ Mg, const float* __restrict__ x,
+__global__ __launch_bounds__(kInferBlock, 3) void k_mlp_infer(const DevModel* __restrict__ Mg, const float* __restrict__ x,
                                                            uint32_t n, float* __restrict__ y, int normalise,
                                                            int ystride, int ocol, int residual)
 {
