@@ -69,9 +69,13 @@ def build(r=0.008, length=10.0, height=2.0, width=1.0, depth=0.9, amplitude=0.1,
     piston = dict(type=N.BODY_SINE, position=(px, d + plate_c[1], 2 * d + plate_c[2]), axis=(1.0, 0.0, 0.0),
                   amplitude=amplitude, period=period)
     plate = plate - plate_c
-    x0 = px + amplitude + 1.5 * d
-    # jittered water block from just in front of the piston's farthest place
-    counts = (int((length - x0 - d) / d), int(round(depth / d)), int((width - d) / d) - 1)
+    # jittered water block from one spacing in front of the plate at its rest
+    # place to one spacing from the far wall, and to 1-1.5 spacings from the
+    # side walls (wider gaps let a deep column collapse into them and throw
+    # jets up the walls)
+    counts = (int(np.floor((length - d - (px + 1.5 * d)) / d + 1e-6)) + 1, int(round(depth / d)),
+              int(np.floor(width / d - 2.0 + 1e-6)) + 1)
+    x0 = length - d - (counts[0] - 1) * d
     rng = np.random.default_rng(seed)
     fl = fluid_block((x0, d, d), counts, d)
     if jitter:
