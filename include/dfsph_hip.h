@@ -381,9 +381,9 @@ int dfsph_rigid_body_count(void* handle, uint32_t* n);
  * outputs are corrections to those reference warm starts in dt-invariant units:
  * the step's converged totals (kappaV h, kappa h_f^2) less (f5 / 2, f4), where
  * f4 and f5 are the features that hold the previous step's totals
- * (kappa h_f^2, kappaV h); the learned warm start adds them back, and its
- * divergence part asks for no more pressure than the reference's
- * (max(y0 + f5 / 2, f5 / 2), kappaV <= 0).
+ * (kappa h_f^2, kappaV h); the learned warm start adds them back and asks
+ * for no more pressure than the reference's: (max(y0 + f5 / 2, f5 / 2),
+ * max(y1 + f4, f4)), kappa and kappaV <= 0.
  * Parameters use one flat FP32 layout of DFSPH_MLP_NPARAM values:
  *   W1[64x12] b1[64] W2[64x64] b2[64] W3[64x64] b3[64] W4[2x64] b4[2]
  * (row-major, out x in); norm holds in_mean[12] in_std[12] out_mean[2] out_std[2]. */
@@ -416,7 +416,7 @@ int dfsph_mlp_train(void* handle, uint32_t steps, uint32_t batch, float lr, uint
                     float* loss_last);
 /* features and predictions of the current step (device order, n particles);
  * the predictions are the warm starts the solver used: MLP output y plus the
- * reference warm start, (max(y0 + f5 / 2, f5 / 2), y1 + f4) */
+ * reference warm start, (max(y0 + f5 / 2, f5 / 2), max(y1 + f4, f4)) */
 int dfsph_mlp_step_features(void* handle, float* feat, float* pred_v, float* pred_p, uint32_t n);
 
 #ifdef __cplusplus

@@ -787,14 +787,13 @@ __host__ __device__ __forceinline__ float ws_base(const float* feat_row, int out
 }
 
 // The warm start from the network's output y: the reference warm start plus the
-// correction, where the divergence warm start asks for no more pressure than
-// the particle's previous divergence total (kappaV <= 0; at rest that total is
-// near zero, and a stronger start pushes the particles apart, which the
-// divergence error, a sum of compressions only, does not see).
+// correction, bounded to ask for no more pressure than the reference warm
+// start (kappa, kappaV <= 0).  A stronger start pushes the particles apart,
+// which the solvers' errors, sums of compressions only, do not see.
 __host__ __device__ __forceinline__ float ws_learned(const float* feat_row, int out, float y)
 {
     const float w = y + ws_base(feat_row, out);
-    return out == 0 ? fmaxf(w, ws_base(feat_row, 0)) : w;
+    return fmaxf(w, ws_base(feat_row, out));
 }
 
 }  // namespace dfsph

@@ -51,8 +51,8 @@ def test_step_warm_start_is_reference_plus_correction():
     """The warm starts of a learned step, read back with the step's features,
     equal the device forward pass of those features plus the reference warm
     start they hold (f5 / 2 for the divergence solver, f4 for the density
-    solver), the divergence one no stronger than the reference's (both are
-    kappa <= 0 totals).  Same kernel arithmetic on both sides: exact."""
+    solver), neither stronger than the reference's (both are kappa <= 0
+    totals).  Same kernel arithmetic on both sides: exact."""
     from splishsplash_for_cuda_amd._native import WARMSTART_LEARNED
     sc = scene.seeded_dam_break(5, 20_000)
     params, norm = learned.load()
@@ -65,9 +65,9 @@ def test_step_warm_start_is_reference_plus_correction():
     s.close()
     bv = np.float32(0.5) * f[:, 5]
     bp = f[:, 4]
-    assert (bp < 0).mean() > 0.5 and (bv < 0).mean() > 0.1        # warm starts in play
+    assert (bp < 0).any() and (bv < 0).mean() > 0.1                # warm starts in play
     np.testing.assert_array_equal(pv, np.maximum(y[:, 0] + bv, bv))
-    np.testing.assert_array_equal(pp, y[:, 1] + bp)
+    np.testing.assert_array_equal(pp, np.maximum(y[:, 1] + bp, bp))
     assert (pv > bv).any() and (pv == bv).any()                    # the bound is active somewhere, not everywhere
 
 

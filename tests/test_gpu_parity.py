@@ -450,8 +450,8 @@ def test_water_at_rest_stays_at_rest(ws):
     """Still water in a closed tank (1 x 0.8 x 1 m, 0.3 m deep, r = 12.5 mm)
     for 1 s: the largest speed stays below 1 m/s while the lattice start
     settles and below 0.4 m/s over the last half second with the reference
-    warm start (measured 0.50 and 0.36), below 2 and 1 m/s with the committed
-    learned model (measured 1.16 and 0.66), and every step meets the
+    warm start (measured 0.50 and 0.36), below 1 and 0.6 m/s with the
+    committed learned model (measured 0.58 and 0.38), and every step meets the
     tolerances.  The water is outside the learned model's training set (dam
     breaks)."""
     r = 0.0125
@@ -475,6 +475,6 @@ def test_water_at_rest_stays_at_rest(ws):
         vmax.append(float(np.abs(s.download(S.FIELD_VEL)).max()))
     late = vmax[len(vmax) // 2:]
     print("max speed over 1 s: %.3f m/s, over the last 0.5 s %.3f (last %.3f)" % (max(vmax), max(late), vmax[-1]))
-    bound, bound_late = (2.0, 1.0) if lw else (1.0, 0.4)
+    bound, bound_late = (1.0, 0.6) if lw else (1.0, 0.4)
     assert max(vmax) < bound and max(late) < bound_late and st.n_fluid == len(fl)
     s.close()
