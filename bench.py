@@ -1,17 +1,28 @@
 """Benchmark of the DFSPH time step on one MI355X.
 
-Workload (default): the plain DFSPH dam break of 1,048,576 fluid particles
-(128 x 128 x 64 block, r = 0.01 m) in a 6 x 3 x 2 m open-top tank with
-600,801 wall particles at spacing r; FP32 bulk arithmetic, FP64 convergence
-reductions, divergence and density tolerances 0.1 %, CFL 0.4 with dt in
-[0.1, 5] ms.  A "step" is one DFSPHCUDA::step() (neighbor search, both
-solvers, forces, integration, position check) over the whole particle set.
+Workload (default, --config c4): the dam break of 4,194,304 fluid particles
+(256 x 128 x 128 block, r = 7.5 mm) with a 0.5 m cube obstacle on the floor
+of a closed 6 x 3 x 2 m tank (walls and obstacle sampled at spacing 2r);
+FP32 bulk arithmetic, FP64 convergence reductions, divergence and density
+tolerances 0.1 %, CFL 0.4 with dt in [0.1, 5] ms, minimum iterations (3, 2)
+as the reference GPU solver.  A "step" is one DFSPHCUDA::step() (neighbor
+search, both solvers, forces, integration, position check) over the whole
+particle set.
 
-Prints ONE JSON line: particle-steps/s over the timed steps, the roofline of
-the dominant kernel (algorithmic bytes per launch / its average duration from
-HIP events recorded on the solver's stream during the timed steps), and the
-CPU baseline (the FP64 OpenMP oracle in the reference CPU semantics, timed on
-a bounded sample of the same workload).
+Legs, each on its own solver, the same steps:
+  plain    the reference warm start: warm-up, then the timed steps (no
+           profiling events inside the timed region), then the same number
+           of profiled steps (HIP events around every solver kernel) for the
+           roofline and the kernel shares;
+  learned  the committed learned warm start, timed and profiled the same way;
+           its ms per step, iterations per step, error maxima, the time of its
+           feature sweep and inference, and the final density field against
+           the plain run's (mean |rho_learned - rho_plain| / rho0 over the
+           particles matched by id, after the same number of steps);
+  cpu      the FP64 OpenMP oracle in the reference CPU semantics, timed on a
+           bounded sample of the same scene (context only).
+
+Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -31,17 +42,14 @@ from splishsplash_for_cuda_amd import scene  # noqa: E402
 from splishsplash_for_cuda_amd._native import KERNEL_NAMES  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, AMD published
+RHO0 = 1000.0
 
-# Compulsory per-particle state of each sweep in bytes (FP32 SoA; uniform
-# fluid mass not counted), the index per stored neighbor (2 B in the 16-bit
-# lists of LDS-staged blocks, 4 B in 32-bit lists) and the 16 B boundary
-# position + psi are added per launch:
-#   B = N_f * S_k + 2 * sum n_fluid,staged + 4 * (sum n_fluid,unstaged + sum n_boundary) + 16 * N_b
-# Algorithmic bytes per launch, the per-unit figures of the hot-path contract:
-# B_k = N_f S_k + 4 (sum n_f + sum n_b) + 16 N_b -- S_k the compulsory
-# per-particle FP32 state of the reference kernel (mass uniform, not counted),
-# one 32-bit index per stored neighbor, the boundary position + psi once.
-# Fused kernels count the reference kernels they replace.
+# Algorithmic bytes per launch of each neighbor sweep:
+#   B_k = N_f * S_k + 4 * (sum n_f + sum n_b) + 16 * N_b
+# S_k the compulsory per-particle FP32 state of the reference kernel that the
+# sweep replaces (mass uniform, not counted), one 32-bit index per stored
+# neighbor, the boundary position + psi read once.  A fused sweep counts the
+# reference kernels it replaces (each reads the lists once).
 SWEEP_STATE_BYTES = {
     "div_ws_init": 44 + 44,   # fused rho/alpha/div pass (x12 v12 r; rho alpha rho_adv 12 w; kappaV8 rw) + divergence warm start (x12 kappaV4 rho_adv4 v24 rw)
     "div_init": 32,           # x12 v12 r; rho_adv4 kappaV4 w
@@ -53,8 +61,8 @@ SWEEP_STATE_BYTES = {
     "dens_jacobi": 44,        # x12 v24 rw, rho_adv4, alpha4
     "dens_loop_end": 44,      # x12 v12 rho4 alpha4 r; rho_adv4 w; kappa8 rw
 }
-# the reference kernels a fused launch replaces (each reads the neighbor lists once)
 SWEEP_PASSES = {"div_ws_init": 2, "pressure_ws": 1}
+SCENES = {"c1": scene.config1, "c2": scene.config2, "c4": scene.config4}
 
 
 def kernel_bytes(name, n_f, n_b, sum_nf, sum_nb):
@@ -65,16 +73,123 @@ def kernel_bytes(name, n_f, n_b, sum_nf, sum_nb):
     return n_f * s + passes * (4 * (sum_nf + sum_nb) + 16 * n_b)
 
 
-def n_f_mean_over(inf0, inf1):
-    """fluid particles per step over the timed region (open boundaries change the count)"""
-    return (inf0.n_fluid + inf1.n_fluid) / 2.0
-
-
-def make_params(sc, mode, max_neighbors):
-    return S.default_params(
+def make_params(sc, mode, max_neighbors, min_iters, floor_only=False):
+    p = S.default_params(
         particle_radius=sc.particle_radius, dt=0.001, use_cfl=1, cfl_factor=1.0, dt_min=1e-4, dt_max=5e-3,
         max_error=0.1, max_error_v=0.1, max_neighbors=max_neighbors,
         warm_start_mode=S.WARMSTART_LEARNED if mode == "learned" else S.WARMSTART_REFERENCE)
+    p.min_iter_v, p.min_iter = min_iters
+    if floor_only:
+        # cost of a step whose solves stop at their floors: what a perfect warm
+        # start would cost (tolerances not enforced)
+        p.max_iter_v, p.max_iter = min_iters
+    return p
+
+
+def make_solver(sc, p, tank, mode):
+    if tank is not None:
+        from splishsplash_for_cuda_amd import wave_tank
+        return wave_tank.make_solver(tank, p, learned=mode == "learned")
+    hs = S.HostSolver(p, sc.boundary, sc.fluid)
+    if mode == "learned":
+        from splishsplash_for_cuda_amd import learned
+        learned.attach(hs.view)
+    return hs
+
+
+def run_leg(sc, p, tank, mode, steps, warmup, prof_steps):
+    """One solver through SPH::DFSPHCUDA::step(): warm-up, timed steps (no
+    events), profiled steps (events on every kernel); the final density field
+    by particle id."""
+    hs = make_solver(sc, p, tank, mode)
+    solver = hs.view
+    for _ in range(warmup):
+        hs.step()
+    inf0 = solver.info()
+    rec = {k: [] for k in ("iters_v", "iters", "err_v", "err", "h", "max_seen", "overflow")}
+    solver.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = hs.step()
+        rec["iters_v"].append(st.iters_v)
+        rec["iters"].append(st.iters)
+        rec["err_v"].append(st.err_v)
+        rec["err"].append(st.err)
+        rec["h"].append(st.h)
+        rec["max_seen"].append(st.max_neighbors_seen)
+        rec["overflow"].append(st.overflow)
+    solver.synchronize()
+    elapsed = time.perf_counter() - t0
+    inf1 = solver.info()
+    # profiled pass: the same number of steps, events around every kernel
+    solver.profile_enable((1 << len(KERNEL_NAMES)) - 1)
+    solver.profile_reset()
+    for _ in range(prof_steps):
+        hs.step()
+    solver.synchronize()
+    prof = solver.profile()
+    inf2 = solver.info()
+    solver.profile_enable(0)
+    ids = solver.download(S.FIELD_ID)
+    rho = solver.download(S.FIELD_DENSITY)
+    density = np.full(int(ids.max()) + 1 if ids.size else 0, np.nan)
+    density[ids] = rho
+    hs.close()
+    n_f = (inf0.n_fluid + inf1.n_fluid) / 2.0
+    # per-step errors as fractions of their tolerances' units: density
+    # err / rho0, divergence err * h / rho0 (the thresholds of the solves,
+    # DFSPH_core_cuda.cu:822,1372), in percent
+    err_d = 100.0 * np.asarray(rec["err"]) / RHO0
+    err_v = 100.0 * np.asarray(rec["err_v"]) * np.asarray(rec["h"]) / RHO0
+    return {
+        "elapsed": elapsed, "n_f": n_f, "inf0": inf0, "inf1": inf1, "inf2": inf2, "prof": prof,
+        "prof_steps": prof_steps, "rec": rec, "density": density, "err_density_pct": err_d,
+        "err_divergence_pct": err_v, "time": inf2.time,
+    }
+
+
+def roofline_of(leg, peak=HBM_PEAK_GBS):
+    """Dominant sweep and sum over the solver sweeps, from the profiled pass."""
+    inf1, inf2, prof = leg["inf1"], leg["inf2"], leg["prof"]
+    n_f = (inf1.n_fluid + inf2.n_fluid) / 2.0
+    n_b = inf2.n_boundary
+    sum_nf = (inf1.neighbors_fluid_total + inf2.neighbors_fluid_total) / 2.0
+    sum_nb = (inf1.neighbors_boundary_total + inf2.neighbors_boundary_total) / 2.0
+    dominant = max(SWEEP_STATE_BYTES, key=lambda k: prof[k][1])
+    launches, ms = prof[dominant]
+    nbytes = kernel_bytes(dominant, n_f, n_b, sum_nf, sum_nb)
+    avg_s = ms / 1e3 / max(launches, 1)
+    tot_b = sum(kernel_bytes(k, n_f, n_b, sum_nf, sum_nb) * prof[k][0] for k in SWEEP_STATE_BYTES)
+    tot_s = sum(prof[k][1] for k in SWEEP_STATE_BYTES) / 1e3
+    return dominant, {
+        "bound": "hbm",
+        "kernel": dominant,
+        "achieved": nbytes / avg_s / 1e9,
+        "peak": peak,
+        "unit": "GB/s",
+        "frac": nbytes / avg_s / 1e9 / peak,
+        "traffic": None,
+        "bytes_model": "N_f*S_k + 4*(sum n_f + sum n_b) + 16*N_b, S_k = %d B" % SWEEP_STATE_BYTES[dominant],
+        "algorithmic_bytes_per_launch": nbytes,
+        "avg_launch_us": avg_s * 1e6,
+        "launches": launches,
+        "solver_sweeps": {"achieved": tot_b / max(tot_s, 1e-12) / 1e9,
+                          "frac": tot_b / max(tot_s, 1e-12) / 1e9 / peak,
+                          "bytes": tot_b, "seconds": tot_s},
+    }
+
+
+def iters_summary(rec):
+    iv, ip = np.asarray(rec["iters_v"]), np.asarray(rec["iters"])
+    return {"divergence_mean": float(iv.mean()), "density_mean": float(ip.mean()),
+            "divergence_p95": float(np.percentile(iv, 95)), "density_p95": float(np.percentile(ip, 95))}
+
+
+def errors_summary(leg):
+    return {"density_max_pct": float(leg["err_density_pct"].max()),
+            "divergence_max_pct": float(leg["err_divergence_pct"].max()),
+            "density_mean_pct": float(leg["err_density_pct"].mean()),
+            "divergence_mean_pct": float(leg["err_divergence_pct"].mean())}
 
 
 def cpu_baseline(sc, budget_s, max_steps, tank=None):
@@ -105,48 +220,35 @@ def cpu_baseline(sc, budget_s, max_steps, tank=None):
                       "TimeStepDFSPH with OpenMP (%.1f s)" % (steps, sc.n_fluid, t)}
 
 
-def learned_leg(sc, p, tank, steps, warmup):
-    """The same workload through SPH::DFSPHCUDA::step() with the committed
-    learned warm start: ms per step, throughput and iterations per step, run
-    after the plain solver is closed (untimed setup, the same warm-up)."""
-    from splishsplash_for_cuda_amd import learned
-    p.warm_start_mode = S.WARMSTART_LEARNED
+def workload_text(args, sc, tank, first, last):
     if tank is not None:
-        from splishsplash_for_cuda_amd import wave_tank
-        hs = wave_tank.make_solver(tank, p, learned=True)
-    else:
-        hs = S.HostSolver(p, sc.boundary, sc.fluid)
-        learned.attach(hs.view)
-    for _ in range(warmup + 5):
-        hs.step()
-    inf0 = hs.view.info()
-    iv, ip = [], []
-    hs.view.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        st = hs.step()
-        iv.append(st.iters_v)
-        ip.append(st.iters)
-    hs.view.synchronize()
-    elapsed = time.perf_counter() - t0
-    inf1 = hs.view.info()
-    hs.close()
-    return {"ms_per_step": 1e3 * elapsed / steps, "value": n_f_mean_over(inf0, inf1) * steps / elapsed,
-            "iters_per_step": {"divergence_mean": float(np.mean(iv)), "density_mean": float(np.mean(ip))}}
+        return ("C5: %s DFSPH open-boundary wave tank 10x2x1 m, %d fluid (start) + %d wall particles, r=%g m, "
+                "sinusoidal piston (0.1 m, 1.5 s) and a floating box (500 kg/m^3) as rigid bodies, absorbing open "
+                "boundary with per-step inflow / outflow compaction, tolerances 0.1%%, CFL dt in [0.1,5] ms, "
+                "steps %d-%d of the run" % (args.mode, sc.n_fluid, sc.n_boundary, sc.particle_radius, first, last))
+    what = {"c1": "dam break", "c2": "dam break",
+            "c4": "dam break with a 0.5 m cube obstacle"}[args.config]
+    return ("%s: %s DFSPH %s, %d fluid + %d boundary particles, r=%g m, tolerances 0.1%%, CFL 0.4 dt in "
+            "[0.1,5] ms, min iters (%s), steps %d-%d of the run"
+            % (args.config.upper(), args.mode, what, sc.n_fluid, sc.n_boundary, sc.particle_radius,
+               args.min_iters, first, last))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c4", "c5"],
-                    help="c5: the open-boundary wave tank (piston, absorbing end, floating box)")
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="c4", choices=["c1", "c2", "c4", "c5"],
+                    help="c4: the 4M-particle dam break with obstacle; c5: the open-boundary wave tank")
     ap.add_argument("--mode", default="plain", choices=["plain", "learned"])
-    ap.add_argument("--path", default="host", choices=["host", "abi"],
-                    help="host: SPH::DFSPHCUDA::step() (C++); abi: dfsph_step")
     ap.add_argument("--min-iters", default="3,2",
                     help="minimum divergence,density Jacobi iterations (the reference GPU solver's are 3,2)")
+    ap.add_argument("--prof-steps", type=int, default=0,
+                    help="profiled steps after the timed ones (0: as many as timed, at most 50)")
+    ap.add_argument("--floor-only", action="store_true",
+                    help="stop both solves at their minimum iterations: the cost of a step with a perfect warm "
+                         "start (tolerances not enforced; not a valid throughput)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned-leg", action="store_true",
@@ -163,127 +265,79 @@ def main():
         tank = wave_tank.build()
         sc = scene.Scene("C5_wave_tank", tank.particle_radius, tank.fluid, tank.walls)
     else:
-        sc = {"c1": scene.config1, "c2": scene.config2, "c4": scene.config4}[args.config]()
+        sc = SCENES[args.config]()
     max_nbr = 90 if args.config == "c1" else 128
-    p = make_params(sc, args.mode, max_nbr)
-    p.min_iter_v, p.min_iter = (int(v) for v in args.min_iters.split(","))
-    if tank is not None:
-        if args.path != "host":
-            print("c5 runs through SPH::DFSPHCUDA::step() (open boundaries): use --path host", file=sys.stderr)
-            sys.exit(2)
-        hs = wave_tank.make_solver(tank, p, learned=args.mode == "learned")
-        solver = hs.view
-        step = hs.step
-    elif args.path == "host":
-        hs = S.HostSolver(p, sc.boundary, sc.fluid)
-        solver = hs.view
-        step = hs.step
-    else:
-        solver = S.Solver(p)
-        solver.upload_boundary(sc.boundary)
-        solver.upload_fluid(sc.fluid)
-        step = solver.step
-    if args.mode == "learned" and tank is None:
-        from splishsplash_for_cuda_amd import learned
-        learned.attach(solver)
+    min_iters = tuple(int(v) for v in args.min_iters.split(","))
+    prof_steps = args.prof_steps or min(args.steps, 50)
+    p = make_params(sc, args.mode, max_nbr, min_iters, args.floor_only)
 
-    for _ in range(args.warmup):
-        step()
-    # find the dominant sweep over a few profiled steps (outside the timed region)
-    solver.profile_enable((1 << len(KERNEL_NAMES)) - 1)
-    solver.profile_reset()
-    for _ in range(5):
-        step()
-    prof = solver.profile()
-    solver.profile_enable(0)
-    dominant = max((k for k in SWEEP_STATE_BYTES), key=lambda k: prof[k][1])
-
-    # timed region: event timing of the dominant sweep only
-    solver.profile_enable(1 << KERNEL_NAMES.index(dominant))
-    solver.profile_reset()
-    inf0 = solver.info()
-    iters_v, iters = [], []
-    solver.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = step()
-        iters_v.append(st.iters_v)
-        iters.append(st.iters)
-    solver.synchronize()
-    elapsed = time.perf_counter() - t0
-    inf1 = solver.info()
-    launches, ms = solver.profile()[dominant]
-    solver.profile_enable(0)
-
-    n_f = (inf0.n_fluid + inf1.n_fluid) / 2.0
-    sum_nf = (inf0.neighbors_fluid_total + inf1.neighbors_fluid_total) / 2.0
-    sum_nb = (inf0.neighbors_boundary_total + inf1.neighbors_boundary_total) / 2.0
-    sum_nfu = (inf0.neighbors_fluid_unstaged + inf1.neighbors_fluid_unstaged) / 2.0
-    nbytes = kernel_bytes(dominant, n_f, inf1.n_boundary, sum_nf, sum_nb)
-    avg_s = ms / 1e3 / max(launches, 1)
-    achieved = nbytes / avg_s / 1e9
-    traffic = None
+    leg = run_leg(sc, p, tank, args.mode, args.steps, args.warmup, prof_steps)
+    dominant, roof = roofline_of(leg)
     tfile = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (dominant, args.config))
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
-
-    value = n_f_mean_over(inf0, inf1) * args.steps / elapsed
+        roof["traffic"] = json.load(open(tfile)).get("bytes_per_launch")
+    prof = leg["prof"]
+    total_ms = max(sum(v[1] for v in prof.values()), 1e-9)
+    rec = leg["rec"]
+    first, last = args.warmup + 1, args.warmup + args.steps
     line = {
         "metric": "particle-steps/s at ρ-err≤0.1%; pressure iters/step learned vs plain DFSPH",
-        "value": value,
+        "value": leg["n_f"] * args.steps / leg["elapsed"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": 1e3 * leg["elapsed"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": ("%s: %s DFSPH dam break, %d fluid + %d boundary particles, r=%g m, tolerances 0.1%%, "
-                         "CFL 0.4 dt in [0.1,5] ms, steps %d-%d of the run"
-                         % (args.config.upper(), args.mode, sc.n_fluid, sc.n_boundary, sc.particle_radius,
-                            args.warmup + 5, args.warmup + 5 + args.steps)) if tank is None else
-                        ("C5: %s DFSPH open-boundary wave tank 10x2x1 m, %d fluid (start) + %d wall particles, "
-                         "r=%g m, sinusoidal piston (0.1 m, 1.5 s) and a floating box (500 kg/m^3) as rigid "
-                         "bodies, absorbing open boundary with per-step inflow / outflow compaction, tolerances "
-                         "0.1%%, CFL dt in [0.1,5] ms, steps %d-%d of the run"
-                         % (args.mode, sc.n_fluid, sc.n_boundary, sc.particle_radius, args.warmup + 5,
-                            args.warmup + 5 + args.steps)),
-            "path": "SPH::DFSPHCUDA::step()" if args.path == "host" else "dfsph_step",
-            "iters_per_step": {"divergence_mean": float(np.mean(iters_v)), "density_mean": float(np.mean(iters)),
-                               "divergence_p95": float(np.percentile(iters_v, 95)),
-                               "density_p95": float(np.percentile(iters, 95))},
+            "workload": workload_text(args, sc, tank, first, last),
+            "path": "SPH::DFSPHCUDA::step()",
+            "iters_per_step": iters_summary(rec),
+            "errors": errors_summary(leg),
+            "tolerances_pct": {"density": 0.1, "divergence": 0.1},
             "max_neighbors": max_nbr,
-            "min_iters": [int(p.min_iter_v), int(p.min_iter)],
-            "unstaged_blocks": int(inf1.unstaged_blocks),
-            "fluid_neighbors_unstaged_frac": float(sum_nfu / max(sum_nf, 1.0)),
+            "max_neighbors_seen": int(max(rec["max_seen"])),
+            "overflow": int(max(rec["overflow"])),
+            "min_iters": list(min_iters),
+            "unstaged_blocks": int(leg["inf1"].unstaged_blocks),
+            "timing": "timed steps without profiling events; roofline and kernel shares from %d further "
+                      "profiled steps (HIP events on the solver's stream)" % prof_steps,
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dominant,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "bytes_model": "N_f*S_k + 4*(sum n_f + sum n_b) + 16*N_b, S_k = %d B" % SWEEP_STATE_BYTES[dominant],
-            "traffic": traffic,
-            "algorithmic_bytes_per_launch": nbytes,
-            "avg_launch_us": avg_s * 1e6,
-            "launches": launches,
-        },
-        "kernel_share": {k: round(v[1] / max(sum(x[1] for x in prof.values()), 1e-9), 4) for k, v in prof.items()},
+        "roofline": roof,
+        "kernel_share": {k: round(v[1] / total_ms, 4) for k, v in prof.items() if v[0]},
+        "kernel_ms_per_step": {k: round(v[1] / prof_steps, 4) for k, v in prof.items() if v[0]},
     }
-    if args.path == "host":
-        hs.close()
-    else:
-        solver.close()
-    if args.mode == "plain" and args.path == "host" and not args.no_learned_leg:
-        lrn = learned_leg(sc, p, tank, args.steps, args.warmup)
-        lrn["speedup_vs_plain"] = line["ms_per_step"] / lrn["ms_per_step"]
-        line["learned_vs_plain"] = lrn
+    if args.floor_only:
+        line["config"]["floor_only"] = "solves stopped at their minimum iterations: cost ceiling, not a valid throughput"
+    if args.mode == "plain" and not args.no_learned_leg and not args.floor_only:
+        pl = S.default_params()
+        pl = make_params(sc, "learned", max_nbr, min_iters)
+        lrn = run_leg(sc, pl, tank, "learned", args.steps, args.warmup, prof_steps)
+        ms = 1e3 * lrn["elapsed"] / args.steps
+        lp = lrn["prof"]
+        nd = min(leg["density"].size, lrn["density"].size)
+        both = ~np.isnan(leg["density"][:nd]) & ~np.isnan(lrn["density"][:nd])
+        dfield = np.abs(lrn["density"][:nd][both] - leg["density"][:nd][both]) / RHO0
+        line["learned_vs_plain"] = {
+            "ms_per_step": ms,
+            "value": lrn["n_f"] * args.steps / lrn["elapsed"],
+            "speedup_vs_plain": line["ms_per_step"] / ms,
+            "iters_per_step": iters_summary(lrn["rec"]),
+            "errors": errors_summary(lrn),
+            "max_neighbors_seen": int(max(lrn["rec"]["max_seen"])),
+            "overflow": int(max(lrn["rec"]["overflow"])),
+            "overhead_ms_per_step": {k: round(lp[k][1] / prof_steps, 4) for k in ("features", "mlp_infer")},
+            "density_field_vs_plain": {
+                "after_steps": args.warmup + args.steps + prof_steps,
+                "mean_abs_rel": float(dfield.mean()) if dfield.size else None,
+                "max_abs_rel": float(dfield.max()) if dfield.size else None,
+                "matched": int(both.sum()),
+                "time_plain_s": leg["time"], "time_learned_s": lrn["time"]},
+        }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(sc, args.cpu_budget, 1000, tank)
     else:

@@ -116,6 +116,8 @@ typedef struct dfsph_info {
 #define DFSPH_FIELD_BOUNDARY_POS 12 /* float xyz (3 n_boundary), cell-sorted */
 #define DFSPH_FIELD_BOUNDARY_PSI 13 /* float (n_boundary), cell-sorted */
 #define DFSPH_FIELD_TAG 14         /* uint32 dynamic-window / open-boundary tag (also uploadable) */
+#define DFSPH_FIELD_KAPPA_TOTAL 15  /* float: last density solve's total, kappa h_f^2 (also uploadable) */
+#define DFSPH_FIELD_KAPPAV_TOTAL 16 /* float: last divergence solve's total, kappaV h (also uploadable) */
 
 void dfsph_default_params(dfsph_params* p);
 const char* dfsph_last_error(void);
@@ -220,7 +222,8 @@ int dfsph_step(void* handle, dfsph_step_stats* stats);
  * copy a field of the active particles (device order) to host memory.
  * count must equal the number of elements of the field. */
 int dfsph_download(void* handle, int field, void* dst, uint32_t count);
-/* overwrite a fluid field (device order): POS, VEL, MASS, KAPPA, KAPPAV */
+/* overwrite a fluid field (device order): POS, VEL, MASS, KAPPA, KAPPAV, TAG,
+ * KAPPA_TOTAL, KAPPAV_TOTAL */
 int dfsph_upload_field(void* handle, int field, const void* src, uint32_t count);
 
 /* neighbor lists of the last search, device order, row-major [i*max + k]:
@@ -230,6 +233,13 @@ int dfsph_get_neighbors(void* handle, uint32_t* n_fluid, uint32_t* n_boundary,
 
 int dfsph_get_info(void* handle, dfsph_info* info);
 int dfsph_synchronize(void* handle);
+/* Statistics of the last neighbor search, from the per-particle counts that
+ * DFSPH_neighborsSearch_kernel stores (DFSPH_core_cuda.cu:2513-2516): the
+ * largest list, the particles whose list was truncated at max_neighbors and
+ * the stored neighbor totals.  No stream synchronisation when a solve has
+ * already brought them back after that search. */
+int dfsph_neighbor_stats(void* handle, uint32_t* max_seen, uint32_t* overflow, uint64_t* fluid_total,
+                         uint64_t* boundary_total);
 
 /* Per-kernel HIP-event timing, the device-side counterpart of the phase timers
  * of DFSPHCUDA::step (DFSPH_CUDA.cpp:185-194, 480-495) and SegmentedTiming
@@ -249,7 +259,9 @@ int dfsph_synchronize(void* handle);
 #define DFSPH_KERNEL_DENS_JACOBI 9   /* density velocity sweep                  */
 #define DFSPH_KERNEL_DENS_LOOP_END 10 /* density source + FP64 reduction        */
 #define DFSPH_KERNEL_UPDATE_POS 11   /* position update + escape count          */
-#define DFSPH_KERNEL_COUNT 12
+#define DFSPH_KERNEL_FEATURES 12     /* learned warm start: feature sweep       */
+#define DFSPH_KERNEL_MLP_INFER 13    /* learned warm start: MLP inference       */
+#define DFSPH_KERNEL_COUNT 14
 int dfsph_profile_enable(void* handle, uint32_t mask);
 int dfsph_profile_reset(void* handle);
 /* launches and summed duration (ms) of one kernel id since the last reset */
@@ -378,12 +390,15 @@ int dfsph_rigid_body_count(void* handle, uint32_t* n);
  * (12 -> 64 -> 64 -> 64 -> 2, SiLU; BF16 weights on the matrix cores, FP32
  * accumulate) that predicts the warm starts the reference forms at
  * DFSPH_core_cuda.cu:240 (kappaV) and :3063-3068 (kappa).  The network's two
- * outputs are corrections to those reference warm starts in dt-invariant units:
- * the step's converged totals (kappaV h, kappa h_f^2) less (f5 / 2, f4), where
- * f4 and f5 are the features that hold the previous step's totals
- * (kappa h_f^2, kappaV h); the learned warm start adds them back and asks
- * for no more pressure than the reference's: (max(y0 + f5 / 2, f5 / 2),
- * max(y1 + f4, f4)), kappa and kappaV <= 0.
+ * outputs are corrections to those reference warm starts in dt-invariant units
+ * (kappaV h, kappa h_f^2): the increments the plain solver's Jacobi
+ * iterations still add after the reference warm start.  Features 4 and 5 are
+ * the last solves' increments in the same units (kappa h_f^2, kappaV h).  A
+ * sweep applies ref + y / (h or h_f^2), bounded between min(ref, prev) and
+ * max(ref, 0), where ref is the reference warm start and prev the particle's
+ * converged total of the last step (both in raw units, clamped at -0.5): no
+ * more pressure than the larger of the two, and a network that outputs zero
+ * gives the reference warm start exactly.
  * Parameters use one flat FP32 layout of DFSPH_MLP_NPARAM values:
  *   W1[64x12] b1[64] W2[64x64] b2[64] W3[64x64] b3[64] W4[2x64] b4[2]
  * (row-major, out x in); norm holds in_mean[12] in_std[12] out_mean[2] out_std[2]. */
@@ -400,10 +415,11 @@ int dfsph_mlp_forward(void* handle, const float* x, uint32_t n, float* y, int no
 int dfsph_mlp_loss_grad(void* handle, const float* x, const float* t, uint32_t n, float* grad, double* loss,
                         int normalise);
 /* Training set harvest: while capacity is not reached, every step (dfsph_step or
- * the phase calls) appends each stride-th particle's 12 features (computed after
- * the neighbor search) and the network's targets: the step's converged totals
- * (kappaV h, kappa h_f^2) less (f5 / 2, f4).
- * capacity 0 stops harvesting. */
+ * the phase calls) with the reference warm start appends each stride-th
+ * particle's 12 features (computed after the neighbor search) and the
+ * network's targets: the increments of the step's two solves (kappaV h,
+ * kappa h_f^2).  A pressure solve in another warm-start mode while harvesting
+ * returns DFSPH_ERR_STATE.  capacity 0 stops harvesting. */
 int dfsph_mlp_harvest(void* handle, uint32_t capacity, uint32_t stride);
 int dfsph_mlp_dataset_size(void* handle, uint32_t* n);
 int dfsph_mlp_dataset_download(void* handle, float* x, float* t, uint32_t n);
@@ -415,8 +431,8 @@ int dfsph_mlp_fit_normalization(void* handle);
 int dfsph_mlp_train(void* handle, uint32_t steps, uint32_t batch, float lr, uint32_t seed, float* loss_first,
                     float* loss_last);
 /* features and predictions of the current step (device order, n particles);
- * the predictions are the warm starts the solver used: MLP output y plus the
- * reference warm start, (max(y0 + f5 / 2, f5 / 2), max(y1 + f4, f4)) */
+ * the predictions are the network's corrections (kappaV h, kappa h_f^2) that
+ * the sweeps add to the reference warm starts */
 int dfsph_mlp_step_features(void* handle, float* feat, float* pred_v, float* pred_p, uint32_t n);
 
 #ifdef __cplusplus

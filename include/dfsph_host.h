@@ -25,6 +25,9 @@ typedef struct dfsph_cuda_stats {
     uint32_t n_fluid;
     int32_t removed;
     uint32_t inflow_added, outflow_removed;   /* open boundaries, at the start of the step */
+    uint32_t max_neighbors_seen;  /* largest neighbor list of the step's search */
+    uint32_t overflow;            /* particles whose list was truncated; the step then
+                                     returns DFSPH_ERR_NEIGHBOR_OVERFLOW after it completed */
 } dfsph_cuda_stats;
 
 /* new SPH::DFSPHCUDA(params, boundary, fluid); NULL on failure (message in err) */

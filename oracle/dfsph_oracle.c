@@ -1180,6 +1180,65 @@ uint32_t or_outflow(const or_state* s, const or_surface* interior, const or_surf
 }
 
 /* ------------------------------------------------------------------ */
+/* Features and training targets of the learned warm start (no reference */
+/* counterpart; inserted where the reference forms its warm starts,     */
+/* DFSPH_core_cuda.cu:240 and :3063-3068).  Mode GPU state: kappa and    */
+/* kappaV hold the last solves' increments (raw units).                  */
+/* ------------------------------------------------------------------ */
+void or_features(const or_state* s, double* out)
+{
+    const double hk = s->hk, rho0 = s->p.density0, diameter = 2.0 * s->p.particle_radius;
+#pragma omp parallel for schedule(static)
+    for (long il = 0; il < (long)s->n; ++il) {
+        const uint32_t i = (uint32_t)il;
+        const uint32_t* row = ROW(i);
+        const double* xi = XI(i);
+        const double* vi = VI(i);
+        double dfl = 0.0, dbd = 0.0, off[3] = {0, 0, 0}, dist = 0.0, d[3];
+        for (uint32_t k = 0; k < s->nf[i]; ++k) {
+            const uint32_t j = row[k];
+            sub3(xi, XI(j), d);
+            dfl += s->m[j] * kW(s, d);
+            off[0] -= d[0]; off[1] -= d[1]; off[2] -= d[2];
+            dist += sqrt(dot3(d, d));
+        }
+        for (uint32_t k = 0; k < s->nbc[i]; ++k) {
+            const uint32_t j = row[s->nf[i] + k];
+            sub3(xi, XB(j), d);
+            dbd += s->psi[j] * kW(s, d);
+            off[0] -= d[0]; off[1] -= d[1]; off[2] -= d[2];
+            dist += sqrt(dot3(d, d));
+        }
+        const double rho = s->m[i] * s->W0 + dfl + dbd;
+        const double nn = (double)(s->nf[i] + s->nbc[i]);
+        const double inv_n = nn > 0.0 ? 1.0 / nn : 0.0;
+        double* f = out + (size_t)i * 12;
+        f[0] = rho / rho0 - 1.0;                                 /* density ratio                     */
+        f[1] = density_change_sum(s, i) * s->h / rho0;           /* divergence source, unclamped      */
+        f[2] = nn / 40.0;                                        /* neighbor count                    */
+        f[3] = sqrt(dot3(vi, vi)) * s->h / diameter;             /* speed in diameters per step       */
+        f[4] = s->kappa[i] * s->h * s->h;                        /* last density increments kappa h_f^2 */
+        f[5] = s->kappaV[i] * s->h_past;                         /* last divergence increments kappaV h */
+        f[6] = (double)s->nbc[i] * inv_n;                        /* boundary share of the neighbors   */
+        f[7] = off[0] * inv_n / hk;                              /* mean offset to the neighbors / hk */
+        f[8] = off[1] * inv_n / hk;
+        f[9] = off[2] * inv_n / hk;
+        f[10] = dist * inv_n / hk;                               /* mean neighbor distance / hk       */
+        f[11] = dfl / rho;                                       /* fluid share of the density        */
+    }
+}
+
+void or_warm_start_targets(const or_state* s, double* out)
+{
+    /* after a step's solves: the increments the reference warm start left to
+     * the Jacobi iterations, in dt-invariant units (kappaV h, kappa h_f^2) */
+    for (uint32_t i = 0; i < s->n; ++i) {
+        out[2 * i + 0] = s->kappaV[i] * s->h;
+        out[2 * i + 1] = s->kappa[i] * s->h_future * s->h_future;
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* FP64 MLP of the learned warm start (no reference counterpart)       */
 /* ------------------------------------------------------------------ */
 #define MLP_IN 12

@@ -134,6 +134,8 @@ def lib():
             "or_kernel_W": (C.c_double, [C.c_double, C.c_double, C.c_int]),
             "or_kernel_gradW": (None, [_DP, C.c_double, C.c_int, _DP]),
             "or_mlp_forward": (None, [C.c_uint32, _DP, _DP, _DP]),
+            "or_features": (None, [C.c_void_p, _DP]),
+            "or_warm_start_targets": (None, [C.c_void_p, _DP]),
             "or_mlp_loss_grad": (C.c_double, [C.c_uint32, _DP, _DP, _DP, _DP]),
             "or_set_restriction": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_longlong, _UP]),
             "or_surface_inside": (C.c_int, [C.POINTER(OrSurface), _DP, _DP]),
@@ -281,6 +283,18 @@ class Oracle:
         st = OrStats()
         self.L.or_step(self.s, C.byref(st))
         return st
+
+    def features(self):
+        """learned warm-start features (n, 12) of the current state"""
+        out = np.zeros((self.n, 12))
+        self.L.or_features(self.s, out.ctypes.data_as(_DP))
+        return out
+
+    def warm_start_targets(self):
+        """training targets (n, 2) after a step's solves"""
+        out = np.zeros((self.n, 2))
+        self.L.or_warm_start_targets(self.s, out.ctypes.data_as(_DP))
+        return out
 
     def add_body(self, desc, x_body, psi=None):
         x = _d(x_body).reshape(-1)

@@ -46,6 +46,8 @@ FIELD_NBR_BOUNDARY = 11
 FIELD_BOUNDARY_POS = 12
 FIELD_BOUNDARY_PSI = 13
 FIELD_TAG = 14
+FIELD_KAPPA_TOTAL = 15
+FIELD_KAPPAV_TOTAL = 16
 _VECTOR_FIELDS = {FIELD_POS, FIELD_VEL, FIELD_ACC, FIELD_BOUNDARY_POS}
 _UINT_FIELDS = {FIELD_ID, FIELD_NBR_FLUID, FIELD_NBR_BOUNDARY, FIELD_TAG}
 
@@ -131,6 +133,8 @@ class DFSPHCudaStats(C.Structure):
         ("removed", C.c_int32),
         ("inflow_added", C.c_uint32),
         ("outflow_removed", C.c_uint32),
+        ("max_neighbors_seen", C.c_uint32),
+        ("overflow", C.c_uint32),
     ]
 
 
@@ -254,6 +258,8 @@ HIP_SIGNATURES = {
     "dfsph_get_neighbors": (_I, [_P, _U32P, _U32P, _U32P, _U32]),
     "dfsph_get_info": (_I, [_P, C.POINTER(DFSPHInfo)]),
     "dfsph_synchronize": (_I, [_P]),
+    "dfsph_neighbor_stats": (_I, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                 C.POINTER(C.c_uint64)]),
     "dfsph_profile_enable": (_I, [_P, _U32]),
     "dfsph_profile_reset": (_I, [_P]),
     "dfsph_profile_read": (_I, [_P, _U32, _U32P, C.POINTER(_D)]),
@@ -275,7 +281,7 @@ MLP_NNORM = 28
 MLP_NFEAT = 12
 
 KERNEL_NAMES = ["sort", "neighbors", "div_ws_init", "div_init", "div_jacobi", "div_loop_end", "xsph",
-                "pressure_ws", "dens_init", "dens_jacobi", "dens_loop_end", "update_pos"]
+                "pressure_ws", "dens_init", "dens_jacobi", "dens_loop_end", "update_pos", "features", "mlp_infer"]
 
 HOST_SIGNATURES = {
     "dfsph_cuda_new": (C.c_void_p, [C.POINTER(DFSPHParams), _F32P, _U32, _F32P, _F32P, _U32, C.c_char_p, _U32]),
@@ -660,7 +666,7 @@ class Solver:
         return f, pv, pp
 
     # ---- per-kernel event timing ----
-    def profile_enable(self, mask=(1 << 12) - 1):
+    def profile_enable(self, mask=(1 << len(KERNEL_NAMES)) - 1):
         _check(self.lib.dfsph_profile_enable(self.h, mask), "dfsph_profile_enable")
 
     def profile_reset(self):
@@ -677,6 +683,14 @@ class Solver:
         return out
 
     # ---- data out ----
+    def neighbor_stats(self):
+        """(max_seen, overflow, fluid_total, boundary_total) of the last neighbor search"""
+        ms, of = C.c_uint32(), C.c_uint32()
+        nf, nb = C.c_uint64(), C.c_uint64()
+        _check(self.lib.dfsph_neighbor_stats(self.h, C.byref(ms), C.byref(of), C.byref(nf), C.byref(nb)),
+               "dfsph_neighbor_stats")
+        return ms.value, of.value, nf.value, nb.value
+
     def info(self) -> DFSPHInfo:
         inf = DFSPHInfo()
         _check(self.lib.dfsph_get_info(self.h, C.byref(inf)), "dfsph_get_info")
@@ -775,10 +789,10 @@ class HostSolver:
         if rc != DFSPH_OK:
             raise DFSPHError(rc, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
 
-    def step(self) -> DFSPHCudaStats:
+    def step(self, raise_on_overflow=True) -> DFSPHCudaStats:
         st = DFSPHCudaStats()
         rc = self.lib.dfsph_cuda_step(self.s, C.byref(st))
-        if rc != DFSPH_OK:
+        if rc != DFSPH_OK and (raise_on_overflow or rc != DFSPH_ERR_NEIGHBOR_OVERFLOW):
             raise DFSPHError(rc, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
         return st
 

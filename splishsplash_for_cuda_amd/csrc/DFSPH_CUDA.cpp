@@ -3,6 +3,7 @@
 #include "DFSPH_CUDA.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -20,7 +21,9 @@ void check(int rc, const char* what)
 }
 
 constexpr uint32_t kSnapMagic = 0x48505344u;   // "DSPH"
-constexpr uint32_t kSnapVersion = 1;
+// version 2 adds the warm-start totals of the last step (the learned warm
+// start's bound) and the rigid bodies' states; version 1 files still load
+constexpr uint32_t kSnapVersion = 2;
 
 }  // namespace
 
@@ -266,12 +269,19 @@ void DFSPHCUDA::step()
     check(dfsph_update_vel(h), "dfsph_update_vel");
     check(dfsph_pressure_solve(h, m_maxIterations, m_maxError, &m_iterations, &m_densityError), "dfsph_pressure_solve");
     if (m_data.params.use_cfl) check(dfsph_get_time(h, nullptr, &m_data.h_future, nullptr), "dfsph_get_time");
+    // the search's list statistics, brought back by the pressure solve's loop
+    check(dfsph_neighbor_stats(h, &m_maxNeighborsSeen, &m_neighborOverflow, nullptr, nullptr),
+          "dfsph_neighbor_stats");
     check(dfsph_update_pos(h), "dfsph_update_pos");
     m_data.onSimulationStepEnd();
     m_removedLastStep = 0;
     if (m_data.params.check_positions_mode >= 0)
         m_removedLastStep = m_data.checkParticlesPositions(m_data.params.check_positions_mode, false);
     count_steps++;
+    // as dfsph_step: the step is complete, but some list was truncated
+    if (m_neighborOverflow)
+        throw DFSPHError(DFSPH_ERR_NEIGHBOR_OVERFLOW,
+                         std::to_string(m_neighborOverflow) + " particles exceed max_neighbors");
 }
 
 void DFSPHCUDA::reset()
@@ -305,6 +315,24 @@ void DFSPHCUDA::handleSimulationSave(const std::string& path) const
     f.write(reinterpret_cast<const char*>(k.data()), sizeof(float) * k.size());
     f.write(reinterpret_cast<const char*>(kv.data()), sizeof(float) * kv.size());
     f.write(reinterpret_cast<const char*>(id.data()), sizeof(uint32_t) * id.size());
+    const std::vector<float> kt = m_data.readField(DFSPH_FIELD_KAPPA_TOTAL),
+                             kvt = m_data.readField(DFSPH_FIELD_KAPPAV_TOTAL);
+    f.write(reinterpret_cast<const char*>(kt.data()), sizeof(float) * kt.size());
+    f.write(reinterpret_cast<const char*>(kvt.data()), sizeof(float) * kvt.size());
+    uint32_t nbody = 0;
+    check(dfsph_rigid_body_count(h, &nbody), "dfsph_rigid_body_count");
+    f.write(reinterpret_cast<const char*>(&nbody), sizeof nbody);
+    for (uint32_t b = 0; b < nbody; ++b) {
+        dfsph_rigid_body rb;
+        check(dfsph_get_rigid_body(h, b, &rb, nullptr, nullptr), "dfsph_get_rigid_body");
+        if (rb.type == DFSPH_BODY_SINE) {
+            // dfsph_set_rigid_body takes a sinusoidal body's center at rest:
+            // recovered from the current center and the time (to rounding)
+            const double sn = std::sin(2.0 * M_PI * info.time / rb.period);
+            for (int a = 0; a < 3; ++a) rb.position[a] -= rb.amplitude * sn * rb.axis[a];
+        }
+        f.write(reinterpret_cast<const char*>(&rb), sizeof rb);
+    }
     if (!f) throw DFSPHError(DFSPH_ERR_ARG, "write failed: " + path);
 }
 
@@ -316,7 +344,8 @@ void DFSPHCUDA::handleSimulationLoad(const std::string& path)
     double tm[3];
     f.read(reinterpret_cast<char*>(hdr), sizeof hdr);
     f.read(reinterpret_cast<char*>(tm), sizeof tm);
-    if (!f || hdr[0] != kSnapMagic || hdr[1] != kSnapVersion) throw DFSPHError(DFSPH_ERR_ARG, "not a snapshot: " + path);
+    if (!f || hdr[0] != kSnapMagic || hdr[1] < 1 || hdr[1] > kSnapVersion)
+        throw DFSPHError(DFSPH_ERR_ARG, "not a snapshot: " + path);
     const uint32_t n = hdr[2];
     ParticleArrays p;
     p.pos.resize(3 * (size_t)n);
@@ -330,15 +359,39 @@ void DFSPHCUDA::handleSimulationLoad(const std::string& path)
     f.read(reinterpret_cast<char*>(k.data()), sizeof(float) * n);
     f.read(reinterpret_cast<char*>(kv.data()), sizeof(float) * n);
     f.read(reinterpret_cast<char*>(p.id.data()), sizeof(uint32_t) * n);
+    std::vector<float> kt, kvt;
+    std::vector<dfsph_rigid_body> bodies;
+    if (hdr[1] >= 2) {
+        kt.resize(n);
+        kvt.resize(n);
+        f.read(reinterpret_cast<char*>(kt.data()), sizeof(float) * n);
+        f.read(reinterpret_cast<char*>(kvt.data()), sizeof(float) * n);
+        uint32_t nbody = 0;
+        f.read(reinterpret_cast<char*>(&nbody), sizeof nbody);
+        bodies.resize(f ? nbody : 0);
+        for (auto& rb : bodies) f.read(reinterpret_cast<char*>(&rb), sizeof rb);
+    }
     if (!f) throw DFSPHError(DFSPH_ERR_ARG, "truncated snapshot: " + path);
-    m_data.reset(p);
     void* h = m_data.handle();
+    uint32_t have = 0;
+    check(dfsph_rigid_body_count(h, &have), "dfsph_rigid_body_count");
+    if (!bodies.empty() && bodies.size() != have)
+        throw DFSPHError(DFSPH_ERR_STATE, "the snapshot holds " + std::to_string(bodies.size()) +
+                                              " rigid bodies, the solver " + std::to_string(have));
+    m_data.reset(p);
     check(dfsph_upload_field(h, DFSPH_FIELD_KAPPA, k.data(), n), "dfsph_upload_field");
     check(dfsph_upload_field(h, DFSPH_FIELD_KAPPAV, kv.data(), n), "dfsph_upload_field");
+    if (!kt.empty()) {
+        check(dfsph_upload_field(h, DFSPH_FIELD_KAPPA_TOTAL, kt.data(), n), "dfsph_upload_field");
+        check(dfsph_upload_field(h, DFSPH_FIELD_KAPPAV_TOTAL, kvt.data(), n), "dfsph_upload_field");
+    }
     check(dfsph_set_time(h, tm[0], tm[1], tm[2]), "dfsph_set_time");
     m_data.h = m_data.h_future = tm[0];
     m_data.h_past = tm[1];
     m_data.time = tm[2];
+    // bodies after the time: a sinusoidal body's phase is taken from it
+    for (uint32_t b = 0; b < (uint32_t)bodies.size(); ++b)
+        check(dfsph_set_rigid_body(h, b, &bodies[b]), "dfsph_set_rigid_body");
     count_steps = (int)hdr[3];
 }
 
@@ -427,8 +480,8 @@ int dfsph_cuda_step(void* s, dfsph_cuda_stats* st)
 {
     return guarded([&] {
         SPH::DFSPHCUDA* d = S(s);
-        d->step();
-        if (st) {
+        auto fill = [&] {
+            if (!st) return;
             st->iters_v = d->getIterationsV();
             st->iters = d->getIterations();
             st->err_v = d->getDivergenceError();
@@ -439,7 +492,18 @@ int dfsph_cuda_step(void* s, dfsph_cuda_stats* st)
             st->removed = d->getRemovedLastStep();
             st->inflow_added = d->getInflowAddedLastStep();
             st->outflow_removed = d->getOutflowRemovedLastStep();
+            st->max_neighbors_seen = d->getMaxNeighborsSeen();
+            st->overflow = d->getNeighborOverflow();
+        };
+        try {
+            d->step();
+        } catch (const SPH::DFSPHError& e) {
+            // a truncated neighbor list is reported after a whole step: the
+            // statistics describe that step
+            if (e.code == DFSPH_ERR_NEIGHBOR_OVERFLOW) fill();
+            throw;
         }
+        fill();
     });
 }
 

@@ -112,6 +112,10 @@ public:
     double getDensityError() const { return m_densityError; }
     double getDivergenceError() const { return m_divergenceError; }
     int getRemovedLastStep() const { return m_removedLastStep; }
+    // largest neighbor list of this step's search, and the particles whose
+    // list was truncated at max_neighbors (step() throws when there are any)
+    uint32_t getMaxNeighborsSeen() const { return m_maxNeighborsSeen; }
+    uint32_t getNeighborOverflow() const { return m_neighborOverflow; }
 
     // DFSPHCUDA::zeroFluidVelocities
     void zeroFluidVelocities() { m_data.zeroFluidVelocities(); }
@@ -170,6 +174,7 @@ protected:
     bool m_openBoundaries = false;
     OpenBoundaries m_ob;
     uint32_t m_inflowAdded = 0, m_outflowRemoved = 0;
+    uint32_t m_maxNeighborsSeen = 0, m_neighborOverflow = 0;
 };
 
 }  // namespace SPH

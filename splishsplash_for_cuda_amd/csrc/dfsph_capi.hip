@@ -129,7 +129,7 @@ struct Handle {
     dfsph::HostRecord last{};
     // ABI calls on this handle (counted by H()); the op of the last record
     // wait and of the last step end, for the no-synchronisation fast paths
-    uint64_t ops = 0, rec_op = 0, end_op = 0, pos_op = 0;
+    uint64_t ops = 0, rec_op = 0, end_op = 0, pos_op = 0, nbr_op = 0;
     uint32_t end_seq = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
 
@@ -402,6 +402,7 @@ int neighbor_search(Handle* h, bool sort)
     HIPCHK(hipGetLastError());
     h->lists_valid = true;
     h->acc_valid = false;
+    h->nbr_op = h->ops;
     return DFSPH_OK;
 }
 
@@ -496,11 +497,19 @@ int learned_predict(Handle* h)
     if (learned && !(h->mlp && dfsph::mlp_loaded(h->mlp)))
         return fail(DFSPH_ERR_NO_MODEL, "learned warm start selected but no model loaded");
     CHK(dfsph::mlp_ensure_particles(h->mlp, h->cap));
+    const int pf = pbeg(h, DFSPH_KERNEL_FEATURES);
     dfsph::launch_features(h->s, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h), h->vel[h->vc],
-                           h->ktot[h->pc], h->kvtot[h->pc], (float)(2.0 * h->p.particle_radius),
-                           dfsph::mlp_feat(h->mlp));
+                           h->kap[h->pc], h->kapv[h->pc], h->ktot[h->pc], h->kvtot[h->pc],
+                           (float)(2.0 * h->p.particle_radius), dfsph::mlp_feat(h->mlp), dfsph::mlp_prev_p(h->mlp),
+                           dfsph::mlp_prev_v(h->mlp));
+    pend(h, pf);
     HIPCHK(hipGetLastError());
-    if (learned) CHK(dfsph::mlp_infer_particles(h->mlp, h->s, h->n));
+    if (learned) {
+        const int pm = pbeg(h, DFSPH_KERNEL_MLP_INFER);
+        const int rc = dfsph::mlp_infer_particles(h->mlp, h->s, h->n);
+        pend(h, pm);
+        CHK(rc);
+    }
     h->pred_valid = true;
     return DFSPH_OK;
 }
@@ -510,13 +519,12 @@ int divergence_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* i
     CHK(flush_vel(h));
     if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale: call dfsph_neighbor_search");
     const int ws = h->p.warm_start_mode;
-    const float* kws = h->kapv[h->pc];
     CHK(learned_predict(h));
-    if (ws == DFSPH_WARMSTART_LEARNED) kws = dfsph::mlp_pred_v(h->mlp);
+    const dfsph::WsLearned lw = ws == DFSPH_WARMSTART_LEARNED ? dfsph::mlp_ws_v(h->mlp) : dfsph::WsLearned{};
     const int vo = h->vc ^ 1;
     int pr = pbeg(h, DFSPH_KERNEL_DIV_WS_INIT);
     dfsph::launch_div_ws_init(h->s, ws, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h), h->vel[h->vc],
-                              h->vel[vo], kws, h->rho, h->alpha, h->radv, h->kvtot[h->pc]);
+                              h->vel[vo], h->kapv[h->pc], lw, h->rho, h->alpha, h->radv, h->kvtot[h->pc]);
     pend(h, pr);
     h->vc = vo;
     pr = pbeg(h, DFSPH_KERNEL_DIV_INIT);
@@ -539,7 +547,8 @@ int density_only(Handle* h)
     CHK(learned_predict(h));
     const int vo = h->vc ^ 1;
     dfsph::launch_div_ws_init(h->s, DFSPH_WARMSTART_NONE, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h),
-                              h->vel[h->vc], h->vel[vo], nullptr, h->rho, h->alpha, h->radv, h->kvtot[h->pc]);
+                              h->vel[h->vc], h->vel[vo], nullptr, dfsph::WsLearned{}, h->rho, h->alpha, h->radv,
+                              h->kvtot[h->pc]);
     h->vc = vo;
     HIPCHK(hipGetLastError());
     return DFSPH_OK;
@@ -562,10 +571,10 @@ int pressure_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* ite
 {
     if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale: call dfsph_neighbor_search");
     const int ws = h->p.warm_start_mode;
-    const float* kws = h->kap[h->pc];
+    dfsph::WsLearned lw{};
     if (ws == DFSPH_WARMSTART_LEARNED) {
         CHK(learned_predict(h));
-        kws = dfsph::mlp_pred_p(h->mlp);
+        lw = dfsph::mlp_ws_p(h->mlp);
     }
     if (ws == DFSPH_WARMSTART_NONE) {
         CHK(flush_vel(h));
@@ -574,7 +583,7 @@ int pressure_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* ite
         const bool fused = h->pend_vel;
         const int pw = pbeg(h, DFSPH_KERNEL_PRESSURE_WS);
         dfsph::launch_pressure_ws(h->s, ws, fused, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h),
-                                  h->acc, h->vel[h->vc], kws, fused ? 1 : 0, h->ktot[h->pc]);
+                                  h->acc, h->vel[h->vc], h->kap[h->pc], lw, fused ? 1 : 0, h->ktot[h->pc]);
         pend(h, pw);
         h->pend_vel = false;
     }
@@ -588,8 +597,12 @@ int pressure_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* ite
     CHK(jacobi_loop(h, false, max_iter, max_error, iters, err));
     // the step's density total (applied warm start + increments) in kappa h_f^2
     dfsph::launch_finish_totals(h->s, h->tm, h->n, 1, h->kap[h->pc], h->ktot[h->pc]);
-    if (h->mlp && dfsph::mlp_harvesting(h->mlp))
-        CHK(dfsph::mlp_harvest_step(h->mlp, h->s, h->n, h->kvtot[h->pc], h->ktot[h->pc]));
+    if (h->mlp && dfsph::mlp_harvesting(h->mlp)) {
+        // targets: what the reference warm start left to the iterations
+        if (ws != DFSPH_WARMSTART_REFERENCE)
+            return fail(DFSPH_ERR_STATE, "a training set is harvested with the reference warm start");
+        CHK(dfsph::mlp_harvest_step(h->mlp, h->s, h->tm, h->n, h->kapv[h->pc], h->kap[h->pc]));
+    }
     HIPCHK(hipGetLastError());
     return DFSPH_OK;
 }
@@ -770,7 +783,10 @@ int body_from_abi(const dfsph_rigid_body* b, double t_now, dfsph::DevBody& d)
         d.c0[a] = b->position[a];
         d.axis[a] = an > 0.0 ? b->axis[a] / an : 0.0;
     }
-    for (int a = 0; a < 4; ++a) d.q[a] = b->rotation[a] / qn;
+    // a quaternion already of unit length is kept as it is, so that a state read
+    // back by dfsph_get_rigid_body restores bit for bit
+    const double qs = std::fabs(qn - 1.0) > 1e-12 ? 1.0 / qn : 1.0;
+    for (int a = 0; a < 4; ++a) d.q[a] = b->rotation[a] * qs;
     d.clock = t_now;
     if (b->type == DFSPH_BODY_SINE) {
         d.amp = b->amplitude;
@@ -1251,6 +1267,8 @@ int dfsph_download(void* hv, int field, void* dst, uint32_t count)
     case DFSPH_FIELD_DENSITY_ADV: return get_raw(h->radv, 4ull * n);
     case DFSPH_FIELD_KAPPA: return get_raw(h->kap[h->pc], 4ull * n);
     case DFSPH_FIELD_KAPPAV: return get_raw(h->kapv[h->pc], 4ull * n);
+    case DFSPH_FIELD_KAPPA_TOTAL: return get_raw(h->ktot[h->pc], 4ull * n);
+    case DFSPH_FIELD_KAPPAV_TOTAL: return get_raw(h->kvtot[h->pc], 4ull * n);
     case DFSPH_FIELD_ID: return get_raw(h->id[h->pc], 4ull * n);
     case DFSPH_FIELD_TAG: return get_raw(h->tag[h->pc], 4ull * n);
     case DFSPH_FIELD_NBR_FLUID:
@@ -1293,6 +1311,12 @@ int dfsph_upload_field(void* hv, int field, const void* src, uint32_t count)
     if (field == DFSPH_FIELD_KAPPA) { HIPCHK(hipMemcpy(h->kap[h->pc], src, 4ull * n, hipMemcpyHostToDevice)); return DFSPH_OK; }
     if (field == DFSPH_FIELD_KAPPAV) { HIPCHK(hipMemcpy(h->kapv[h->pc], src, 4ull * n, hipMemcpyHostToDevice)); return DFSPH_OK; }
     if (field == DFSPH_FIELD_TAG) { HIPCHK(hipMemcpy(h->tag[h->pc], src, 4ull * n, hipMemcpyHostToDevice)); return DFSPH_OK; }
+    if (field == DFSPH_FIELD_KAPPA_TOTAL || field == DFSPH_FIELD_KAPPAV_TOTAL) {
+        float* d = field == DFSPH_FIELD_KAPPA_TOTAL ? h->ktot[h->pc] : h->kvtot[h->pc];
+        HIPCHK(hipMemcpy(d, src, 4ull * n, hipMemcpyHostToDevice));
+        h->pred_valid = false;
+        return DFSPH_OK;
+    }
     return fail(DFSPH_ERR_ARG, "field cannot be uploaded");
 }
 
@@ -1359,6 +1383,33 @@ int dfsph_get_info(void* hv, dfsph_info* info)
 }
 
 int dfsph_synchronize(void* hv) { return sync(H(hv)); }
+
+int dfsph_neighbor_stats(void* hv, uint32_t* max_seen, uint32_t* overflow, uint64_t* fluid_total,
+                         uint64_t* boundary_total)
+{
+    Handle* h = H(hv);
+    uint32_t ms, of;
+    uint64_t nf, nb;
+    if (h->rec_op > h->nbr_op && h->last.seq == h->seq) {
+        // the last record the host has seen was queued after the last search
+        // and copied its statistics
+        ms = h->last.max_seen;
+        of = h->last.overflow;
+        nf = h->last.nbr_fluid;
+        nb = h->last.nbr_boundary;
+    } else {
+        CHK(read_ctl(h));
+        ms = h->hctl->max_seen;
+        of = h->hctl->overflow;
+        nf = h->hctl->nbr_fluid;
+        nb = h->hctl->nbr_boundary;
+    }
+    if (max_seen) *max_seen = ms;
+    if (overflow) *overflow = of;
+    if (fluid_total) *fluid_total = nf;
+    if (boundary_total) *boundary_total = nb;
+    return DFSPH_OK;
+}
 
 // ---- dynamic window: restriction and tags ----
 int dfsph_set_restriction(void* hv, int mode, uint32_t count_active, uint32_t count_active_neighbors,

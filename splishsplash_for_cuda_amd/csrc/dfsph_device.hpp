@@ -774,26 +774,35 @@ __device__ __forceinline__ double block_sum(double v, double* lds4)
     return t;   // valid in thread 0
 }
 
-// The learned warm start predicts a correction to the reference warm start, in
-// the dt-invariant units (kappaV h, kappa h_f^2).  The reference forms its warm
-// starts from the previous step's totals, which are features 5 (kappaV h) and
-// 4 (kappa h_f^2) of a particle: kappaV h / 2 for the divergence solver
-// (DFSPH_core_cuda.cu:240) and kappa h_f^2 for the density solver (:3063-3068),
-// before the rescale to the new step and the clamp at -0.5 that the sweeps apply.
+// The learned warm start adds the network's correction to the reference warm
+// start.  Features 4 and 5 of a particle hold the last solves' increments in
+// the dt-invariant units (kappa h_f^2, kappaV h), from which the reference
+// forms its warm starts (DFSPH_core_cuda.cu:240, :3063-3068); the network's
+// targets are the increments the next solves still need beyond those warm
+// starts, in the same units.
 constexpr int kFeatKappaP = 4, kFeatKappaV = 5;
-__host__ __device__ __forceinline__ float ws_base(const float* feat_row, int out)
+
+// The warm start a sweep applies in learned mode, in raw units: the
+// reference's (ref, the value of the reference warm start), plus the
+// network's correction dy, bounded between min(ref, prev) and max(ref, 0),
+// prev the particle's converged total of the last step in the current raw
+// units (clamped at -0.5 like the reference).  It asks for no more pressure
+// (a more negative kappa) than the larger of the reference warm start and
+// the pressure the last step converged to, and for none where the reference
+// asks for none.  A stronger start pushes the particles apart, which the
+// solvers' errors, sums of compressions only, do not see.  dy = 0 gives ref
+// exactly.
+__host__ __device__ __forceinline__ float ws_learned(float ref, float prev, float dy)
 {
-    return out == 0 ? 0.5f * feat_row[kFeatKappaV] : feat_row[kFeatKappaP];
+    return fminf(fmaxf(ref + dy, fminf(ref, prev)), fmaxf(ref, 0.0f));
 }
 
-// The warm start from the network's output y: the reference warm start plus the
-// correction, bounded to ask for no more pressure than the reference warm
-// start (kappa, kappaV <= 0).  A stronger start pushes the particles apart,
-// which the solvers' errors, sums of compressions only, do not see.
-__host__ __device__ __forceinline__ float ws_learned(const float* feat_row, int out, float y)
-{
-    const float w = y + ws_base(feat_row, out);
-    return fmaxf(w, ws_base(feat_row, out));
-}
+// The learned warm start's inputs to the sweeps: the network's corrections
+// (dt-invariant units) and the last step's converged totals (same units), one
+// float per particle each.
+struct WsLearned {
+    const float* dy;
+    const float* prev;
+};
 
 }  // namespace dfsph

@@ -874,31 +874,30 @@ __global__ __launch_bounds__(kBlock) void k_solve_finalize(SolveCtl* ctl, const 
 // rho_i = m_i W0 + sum m_j W_ij; alpha_i = -(1/h) / max(sum |m_j gradW|^2 + |sum m_j gradW|^2, eps);
 // rho_adv_i = 0 if fewer than 20 neighbors else max(sum m_j (v_i - v_j).gradW, 0).
 // Warm start (only where rho_adv_i > 0): v_i += h sum (k_i + k_j) m_j gradW with
-// k = max(kappaV h_ratio / 2, -0.5) (WS == 0), the learned prediction of
-// kappaV h (dt-invariant) / h, clamped like the reference (WS == 1), or nothing
-// (WS == 2).  Velocities are read from vel_i and written to vel_o.  The
-// applied k_i starts the particle's total of the step in kvtot (when given).
+// k = max(kappaV h_ratio / 2, -0.5) (WS == 0), that value plus the learned
+// correction, bounded by ws_learned (WS == 1), or nothing (WS == 2).  kws holds
+// the last solve's kappaV increments.  Velocities are read from vel_i and
+// written to vel_o.  The applied k_i starts the particle's total of the step
+// in kvtot (when given).
 template <int WS, bool SOL>
 __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_ws_init(SimConst c, const DevTime* t, uint32_t n,
                                                         const float4* __restrict__ pos,
                                                         const float4* __restrict__ bpos, NbrLists L,
                                                         const float4* __restrict__ vel_i,
                                                         float4* __restrict__ vel_o,
-                                                        const float* __restrict__ kws,
+                                                        const float* __restrict__ kws, WsLearned lw,
                                                         float* __restrict__ rho, float* __restrict__ alpha,
                                                         float* __restrict__ radv, float* __restrict__ kvtot)
 {
     __shared__ float4 sP[kStageMaxPV];
     __shared__ float4 sV[kStageMaxPV];           // (v, warm-start k) of the staged set
     const float ratio = t->ratio, h = t->h, inv_h = t->inv_h;
-    auto ws_val = [&](float raw) -> float {
-        if (WS == 0) return fmaxf(raw * ratio / 2.0f, -0.5f);
-        // learned: clamped like the reference below and at 0 above, the range
-        // of the converged total (a sum of rho_adv alpha <= 0 terms)
-        if (WS == 1) return fminf(fmaxf(raw * inv_h, -0.5f), 0.0f);
-        return 0.0f;
+    auto ws_of = [&](uint32_t j) -> float {
+        if (WS == 2) return 0.0f;
+        const float ref = fmaxf(kws[j] * ratio / 2.0f, -0.5f);
+        if (WS == 0) return ref;
+        return ws_learned(ref, fmaxf(lw.prev[j] * inv_h, -0.5f), lw.dy[j] * inv_h);
     };
-    auto ws_of = [&](uint32_t j) -> float { return WS == 2 ? 0.0f : ws_val(kws[j]); };
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t ic = i < n ? i : 0u;         // own loads before staging, so they overlap it
@@ -906,13 +905,12 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_ws_init(SimConst
     const Head hd = list_head(L, sn0, c.cap, ic);
     const float4 vi = vel_i[ic];
     const float4 pi = pos[ic];
-    const float kraw = WS == 2 ? 0.0f : kws[ic];
+    const float ki = ws_of(ic);
     const uint32_t sn = stage_block<kStageMaxPV>(L, b, sn0, [&](uint32_t g) {
         const float4 v = vel_i[g];
         return PV{pos[g], make_float4(v.x, v.y, v.z, ws_of(g))};
     }, [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
     if (i >= n) return;
-    const float ki = ws_val(kraw);
     if (kvtot) kvtot[i] = ki;
     if (!is_source(L.rs, i)) { vel_o[i] = vi; return; }
     const uint32_t cn = hd.cnt;
@@ -1206,16 +1204,17 @@ __global__ void k_update_vel(const DevTime* t, uint32_t n, const float4* acc, fl
 // update of DFSPH_update_vel_kernel (APPLY_ACC):
 // v_i += h a_i;  v_i += h_f (sum_f [|k_i+k_j|>eps] (k_i+k_j) m_j gradW + [|k_i|>eps] sum_b k_i psi_j gradW)
 // with k = max(kappa ratio2, -0.5) (WS == 0; ratio2 = 1 when already applied),
-// or the learned prediction of kappa h_f^2 (dt-invariant) / h_f^2, clamped like
-// the reference (WS == 1).  No gate (unlike the divergence warm start).  The
-// applied k_i starts the particle's total of the step in ktot (when given).
+// or that value plus the learned correction, bounded by ws_learned (WS == 1).
+// kws holds the last solve's kappa increments.  No gate (unlike the divergence
+// warm start).  The applied k_i starts the particle's total of the step in
+// ktot (when given).
 template <int WS, bool APPLY_ACC, bool SOL>
 __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTime* t, uint32_t n,
                                                         const float4* __restrict__ pos,
                                                         const float4* __restrict__ bpos, NbrLists L,
                                                         const float4* __restrict__ acc,
                                                         float4* __restrict__ vel,
-                                                        const float* __restrict__ kws, int rescale,
+                                                        const float* __restrict__ kws, WsLearned lw, int rescale,
                                                         float* __restrict__ ktot)
 {
     __shared__ float4 sP[kStageMax];
@@ -1223,8 +1222,9 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
     const float ratio2 = rescale ? t->ratio2 : 1.0f;
     const float ihf2 = t->inv_h_future * t->inv_h_future;
     auto ws_of = [&](uint32_t j) -> float {
-        if (WS == 0) return fmaxf(kws[j] * ratio2, -0.5f);
-        return fminf(fmaxf(kws[j] * ihf2, -0.5f), 0.0f);        // learned: see k_div_ws_init
+        const float ref = fmaxf(kws[j] * ratio2, -0.5f);
+        if (WS == 0) return ref;
+        return ws_learned(ref, fmaxf(lw.prev[j] * ihf2, -0.5f), lw.dy[j] * ihf2);
     };
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
@@ -1409,7 +1409,7 @@ __global__ void k_finish_totals(const DevTime* t, uint32_t n, int pressure, cons
 // (before the divergence warm start):
 //  0 rho / rho0 - 1            1 divergence source h / rho0 (unclamped)
 //  2 (n_fluid + n_boundary) / 40   3 |v| h / (2r)
-//  4 previous total kappa h_f^2    5 previous total kappaV h
+//  4 last density increments kappa h_f^2   5 last divergence increments kappaV h
 //  6 boundary fraction n_b / n     7-9 mean neighbor offset / h_k
 // 10 mean neighbor distance / h_k 11 fluid share of the density
 template <bool SOL>
@@ -1417,9 +1417,12 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_features(SimConst c,
                                                      const float4* __restrict__ pos,
                                                      const float4* __restrict__ bpos, NbrLists L,
                                                      const float4* __restrict__ vel,
+                                                     const float* __restrict__ kap,
+                                                     const float* __restrict__ kapv,
                                                      const float* __restrict__ ktot,
                                                      const float* __restrict__ kvtot, float diameter,
-                                                     float* __restrict__ feat)
+                                                     float* __restrict__ feat, float* __restrict__ prev_p,
+                                                     float* __restrict__ prev_v)
 {
     __shared__ float4 sP[kStageMaxPV];
     __shared__ float4 sV[kStageMaxPV];
@@ -1474,21 +1477,24 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_features(SimConst c,
     f[1] = dr * t->h / c.density0;
     f[2] = nn * (1.0f / 40.0f);
     f[3] = __builtin_sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z) * t->h / diameter;
-    f[kFeatKappaP] = ktot[i];
-    f[kFeatKappaV] = kvtot[i];
+    f[kFeatKappaP] = kap[i] * (t->h * t->h);
+    f[kFeatKappaV] = kapv[i] * t->h_past;
     f[6] = (float)nb * inv_n;
     f[7] = ox * s;
     f[8] = oy * s;
     f[9] = oz * s;
     f[10] = od * s;
     f[11] = dfl / dens;
+    prev_p[i] = ktot[i];
+    prev_v[i] = kvtot[i];
 }
 
-// training set: append the features of the step and the step's totals less
-// the reference warm start formed from the features (ws_base), the correction
-// the network learns
-__global__ void k_harvest(uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap, const float* feat,
-                          const float* kvtot, const float* ktot, float* ds_x, float* ds_t)
+// training set, after a step's solves with the reference warm start: append
+// the step's features and the solves' increments in dt-invariant units
+// (kappaV h, kappa h_f^2), what the reference warm start left to the Jacobi
+// iterations: the correction the network learns
+__global__ void k_harvest(const DevTime* t, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap,
+                          const float* feat, const float* kapv, const float* kap, float* ds_x, float* ds_t)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t i = k * stride;
@@ -1496,8 +1502,8 @@ __global__ void k_harvest(uint32_t n, uint32_t stride, uint32_t offset, uint32_t
     const size_t o = offset + k;
     const float* f = feat + (size_t)i * 12;
     for (int a = 0; a < 12; ++a) ds_x[o * 12 + a] = f[a];
-    ds_t[o * 2 + 0] = kvtot[i] - ws_base(f, 0);
-    ds_t[o * 2 + 1] = ktot[i] - ws_base(f, 1);
+    ds_t[o * 2 + 0] = kapv[i] * t->h;
+    ds_t[o * 2 + 1] = kap[i] * (t->h_future * t->h_future);
 }
 
 // ------------------------------------------------------------------------
@@ -1627,13 +1633,13 @@ void launch_set_w(hipStream_t s, float4* p, const float* w, uint32_t n)
 
 void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime* t, uint32_t n,
                         const float4* pos, const float4* bpos, const NbrLists& L,
-                        const float4* vel_i, float4* vel_o, const float* kws, float* rho, float* alpha,
-                        float* radv, float* kvtot)
+                        const float4* vel_i, float4* vel_o, const float* kws, const WsLearned& lw,
+                        float* rho, float* alpha, float* radv, float* kvtot)
 {
     if (!n) return;
     const uint32_t g = nblk_xcd(n);
     const bool sol = L.sol.bvel != nullptr;
-#define DWS(W, S) k_div_ws_init<W, S><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel_i, vel_o, kws, rho, alpha, radv, kvtot)
+#define DWS(W, S) k_div_ws_init<W, S><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel_i, vel_o, kws, lw, rho, alpha, radv, kvtot)
     if (ws == 0) { if (sol) DWS(0, true); else DWS(0, false); }
     else if (ws == 1) { if (sol) DWS(1, true); else DWS(1, false); }
     else { if (sol) DWS(2, true); else DWS(2, false); }
@@ -1688,12 +1694,13 @@ void launch_update_vel(hipStream_t s, const DevTime* t, uint32_t n, const float4
 
 void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c, const DevTime* t, uint32_t n,
                         const float4* pos, const float4* bpos, const NbrLists& L,
-                        const float4* acc, float4* vel, const float* kws, int rescale, float* ktot)
+                        const float4* acc, float4* vel, const float* kws, const WsLearned& lw, int rescale,
+                        float* ktot)
 {
     if (!n) return;
     const uint32_t g = nblk_xcd(n);
     const bool sol = L.sol.bvel != nullptr;
-#define PW(W, A, S) k_pressure_ws<W, A, S><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, acc, vel, kws, rescale, ktot)
+#define PW(W, A, S) k_pressure_ws<W, A, S><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, acc, vel, kws, lw, rescale, ktot)
     if (ws == 1) {
         if (apply_acc) { if (sol) PW(1, true, true); else PW(1, true, false); }
         else { if (sol) PW(1, false, true); else PW(1, false, false); }
@@ -1748,21 +1755,24 @@ void launch_finish_totals(hipStream_t s, const DevTime* t, uint32_t n, int press
 }
 
 void launch_features(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
-                     const float4* bpos, const NbrLists& L, const float4* vel,
-                     const float* ktot, const float* kvtot, float diameter, float* feat)
+                     const float4* bpos, const NbrLists& L, const float4* vel, const float* kap,
+                     const float* kapv, const float* ktot, const float* kvtot, float diameter, float* feat,
+                     float* prev_p, float* prev_v)
 {
     if (!n) return;
     if (L.sol.bvel)
-        k_features<true><<<nblk_xcd(n), kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, ktot, kvtot, diameter, feat);
+        k_features<true><<<nblk_xcd(n), kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, kap, kapv, ktot, kvtot,
+                                                        diameter, feat, prev_p, prev_v);
     else
-        k_features<false><<<nblk_xcd(n), kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, ktot, kvtot, diameter, feat);
+        k_features<false><<<nblk_xcd(n), kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, kap, kapv, ktot, kvtot,
+                                                         diameter, feat, prev_p, prev_v);
 }
 
-void launch_harvest(hipStream_t s, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap, const float* feat,
-                    const float* kvtot, const float* ktot, float* ds_x, float* ds_t)
+void launch_harvest(hipStream_t s, const DevTime* t, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap,
+                    const float* feat, const float* kapv, const float* kap, float* ds_x, float* ds_t)
 {
     const uint32_t m = (n + stride - 1) / stride;
-    if (m) k_harvest<<<nblk(m), kBlock, 0, s>>>(n, stride, offset, cap, feat, kvtot, ktot, ds_x, ds_t);
+    if (m) k_harvest<<<nblk(m), kBlock, 0, s>>>(t, n, stride, offset, cap, feat, kapv, kap, ds_x, ds_t);
 }
 
 }  // namespace dfsph

@@ -50,8 +50,8 @@ void launch_wall_vel(hipStream_t s, float4* bvel, uint32_t n);
 
 void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime* t, uint32_t n,
                         const float4* pos, const float4* bpos, const NbrLists& L,
-                        const float4* vel_i, float4* vel_o, const float* kws, float* rho, float* alpha,
-                        float* radv, float* kvtot);
+                        const float4* vel_i, float4* vel_o, const float* kws, const WsLearned& lw,
+                        float* rho, float* alpha, float* radv, float* kvtot);
 void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
                        const float4* bpos, const NbrLists& L, const float4* vel,
                        const float* alpha, float* radv, float* kk, float* kappaV, SolveCtl* ctl, double* partial,
@@ -68,7 +68,8 @@ void launch_update_vel(hipStream_t s, const DevTime* t, uint32_t n, const float4
                        const Restrict& rs);
 void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c, const DevTime* t, uint32_t n,
                         const float4* pos, const float4* bpos, const NbrLists& L,
-                        const float4* acc, float4* vel, const float* kws, int rescale, float* ktot);
+                        const float4* acc, float4* vel, const float* kws, const WsLearned& lw, int rescale,
+                        float* ktot);
 void launch_density_source(hipStream_t s, bool init, const SimConst& c, const DevTime* t, uint32_t n,
                            const float4* pos, const float4* bpos, const NbrLists& L,
                            const float4* vel, const float* rho, float* alpha, float* radv, float* kk,
@@ -81,11 +82,14 @@ void launch_keep_from_u8(hipStream_t s, uint32_t n, const uint8_t* k8, uint32_t*
 void launch_compact(hipStream_t s, uint32_t n, const uint32_t* keep, const uint32_t* rank, const FluidPtrs& in,
                     const FluidPtrs& out);
 void launch_finish_totals(hipStream_t s, const DevTime* t, uint32_t n, int pressure, const float* inc, float* tot);
+// features of the learned warm start; also copies the last step's totals
+// (ktot, kvtot) into prev_p, prev_v for the sweeps' bound
 void launch_features(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n, const float4* pos,
-                     const float4* bpos, const NbrLists& L, const float4* vel,
-                     const float* ktot, const float* kvtot, float diameter, float* feat);
-void launch_harvest(hipStream_t s, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap, const float* feat,
-                    const float* kvtot, const float* ktot, float* ds_x, float* ds_t);
+                     const float4* bpos, const NbrLists& L, const float4* vel, const float* kap,
+                     const float* kapv, const float* ktot, const float* kvtot, float diameter, float* feat,
+                     float* prev_p, float* prev_v);
+void launch_harvest(hipStream_t s, const DevTime* t, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap,
+                    const float* feat, const float* kapv, const float* kap, float* ds_x, float* ds_t);
 
 // open boundary / dynamic window (dfsph_window.hip)
 void launch_surface_keep(hipStream_t s, const float4* p, uint32_t n, const Surface& S, int want_inside, uint32_t* keep);

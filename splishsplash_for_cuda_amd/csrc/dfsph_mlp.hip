@@ -151,8 +151,7 @@ __device__ void forward_wave(const DevModel& M, const WaveBufs& w, float out[4][
 
 // ---------------------------------------------------------------- inference
 // x: [n][12] raw features (FP32); output o of row r goes to y[r * ystride + o * ocol]
-// (denormalised when normalise is set; plus the reference warm start of the
-// row's features when residual is set, the solver's prediction).
+// (denormalised when normalise is set: the solver's corrections).
 //
 // Register-resident form: each layer is computed transposed, H_out^T = W H_in^T,
 // with W as the MFMA A operand and the particles as the N dimension.  The
@@ -197,7 +196,7 @@ __device__ __forceinline__ bf16x4 act_frag(const f32x4& z)
 
 __global__ __launch_bounds__(kInferBlock, 3) void k_mlp_infer(const DevModel* __restrict__ Mg, const float* __restrict__ x,
                                                            uint32_t n, float* __restrict__ y, int normalise,
-                                                           int ystride, int ocol, int residual)
+                                                           int ystride, int ocol)
 {
     __shared__ __align__(16) float sb[3 * kH + kOutPad];
     const DevModel& M = *Mg;
@@ -264,7 +263,6 @@ __global__ __launch_bounds__(kInferBlock, 3) void k_mlp_infer(const DevModel* __
             for (int o = 0; o < kOut; ++o) {
                 float val = acc[o];
                 if (normalise) val = val * M.norm[2 * kIn + kOut + o] + M.norm[2 * kIn + o];
-                if (residual) val = ws_learned(x + (size_t)p * kIn, o, val);
                 y[(size_t)p * ystride + (size_t)o * ocol] = val;
             }
         }
@@ -479,7 +477,8 @@ struct MlpState {
     bool loaded = false;
     // per-particle buffers of the solver integration
     float* feat = nullptr;      // [cap][12]
-    float* pred = nullptr;      // [cap][2]  dt-invariant (kappaV h, kappa h_f^2)
+    float* pred = nullptr;      // [4][cap]: corrections (kappaV h, kappa h_f^2), then the last
+                                // step's converged totals in the same units (the sweeps' bound)
     float* kvws = nullptr;      // predicted kappaV warm start (GPU units)
     uint32_t cap = 0;
     // training set
@@ -604,7 +603,7 @@ int dfsph_mlp_forward(void* handle, const float* x, uint32_t n, float* y, int no
     if (!rc) {
         (void)hipMemcpyAsync(dx, x, sizeof(float) * n * dfsph::kIn, hipMemcpyHostToDevice, s);
         dfsph::k_mlp_infer<<<dfsph::infer_grid(n), dfsph::kInferBlock, 0, s>>>(m->model, dx, n, dy, normalise,
-                                                                                  dfsph::kOut, 1, 0);
+                                                                                  dfsph::kOut, 1);
         (void)hipMemcpyAsync(y, dy, sizeof(float) * n * dfsph::kOut, hipMemcpyDeviceToHost, s);
         if (hipStreamSynchronize(s) != hipSuccess) rc = mfail(DFSPH_ERR_HIP, "mlp forward");
     }
@@ -653,15 +652,17 @@ namespace dfsph {
 bool mlp_loaded(const MlpState* m) { return m && m->loaded; }
 bool mlp_harvesting(const MlpState* m) { return m && m->ds_cap > 0 && m->ds_n < m->ds_cap; }
 float* mlp_feat(MlpState* m) { return m->feat; }
-const float* mlp_pred_v(const MlpState* m) { return m->pred; }
-const float* mlp_pred_p(const MlpState* m) { return m->pred + m->cap; }
+WsLearned mlp_ws_v(const MlpState* m) { return WsLearned{m->pred, m->pred + 2 * (size_t)m->cap}; }
+WsLearned mlp_ws_p(const MlpState* m) { return WsLearned{m->pred + m->cap, m->pred + 3 * (size_t)m->cap}; }
+float* mlp_prev_p(MlpState* m) { return m->pred + 3 * (size_t)m->cap; }
+float* mlp_prev_v(MlpState* m) { return m->pred + 2 * (size_t)m->cap; }
 
 int mlp_ensure_particles(MlpState* m, uint32_t cap)
 {
     if (!m) return mfail(DFSPH_ERR_STATE, "no learned warm start state");
     if (cap <= m->cap && m->feat) return DFSPH_OK;
     if (int rc = malloc_dev(m->feat, (size_t)cap * kIn)) return rc;
-    if (int rc = malloc_dev(m->pred, (size_t)cap * kOut)) return rc;
+    if (int rc = malloc_dev(m->pred, (size_t)cap * 4)) return rc;
     m->cap = cap;
     return DFSPH_OK;
 }
@@ -669,16 +670,16 @@ int mlp_ensure_particles(MlpState* m, uint32_t cap)
 int mlp_infer_particles(MlpState* m, hipStream_t s, uint32_t n)
 {
     if (n == 0) return DFSPH_OK;
-    k_mlp_infer<<<infer_grid(n), kInferBlock, 0, s>>>(m->model, m->feat, n, m->pred, 1, 1, (int)m->cap, 1);
+    k_mlp_infer<<<infer_grid(n), kInferBlock, 0, s>>>(m->model, m->feat, n, m->pred, 1, 1, (int)m->cap);
     MHIP(hipGetLastError());
     return DFSPH_OK;
 }
 
-int mlp_harvest_step(MlpState* m, hipStream_t s, uint32_t n, const float* kvtot, const float* ktot)
+int mlp_harvest_step(MlpState* m, hipStream_t s, const DevTime* t, uint32_t n, const float* kapv, const float* kap)
 {
     if (!mlp_harvesting(m) || n == 0) return DFSPH_OK;
     const uint32_t stride = m->ds_stride ? m->ds_stride : 1;
-    launch_harvest(s, n, stride, m->ds_n, m->ds_cap, m->feat, kvtot, ktot, m->ds_x, m->ds_t);
+    launch_harvest(s, t, n, stride, m->ds_n, m->ds_cap, m->feat, kapv, kap, m->ds_x, m->ds_t);
     MHIP(hipGetLastError());
     const uint32_t add = (n + stride - 1) / stride;
     m->ds_n = m->ds_n + add > m->ds_cap ? m->ds_cap : m->ds_n + add;

@@ -27,10 +27,14 @@ int mlp_ensure_particles(MlpState* m, uint32_t cap);
 float* mlp_feat(MlpState* m);            // [cap][12] features of the current step
 // predictions from the features of the first n particles
 int mlp_infer_particles(MlpState* m, hipStream_t s, uint32_t n);
-const float* mlp_pred_v(const MlpState* m);   // kappaV h     (dt-invariant)
-const float* mlp_pred_p(const MlpState* m);   // kappa h_f^2  (dt-invariant)
-// append (features, totals) of the step to the training set
-int mlp_harvest_step(MlpState* m, hipStream_t s, uint32_t n, const float* kvtot, const float* ktot);
+// the sweeps' inputs: corrections and last totals, kappaV h and kappa h_f^2
+WsLearned mlp_ws_v(const MlpState* m);
+WsLearned mlp_ws_p(const MlpState* m);
+// where the feature sweep copies the last step's totals
+float* mlp_prev_p(MlpState* m);
+float* mlp_prev_v(MlpState* m);
+// append (features, increments of the step's solves) to the training set
+int mlp_harvest_step(MlpState* m, hipStream_t s, const DevTime* t, uint32_t n, const float* kapv, const float* kap);
 void mlp_destroy(MlpState* m);
 
 }  // namespace dfsph

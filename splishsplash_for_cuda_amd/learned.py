@@ -1,15 +1,15 @@
 """Learned warm start: training-set harvest, GPU training, model files.
 
-The network (12 -> 64 -> 64 -> 64 -> 2, SiLU) predicts each particle's
-converged (kappaV h, kappa h_f^2) of a step from features of the step's
-initial state (see k_features in csrc/dfsph_kernels.hip), as a correction to
-the reference warm start: its targets are the converged totals less
-(f5 / 2, f4), the reference's warm starts formed from the previous step's
-totals that features 4 and 5 hold, and the solver adds those back to the
-network's outputs.  A network that outputs zero is therefore the reference
-warm start, and at rest, where the totals repeat from step to step, the
-targets are near zero.  Targets come from
-the plain solver, which keeps iterating to the same tolerances when the
+The network (12 -> 64 -> 64 -> 64 -> 2, SiLU) predicts a correction to each
+particle's reference warm starts of a step, from features of the step's
+initial state (see k_features in csrc/dfsph_kernels.hip).  Its targets are
+the increments that the plain solver's Jacobi iterations add after the
+reference warm start, in dt-invariant units (kappaV h, kappa h_f^2), harvested
+from runs with the reference warm start.  The solver adds the correction to
+the reference warm start, bounded so that it asks for no more pressure than
+the larger of the reference warm start and the particle's converged total
+of the last step: a network that outputs zero is the reference warm start,
+exactly.  The plain solver keeps iterating to the same tolerances when the
 learned warm start is used: the network only changes where the Jacobi
 iterations start.
 """
@@ -58,7 +58,8 @@ def make_solver(sc, params):
 
 def harvest_scene(sc, steps, max_samples, **kw):
     """Run the plain solver on one scene and collect (features, targets): the
-    targets are the converged totals less the reference warm start."""
+    targets are the increments of each step's solves after the reference
+    warm start, in dt-invariant units."""
     s = make_solver(sc, solver_params(sc, **kw))
     stride = max(1, int(np.ceil(sc.n_fluid * steps / max_samples)))
     s.mlp_harvest(max_samples, stride)

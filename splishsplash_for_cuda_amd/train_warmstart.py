@@ -3,7 +3,7 @@
     python -m splishsplash_for_cuda_amd.train_warmstart [--scenes 32] [--samples 2000000]
         [--steps 200] [--train-steps 6000] [--out splishsplash_for_cuda_amd/models/warmstart_mlp.npz]
 
-1. Harvest (features, converged totals) pairs from `--scenes` seeded dam breaks
+1. Harvest (features, solver increments) pairs from `--scenes` seeded dam breaks
    (seed 1000 + k, 50k-200k particles, block aspect ratios in [0.5, 2]) run by
    the plain solver with the reference warm start.
 2. Fit the input/output standardisation and train the network with the

@@ -1,7 +1,9 @@
 """The learned warm start end to end on the GPU: the committed model cuts the
 divergence iterations on a scene outside its training set while the solver
 still meets the same tolerances, the warm starts a step uses are the network's
-corrections added to the reference warm start, and GPU training lowers the loss.
+corrections added to the reference warm start (a zero network is the
+reference warm start exactly), the features and targets match the oracle's,
+and GPU training lowers the loss.
 
 Training scenes are seeded_dam_break(k, ...) for k < 32; the held-out scene
 here uses k = 77.  Bounds:
@@ -47,12 +49,37 @@ def test_trained_model_cuts_divergence_iterations_on_a_new_scene():
     assert lrn["rho_diff_pct"] <= 3.0 * none["rho_diff_pct"]
 
 
-def test_step_warm_start_is_reference_plus_correction():
-    """The warm starts of a learned step, read back with the step's features,
-    equal the device forward pass of those features plus the reference warm
-    start they hold (f5 / 2 for the divergence solver, f4 for the density
-    solver), neither stronger than the reference's (both are kappa <= 0
-    totals).  Same kernel arithmetic on both sides: exact."""
+def test_zero_network_is_the_reference_warm_start():
+    """A network whose outputs are zero (all parameters 0, output mean 0)
+    gives exactly the reference warm start: the learned run's states equal
+    the reference run's bit for bit over 8 steps, so the learned sweeps add
+    nothing but the bounded correction."""
+    from splishsplash_for_cuda_amd._native import MLP_NNORM, MLP_NPARAM, WARMSTART_LEARNED, WARMSTART_REFERENCE
+    sc = scene.seeded_dam_break(5, 20_000)
+    norm = np.zeros(MLP_NNORM, np.float32)
+    norm[12:24] = 1.0       # input std
+    norm[26:28] = 1.0       # output std; output mean 0
+    out = {}
+    for ws in (WARMSTART_REFERENCE, WARMSTART_LEARNED):
+        s = learned.make_solver(sc, learned.solver_params(sc, warm_start=ws))
+        if ws == WARMSTART_LEARNED:
+            s.mlp_set_params(np.zeros(MLP_NPARAM, np.float32), norm)
+        its = [(st.iters_v, st.iters) for st in (s.step() for _ in range(8))]
+        out[ws] = (s.state_by_id(), its)
+        if ws == WARMSTART_LEARNED:
+            f, pv, pp = s.mlp_step_features()
+            assert not pv.any() and not pp.any()
+        s.close()
+    a, b = out[WARMSTART_REFERENCE], out[WARMSTART_LEARNED]
+    assert a[1] == b[1]
+    for k in ("pos", "vel", "kappa", "kappaV", "density"):
+        assert np.array_equal(a[0][k], b[0][k]), k
+
+
+def test_step_predictions_are_the_device_forward_pass():
+    """The corrections a learned step uses are the device forward pass of
+    the step's features (same kernel arithmetic: exact), and the committed
+    model's corrections are not all zero."""
     from splishsplash_for_cuda_amd._native import WARMSTART_LEARNED
     sc = scene.seeded_dam_break(5, 20_000)
     params, norm = learned.load()
@@ -63,12 +90,67 @@ def test_step_warm_start_is_reference_plus_correction():
     f, pv, pp = s.mlp_step_features()
     y = s.mlp_forward(f, normalise=True)
     s.close()
-    bv = np.float32(0.5) * f[:, 5]
-    bp = f[:, 4]
-    assert (bp < 0).any() and (bv < 0).mean() > 0.1                # warm starts in play
-    np.testing.assert_array_equal(pv, np.maximum(y[:, 0] + bv, bv))
-    np.testing.assert_array_equal(pp, np.maximum(y[:, 1] + bp, bp))
-    assert (pv > bv).any() and (pv == bv).any()                    # the bound is active somewhere, not everywhere
+    np.testing.assert_array_equal(pv, y[:, 0])
+    np.testing.assert_array_equal(pp, y[:, 1])
+    assert (f[:, 5] < 0).mean() > 0.1 and (f[:, 4] < 0).any()     # increments in play
+    assert np.abs(pv).max() > 0
+
+
+def test_features_and_targets_match_oracle(oracle_mod):
+    """Features of a step (GPU k_features) and the training targets of the
+    same step (k_harvest) against the oracle's restatement, both sides from
+    the oracle's state after 4 steps of a small dam break (matched by id).
+    Tolerances: features 0, 3, 10, 11 rel 1e-5; 1 (a source) 1e-4 of its
+    largest magnitude; 2 and 6 exact; 4, 5 rel 1e-6 of the largest; 7-9
+    (offset means, sums of differences) 1e-5 absolute (in hk); targets as
+    kappa after a step (FP32 Jacobi sums)."""
+    import splishsplash_for_cuda_amd as S
+    from test_gpu_parity import _resync
+    om = oracle_mod
+    sc = scene.small_dam_break()
+    dt = 0.002
+    o = om.Oracle(particle_radius=sc.particle_radius, dt=dt, fp32_predicate=1)
+    o.set_boundary(sc.boundary.astype(np.float64))
+    o.set_fluid(sc.fluid.astype(np.float64))
+    for _ in range(4):
+        o.step()
+    s = S.Solver(S.default_params(particle_radius=sc.particle_radius, dt=dt))
+    s.upload_boundary(sc.boundary)
+    s.upload_fluid(sc.fluid)
+    _resync(s, o, om)
+    s.mlp_harvest(sc.n_fluid, 1)
+    s.neighbor_search()
+    o.neighbor_search()
+    ids = s.download(S.FIELD_ID)
+    where = {int(v): k for k, v in enumerate(o.ids())}
+    idx = np.array([where[int(v)] for v in ids])
+    fo = o.features()[idx]
+    for x in (s, o):
+        x.divergence_solve()
+        x.external_forces()
+        x.update_time_step(dt)
+        x.update_vel()
+    it_g, _ = s.pressure_solve()
+    it_o, _ = o.pressure_solve()
+    assert it_g == it_o
+    to = o.warm_start_targets()[idx]
+    fg, tg = s.mlp_dataset()
+    s.close()
+    assert fg.shape == (sc.n_fluid, 12)
+    print("feature |max err|:", np.abs(fg - fo).max(0))
+    for a in (0, 3, 10, 11):
+        np.testing.assert_allclose(fg[:, a], fo[:, a], rtol=1e-5, atol=1e-6, err_msg=str(a))
+    assert np.abs(fg[:, 1] - fo[:, 1]).max() <= 1e-4 * np.abs(fo[:, 1]).max()
+    for a in (2, 6):
+        np.testing.assert_array_equal(fg[:, a], fo[:, a].astype(np.float32))
+    for a in (4, 5):
+        assert np.abs(fg[:, a] - fo[:, a]).max() <= 1e-6 * np.abs(fo[:, a]).max(), a
+    assert np.abs(fg[:, 7:10] - fo[:, 7:10]).max() < 1e-5
+    for c in (0, 1):
+        sc_ = np.abs(to[:, c]).max()
+        assert sc_ > 0
+        err = np.abs(tg[:, c] - to[:, c])
+        assert (err > 1e-3 * sc_).mean() < 1e-3 and err.max() < 2e-2 * sc_, c
 
 
 def test_gpu_training_lowers_the_loss():

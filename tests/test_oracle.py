@@ -450,3 +450,320 @@ def test_restriction_freezes_the_inactive(oracle_mod):
     assert np.all(rho[ca + can:] == 0.0)            # never computed
     assert np.all(rho[ca:ca + can] > 0.0)           # source band: densities
     assert np.abs(vs[:ca]).max() > 0.0              # the active part moved
+
+
+# ------------------------------------------- closed forms of the cubic spline
+
+def _W(q, hk):
+    """Cubic spline by its definition (SPHKernels.h:13-92), q = r / hk:
+    k (6q^3 - 6q^2 + 1) on [0, 1/2], 2k (1 - q)^3 on [1/2, 1], k = 8 / (pi hk^3)."""
+    k = 8.0 / (math.pi * hk ** 3)
+    if q <= 0.5:
+        return k * (6 * q ** 3 - 6 * q ** 2 + 1)
+    return 2 * k * (1 - q) ** 3 if q <= 1.0 else 0.0
+
+
+# ---------------------------------------------------------------- boundary psi
+
+@pytest.mark.parametrize("spacing_over_r", [2, 1])
+def test_boundary_psi_flat_wall_by_enumeration(oracle_mod, spacing_over_r):
+    """psi = rho0 / (W(0) + sum_j W_ij) over the other wall particles
+    (FluidModel::computeBoundaryPsi, FluidModel.cpp:233-258) on a flat square
+    lattice of wall particles: at spacing 2r = hk/2 the center has 4 neighbors
+    at hk/2 and 4 at hk/sqrt(2), an edge particle 3 and 2, a corner 2 and 1
+    (closed forms); at spacing r the offsets (a, b) with a^2 + b^2 < 16 are
+    enumerated.  A dropped self term, a wrong numerator or a wrong support
+    radius changes psi by far more than the tolerance."""
+    r = 0.025
+    hk = 4 * r
+    s = spacing_over_r * r
+    n = 21
+    a, b = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    x = np.stack([a.ravel() * s, np.zeros(n * n), b.ravel() * s], axis=1) + 0.5
+    o = oracle_mod.Oracle(particle_radius=r)
+    o.set_boundary(x)
+    psi = o.boundary_psi(n * n)
+    k = 8.0 / (math.pi * hk ** 3)
+    center, edge, corner = (n // 2) * n + n // 2, (n // 2) * n, 0
+    if spacing_over_r == 2:
+        w_half, w_diag = k / 4.0, 2 * k * (1 - math.sqrt(0.5)) ** 3
+        want = {center: k + 4 * w_half + 4 * w_diag, edge: k + 3 * w_half + 2 * w_diag,
+                corner: k + 2 * w_half + w_diag}
+    else:
+        def delta(ia, ib):
+            tot = 0.0
+            for da in range(-4, 5):
+                for db in range(-4, 5):
+                    if 0 <= ia + da < n and 0 <= ib + db < n and da * da + db * db < 16:
+                        tot += _W(math.sqrt(da * da + db * db) * s / hk, hk)
+            return tot
+        want = {center: delta(n // 2, n // 2), edge: delta(n // 2, 0), corner: delta(0, 0)}
+    for i, dlt in want.items():
+        assert psi[i] == pytest.approx(1000.0 / dlt, rel=1e-12), i
+
+
+# ---------------------------------------------------------------- XSPH (A13)
+
+def test_xsph_three_particles_closed_form(oracle_mod):
+    """Three fluid particles on a line, 0.6 hk apart (the outer two are not
+    neighbors), v = (1, 0, 0) on the first, 0 elsewhere, no gravity.
+    a_i = -(nu / dt) sum_j (m_j / rho_j) (v_i - v_j) W_ij
+    (DFSPH_core_cuda.cu:1460-1462, Viscosity_XSPH.cpp:42-47) with
+    rho = m (W(0) + sum W): the mass cancels and
+      a_0 = -(nu/dt) W(.6) / (W(0) + 2 W(.6))      (rho of the middle one)
+      a_1 = +(nu/dt) W(.6) / (W(0) + W(.6))        (rho of the first one)
+      a_2 = 0
+    A factor 1/hk instead of 1/dt, a missing nu or rho_i in place of rho_j
+    fails by orders of magnitude or by the ratio of the two densities."""
+    r = 0.025
+    hk = 4 * r
+    dt, nu = 0.002, 0.01
+    x = np.array([[0.3, 0.3, 0.3], [0.3 + 0.6 * hk, 0.3, 0.3], [0.3 + 1.2 * hk, 0.3, 0.3]])
+    v = np.array([[1.0, 0, 0], [0, 0, 0], [0, 0, 0]])
+    o = oracle_mod.Oracle(particle_radius=r, dt=dt, viscosity=nu, max_iter_v=0, min_iter_v=0,
+                          gravity=(0.0, 0.0, 0.0))
+    o.set_boundary(_far_boundary())
+    o.set_fluid(x, v)
+    o.neighbor_search()
+    o.divergence_solve()
+    o.external_forces()
+    acc = o.get(oracle_mod.F_ACC)
+    w0, w6 = _W(0.0, hk), _W(0.6, hk)
+    assert acc[0, 0] == pytest.approx(-(nu / dt) * w6 / (w0 + 2 * w6), rel=1e-12)
+    assert acc[1, 0] == pytest.approx((nu / dt) * w6 / (w0 + w6), rel=1e-12)
+    assert acc[2, 0] == 0.0
+    assert np.abs(acc[:, 1:]).max() == 0.0
+
+
+def test_xsph_quadratic_shear_on_a_lattice(oracle_mod):
+    """v_x = c y^2 on a cubic lattice of spacing 2r = hk/2, interior particle:
+    the linear part cancels on the symmetric stencil and
+      a_x = (nu/dt) c s^2 (2 W(1/2) + 8 W(1/sqrt2) + 8 W(sqrt3/2)) / (W(0) + 6 W(1/2) + 12 W(1/sqrt2) + 8 W(sqrt3/2))
+    (the 26 neighbors of the interior, of which 2, 8 and 8 have |dy| = s)."""
+    r = 0.025
+    hk = 4 * r
+    s = 2 * r
+    dt, nu, c = 0.002, 0.01, 3.0
+    n = 9
+    g = np.stack(np.meshgrid(*(np.arange(n),) * 3, indexing="ij"), -1).reshape(-1, 3) * s + 0.2
+    v = np.zeros_like(g)
+    v[:, 0] = c * g[:, 1] ** 2
+    o = oracle_mod.Oracle(particle_radius=r, dt=dt, viscosity=nu, max_iter_v=0, min_iter_v=0,
+                          gravity=(0.0, 0.0, 0.0))
+    o.set_boundary(_far_boundary())
+    o.set_fluid(g, v)
+    o.neighbor_search()
+    o.divergence_solve()
+    o.external_forces()
+    ctr = (n // 2) * (n * n + n + 1)
+    w1, w2, w3 = _W(0.5, hk), _W(math.sqrt(0.5), hk), _W(math.sqrt(0.75), hk)
+    want = (nu / dt) * c * s * s * (2 * w1 + 8 * w2 + 8 * w3) / (_W(0.0, hk) + 6 * w1 + 12 * w2 + 8 * w3)
+    acc = o.get(oracle_mod.F_ACC)[ctr]
+    assert acc[0] == pytest.approx(want, rel=1e-9)
+    assert abs(acc[1]) < 1e-12 * want and abs(acc[2]) < 1e-12 * want
+
+
+# ------------------------------------------------- warm-start rescales (A9, A14)
+
+def test_pressure_warm_start_rescale_by_hand(oracle_mod):
+    """DFSPH_update_vel_kernel (DFSPH_core_cuda.cu:3060-3068): v += h a and
+    kappa <- max(kappa (h / h_future)^2, -0.5), h_future from updateTimeStep
+    (DFSPH_c_arrays_structure.cpp:1100-1106).  h = 2 ms, h_future = 1 ms:
+    kappa -0.1 -> -0.4, -0.2 -> -0.5 (clamped), 0.05 -> 0.2; then
+    h_future = 4 ms: -0.4 -> -0.1."""
+    r = 0.025
+    x = np.array([[0.3, 0.3, 0.3], [0.6, 0.3, 0.3], [0.9, 0.3, 0.3]])
+    o = oracle_mod.Oracle(particle_radius=r, dt=0.002, gravity=(0.0, -10.0, 0.0), max_iter_v=0, min_iter_v=0)
+    o.set_boundary(_far_boundary())
+    o.set_fluid(x)
+    o.neighbor_search()
+    o.divergence_solve()
+    o.external_forces()
+    o.set(oracle_mod.F_KAPPA, np.array([-0.1, -0.2, 0.05]))
+    o.update_time_step(0.001)
+    o.update_vel()
+    np.testing.assert_allclose(o.get(oracle_mod.F_KAPPA), [-0.4, -0.5, 0.2], rtol=1e-14)
+    np.testing.assert_allclose(o.get(oracle_mod.F_VEL)[:, 1], -10.0 * 0.002, rtol=1e-14)
+    o.set(oracle_mod.F_KAPPA, np.array([-0.4, -0.4, -0.4]))
+    o.update_time_step(0.004)
+    o.update_vel()
+    np.testing.assert_allclose(o.get(oracle_mod.F_KAPPA), [-0.1, -0.1, -0.1], rtol=1e-14)
+
+
+def _div_warm_start_dv(om, h_prev, h_now, kv):
+    """Velocity change of the divergence solve's warm start alone (no
+    iterations) on a jittered, compressed lattice, after a step of h_prev
+    followed by updateTimeStep(h_now) and the step end; kappaV = kv
+    everywhere."""
+    r = 0.025
+    s = 2 * r
+    n = 7
+    rng = np.random.default_rng(5)
+    g = np.stack(np.meshgrid(*(np.arange(n),) * 3, indexing="ij"), -1).reshape(-1, 3) * s + 0.2
+    g = g + rng.uniform(-0.1 * s, 0.1 * s, size=g.shape)
+    v = -0.5 * (g - g.mean(0))                # compression: positive source in the bulk
+    o = om.Oracle(particle_radius=r, dt=h_prev, max_iter_v=0, min_iter_v=0)
+    o.set_boundary(_far_boundary())
+    o.set_fluid(g, v)
+    o.update_time_step(h_now)
+    o.step_end()
+    o.set(om.F_KAPPAV, np.full(len(g), kv))
+    o.neighbor_search()
+    o.divergence_solve()
+    return o.get(om.F_VEL) - v, g
+
+
+def test_divergence_warm_start_rescale_by_hand(oracle_mod):
+    """DFSPH_divergence_warmstart_init_kernel (DFSPH_core_cuda.cu:240):
+    kappaV <- max(kappaV h_ratio_to_past / 2, -0.5), h_ratio_to_past =
+    h_prev / h_now, then v_i += h_now sum_j (k_i + k_j) m_j gradW_ij where the
+    source is positive (:507-511).  With kappaV uniform the change is
+    h_now k' (2 m sum_j gradW_ij): relative to case A (2 ms, 2 ms, -0.1 ->
+    k' = -0.05) the cases give
+      B 2 ms -> 1 ms, -0.1: k' = -0.1, h k' the same          ratio 1
+      C 1 ms -> 2 ms, -0.1: k' = -0.025                       ratio 1/2
+      D 2 ms -> 2 ms, -3:   k' = -1.5 clamped to -0.5         ratio 10
+      E 2 ms -> 2 ms, +0.4: k' = 0.2 (no clamp above)         ratio -4
+    and case A's change itself for one particle by a brute-force sum over
+    all particles closer than hk (2 k' h m sum_j gradW_ij)."""
+    om = oracle_mod
+    dA, g = _div_warm_start_dv(om, 0.002, 0.002, -0.1)
+    scale = np.abs(dA).max()
+    moving = np.abs(dA).max(1) > 1e-3 * scale
+    assert moving.sum() > 100 and scale > 1e-3
+    ratios = {"B": ((0.002, 0.001, -0.1), 1.0), "C": ((0.001, 0.002, -0.1), 0.5),
+              "D": ((0.002, 0.002, -3.0), 10.0), "E": ((0.002, 0.002, 0.4), -4.0)}
+    for name, (args, want) in ratios.items():
+        d, _ = _div_warm_start_dv(om, *args)
+        np.testing.assert_allclose(d, want * dA, rtol=1e-9, atol=1e-12 * scale, err_msg=name)
+        assert np.array_equal(np.abs(d).max(1) > 0, np.abs(dA).max(1) > 0), name
+    r = 0.025
+    hk, m = 4 * r, 0.8 * (2 * r) ** 3 * 1000.0
+    i = int(np.argmax(np.abs(dA).max(1)))
+    gsum = np.zeros(3)
+    for j in range(len(g)):
+        dx = g[i] - g[j]
+        if j != i and np.dot(dx, dx) < hk * hk:
+            gsum += om.kernel_gradW(dx, hk)
+    np.testing.assert_allclose(dA[i], 0.002 * 2 * -0.05 * m * gsum, rtol=1e-10)
+
+
+# ------------------------------------------ learned warm start features (A22)
+
+def _dW(q, hk):
+    """dW/dr of the cubic spline at r = q hk."""
+    k = 8.0 / (math.pi * hk ** 3)
+    if q <= 0.5:
+        return k * (18 * q * q - 12 * q) / hk
+    return -6 * k * (1 - q) ** 2 / hk if q <= 1.0 else 0.0
+
+
+def test_features_lattice_interior_closed_form(oracle_mod):
+    """Center of a perfect lattice of spacing s = 0.5001 hk under uniform
+    compression v = -c (x - x_c), after a 2 ms step followed by a 1 ms one,
+    kappa = -0.3, kappaV = -0.2: its 26 neighbors lie at s (6), s sqrt2 (12)
+    and s sqrt3 (8), q = s / hk times 1, sqrt2, sqrt3, so
+      f0 = m (W0 + 6 W1 + 12 W2 + 8 W3) / rho0 - 1
+      f1 = h sum_j m (v_i - v_j).gradW_ij / rho0 = -c h m / rho0 sum_j |d_j| W'(|d_j|)
+      f2 = 26/40, f3 = 0 (v_c = 0), f4 = kappa h^2, f5 = kappaV h_past,
+      f6 = 0, f7..9 = 0, f10 = q (6 + 12 sqrt2 + 8 sqrt3) / 26,
+      f11 = (6 W1 + 12 W2 + 8 W3) / (W0 + 6 W1 + 12 W2 + 8 W3)."""
+    om = oracle_mod
+    r = 0.025
+    hk = 4 * r
+    s = 0.5001 * hk            # second-shell axis neighbors clearly beyond hk
+    m = 0.8 * (2 * r) ** 3 * 1000.0
+    n, c = 9, 0.7
+    g = np.stack(np.meshgrid(*(np.arange(n),) * 3, indexing="ij"), -1).reshape(-1, 3) * s + 0.2
+    ctr = (n // 2) * (n * n + n + 1)
+    v = -c * (g - g[ctr])
+    o = om.Oracle(particle_radius=r, dt=0.002)
+    o.set_boundary(_far_boundary())
+    o.set_fluid(g, v)
+    o.update_time_step(0.001)
+    o.step_end()
+    o.set(om.F_KAPPA, np.full(len(g), -0.3))
+    o.set(om.F_KAPPAV, np.full(len(g), -0.2))
+    o.neighbor_search()
+    f = o.features()[ctr]
+    a = s / hk
+    qs = [(a, 6), (a * math.sqrt(2), 12), (a * math.sqrt(3), 8)]
+    sw = sum(cnt * _W(q, hk) for q, cnt in qs)
+    want = np.zeros(12)
+    want[0] = m * (_W(0.0, hk) + sw) / 1000.0 - 1.0
+    want[1] = -c * 0.001 * m / 1000.0 * sum(cnt * q * hk * _dW(q, hk) for q, cnt in qs)
+    want[2] = 26 / 40
+    want[4] = -0.3 * 0.001 ** 2
+    want[5] = -0.2 * 0.002
+    want[10] = sum(cnt * q for q, cnt in qs) / 26
+    want[11] = sw / (_W(0.0, hk) + sw)
+    np.testing.assert_allclose(f, want, rtol=1e-12, atol=1e-14)
+    assert want[1] > 0 and o.neighbors()[0][ctr] == 26     # compression: positive source
+
+
+def test_features_fluid_and_boundary_pair_by_hand(oracle_mod):
+    """One fluid particle moving at (0, 1, 0) with one wall particle of psi =
+    0.05 at 0.4 hk above it: f0 = (m W0 + psi W(.4)) / rho0 - 1,
+    f1 = h psi (v . gradW(x_i - x_b)) / rho0 = -h psi W'(.4) / rho0,
+    f2 = 1/40, f3 = h / (2r), f6 = 1, f7..9 = (0, 0.4, 0), f10 = 0.4, f11 = 0."""
+    om = oracle_mod
+    r = 0.025
+    hk = 4 * r
+    dt = 0.002
+    m = 0.8 * (2 * r) ** 3 * 1000.0
+    xf = np.array([[0.5, 0.5, 0.5]])
+    xb = np.array([[0.5, 0.5 + 0.4 * hk, 0.5], [5.0, 5.0, 5.0]])
+    o = om.Oracle(particle_radius=r, dt=dt)
+    o.set_boundary(xb, np.array([0.05, 0.05]))
+    o.set_fluid(xf, np.array([[0.0, 1.0, 0.0]]))
+    o.neighbor_search()
+    f = o.features()[0]
+    want = np.zeros(12)
+    want[0] = (m * _W(0.0, hk) + 0.05 * _W(0.4, hk)) / 1000.0 - 1.0
+    want[1] = -dt * 0.05 * _dW(0.4, hk) / 1000.0
+    want[2] = 1 / 40
+    want[3] = dt / (2 * r)
+    want[6] = 1.0
+    want[8] = 0.4
+    want[10] = 0.4
+    np.testing.assert_allclose(f, want, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("which", ["divergence", "density"])
+def test_warm_start_targets_are_dt_invariant(oracle_mod, which):
+    """The targets are the solves' increments in dt-invariant units
+    (kappaV h, kappa h_f^2): with the iteration counts fixed, the same
+    configuration stepped with h = 1 ms and h = 2 ms gives the same targets.
+    Divergence: alpha ~ 1/h, the source does not depend on h and the velocity
+    change h (k_i + k_j) is h-free.  Density (at rest, compressed, h = h_f,
+    no gravity): alpha/h_f ~ 1/h^2 and rho_adv does not depend on h.  A
+    target in raw units, or scaled by h instead of h^2, fails by 2x or 4x."""
+    om = oracle_mod
+    r = 0.025
+    s = 2 * r
+    n = 8
+    rng = np.random.default_rng(9)
+    out = []
+    for h in (0.001, 0.002):
+        g = np.stack(np.meshgrid(*(np.arange(n),) * 3, indexing="ij"), -1).reshape(-1, 3) * s + 0.2
+        g = g + rng.uniform(-0.05 * s, 0.05 * s, size=g.shape) if not out else g0
+        g0 = g
+        if which == "divergence":
+            v = -0.4 * (g - g.mean(0))
+            kw = dict(min_iter_v=3, max_iter_v=3, min_iter=0, max_iter=0)
+        else:
+            g = g.mean(0) + 0.93 * (g - g.mean(0))     # compressed, at rest
+            v = np.zeros_like(g)
+            kw = dict(min_iter_v=0, max_iter_v=0, min_iter=2, max_iter=2)
+        o = om.Oracle(particle_radius=r, dt=h, gravity=(0.0, 0.0, 0.0), **kw)
+        o.set_boundary(_far_boundary())
+        o.set_fluid(g, v)
+        o.neighbor_search()
+        o.divergence_solve()
+        o.external_forces()
+        o.update_time_step(h)
+        o.update_vel()
+        o.pressure_solve()
+        out.append(o.warm_start_targets()[:, 0 if which == "divergence" else 1])
+    assert np.abs(out[0]).max() > 0
+    np.testing.assert_allclose(out[1], out[0], rtol=1e-9, atol=1e-12 * np.abs(out[0]).max())
