@@ -218,6 +218,13 @@ int dfsph_add_particles(void* handle, const float* xyz, const float* vel, const 
  * reference order, with the library's kernel fusions. */
 int dfsph_step(void* handle, dfsph_step_stats* stats);
 
+/* change_max_particle_number (DFSPH_memory_management_cuda.cu:962-1018): grow
+ * the fluid capacity to max_particles (no change when it is not larger),
+ * keeping the particles in their order; the next step rebuilds the neighbor
+ * lists.  dfsph_open_boundary_apply calls it by itself (capacity doubled once
+ * more than 3/4 is in use, OpenBoundariesSimple.cu:570-573). */
+int dfsph_change_max_particle_number(void* handle, uint32_t max_particles);
+
 /* read_UnifiedParticleSet_cuda (DFSPH_memory_management_cuda.cu:652-674):
  * copy a field of the active particles (device order) to host memory.
  * count must equal the number of elements of the field. */
@@ -396,9 +403,10 @@ int dfsph_rigid_body_count(void* handle, uint32_t* n);
  * the last solves' increments in the same units (kappa h_f^2, kappaV h).  A
  * sweep applies ref + y / (h or h_f^2), bounded between min(ref, prev) and
  * max(ref, 0), where ref is the reference warm start and prev the particle's
- * converged total of the last step (both in raw units, clamped at -0.5): no
- * more pressure than the larger of the two, and a network that outputs zero
- * gives the reference warm start exactly.
+ * converged total of the last step (density solver) or half of it
+ * (divergence solver), both in raw units and clamped at -0.5: no more
+ * pressure than the larger of the two, and a network that outputs zero gives
+ * the reference warm start exactly.
  * Parameters use one flat FP32 layout of DFSPH_MLP_NPARAM values:
  *   W1[64x12] b1[64] W2[64x64] b2[64] W3[64x64] b3[64] W4[2x64] b4[2]
  * (row-major, out x in); norm holds in_mean[12] in_std[12] out_mean[2] out_std[2]. */

@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# ORACLE_LIB: another build of the same library (the sanitizer build of oracle/sanitize.sh)
+LIB = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 MODE_GPU = 0
 MODE_CPU = 1

@@ -13,7 +13,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 HIP_LIB = os.path.join(HERE, "libdfsph_hip.so")
-HOST_LIB = os.path.join(HERE, "libdfsph_host.so")
+# DFSPH_HOST_LIB: another build of the host library (the sanitizer build of oracle/sanitize.sh)
+HOST_LIB = os.environ.get("DFSPH_HOST_LIB") or os.path.join(HERE, "libdfsph_host.so")
 
 DFSPH_OK = 0
 DFSPH_ERR_HIP = 1
@@ -258,6 +259,7 @@ HIP_SIGNATURES = {
     "dfsph_get_neighbors": (_I, [_P, _U32P, _U32P, _U32P, _U32]),
     "dfsph_get_info": (_I, [_P, C.POINTER(DFSPHInfo)]),
     "dfsph_synchronize": (_I, [_P]),
+    "dfsph_change_max_particle_number": (_I, [_P, _U32]),
     "dfsph_neighbor_stats": (_I, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                  C.POINTER(C.c_uint64)]),
     "dfsph_profile_enable": (_I, [_P, _U32]),
@@ -683,6 +685,10 @@ class Solver:
         return out
 
     # ---- data out ----
+    def change_max_particle_number(self, max_particles):
+        _check(self.lib.dfsph_change_max_particle_number(self.h, int(max_particles)),
+               "dfsph_change_max_particle_number")
+
     def neighbor_stats(self):
         """(max_seen, overflow, fluid_total, boundary_total) of the last neighbor search"""
         ms, of = C.c_uint32(), C.c_uint32()

@@ -698,6 +698,52 @@ int alloc_fluid(Handle* h, uint32_t cap)
     return DFSPH_OK;
 }
 
+// change_max_particle_number (DFSPH_memory_management_cuda.cu:962-1018): every
+// per-particle buffer reallocated for new_cap particles, the active particles'
+// persistent fields (position and mass, velocity, kappa, kappaV, the totals,
+// id, tag) kept in their order; the neighbor lists are rebuilt by the next
+// search.
+int grow_capacity(Handle* h, uint32_t new_cap)
+{
+    if (new_cap <= h->cap) return DFSPH_OK;
+    CHK(flush_vel(h));
+    const uint32_t n = h->n;
+    float4 *tp = nullptr, *tv = nullptr;
+    float *tk = nullptr, *tkv = nullptr, *tkt = nullptr, *tkvt = nullptr;
+    uint32_t *tid = nullptr, *ttag = nullptr;
+    auto release = [&] { dfree(tp); dfree(tv); dfree(tk); dfree(tkv); dfree(tkt); dfree(tkvt); dfree(tid); dfree(ttag); };
+    int rc = DFSPH_OK;
+    auto keep = [&](auto*& tmp, auto* src) -> int {
+        CHK(dalloc(tmp, n));
+        if (n) HIPCHK(hipMemcpyAsync(tmp, src, sizeof(*src) * n, hipMemcpyDeviceToDevice, h->s));
+        return DFSPH_OK;
+    };
+    if ((rc = keep(tp, h->pos[h->pc])) || (rc = keep(tv, h->vel[h->vc])) || (rc = keep(tk, h->kap[h->pc])) ||
+        (rc = keep(tkv, h->kapv[h->pc])) || (rc = keep(tkt, h->ktot[h->pc])) || (rc = keep(tkvt, h->kvtot[h->pc])) ||
+        (rc = keep(tid, h->id[h->pc])) || (rc = keep(ttag, h->tag[h->pc])) || (rc = sync(h)) ||
+        (rc = alloc_fluid(h, new_cap))) {
+        release();
+        return rc;
+    }
+    h->pc = 0;
+    h->vc = 0;
+    auto back = [&](auto* dst, const auto* tmp) -> int {
+        if (n) HIPCHK(hipMemcpyAsync(dst, tmp, sizeof(*tmp) * n, hipMemcpyDeviceToDevice, h->s));
+        return DFSPH_OK;
+    };
+    if ((rc = back(h->pos[0], tp)) || (rc = back(h->vel[0], tv)) || (rc = back(h->kap[0], tk)) ||
+        (rc = back(h->kapv[0], tkv)) || (rc = back(h->ktot[0], tkt)) || (rc = back(h->kvtot[0], tkvt)) ||
+        (rc = back(h->id[0], tid)) || (rc = back(h->tag[0], ttag)) || (rc = sync(h))) {
+        release();
+        return rc;
+    }
+    release();
+    h->lists_valid = false;
+    h->acc_valid = false;
+    h->pred_valid = false;
+    return DFSPH_OK;
+}
+
 int write_fluid_range(Handle* h, uint32_t at, const float* xyz, const float* vel, const float* mass,
                       const uint32_t* id, uint32_t m)
 {
@@ -712,6 +758,11 @@ int write_fluid_range(Handle* h, uint32_t at, const float* xyz, const float* vel
         I[i] = id ? id[i] : at + i;
         h->next_id = std::max(h->next_id, I[i] + 1u);
     }
+    // uniform fluid mass: the source sweeps then stage 24-byte records
+    bool same = at == 0 || h->c.m_uniform > 0.0f;
+    const float mref = at == 0 ? (m ? P[0].w : 0.0f) : h->c.m_uniform;
+    for (uint32_t i = 0; i < m && same; ++i) same = P[i].w == mref;
+    h->c.m_uniform = same && mref > 0.0f ? mref : 0.0f;
     HIPCHK(hipMemcpyAsync(h->pos[h->pc] + at, P.data(), sizeof(float4) * m, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipMemcpyAsync(h->vel[h->vc] + at, V.data(), sizeof(float4) * m, hipMemcpyHostToDevice, h->s));
     HIPCHK(hipMemcpyAsync(h->id[h->pc] + at, I.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, h->s));
@@ -1303,6 +1354,11 @@ int dfsph_upload_field(void* hv, int field, const void* src, uint32_t count)
             if (field == DFSPH_FIELD_MASS) tmp[i].w = f[i];
             else { tmp[i].x = f[3 * i]; tmp[i].y = f[3 * i + 1]; tmp[i].z = f[3 * i + 2]; }
         }
+        if (field == DFSPH_FIELD_MASS) {
+            bool same = f[0] > 0.0f;
+            for (uint32_t i = 1; i < n && same; ++i) same = f[i] == f[0];
+            h->c.m_uniform = same ? f[0] : 0.0f;
+        }
         HIPCHK(hipMemcpy(dptr, tmp.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
         if (field != DFSPH_FIELD_VEL) h->lists_valid = false;
         h->acc_valid = false;
@@ -1630,6 +1686,9 @@ int dfsph_open_boundary_apply(void* hv, const double* window_displacement, float
     if (use_inflow) {
         if (!(allowed_new_distance > 0.0f) || !(allowed_new_density > 0.0f))
             return fail(DFSPH_ERR_ARG, "inflow needs a positive distance and density");
+        // room for the inflow: the capacity doubles once more than 3/4 of it
+        // is in use (OpenBoundariesSimple.cu:570-573)
+        if (h->n > 0.75 * (double)h->cap) CHK(grow_capacity(h, 2 * h->cap));
         const uint32_t n = h->n, nc = h->ob_n;
         if (nc && n) {
             // cell table of the current positions without reordering
@@ -1648,7 +1707,12 @@ int dfsph_open_boundary_apply(void* hv, const double* window_displacement, float
             CHK(sync(h));
             const uint32_t m = h->hctl->pad;
             int rc = DFSPH_OK;
-            if (n + m > h->cap) rc = fail(DFSPH_ERR_CAPACITY, "inflow beyond the fluid capacity");
+            if (n + m > h->cap) {
+                // more inflow than the doubled capacity holds: grow to fit (the
+                // candidate flags, ranks and velocities are inflow scratch and
+                // survive; the cell table is not used any more)
+                rc = grow_capacity(h, std::max(2 * h->cap, n + m));
+            }
             if (rc == DFSPH_OK && m) {
                 dfsph::launch_append_inflow(h->s, h->ob_cand, nc, k, r, vnew, n, mass, h->next_id,
                                             fluid_ptrs(h, h->pc, h->vc));
@@ -1668,6 +1732,14 @@ int dfsph_open_boundary_apply(void* hv, const double* window_displacement, float
         CHK(compact(h, removed));
     }
     return DFSPH_OK;
+}
+
+int dfsph_change_max_particle_number(void* hv, uint32_t max_particles)
+{
+    Handle* h = H(hv);
+    if (max_particles < h->n) return fail(DFSPH_ERR_CAPACITY, "the new capacity is below the particle count");
+    if (!h->pos[0]) return fail(DFSPH_ERR_STATE, "no fluid uploaded");
+    return grow_capacity(h, max_particles);
 }
 
 int dfsph_profile_enable(void* hv, uint32_t mask)

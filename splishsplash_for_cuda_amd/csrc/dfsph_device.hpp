@@ -52,6 +52,7 @@ struct SimConst {
     uint32_t max_nbr;
     uint32_t stage_lds;            // largest neighbor set staged in LDS (<= kStageMax)
     uint32_t stage_cap;            // largest neighbor set with 16-bit lists (<= kStageCap)
+    float m_uniform;               // the mass of every fluid particle when they are all equal, else 0
 };
 
 // Time-step scalars live on the device so a whole step can be queued
@@ -784,14 +785,15 @@ constexpr int kFeatKappaP = 4, kFeatKappaV = 5;
 
 // The warm start a sweep applies in learned mode, in raw units: the
 // reference's (ref, the value of the reference warm start), plus the
-// network's correction dy, bounded between min(ref, prev) and max(ref, 0),
-// prev the particle's converged total of the last step in the current raw
-// units (clamped at -0.5 like the reference).  It asks for no more pressure
-// (a more negative kappa) than the larger of the reference warm start and
-// the pressure the last step converged to, and for none where the reference
-// asks for none.  A stronger start pushes the particles apart, which the
-// solvers' errors, sums of compressions only, do not see.  dy = 0 gives ref
-// exactly.
+// network's correction dy, bounded between min(ref, prev) and max(ref, 0).
+// prev is the particle's converged total of the last step in the current raw
+// units (clamped at -0.5 like the reference): the whole total for the density
+// solver, half of it for the divergence solver (a warm start of the whole
+// divergence total set water at rest in motion).  It asks for no more
+// pressure (a more negative kappa) than the larger of the reference warm
+// start and prev, and for none where the reference asks for none.  A
+// stronger start pushes the particles apart, which the solvers' errors, sums
+// of compressions only, do not see.  dy = 0 gives ref exactly.
 __host__ __device__ __forceinline__ float ws_learned(float ref, float prev, float dy)
 {
     return fminf(fmaxf(ref + dy, fminf(ref, prev)), fmaxf(ref, 0.0f));

@@ -829,9 +829,57 @@ __global__ void k_set_w(float4* p, const float* w, uint32_t n)
 struct PV {
     float4 p, v;                   // position + mass, velocity (+ a per-kernel scalar in w)
 };
+
+// LDS stage of (position, velocity) records.  UM (every fluid particle has the
+// mass c.m_uniform): 24 B per particle, (x y z vx) and (vy vz), so that three
+// 256-particle blocks fit a CU (48 KB each) instead of two; the mass is put
+// back from the constant, the same value bit for bit.  Entries of the 16-bit
+// lists are byte offsets of 16-byte slots; the 8-byte slots are at half.
+template <bool UM>
+struct PVStage {
+    float4 p[kStageMaxPV], v[kStageMaxPV];
+    __device__ __forceinline__ void put(uint32_t k, const PV& d) { p[k] = d.p; v[k] = d.v; }
+    __device__ __forceinline__ PV get(uint32_t off, float) const { return PV{lds_f4(p, off), lds_f4(v, off)}; }
+};
+template <>
+struct PVStage<true> {
+    float4 a[kStageMaxPV];
+    float2 b[kStageMaxPV];
+    __device__ __forceinline__ void put(uint32_t k, const PV& d)
+    {
+        a[k] = make_float4(d.p.x, d.p.y, d.p.z, d.v.x);
+        b[k] = make_float2(d.v.y, d.v.z);
+    }
+    __device__ __forceinline__ PV get(uint32_t off, float m) const
+    {
+        const float4 u = lds_f4(a, off);
+        const float2 w = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(b) + (off >> 1));
+        return PV{make_float4(u.x, u.y, u.z, m), make_float4(u.w, w.x, w.y, 0.0f)};
+    }
+};
 struct PK {
     float4 p;                      // position + mass
     float k;                       // k_j of the Jacobi sweep / the warm start
+};
+
+// LDS stage of (position, k) records: 20 B per particle, or 16 B (x y z k)
+// with a uniform fluid mass (UM; five blocks per CU instead of four).
+template <bool UM>
+struct PKStage {
+    float4 p[kStageMax];
+    float k[kStageMax];
+    __device__ __forceinline__ void put(uint32_t i, const PK& d) { p[i] = d.p; k[i] = d.k; }
+    __device__ __forceinline__ PK get(uint32_t off, float) const { return PK{lds_f4(p, off), lds_f1(k, off)}; }
+};
+template <>
+struct PKStage<true> {
+    float4 a[kStageMax];
+    __device__ __forceinline__ void put(uint32_t i, const PK& d) { a[i] = make_float4(d.p.x, d.p.y, d.p.z, d.k); }
+    __device__ __forceinline__ PK get(uint32_t off, float m) const
+    {
+        const float4 u = lds_f4(a, off);
+        return PK{make_float4(u.x, u.y, u.z, m), u.w};
+    }
 };
 
 // Reducing sweeps write one FP64 partial per block with a plain store; the
@@ -875,7 +923,8 @@ __global__ __launch_bounds__(kBlock) void k_solve_finalize(SolveCtl* ctl, const 
 // rho_adv_i = 0 if fewer than 20 neighbors else max(sum m_j (v_i - v_j).gradW, 0).
 // Warm start (only where rho_adv_i > 0): v_i += h sum (k_i + k_j) m_j gradW with
 // k = max(kappaV h_ratio / 2, -0.5) (WS == 0), that value plus the learned
-// correction, bounded by ws_learned (WS == 1), or nothing (WS == 2).  kws holds
+// correction, bounded by ws_learned with half the last converged total
+// (WS == 1), or nothing (WS == 2).  kws holds
 // the last solve's kappaV increments.  Velocities are read from vel_i and
 // written to vel_o.  The applied k_i starts the particle's total of the step
 // in kvtot (when given).
@@ -896,7 +945,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_ws_init(SimConst
         if (WS == 2) return 0.0f;
         const float ref = fmaxf(kws[j] * ratio / 2.0f, -0.5f);
         if (WS == 0) return ref;
-        return ws_learned(ref, fmaxf(lw.prev[j] * inv_h, -0.5f), lw.dy[j] * inv_h);
+        return ws_learned(ref, fmaxf(0.5f * lw.prev[j] * inv_h, -0.5f), lw.dy[j] * inv_h);
     };
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
@@ -989,7 +1038,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_ws_init(SimConst
 // kappaV = 0) and DFSPH_divergence_loop_end_kernel (:732-748) + the FP64 error
 // reduction that replaces cub::DeviceReduce::Sum + cudaMemcpy (:774-781).
 // Also stores k_i = rho_adv_i * alpha_i for the next Jacobi sweep.
-template <bool INIT, bool SOL>
+template <bool INIT, bool SOL, bool UM>
 __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_source(SimConst c, uint32_t n, const float4* __restrict__ pos,
                                                        const float4* __restrict__ bpos, NbrLists L,
                                                        const float4* __restrict__ vel,
@@ -1000,8 +1049,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_source(SimConst 
 {
     if (INIT && ci.sc && blockIdx.x == 0 && threadIdx.x == 0) ctl_init(ci);
     __shared__ double lds4[kBlock / 64];
-    __shared__ float4 sP[kStageMaxPV];
-    __shared__ float4 sV[kStageMaxPV];
+    __shared__ PVStage<UM> st;
     if (!INIT && ctl->done) return;
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
@@ -1012,7 +1060,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_source(SimConst 
     const float4 vi = vel[ic];
     const float ai = alpha[ic];
     const uint32_t sn = stage_block<kStageMaxPV>(L, b, sn0, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
-                                    [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
+                                    [&](uint32_t k, const PV& d) { st.put(k, d); });
     float r = 0.0f;
     if (i < n && !is_source(L.rs, i)) r = radv[i];      // restricted out: stored value in the error sum
     if (i < n && is_source(L.rs, i)) {
@@ -1025,7 +1073,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_source(SimConst 
             const float mg = keep(valid && kp.ok, pj.w * kp.g);
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         };
-        fluid_nbrs<kStageMaxPV>(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+        fluid_nbrs<kStageMaxPV>(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return st.get(e, c.m_uniform); },
                    [&](uint32_t j) { return PV{pos[j], vel[j]}; },
                    [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
         for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, hd.b0, hd.b1,
@@ -1050,15 +1098,14 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_source(SimConst 
 // Jacobi velocity update: divergenceSolveParticle<false> (DFSPH_core_cuda.cu:400-481,
 // DIV: kappaV += k_i, step h) and pressureSolveParticle<false> (:867-951, step h_future):
 // v_i += step (sum_f [|k_i + k_j| > eps] (k_i + k_j) m_j gradW + [|k_i| > eps] sum_b k_i psi_j gradW)
-template <bool DIV, bool SOL>
+template <bool DIV, bool SOL, bool UM>
 __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t, uint32_t n,
                                                    const float4* __restrict__ pos,
                                                    const float4* __restrict__ bpos, NbrLists L,
                                                    const float* __restrict__ kk, float* __restrict__ kappaV,
                                                    float4* __restrict__ vel, const SolveCtl* ctl)
 {
-    __shared__ float4 sP[kStageMax];
-    __shared__ float sK[kStageMax];
+    __shared__ PKStage<UM> st;
     if (ctl->done) return;
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
@@ -1069,7 +1116,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t,
     const float4 pi = pos[ic];
     float4 v = vel[ic];
     const uint32_t sn = stage_block(L, b, sn0, [&](uint32_t g) { return PK{pos[g], kk[g]}; },
-                                    [&](uint32_t k, const PK& d) { sP[k] = d.p; sK[k] = d.k; });
+                                    [&](uint32_t k, const PK& d) { st.put(k, d); });
     if (i >= n || !is_compute(L.rs, i)) return;
     if (DIV) kappaV[i] = kv + ki;
     const uint32_t cn = L.cnt[i];
@@ -1082,7 +1129,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t,
         const float s = keep(valid && kp.ok && fabsf(ks) > kEps, ks * pj.w * kp.g);
         ax += s * dx; ay += s * dy; az += s * dz;
     };
-    fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t e) { return PK{lds_f4(sP, e), lds_f1(sK, e)}; },
+    fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t e) { return st.get(e, c.m_uniform); },
                [&](uint32_t j) { return PK{pos[j], kk[j]}; },
                [&](const PK& d, bool valid) { fluid(d.p, d.k, valid); });
     if (fabsf(ki) > kEps) {
@@ -1278,7 +1325,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
 // INIT: DFSPH_pressure_init_kernel (:1217-1262): alpha *= 1/h_f, kappa = 0 first.
 // Otherwise DFSPH_pressure_loop_end_kernel (:1264-1284) + FP64 error reduction
 // (replaces cub Sum + cudaMemcpy, :1321-1325).
-template <bool INIT, bool SOL>
+template <bool INIT, bool SOL, bool UM>
 __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_density_source(SimConst c, const DevTime* t, uint32_t n,
                                                            const float4* __restrict__ pos,
                                                            const float4* __restrict__ bpos, NbrLists L,
@@ -1290,8 +1337,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_density_source(SimCo
 {
     if (INIT && ci.sc && blockIdx.x == 0 && threadIdx.x == 0) ctl_init(ci);
     __shared__ double lds4[kBlock / 64];
-    __shared__ float4 sP[kStageMaxPV];
-    __shared__ float4 sV[kStageMaxPV];
+    __shared__ PVStage<UM> st;
     if (!INIT && ctl->done) return;
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
@@ -1303,7 +1349,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_density_source(SimCo
     const float a0 = alpha[ic], rho_i = rho[ic];
     const float kap0 = INIT ? 0.0f : kappa[ic];
     const uint32_t sn = stage_block<kStageMaxPV>(L, b, sn0, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
-                                    [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
+                                    [&](uint32_t k, const PV& d) { st.put(k, d); });
     float r = 0.0f;
     if (i < n && !is_source(L.rs, i)) r = radv[i];      // restricted out: stored value in the error sum
     if (i < n && is_source(L.rs, i)) {
@@ -1317,7 +1363,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_density_source(SimCo
             const float mg = keep(valid && kp.ok, pj.w * kp.g);
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         };
-        fluid_nbrs<kStageMaxPV>(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+        fluid_nbrs<kStageMaxPV>(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return st.get(e, c.m_uniform); },
                    [&](uint32_t j) { return PV{pos[j], vel[j]}; },
                    [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
         (void)nf;
@@ -1412,7 +1458,7 @@ __global__ void k_finish_totals(const DevTime* t, uint32_t n, int pressure, cons
 //  4 last density increments kappa h_f^2   5 last divergence increments kappaV h
 //  6 boundary fraction n_b / n     7-9 mean neighbor offset / h_k
 // 10 mean neighbor distance / h_k 11 fluid share of the density
-template <bool SOL>
+template <bool SOL, bool UM>
 __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_features(SimConst c, const DevTime* t, uint32_t n,
                                                      const float4* __restrict__ pos,
                                                      const float4* __restrict__ bpos, NbrLists L,
@@ -1424,8 +1470,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_features(SimConst c,
                                                      float* __restrict__ feat, float* __restrict__ prev_p,
                                                      float* __restrict__ prev_v)
 {
-    __shared__ float4 sP[kStageMaxPV];
-    __shared__ float4 sV[kStageMaxPV];
+    __shared__ PVStage<UM> st;
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t ic = i < n ? i : 0u;         // own loads before staging, so they overlap it
@@ -1434,7 +1479,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_features(SimConst c,
     const float4 pi = pos[ic];
     const float4 vi = vel[ic];
     const uint32_t sn = stage_block<kStageMaxPV>(L, b, sn0, [&](uint32_t g) { return PV{pos[g], vel[g]}; },
-                                    [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
+                                    [&](uint32_t k, const PV& d) { st.put(k, d); });
     if (i >= n) return;
     const uint32_t cn = hd.cnt;
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
@@ -1451,7 +1496,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_features(SimConst c,
         ox -= m * dx; oy -= m * dy; oz -= m * dz;
         od += m * __builtin_sqrtf(r2);
     };
-    fluid_nbrs<kStageMaxPV>(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+    fluid_nbrs<kStageMaxPV>(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return st.get(e, c.m_uniform); },
                [&](uint32_t j) { return PV{pos[j], vel[j]}; },
                [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
     float dbd = 0.0f;
@@ -1653,9 +1698,12 @@ void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, 
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
     const bool sol = L.sol.bvel != nullptr;
-#define DS(I, S, CI) k_div_source<I, S><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial, CI)
-    if (init) { if (sol) DS(true, true, ci); else DS(true, false, ci); }
-    else { if (sol) DS(false, true, CtlInit{}); else DS(false, false, CtlInit{}); }
+    const bool um = c.m_uniform > 0.0f;
+#define DS(I, S, U, CI) k_div_source<I, S, U><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial, CI)
+#define DSU(I, S, CI) do { if (um) DS(I, S, true, CI); else DS(I, S, false, CI); } while (0)
+    if (init) { if (sol) DSU(true, true, ci); else DSU(true, false, ci); }
+    else { if (sol) DSU(false, true, CtlInit{}); else DSU(false, false, CtlInit{}); }
+#undef DSU
 #undef DS
 }
 
@@ -1671,9 +1719,12 @@ void launch_jacobi(hipStream_t s, bool div, const SimConst& c, const DevTime* t,
     if (!n) return;
     const uint32_t g = nblk_xcd(n);
     const bool sol = L.sol.bvel != nullptr;
-#define JA(D, S) k_jacobi<D, S><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, kk, kappaV, vel, ctl)
-    if (div) { if (sol) JA(true, true); else JA(true, false); }
-    else { if (sol) JA(false, true); else JA(false, false); }
+    const bool um = c.m_uniform > 0.0f;
+#define JA(D, S, U) k_jacobi<D, S, U><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, kk, kappaV, vel, ctl)
+#define JAU(D, S) do { if (um) JA(D, S, true); else JA(D, S, false); } while (0)
+    if (div) { if (sol) JAU(true, true); else JAU(true, false); }
+    else { if (sol) JAU(false, true); else JAU(false, false); }
+#undef JAU
 #undef JA
 }
 
@@ -1718,9 +1769,12 @@ void launch_density_source(hipStream_t s, bool init, const SimConst& c, const De
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
     const bool sol = L.sol.bvel != nullptr;
-#define DN(I, S, CI) k_density_source<I, S><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl, partial, CI)
-    if (init) { if (sol) DN(true, true, ci); else DN(true, false, ci); }
-    else { if (sol) DN(false, true, CtlInit{}); else DN(false, false, CtlInit{}); }
+    const bool um = c.m_uniform > 0.0f;
+#define DN(I, S, U, CI) k_density_source<I, S, U><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl, partial, CI)
+#define DNU(I, S, CI) do { if (um) DN(I, S, true, CI); else DN(I, S, false, CI); } while (0)
+    if (init) { if (sol) DNU(true, true, ci); else DNU(true, false, ci); }
+    else { if (sol) DNU(false, true, CtlInit{}); else DNU(false, false, CtlInit{}); }
+#undef DNU
 #undef DN
 }
 
@@ -1760,12 +1814,12 @@ void launch_features(hipStream_t s, const SimConst& c, const DevTime* t, uint32_
                      float* prev_p, float* prev_v)
 {
     if (!n) return;
-    if (L.sol.bvel)
-        k_features<true><<<nblk_xcd(n), kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, kap, kapv, ktot, kvtot,
-                                                        diameter, feat, prev_p, prev_v);
-    else
-        k_features<false><<<nblk_xcd(n), kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, kap, kapv, ktot, kvtot,
-                                                         diameter, feat, prev_p, prev_v);
+    const bool um = c.m_uniform > 0.0f;
+#define KF(S, U) k_features<S, U><<<nblk_xcd(n), kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, kap, kapv, ktot, kvtot, \
+                                                              diameter, feat, prev_p, prev_v)
+    if (L.sol.bvel) { if (um) KF(true, true); else KF(true, false); }
+    else { if (um) KF(false, true); else KF(false, false); }
+#undef KF
 }
 
 void launch_harvest(hipStream_t s, const DevTime* t, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap,

@@ -8,8 +8,8 @@ reference warm start, in dt-invariant units (kappaV h, kappa h_f^2), harvested
 from runs with the reference warm start.  The solver adds the correction to
 the reference warm start, bounded so that it asks for no more pressure than
 the larger of the reference warm start and the particle's converged total
-of the last step: a network that outputs zero is the reference warm start,
-exactly.  The plain solver keeps iterating to the same tolerances when the
+of the last step (half of it for the divergence solver): a network that
+outputs zero is the reference warm start, exactly.  The plain solver keeps iterating to the same tolerances when the
 learned warm start is used: the network only changes where the Jacobi
 iterations start.
 """

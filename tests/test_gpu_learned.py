@@ -101,7 +101,8 @@ def test_features_and_targets_match_oracle(oracle_mod):
     same step (k_harvest) against the oracle's restatement, both sides from
     the oracle's state after 4 steps of a small dam break (matched by id).
     Tolerances: features 0, 3, 10, 11 rel 1e-5; 1 (a source) 1e-4 of its
-    largest magnitude; 2 and 6 exact; 4, 5 rel 1e-6 of the largest; 7-9
+    largest magnitude; 2 and 6 (ratios of counts) one FP32 rounding (rel
+    2.5e-7); 4, 5 rel 1e-6 of the largest; 7-9
     (offset means, sums of differences) 1e-5 absolute (in hk); targets as
     kappa after a step (FP32 Jacobi sums)."""
     import splishsplash_for_cuda_amd as S
@@ -141,8 +142,8 @@ def test_features_and_targets_match_oracle(oracle_mod):
     for a in (0, 3, 10, 11):
         np.testing.assert_allclose(fg[:, a], fo[:, a], rtol=1e-5, atol=1e-6, err_msg=str(a))
     assert np.abs(fg[:, 1] - fo[:, 1]).max() <= 1e-4 * np.abs(fo[:, 1]).max()
-    for a in (2, 6):
-        np.testing.assert_array_equal(fg[:, a], fo[:, a].astype(np.float32))
+    for a in (2, 6):       # one FP32 rounding of a ratio of counts
+        np.testing.assert_allclose(fg[:, a], fo[:, a], rtol=2.5e-7, atol=0)
     for a in (4, 5):
         assert np.abs(fg[:, a] - fo[:, a]).max() <= 1e-6 * np.abs(fo[:, a]).max(), a
     assert np.abs(fg[:, 7:10] - fo[:, 7:10]).max() < 1e-5

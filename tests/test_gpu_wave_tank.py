@@ -77,3 +77,43 @@ def test_small_wave_tank(learned):
     assert abs(n1 - n0) < 0.1 * n0
     assert np.isfinite(hs.view.download(S.FIELD_POS)).all()
     hs.close()
+
+
+def _full_tank_run(learned, steps):
+    t = W.build()                       # r = 8 mm, 10 x 2 x 1 m basin, 0.9 m of water: about 2.09M particles
+    p = S.default_params(particle_radius=t.particle_radius, dt=0.001, use_cfl=1, cfl_factor=1.0, dt_min=1e-4,
+                         dt_max=5e-3, max_error=0.1, max_error_v=0.1, max_neighbors=128,
+                         warm_start_mode=S.WARMSTART_LEARNED if learned else S.WARMSTART_REFERENCE)
+    hs = W.make_solver(t, p, learned=learned)
+    vmax, iters = [], []
+    for k in range(steps):
+        st = hs.step()
+        iters.append(st.iters_v)
+        assert st.err / p.density0 * 100 <= p.max_error * 1.0001 or st.iters >= p.max_iter
+        assert st.err_v * st.h / p.density0 * 100 <= p.max_error_v * 1.0001 or st.iters_v >= p.max_iter_v
+        if k % 50 == 49:
+            vmax.append(float(np.linalg.norm(hs.view.download(S.FIELD_VEL), axis=1).max()))
+    n1 = hs.view.info().n_fluid
+    time = st.time
+    hs.close()
+    return t.n_fluid, n1, np.array(vmax), float(np.mean(iters)), time
+
+
+@pytest.mark.timeout(600)
+def test_full_wave_tank_stays_bounded():
+    """The full-size tank (about 2.09M fluid particles at r = 8 mm, the
+    piston of 0.1 m at 1.5 s, the box at 500 kg/m^3, the absorbing end), 2000
+    steps with each warm start: every step within the tolerances, the fluid
+    count within 15 % of the start, and the largest speed (sampled every 50
+    steps) below 8 m/s for the reference warm start (the splash at the
+    piston's edges: measured under 6 m/s) and no more than 1.25 x the
+    reference run's peak with the learned one."""
+    ref = _full_tank_run(False, 2000)
+    lrn = _full_tank_run(True, 2000)
+    for name, (n0, n1, vmax, iv, time) in (("reference", ref), ("learned", lrn)):
+        print("%s: %.2f s simulated, fluid %d -> %d, peak speed %.2f m/s (last %.2f), %.2f divergence "
+              "iterations per step" % (name, time, n0, n1, vmax.max(), vmax[-1], iv), flush=True)
+        assert abs(n1 - n0) < 0.15 * n0, name
+        assert np.isfinite(vmax).all(), name
+    assert ref[2].max() < 8.0
+    assert lrn[2].max() <= 1.25 * ref[2].max()

@@ -155,6 +155,42 @@ def test_open_boundary_inflow_and_outflow_match_oracle(oracle_mod):
     assert set(s.download(S.FIELD_ID).tolist()) == set(st1["id"][keep].tolist())
 
 
+def test_inflow_grows_the_fluid_capacity(oracle_mod):
+    """An inflow beyond the capacity grows it (change_max_particle_number,
+    OpenBoundariesSimple.cu:570-573) instead of failing: a solver made with
+    room for only 10 more particles takes the same inflow and then runs the
+    same steps, bit for bit, as one made with room for all of them; an
+    explicit dfsph_change_max_particle_number mid-run changes nothing either."""
+    om = oracle_mod
+    sc, cand, R = _basin()
+    r = sc.particle_radius
+    kw = dict(center=(0, 0, 0), half=(0, 10, 0))
+    out = []
+    for cap in (sc.n_fluid + 10, 3 * sc.n_fluid):
+        s = _solver(sc, dt=0.002, fluid_capacity=cap)
+        for _ in range(5):
+            s.step()
+        gb, _ = _pair(om, "cylinder", radius=R, **kw)
+        gi, _ = _pair(om, "cylinder", radius=R - 3 * r, **kw)
+        gsu, _ = _pair(om, "plane", o=(0, 0.5, 0), n=(0, -1, 0))
+        s.open_boundary_init(gb, gi, gsu, cand)
+        added, _ = s.open_boundary_apply(1.75 * r, 700.0, use_inflow=True, use_outflow=False)
+        inf = s.info()
+        assert added > 10 and inf.n_fluid == sc.n_fluid + added <= inf.fluid_capacity
+        if cap < sc.n_fluid + added:
+            assert inf.fluid_capacity >= 2 * cap
+        for k in range(6):
+            if k == 3 and cap > sc.n_fluid + 10:
+                s.change_max_particle_number(4 * sc.n_fluid)
+                assert s.info().fluid_capacity == 4 * sc.n_fluid
+            s.step()
+        out.append((added, s.state_by_id()))
+        s.close()
+    assert out[0][0] == out[1][0]
+    for f in ("id", "pos", "vel", "kappa", "kappaV", "density"):
+        assert np.array_equal(out[0][1][f], out[1][1][f]), f
+
+
 def test_window_displacement_moves_surfaces_and_inflow(oracle_mod):
     """Moving the window by d is the same as building everything shifted by d."""
     sc, cand, R = _basin()
