@@ -73,11 +73,12 @@ def kernel_bytes(name, n_f, n_b, sum_nf, sum_nb):
     return n_f * s + passes * (4 * (sum_nf + sum_nb) + 16 * n_b)
 
 
-def make_params(sc, mode, max_neighbors, min_iters, floor_only=False):
+def make_params(sc, mode, max_neighbors, min_iters, floor_only=False, gate="above-floors"):
     p = S.default_params(
         particle_radius=sc.particle_radius, dt=0.001, use_cfl=1, cfl_factor=1.0, dt_min=1e-4, dt_max=5e-3,
         max_error=0.1, max_error_v=0.1, max_neighbors=max_neighbors,
-        warm_start_mode=S.WARMSTART_LEARNED if mode == "learned" else S.WARMSTART_REFERENCE)
+        warm_start_mode=S.WARMSTART_LEARNED if mode == "learned" else S.WARMSTART_REFERENCE,
+        learned_gate=S.LEARNED_ABOVE_FLOORS if gate == "above-floors" else S.LEARNED_ALWAYS)
     p.min_iter_v, p.min_iter = min_iters
     if floor_only:
         # cost of a step whose solves stop at their floors: what a perfect warm
@@ -249,6 +250,9 @@ def main():
     ap.add_argument("--floor-only", action="store_true",
                     help="stop both solves at their minimum iterations: the cost of a step with a perfect warm "
                          "start (tolerances not enforced; not a valid throughput)")
+    ap.add_argument("--learned-gate", default="above-floors", choices=["above-floors", "always"],
+                    help="learned warm start every step, or only after a step whose solves ran above their "
+                         "minimum iterations (the reference warm start otherwise)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned-leg", action="store_true",
@@ -269,7 +273,7 @@ def main():
     max_nbr = 90 if args.config == "c1" else 128
     min_iters = tuple(int(v) for v in args.min_iters.split(","))
     prof_steps = args.prof_steps or min(args.steps, 50)
-    p = make_params(sc, args.mode, max_nbr, min_iters, args.floor_only)
+    p = make_params(sc, args.mode, max_nbr, min_iters, args.floor_only, args.learned_gate)
 
     leg = run_leg(sc, p, tank, args.mode, args.steps, args.warmup, prof_steps)
     dominant, roof = roofline_of(leg)
@@ -314,8 +318,7 @@ def main():
     if args.floor_only:
         line["config"]["floor_only"] = "solves stopped at their minimum iterations: cost ceiling, not a valid throughput"
     if args.mode == "plain" and not args.no_learned_leg and not args.floor_only:
-        pl = S.default_params()
-        pl = make_params(sc, "learned", max_nbr, min_iters)
+        pl = make_params(sc, "learned", max_nbr, min_iters, gate=args.learned_gate)
         lrn = run_leg(sc, pl, tank, "learned", args.steps, args.warmup, prof_steps)
         ms = 1e3 * lrn["elapsed"] / args.steps
         lp = lrn["prof"]
@@ -326,6 +329,7 @@ def main():
             "ms_per_step": ms,
             "value": lrn["n_f"] * args.steps / lrn["elapsed"],
             "speedup_vs_plain": line["ms_per_step"] / ms,
+            "learned_gate": args.learned_gate,
             "iters_per_step": iters_summary(lrn["rec"]),
             "errors": errors_summary(lrn),
             "max_neighbors_seen": int(max(lrn["rec"]["max_seen"])),

@@ -63,7 +63,15 @@ typedef struct dfsph_params {
     int32_t warm_start_mode;  /* DFSPH_WARMSTART_*                                            */
     uint32_t check_interval;  /* solver iterations queued per convergence readback (0: 4)     */
     int32_t list_mode;        /* DFSPH_LISTS_*: neighbor-list form (results are identical)     */
+    int32_t learned_gate;     /* DFSPH_LEARNED_*: when the learned warm start runs             */
 } dfsph_params;
+
+/* learned_gate (warm_start_mode DFSPH_WARMSTART_LEARNED only) */
+#define DFSPH_LEARNED_ALWAYS 0       /* every step                                                  */
+#define DFSPH_LEARNED_ABOVE_FLOORS 1 /* only after a step whose divergence or density solve ran
+                                        more than its minimum iterations; otherwise the step uses
+                                        the reference warm start and skips the features and the
+                                        network (at the floors no warm start saves an iteration) */
 
 /* list_mode: how sweep blocks (256 particles) read their fluid neighbors */
 #define DFSPH_LISTS_AUTO 0       /* LDS-staged set when it fits, else 16-bit set ids, else 32-bit */
@@ -218,6 +226,19 @@ int dfsph_add_particles(void* handle, const float* xyz, const float* vel, const 
  * reference order, with the library's kernel fusions. */
 int dfsph_step(void* handle, dfsph_step_stats* stats);
 
+/* The loading of DynamicWindow::loadDataToSimulation (DynamicWindow.cu:2624-3140)
+ * by the rule of the open boundary's inflow (OpenBoundariesSimple.cu:360-508)
+ * without its direction test: each candidate position (e.g. a background fluid
+ * buffer at rest) inside the boundary's box is added when no fluid particle is
+ * within min_distance and the SPH density there (fluid and boundary) is at most
+ * max_density; velocity 0, or the neighbors' (m W)-weighted mean with
+ * interpolate_velocity.  Candidates are decided against the fluid before the
+ * call (not against each other).  New particles: mass of fluid particle 0,
+ * kappa = kappaV = 0, fresh ids, untagged, appended in candidate order; the
+ * capacity grows as needed. */
+int dfsph_load_fluid(void* handle, const float* xyz, uint32_t n, float min_distance, float max_density,
+                     int interpolate_velocity, uint32_t* added);
+
 /* change_max_particle_number (DFSPH_memory_management_cuda.cu:962-1018): grow
  * the fluid capacity to max_particles (no change when it is not larger),
  * keeping the particles in their order; the next step rebuilds the neighbor
@@ -230,7 +251,11 @@ int dfsph_change_max_particle_number(void* handle, uint32_t max_particles);
  * count must equal the number of elements of the field. */
 int dfsph_download(void* handle, int field, void* dst, uint32_t count);
 /* overwrite a fluid field (device order): POS, VEL, MASS, KAPPA, KAPPAV, TAG,
- * KAPPA_TOTAL, KAPPAV_TOTAL */
+ * KAPPA_TOTAL, KAPPAV_TOTAL, and the derived DENSITY and DENSITY_ADV (a
+ * restricted solve keeps them for the particles it leaves out and sums the
+ * stored DENSITY_ADV of those into its average error).  The derived fields
+ * are not carried by reorderings (sorting searches, removals, the active-first
+ * order): they are valid from the next density computation of a particle. */
 int dfsph_upload_field(void* handle, int field, const void* src, uint32_t count);
 
 /* neighbor lists of the last search, device order, row-major [i*max + k]:
@@ -292,7 +317,9 @@ int dfsph_profile_read(void* handle, uint32_t kernel, uint32_t* launches, double
  * [0, count_active + count_active_neighbors) (dfsph_step then searches without
  * reordering).  true_particle_count >= 0 replaces N in the average errors
  * (DFSPH_core_cuda.cu:791-795, 1339-1343).  Restricted-out particles keep
- * their stored rho_adv in the error sums, as the reference's reductions do. */
+ * their stored rho_adv in the error sums, as the reference's reductions do.
+ * Neighbor lists of an unrestricted search cover every particle and stay
+ * valid under a new restriction; those of a restricted search do not. */
 int dfsph_set_restriction(void* handle, int mode, uint32_t count_active, uint32_t count_active_neighbors,
                           int64_t true_particle_count);
 /* tag_neighbors_of_tagged_kernel<untagged_only, true> (RestFluidLoader.cu:2070-2101):

@@ -45,6 +45,33 @@ int dfsph_cuda_set_open_boundaries(void* solver, const dfsph_surface* boundary, 
                                    const dfsph_surface* fluid_surface, const float* inflow_xyz, uint32_t n,
                                    float allowed_new_distance, float allowed_new_density, int use_inflow,
                                    int use_outflow);
+/* DFSPHCUDA::applyDynamicWindow (DFSPH_CUDA.cpp:3988-4139) over
+ * DynamicWindowInterface (DynamicWindow.cu:150-237): every later
+ * dfsph_cuda_step starts by moving the simulated region (`simulation`: a
+ * cuboid, cylinder or sphere at zero displacement) toward rigid body
+ * `tracked_body` in x and z once it is more than `trigger` away (0: 2.5 h),
+ * by at most `max_displacement` (0: 3.5 h): the region's walls (kinematic
+ * body `wall_body`, -1: none) move with it, the existing fluid is kept
+ * inside the region less 3 r of before the move and more than 1.5 h inside
+ * the moved region (the rest is removed), the background buffer (n points at
+ * rest, moving with the region) is loaded into the band of width `band`
+ * (0: max_displacement + 1.5 h) inside the moved region less 3 r (x and z
+ * faces of a cuboid, the radius of a cylinder or sphere) and under `fluid`
+ * where no fluid is within
+ * 1.75 r and the density is at most 0.7 rho0 (dfsph_load_fluid, the inflow's
+ * values), and the loaded particles are settled by 10
+ * density-solver iterations of 3 ms restricted to them and their neighbors,
+ * with a velocity damping of 0.8 -> 0.2 (DynamicWindow::stabilizeFluid; they
+ * leave at rest, the warm starts are cleared, the time and the rigid bodies
+ * are restored afterwards); an open boundary set with
+ * dfsph_cuda_set_open_boundaries follows the region. */
+int dfsph_cuda_set_dynamic_window(void* solver, const dfsph_surface* simulation, const dfsph_surface* fluid,
+                                  const float* background, uint32_t n, int tracked_body, int wall_body,
+                                  double trigger, double max_displacement, double band);
+/* the window's total displacement (3 doubles), its moves so far, and the
+ * particles loaded and removed by the last move */
+int dfsph_cuda_dynamic_window_state(void* solver, double* displacement, uint32_t* moves, uint32_t* loaded,
+                                    uint32_t* removed);
 void dfsph_cuda_set_divergence(void* solver, int enable);
 /* the libdfsph_hip.so handle owned by the solver (for dfsph_download etc.) */
 void* dfsph_cuda_handle(void* solver);

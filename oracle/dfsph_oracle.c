@@ -1217,8 +1217,8 @@ void or_features(const or_state* s, double* out)
         f[1] = density_change_sum(s, i) * s->h / rho0;           /* divergence source, unclamped      */
         f[2] = nn / 40.0;                                        /* neighbor count                    */
         f[3] = sqrt(dot3(vi, vi)) * s->h / diameter;             /* speed in diameters per step       */
-        f[4] = s->kappa[i] * s->h * s->h;                        /* last density increments kappa h_f^2 */
-        f[5] = s->kappaV[i] * s->h_past;                         /* last divergence increments kappaV h */
+        f[4] = s->kappa[i] * s->h * s->h / (diameter * diameter); /* last density increments kappa h_f^2 / d^2 */
+        f[5] = s->kappaV[i] * s->h_past / diameter;              /* last divergence increments kappaV h / d */
         f[6] = (double)s->nbc[i] * inv_n;                        /* boundary share of the neighbors   */
         f[7] = off[0] * inv_n / hk;                              /* mean offset to the neighbors / hk */
         f[8] = off[1] * inv_n / hk;
@@ -1231,10 +1231,12 @@ void or_features(const or_state* s, double* out)
 void or_warm_start_targets(const or_state* s, double* out)
 {
     /* after a step's solves: the increments the reference warm start left to
-     * the Jacobi iterations, in dt-invariant units (kappaV h, kappa h_f^2) */
+     * the Jacobi iterations, in units free of the time step and of the
+     * particle diameter d (kappaV h / d, kappa h_f^2 / d^2) */
+    const double d = 2.0 * s->p.particle_radius;
     for (uint32_t i = 0; i < s->n; ++i) {
-        out[2 * i + 0] = s->kappaV[i] * s->h;
-        out[2 * i + 1] = s->kappa[i] * s->h_future * s->h_future;
+        out[2 * i + 0] = s->kappaV[i] * s->h / d;
+        out[2 * i + 1] = s->kappa[i] * s->h_future * s->h_future / (d * d);
     }
 }
 

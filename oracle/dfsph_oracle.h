@@ -167,12 +167,14 @@ void   or_kernel_gradW(const double* r3, double hk, int table, double* out3);
  * particle (n*12):
  *   0 rho / rho0 - 1 (fluid + boundary)    1 sum m_j (v_i - v_j).gradW h / rho0
  *   2 (n_fluid + n_boundary) / 40          3 |v| h / (2r)
- *   4 kappa h^2 (last density increments)  5 kappaV h_past (last divergence increments)
+ *   4 kappa h^2 / d^2 (last density increments)  5 kappaV h_past / d (last divergence
+ *   increments), d = 2r
  *   6 n_boundary / n                       7-9 mean (x_j - x_i) / hk
  *   10 mean |x_i - x_j| / hk              11 fluid share of the density */
 void or_features(const or_state* s, double* out);
 /* after a step's two solves, before the step end: the training targets, the
- * solves' increments in dt-invariant units (kappaV h, kappa h_future^2), n*2 */
+ * solves' increments in units free of dt and d = 2r (kappaV h / d,
+ * kappa h_future^2 / d^2), n*2 */
 void or_warm_start_targets(const or_state* s, double* out);
 
 /* FP64 per-particle MLP used by the learned warm start (12-64-64-64-2, SiLU).

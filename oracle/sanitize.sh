@@ -12,7 +12,7 @@ mkdir -p "$OUT"
 gcc -O1 -g $SAN -fopenmp -fPIC -ffp-contract=off -std=c11 -Wall -Wno-unknown-pragmas -shared \
     -o "$OUT/liboracle.so" "$ROOT/oracle/dfsph_oracle.c" -lm
 g++ -O1 -g $SAN -std=c++17 -fPIC -shared -Wall -I"$ROOT/include" -I"$PKG/csrc" \
-    "$PKG/csrc/DFSPH_CUDA.cpp" "$PKG/csrc/SceneLoader.cpp" \
+    "$PKG/csrc/DFSPH_CUDA.cpp" "$PKG/csrc/SceneLoader.cpp" "$PKG/csrc/DynamicWindow.cpp" \
     -L"$PKG" -ldfsph_hip -Wl,-rpath,"$PKG" -o "$OUT/libdfsph_host.so"
 export LD_PRELOAD="$(gcc -print-file-name=libasan.so) $(gcc -print-file-name=libubsan.so)"
 export ASAN_OPTIONS=detect_leaks=0:halt_on_error=1

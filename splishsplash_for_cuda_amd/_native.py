@@ -31,6 +31,8 @@ WARMSTART_NONE = 2
 LISTS_AUTO = 0
 LISTS_NO_LDS = 1
 LISTS_GLOBAL = 2
+LEARNED_ALWAYS = 0
+LEARNED_ABOVE_FLOORS = 1
 
 FIELD_POS = 0
 FIELD_VEL = 1
@@ -77,6 +79,7 @@ class DFSPHParams(C.Structure):
         ("warm_start_mode", C.c_int32),
         ("check_interval", C.c_uint32),
         ("list_mode", C.c_int32),
+        ("learned_gate", C.c_int32),
     ]
 
 
@@ -260,6 +263,7 @@ HIP_SIGNATURES = {
     "dfsph_get_info": (_I, [_P, C.POINTER(DFSPHInfo)]),
     "dfsph_synchronize": (_I, [_P]),
     "dfsph_change_max_particle_number": (_I, [_P, _U32]),
+    "dfsph_load_fluid": (_I, [_P, _F32P, _U32, C.c_float, C.c_float, _I, C.POINTER(C.c_uint32)]),
     "dfsph_neighbor_stats": (_I, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                  C.POINTER(C.c_uint64)]),
     "dfsph_profile_enable": (_I, [_P, _U32]),
@@ -293,6 +297,10 @@ HOST_SIGNATURES = {
     "dfsph_cuda_set_open_boundaries": (_I, [_P, C.POINTER(DFSPHSurface), C.POINTER(DFSPHSurface),
                                             C.POINTER(DFSPHSurface), _F32P, _U32, C.c_float, C.c_float, _I, _I]),
     "dfsph_cuda_set_divergence": (None, [_P, _I]),
+    "dfsph_cuda_set_dynamic_window": (_I, [_P, C.POINTER(DFSPHSurface), C.POINTER(DFSPHSurface), _F32P, _U32, _I,
+                                           _I, C.c_double, C.c_double, C.c_double]),
+    "dfsph_cuda_dynamic_window_state": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint32),
+                                             C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "dfsph_cuda_handle": (C.c_void_p, [_P]),
     "dfsph_cuda_save": (_I, [_P, C.c_char_p]),
     "dfsph_cuda_load": (_I, [_P, C.c_char_p]),
@@ -685,6 +693,15 @@ class Solver:
         return out
 
     # ---- data out ----
+    def load_fluid(self, xyz, min_distance, max_density, interpolate_velocity=False):
+        """dfsph_load_fluid: add the candidates far enough from the fluid; returns the count added"""
+        x = _f32(xyz).reshape(-1)
+        a = C.c_uint32()
+        _check(self.lib.dfsph_load_fluid(self.h, _ptr(x, C.c_float), x.size // 3, float(min_distance),
+                                         float(max_density), 1 if interpolate_velocity else 0, C.byref(a)),
+               "dfsph_load_fluid")
+        return a.value
+
     def change_max_particle_number(self, max_particles):
         _check(self.lib.dfsph_change_max_particle_number(self.h, int(max_particles)),
                "dfsph_change_max_particle_number")
@@ -821,6 +838,26 @@ class HostSolver:
                                                      1 if use_inflow else 0, 1 if use_outflow else 0)
         if rc != DFSPH_OK:
             raise DFSPHError(rc, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
+
+    def set_dynamic_window(self, simulation, fluid, background_xyz, tracked_body=-1, wall_body=-1, trigger=0.0,
+                           max_displacement=0.0, band=0.0):
+        """dfsph_cuda_set_dynamic_window: the simulated region follows rigid body tracked_body, its walls
+        (kinematic body wall_body) with it"""
+        b = _f32(background_xyz).reshape(-1)
+        rc = self.lib.dfsph_cuda_set_dynamic_window(self.s, C.byref(simulation), C.byref(fluid), _ptr(b, C.c_float),
+                                                    b.size // 3, int(tracked_body), int(wall_body), float(trigger),
+                                                    float(max_displacement), float(band))
+        if rc != DFSPH_OK:
+            raise DFSPHError(rc, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
+
+    def window_state(self):
+        """(total displacement[3], moves, loaded and removed by the last move)"""
+        d = (C.c_double * 3)()
+        m, l, r = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        rc = self.lib.dfsph_cuda_dynamic_window_state(self.s, d, C.byref(m), C.byref(l), C.byref(r))
+        if rc != DFSPH_OK:
+            raise DFSPHError(rc, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
+        return np.array(d[:]), m.value, l.value, r.value
 
     def set_divergence(self, enable: bool):
         self.lib.dfsph_cuda_set_divergence(self.s, 1 if enable else 0)

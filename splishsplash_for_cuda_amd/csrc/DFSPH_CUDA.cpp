@@ -239,15 +239,28 @@ void DFSPHCUDA::applyOpenBoundaries()
 {
     const float dist = m_ob.allowed_new_distance > 0.0f ? m_ob.allowed_new_distance
                                                         : (float)(1.75 * m_data.params.particle_radius);
-    check(dfsph_open_boundary_apply(m_data.handle(), nullptr, dist, m_ob.allowed_new_density, m_ob.inflow ? 1 : 0,
+    // the open boundary follows the dynamic window (dynamicWindowTotalDisplacement)
+    const double* disp = m_window ? m_window->state().displacement : nullptr;
+    check(dfsph_open_boundary_apply(m_data.handle(), disp, dist, m_ob.allowed_new_density, m_ob.inflow ? 1 : 0,
                                     m_ob.outflow ? 1 : 0, &m_inflowAdded, &m_outflowRemoved),
           "dfsph_open_boundary_apply");
+}
+
+void DFSPHCUDA::setDynamicWindow(const DynamicWindowParams& p)
+{
+    m_window.reset(new DynamicWindow(m_data.handle(), p));
+}
+
+void DFSPHCUDA::applyDynamicWindow()
+{
+    if (m_window) m_window->apply();
 }
 
 void DFSPHCUDA::step()
 {
     void* h = m_data.handle();
     m_inflowAdded = m_outflowRemoved = 0;
+    if (m_window) applyDynamicWindow();              // DFSPH_CUDA.cpp:142-150
     if (m_openBoundaries) applyOpenBoundaries();     // DFSPH_CUDA.cpp:132-140
     check(dfsph_neighbor_search(h, 1), "dfsph_neighbor_search");
     if (m_enableDivergenceSolver) {
@@ -527,6 +540,41 @@ int dfsph_cuda_set_open_boundaries(void* s, const dfsph_surface* boundary, const
         ob.inflow = use_inflow != 0;
         ob.outflow = use_outflow != 0;
         S(s)->setOpenBoundaries(ob);
+    });
+}
+
+int dfsph_cuda_set_dynamic_window(void* s, const dfsph_surface* simulation, const dfsph_surface* fluid,
+                                  const float* background, uint32_t n, int tracked_body, int wall_body,
+                                  double trigger, double max_displacement, double band)
+{
+    return guarded([&] {
+        if (!simulation || !fluid || (n && !background))
+            throw SPH::DFSPHError(DFSPH_ERR_ARG, "surfaces and the background buffer are required");
+        SPH::DynamicWindowParams p;
+        p.simulation = *simulation;
+        p.fluid = *fluid;
+        if (n) p.background.assign(background, background + 3 * (size_t)n);
+        p.tracked_body = tracked_body;
+        p.wall_body = wall_body;
+        p.trigger = trigger;
+        p.max_displacement = max_displacement;
+        p.band = band;
+        S(s)->setDynamicWindow(p);
+    });
+}
+
+int dfsph_cuda_dynamic_window_state(void* s, double* displacement, uint32_t* moves, uint32_t* loaded,
+                                    uint32_t* removed)
+{
+    return guarded([&] {
+        const SPH::DynamicWindow* w = S(s)->dynamicWindow();
+        if (!w) throw SPH::DFSPHError(DFSPH_ERR_STATE, "no dynamic window set");
+        const SPH::DynamicWindowState& st = w->state();
+        if (displacement)
+            for (int a = 0; a < 3; ++a) displacement[a] = st.displacement[a];
+        if (moves) *moves = st.moves;
+        if (loaded) *loaded = st.loaded;
+        if (removed) *removed = st.removed;
     });
 }
 

@@ -14,6 +14,9 @@
 #include <string>
 #include <vector>
 
+#include <memory>
+
+#include "DynamicWindow.h"
 #include "dfsph_hip.h"
 
 namespace SPH {
@@ -149,6 +152,12 @@ public:
     };
     void setOpenBoundaries(const OpenBoundaries& ob);
     void applyOpenBoundaries();
+    // DFSPHCUDA::applyDynamicWindow (DFSPH_CUDA.cpp:3988-4139): every step()
+    // first moves the window when its tracked body asks for it (loading and
+    // stabilization included); the open boundary follows the window.
+    void setDynamicWindow(const DynamicWindowParams& p);
+    void applyDynamicWindow();
+    const DynamicWindow* dynamicWindow() const { return m_window.get(); }
     uint32_t getInflowAddedLastStep() const { return m_inflowAdded; }
     uint32_t getOutflowRemovedLastStep() const { return m_outflowRemoved; }
 
@@ -175,6 +184,7 @@ protected:
     OpenBoundaries m_ob;
     uint32_t m_inflowAdded = 0, m_outflowRemoved = 0;
     uint32_t m_maxNeighborsSeen = 0, m_neighborOverflow = 0;
+    std::unique_ptr<DynamicWindow> m_window;
 };
 
 }  // namespace SPH

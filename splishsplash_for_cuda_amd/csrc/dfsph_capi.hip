@@ -135,8 +135,14 @@ struct Handle {
 
     bool pend_vel = false;            // deferred cuda_update_vel (fused into the pressure warm start)
     bool lists_valid = false;
+    int lists_rmode = 0;              // restriction mode the lists were built under
     bool acc_valid = false;
     bool pred_valid = false;          // features / predictions of this step computed
+    // learned warm start gate: the warm-start mode of the current step (set by
+    // its first solve after a search) and the last solves' iteration counts
+    int ws_step = DFSPH_WARMSTART_REFERENCE;
+    bool ws_step_set = false;
+    uint32_t last_iters_v = UINT32_MAX, last_iters = UINT32_MAX;
 
     dfsph::MlpState* mlp = nullptr;   // learned warm start
 
@@ -401,9 +407,28 @@ int neighbor_search(Handle* h, bool sort)
     pend(h, pn);
     HIPCHK(hipGetLastError());
     h->lists_valid = true;
+    h->lists_rmode = h->rmode;
     h->acc_valid = false;
     h->nbr_op = h->ops;
+    h->ws_step_set = false;
     return DFSPH_OK;
+}
+
+// The warm-start mode of this step: the configured one, except that a gated
+// learned warm start falls back to the reference warm start after a step whose
+// solves both stopped at their minimum iterations (DFSPH_LEARNED_ABOVE_FLOORS).
+int step_warm_start(Handle* h)
+{
+    if (!h->ws_step_set) {
+        int ws = h->p.warm_start_mode;
+        if (ws == DFSPH_WARMSTART_LEARNED && h->p.learned_gate == DFSPH_LEARNED_ABOVE_FLOORS &&
+            h->last_iters_v != UINT32_MAX && h->last_iters != UINT32_MAX &&
+            h->last_iters_v <= h->p.min_iter_v && h->last_iters <= h->p.min_iter)
+            ws = DFSPH_WARMSTART_REFERENCE;
+        h->ws_step = ws;
+        h->ws_step_set = true;
+    }
+    return h->ws_step;
 }
 
 // Queue Jacobi iterations in batches; the device decides when the loop ends
@@ -490,7 +515,7 @@ int jacobi_loop(Handle* h, bool div, uint32_t max_iter, double max_error, uint32
 // Features are also computed while a training set is being harvested.
 int learned_predict(Handle* h)
 {
-    const bool learned = h->p.warm_start_mode == DFSPH_WARMSTART_LEARNED;
+    const bool learned = step_warm_start(h) == DFSPH_WARMSTART_LEARNED;
     const bool harvest = h->mlp && dfsph::mlp_harvesting(h->mlp);
     if (!learned && !harvest) return DFSPH_OK;
     if (h->pred_valid) return DFSPH_OK;
@@ -518,9 +543,10 @@ int divergence_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* i
 {
     CHK(flush_vel(h));
     if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale: call dfsph_neighbor_search");
-    const int ws = h->p.warm_start_mode;
+    const int ws = step_warm_start(h);
     CHK(learned_predict(h));
-    const dfsph::WsLearned lw = ws == DFSPH_WARMSTART_LEARNED ? dfsph::mlp_ws_v(h->mlp) : dfsph::WsLearned{};
+    const float dia = (float)(2.0 * h->p.particle_radius);
+    const dfsph::WsLearned lw = ws == DFSPH_WARMSTART_LEARNED ? dfsph::mlp_ws_v(h->mlp, dia) : dfsph::WsLearned{};
     const int vo = h->vc ^ 1;
     int pr = pbeg(h, DFSPH_KERNEL_DIV_WS_INIT);
     dfsph::launch_div_ws_init(h->s, ws, h->c, h->tm, h->n, h->pos[h->pc], h->bpos, lists(h), h->vel[h->vc],
@@ -534,7 +560,10 @@ int divergence_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* i
     pend(h, pr);
     HIPCHK(hipGetLastError());
     h->acc_valid = false;
-    CHK(jacobi_loop(h, true, max_iter, max_error, iters, err));
+    uint32_t it = 0;
+    CHK(jacobi_loop(h, true, max_iter, max_error, &it, err));
+    if (iters) *iters = it;
+    h->last_iters_v = it;
     // the step's divergence total (applied warm start + increments) in kappaV h
     dfsph::launch_finish_totals(h->s, h->tm, h->n, 0, h->kapv[h->pc], h->kvtot[h->pc]);
     HIPCHK(hipGetLastError());
@@ -570,11 +599,11 @@ int external_forces(Handle* h)
 int pressure_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* iters, double* err)
 {
     if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale: call dfsph_neighbor_search");
-    const int ws = h->p.warm_start_mode;
+    const int ws = step_warm_start(h);
     dfsph::WsLearned lw{};
     if (ws == DFSPH_WARMSTART_LEARNED) {
         CHK(learned_predict(h));
-        lw = dfsph::mlp_ws_p(h->mlp);
+        lw = dfsph::mlp_ws_p(h->mlp, (float)(2.0 * h->p.particle_radius));
     }
     if (ws == DFSPH_WARMSTART_NONE) {
         CHK(flush_vel(h));
@@ -594,14 +623,19 @@ int pressure_solve(Handle* h, uint32_t max_iter, double max_error, uint32_t* ite
     pend(h, pr);
     HIPCHK(hipGetLastError());
     h->acc_valid = false;
-    CHK(jacobi_loop(h, false, max_iter, max_error, iters, err));
+    uint32_t it = 0;
+    CHK(jacobi_loop(h, false, max_iter, max_error, &it, err));
+    if (iters) *iters = it;
+    h->last_iters = it;
+    if (!h->p.enable_divergence) h->last_iters_v = 0;
     // the step's density total (applied warm start + increments) in kappa h_f^2
     dfsph::launch_finish_totals(h->s, h->tm, h->n, 1, h->kap[h->pc], h->ktot[h->pc]);
     if (h->mlp && dfsph::mlp_harvesting(h->mlp)) {
         // targets: what the reference warm start left to the iterations
         if (ws != DFSPH_WARMSTART_REFERENCE)
             return fail(DFSPH_ERR_STATE, "a training set is harvested with the reference warm start");
-        CHK(dfsph::mlp_harvest_step(h->mlp, h->s, h->tm, h->n, h->kapv[h->pc], h->kap[h->pc]));
+        CHK(dfsph::mlp_harvest_step(h->mlp, h->s, h->tm, h->n, h->kapv[h->pc], h->kap[h->pc],
+                                    (float)(2.0 * h->p.particle_radius)));
     }
     HIPCHK(hipGetLastError());
     return DFSPH_OK;
@@ -1367,6 +1401,9 @@ int dfsph_upload_field(void* hv, int field, const void* src, uint32_t count)
     if (field == DFSPH_FIELD_KAPPA) { HIPCHK(hipMemcpy(h->kap[h->pc], src, 4ull * n, hipMemcpyHostToDevice)); return DFSPH_OK; }
     if (field == DFSPH_FIELD_KAPPAV) { HIPCHK(hipMemcpy(h->kapv[h->pc], src, 4ull * n, hipMemcpyHostToDevice)); return DFSPH_OK; }
     if (field == DFSPH_FIELD_TAG) { HIPCHK(hipMemcpy(h->tag[h->pc], src, 4ull * n, hipMemcpyHostToDevice)); return DFSPH_OK; }
+    // derived fields, read by restricted solves for the particles they leave out
+    if (field == DFSPH_FIELD_DENSITY) { HIPCHK(hipMemcpy(h->rho, src, 4ull * n, hipMemcpyHostToDevice)); return DFSPH_OK; }
+    if (field == DFSPH_FIELD_DENSITY_ADV) { HIPCHK(hipMemcpy(h->radv, src, 4ull * n, hipMemcpyHostToDevice)); return DFSPH_OK; }
     if (field == DFSPH_FIELD_KAPPA_TOTAL || field == DFSPH_FIELD_KAPPAV_TOTAL) {
         float* d = field == DFSPH_FIELD_KAPPA_TOTAL ? h->ktot[h->pc] : h->kvtot[h->pc];
         HIPCHK(hipMemcpy(d, src, 4ull * n, hipMemcpyHostToDevice));
@@ -1480,7 +1517,9 @@ int dfsph_set_restriction(void* hv, int mode, uint32_t count_active, uint32_t co
     h->r_active = count_active;
     h->r_active_nbrs = count_active_neighbors;
     h->true_count = true_particle_count;
-    h->lists_valid = false;             // lists are built for the restricted set only
+    // lists of a restricted search cover its restricted set only; those of an
+    // unrestricted one cover every particle and serve any restriction
+    if (h->lists_rmode != 0) h->lists_valid = false;
     return DFSPH_OK;
 }
 
@@ -1732,6 +1771,74 @@ int dfsph_open_boundary_apply(void* hv, const double* window_displacement, float
         CHK(compact(h, removed));
     }
     return DFSPH_OK;
+}
+
+int dfsph_load_fluid(void* hv, const float* xyz, uint32_t nc, float min_distance, float max_density,
+                     int interpolate_velocity, uint32_t* added)
+{
+    Handle* h = H(hv);
+    if (added) *added = 0;
+    if (!h->grid_ready || !h->pos[0]) return fail(DFSPH_ERR_STATE, "boundary and fluid must be uploaded first");
+    if (nc && !xyz) return fail(DFSPH_ERR_ARG, "null candidates");
+    if (!(min_distance > 0.0f) || !(max_density > 0.0f)) return fail(DFSPH_ERR_ARG, "positive distance and density");
+    if (nc == 0 || h->n == 0) return DFSPH_OK;
+    CHK(flush_vel(h));
+    // candidates inside the grid only (the cell table covers the boundary's box)
+    std::vector<float4> P;
+    P.reserve(nc);
+    for (uint32_t i = 0; i < nc; ++i) {
+        const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        if (x >= h->bmin[0] && x <= h->bmax[0] && y >= h->bmin[1] && y <= h->bmax[1] && z >= h->bmin[2] &&
+            z <= h->bmax[2])
+            P.push_back(make_float4(x, y, z, 0.0f));
+    }
+    const uint32_t m = (uint32_t)P.size();
+    if (m == 0) return DFSPH_OK;
+    float4 *cand = nullptr, *vnew = nullptr;
+    uint32_t *k = nullptr, *r = nullptr, *tmp = nullptr;
+    auto release = [&] { dfree(cand); dfree(vnew); dfree(k); dfree(r); dfree(tmp); };
+    int rc = DFSPH_OK;
+    if ((rc = dalloc(cand, m)) || (rc = dalloc(vnew, m)) || (rc = dalloc(k, m + 1)) || (rc = dalloc(r, m + 1)) ||
+        (rc = dalloc(tmp, dfsph::scan_temp_words(m + 1)))) {
+        release();
+        return rc;
+    }
+    auto body = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(cand, P.data(), sizeof(float4) * m, hipMemcpyHostToDevice, h->s));
+        // rigid bodies at their current places in the boundary table (a body
+        // moved since the last search counts where it is now)
+        if (h->ns) CHK(place_bodies(h));
+        const uint32_t n = h->n;
+        CHK(counting_sort(h, h->pos[h->pc], n, h->cstart));
+        float4 p0;
+        HIPCHK(hipMemcpyAsync(&p0, h->pos[h->pc], sizeof(float4), hipMemcpyDeviceToHost, h->s));
+        CHK(sync(h));
+        // no direction rule: a plane surface with a zero normal admits every velocity
+        dfsph::Surface any{};
+        any.type = DFSPH_SURFACE_PLANE;
+        dfsph::launch_inflow(h->s, h->c, cand, m, h->pos[h->pc], h->vel[h->vc], h->cstart, h->perm, h->bpos,
+                             h->bcstart, any, min_distance, max_density, p0.w, k, vnew);
+        if (!interpolate_velocity) dfsph::launch_fill_f32(h->s, reinterpret_cast<float*>(vnew), 0.0f, 4 * m);
+        dfsph::launch_exclusive_scan(h->s, k, r, m + 1, tmp);
+        HIPCHK(hipMemcpyAsync(&h->hctl->pad, r + m, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s));
+        CHK(sync(h));
+        const uint32_t a = h->hctl->pad;
+        if (n + a > h->cap) CHK(grow_capacity(h, std::max(2 * h->cap, n + a)));
+        if (a) {
+            dfsph::launch_append_inflow(h->s, cand, m, k, r, vnew, n, p0.w, h->next_id, fluid_ptrs(h, h->pc, h->vc));
+            HIPCHK(hipGetLastError());
+            h->n = n + a;
+            h->next_id += a;
+            h->lists_valid = false;
+            h->acc_valid = false;
+            h->pred_valid = false;
+        }
+        if (added) *added = a;
+        return sync(h);
+    };
+    rc = body();
+    release();
+    return rc;
 }
 
 int dfsph_change_max_particle_number(void* hv, uint32_t max_particles)

@@ -777,10 +777,12 @@ __device__ __forceinline__ double block_sum(double v, double* lds4)
 
 // The learned warm start adds the network's correction to the reference warm
 // start.  Features 4 and 5 of a particle hold the last solves' increments in
-// the dt-invariant units (kappa h_f^2, kappaV h), from which the reference
-// forms its warm starts (DFSPH_core_cuda.cu:240, :3063-3068); the network's
-// targets are the increments the next solves still need beyond those warm
-// starts, in the same units.
+// units free of the time step and of the particle diameter d (kappa h_f^2 /
+// d^2, kappaV h / d; kappa h_f^2 scales as d^2 and kappaV h as d for the same
+// compression and velocities), from which the reference forms its warm starts
+// (DFSPH_core_cuda.cu:240, :3063-3068); the network's targets are the
+// increments the next solves still need beyond those warm starts, in the same
+// units, so that a model trained at one resolution applies at another.
 constexpr int kFeatKappaP = 4, kFeatKappaV = 5;
 
 // The warm start a sweep applies in learned mode, in raw units: the
@@ -800,11 +802,14 @@ __host__ __device__ __forceinline__ float ws_learned(float ref, float prev, floa
 }
 
 // The learned warm start's inputs to the sweeps: the network's corrections
-// (dt-invariant units) and the last step's converged totals (same units), one
-// float per particle each.
+// and the last step's converged totals (dt-invariant units), one float per
+// particle each.  The network works in units free of the particle size too:
+// its corrections are scaled by `scale` (the particle diameter d for the
+// divergence solver, d^2 for the density solver) before use.
 struct WsLearned {
     const float* dy;
     const float* prev;
+    float scale;
 };
 
 }  // namespace dfsph

@@ -945,7 +945,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_ws_init(SimConst
         if (WS == 2) return 0.0f;
         const float ref = fmaxf(kws[j] * ratio / 2.0f, -0.5f);
         if (WS == 0) return ref;
-        return ws_learned(ref, fmaxf(0.5f * lw.prev[j] * inv_h, -0.5f), lw.dy[j] * inv_h);
+        return ws_learned(ref, fmaxf(0.5f * lw.prev[j] * inv_h, -0.5f), lw.dy[j] * lw.scale * inv_h);
     };
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
@@ -1271,7 +1271,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
     auto ws_of = [&](uint32_t j) -> float {
         const float ref = fmaxf(kws[j] * ratio2, -0.5f);
         if (WS == 0) return ref;
-        return ws_learned(ref, fmaxf(lw.prev[j] * ihf2, -0.5f), lw.dy[j] * ihf2);
+        return ws_learned(ref, fmaxf(lw.prev[j] * ihf2, -0.5f), lw.dy[j] * lw.scale * ihf2);
     };
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
@@ -1455,7 +1455,7 @@ __global__ void k_finish_totals(const DevTime* t, uint32_t n, int pressure, cons
 // (before the divergence warm start):
 //  0 rho / rho0 - 1            1 divergence source h / rho0 (unclamped)
 //  2 (n_fluid + n_boundary) / 40   3 |v| h / (2r)
-//  4 last density increments kappa h_f^2   5 last divergence increments kappaV h
+//  4 last density increments kappa h_f^2 / d^2   5 last divergence increments kappaV h / d
 //  6 boundary fraction n_b / n     7-9 mean neighbor offset / h_k
 // 10 mean neighbor distance / h_k 11 fluid share of the density
 template <bool SOL, bool UM>
@@ -1522,8 +1522,9 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_features(SimConst c,
     f[1] = dr * t->h / c.density0;
     f[2] = nn * (1.0f / 40.0f);
     f[3] = __builtin_sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z) * t->h / diameter;
-    f[kFeatKappaP] = kap[i] * (t->h * t->h);
-    f[kFeatKappaV] = kapv[i] * t->h_past;
+    const float inv_d = 1.0f / diameter;
+    f[kFeatKappaP] = kap[i] * (t->h * t->h) * (inv_d * inv_d);
+    f[kFeatKappaV] = kapv[i] * t->h_past * inv_d;
     f[6] = (float)nb * inv_n;
     f[7] = ox * s;
     f[8] = oy * s;
@@ -1539,7 +1540,8 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_features(SimConst c,
 // (kappaV h, kappa h_f^2), what the reference warm start left to the Jacobi
 // iterations: the correction the network learns
 __global__ void k_harvest(const DevTime* t, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap,
-                          const float* feat, const float* kapv, const float* kap, float* ds_x, float* ds_t)
+                          const float* feat, const float* kapv, const float* kap, float inv_d, float* ds_x,
+                          float* ds_t)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t i = k * stride;
@@ -1547,8 +1549,8 @@ __global__ void k_harvest(const DevTime* t, uint32_t n, uint32_t stride, uint32_
     const size_t o = offset + k;
     const float* f = feat + (size_t)i * 12;
     for (int a = 0; a < 12; ++a) ds_x[o * 12 + a] = f[a];
-    ds_t[o * 2 + 0] = kapv[i] * t->h;
-    ds_t[o * 2 + 1] = kap[i] * (t->h_future * t->h_future);
+    ds_t[o * 2 + 0] = kapv[i] * t->h * inv_d;
+    ds_t[o * 2 + 1] = kap[i] * (t->h_future * t->h_future) * (inv_d * inv_d);
 }
 
 // ------------------------------------------------------------------------
@@ -1823,10 +1825,12 @@ void launch_features(hipStream_t s, const SimConst& c, const DevTime* t, uint32_
 }
 
 void launch_harvest(hipStream_t s, const DevTime* t, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap,
-                    const float* feat, const float* kapv, const float* kap, float* ds_x, float* ds_t)
+                    const float* feat, const float* kapv, const float* kap, float diameter, float* ds_x,
+                    float* ds_t)
 {
     const uint32_t m = (n + stride - 1) / stride;
-    if (m) k_harvest<<<nblk(m), kBlock, 0, s>>>(t, n, stride, offset, cap, feat, kapv, kap, ds_x, ds_t);
+    if (m) k_harvest<<<nblk(m), kBlock, 0, s>>>(t, n, stride, offset, cap, feat, kapv, kap, 1.0f / diameter, ds_x,
+                                                ds_t);
 }
 
 }  // namespace dfsph

@@ -89,7 +89,8 @@ void launch_features(hipStream_t s, const SimConst& c, const DevTime* t, uint32_
                      const float* kapv, const float* ktot, const float* kvtot, float diameter, float* feat,
                      float* prev_p, float* prev_v);
 void launch_harvest(hipStream_t s, const DevTime* t, uint32_t n, uint32_t stride, uint32_t offset, uint32_t cap,
-                    const float* feat, const float* kapv, const float* kap, float* ds_x, float* ds_t);
+                    const float* feat, const float* kapv, const float* kap, float diameter, float* ds_x,
+                    float* ds_t);
 
 // open boundary / dynamic window (dfsph_window.hip)
 void launch_surface_keep(hipStream_t s, const float4* p, uint32_t n, const Surface& S, int want_inside, uint32_t* keep);

@@ -652,8 +652,14 @@ namespace dfsph {
 bool mlp_loaded(const MlpState* m) { return m && m->loaded; }
 bool mlp_harvesting(const MlpState* m) { return m && m->ds_cap > 0 && m->ds_n < m->ds_cap; }
 float* mlp_feat(MlpState* m) { return m->feat; }
-WsLearned mlp_ws_v(const MlpState* m) { return WsLearned{m->pred, m->pred + 2 * (size_t)m->cap}; }
-WsLearned mlp_ws_p(const MlpState* m) { return WsLearned{m->pred + m->cap, m->pred + 3 * (size_t)m->cap}; }
+WsLearned mlp_ws_v(const MlpState* m, float diameter)
+{
+    return WsLearned{m->pred, m->pred + 2 * (size_t)m->cap, diameter};
+}
+WsLearned mlp_ws_p(const MlpState* m, float diameter)
+{
+    return WsLearned{m->pred + m->cap, m->pred + 3 * (size_t)m->cap, diameter * diameter};
+}
 float* mlp_prev_p(MlpState* m) { return m->pred + 3 * (size_t)m->cap; }
 float* mlp_prev_v(MlpState* m) { return m->pred + 2 * (size_t)m->cap; }
 
@@ -675,11 +681,12 @@ int mlp_infer_particles(MlpState* m, hipStream_t s, uint32_t n)
     return DFSPH_OK;
 }
 
-int mlp_harvest_step(MlpState* m, hipStream_t s, const DevTime* t, uint32_t n, const float* kapv, const float* kap)
+int mlp_harvest_step(MlpState* m, hipStream_t s, const DevTime* t, uint32_t n, const float* kapv, const float* kap,
+                     float diameter)
 {
     if (!mlp_harvesting(m) || n == 0) return DFSPH_OK;
     const uint32_t stride = m->ds_stride ? m->ds_stride : 1;
-    launch_harvest(s, t, n, stride, m->ds_n, m->ds_cap, m->feat, kapv, kap, m->ds_x, m->ds_t);
+    launch_harvest(s, t, n, stride, m->ds_n, m->ds_cap, m->feat, kapv, kap, diameter, m->ds_x, m->ds_t);
     MHIP(hipGetLastError());
     const uint32_t add = (n + stride - 1) / stride;
     m->ds_n = m->ds_n + add > m->ds_cap ? m->ds_cap : m->ds_n + add;

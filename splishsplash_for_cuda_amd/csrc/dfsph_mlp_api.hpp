@@ -28,13 +28,14 @@ float* mlp_feat(MlpState* m);            // [cap][12] features of the current st
 // predictions from the features of the first n particles
 int mlp_infer_particles(MlpState* m, hipStream_t s, uint32_t n);
 // the sweeps' inputs: corrections and last totals, kappaV h and kappa h_f^2
-WsLearned mlp_ws_v(const MlpState* m);
-WsLearned mlp_ws_p(const MlpState* m);
+WsLearned mlp_ws_v(const MlpState* m, float diameter);
+WsLearned mlp_ws_p(const MlpState* m, float diameter);
 // where the feature sweep copies the last step's totals
 float* mlp_prev_p(MlpState* m);
 float* mlp_prev_v(MlpState* m);
 // append (features, increments of the step's solves) to the training set
-int mlp_harvest_step(MlpState* m, hipStream_t s, const DevTime* t, uint32_t n, const float* kapv, const float* kap);
+int mlp_harvest_step(MlpState* m, hipStream_t s, const DevTime* t, uint32_t n, const float* kapv, const float* kap,
+                     float diameter);
 void mlp_destroy(MlpState* m);
 
 }  // namespace dfsph

@@ -665,7 +665,7 @@ def test_features_lattice_interior_closed_form(oracle_mod):
     and s sqrt3 (8), q = s / hk times 1, sqrt2, sqrt3, so
       f0 = m (W0 + 6 W1 + 12 W2 + 8 W3) / rho0 - 1
       f1 = h sum_j m (v_i - v_j).gradW_ij / rho0 = -c h m / rho0 sum_j |d_j| W'(|d_j|)
-      f2 = 26/40, f3 = 0 (v_c = 0), f4 = kappa h^2, f5 = kappaV h_past,
+      f2 = 26/40, f3 = 0 (v_c = 0), f4 = kappa h^2 / d^2, f5 = kappaV h_past / d,
       f6 = 0, f7..9 = 0, f10 = q (6 + 12 sqrt2 + 8 sqrt3) / 26,
       f11 = (6 W1 + 12 W2 + 8 W3) / (W0 + 6 W1 + 12 W2 + 8 W3)."""
     om = oracle_mod
@@ -693,8 +693,8 @@ def test_features_lattice_interior_closed_form(oracle_mod):
     want[0] = m * (_W(0.0, hk) + sw) / 1000.0 - 1.0
     want[1] = -c * 0.001 * m / 1000.0 * sum(cnt * q * hk * _dW(q, hk) for q, cnt in qs)
     want[2] = 26 / 40
-    want[4] = -0.3 * 0.001 ** 2
-    want[5] = -0.2 * 0.002
+    want[4] = -0.3 * 0.001 ** 2 / (2 * r) ** 2
+    want[5] = -0.2 * 0.002 / (2 * r)
     want[10] = sum(cnt * q for q, cnt in qs) / 26
     want[11] = sw / (_W(0.0, hk) + sw)
     np.testing.assert_allclose(f, want, rtol=1e-12, atol=1e-14)
@@ -732,7 +732,7 @@ def test_features_fluid_and_boundary_pair_by_hand(oracle_mod):
 @pytest.mark.parametrize("which", ["divergence", "density"])
 def test_warm_start_targets_are_dt_invariant(oracle_mod, which):
     """The targets are the solves' increments in dt-invariant units
-    (kappaV h, kappa h_f^2): with the iteration counts fixed, the same
+    (kappaV h / d, kappa h_f^2 / d^2, d = 2r): with the iteration counts fixed, the same
     configuration stepped with h = 1 ms and h = 2 ms gives the same targets.
     Divergence: alpha ~ 1/h, the source does not depend on h and the velocity
     change h (k_i + k_j) is h-free.  Density (at rest, compressed, h = h_f,
@@ -762,6 +762,44 @@ def test_warm_start_targets_are_dt_invariant(oracle_mod, which):
         o.divergence_solve()
         o.external_forces()
         o.update_time_step(h)
+        o.update_vel()
+        o.pressure_solve()
+        out.append(o.warm_start_targets()[:, 0 if which == "divergence" else 1])
+    assert np.abs(out[0]).max() > 0
+    np.testing.assert_allclose(out[1], out[0], rtol=1e-9, atol=1e-12 * np.abs(out[0]).max())
+
+
+@pytest.mark.parametrize("which", ["divergence", "density"])
+def test_warm_start_targets_are_size_invariant(oracle_mod, which):
+    """The same configuration at r and at 2r (all positions doubled, the same
+    velocities, time step and iteration counts): with m ~ r^3 and gradW ~ 1/r^4
+    the factor scales as r^2 and the sources as 1/r (divergence) or not at all
+    (density, at rest), so kappaV h scales as d and kappa h_f^2 as d^2, and
+    the targets, divided by d and d^2, are equal.  A target without the
+    division, or divided by d for the density solver, fails by 2x.  The
+    velocities are large enough that no k comes near the solver's absolute
+    1e-5 gates, which do not scale."""
+    om = oracle_mod
+    out = []
+    rng = np.random.default_rng(13)
+    base = np.stack(np.meshgrid(*(np.arange(8),) * 3, indexing="ij"), -1).reshape(-1, 3).astype(float)
+    base = base + rng.uniform(-0.05, 0.05, size=base.shape)
+    for r in (0.01, 0.02):
+        g = base * 2 * r + 0.2
+        if which == "divergence":
+            v = -0.4 * (base - base.mean(0)) * 0.5
+            kw = dict(min_iter_v=3, max_iter_v=3, min_iter=0, max_iter=0)
+        else:
+            g = g.mean(0) + 0.93 * (g - g.mean(0))     # compressed, at rest
+            v = np.zeros_like(g)
+            kw = dict(min_iter_v=0, max_iter_v=0, min_iter=2, max_iter=2)
+        o = om.Oracle(particle_radius=r, dt=0.002, gravity=(0.0, 0.0, 0.0), **kw)
+        o.set_boundary(np.array([[50.0, 50.0, 50.0]]))
+        o.set_fluid(g, v)
+        o.neighbor_search()
+        o.divergence_solve()
+        o.external_forces()
+        o.update_time_step(0.002)
         o.update_vel()
         o.pressure_solve()
         out.append(o.warm_start_targets()[:, 0 if which == "divergence" else 1])
