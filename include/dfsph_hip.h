@@ -445,6 +445,17 @@ int dfsph_mlp_set_params(void* handle, const float* params, const float* norm);
 int dfsph_mlp_get_params(void* handle, float* params, float* norm);
 /* y[n][2] = MLP(x[n][12]) on the GPU (normalise: apply the stored normalisation) */
 int dfsph_mlp_forward(void* handle, const float* x, uint32_t n, float* y, int normalise);
+/* No reference counterpart.  dfsph_mlp_forward without any device allocation,
+ * for callers that cannot allocate device memory (code around HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * features and outputs are staged in a static device array of
+ * DFSPH_MLP_NOALLOC_MAX samples (3.7 MB per device, held until the process
+ * ends); n above that returns DFSPH_ERR_ARG.  Needs parameters set first
+ * (dfsph_mlp_set_params), else DFSPH_ERR_NO_MODEL.  Every call on a device
+ * shares that array: calls that overlap on one device (from several threads
+ * or handles) race. */
+#define DFSPH_MLP_NOALLOC_MAX 65536u
+int dfsph_mlp_forward_noalloc(void* handle, const float* x, uint32_t n, float* y, int normalise);
 /* loss = sum 0.5 |MLP(x) - t|^2 / n and its gradient (DFSPH_MLP_NPARAM floats),
  * computed by the training kernels (forward + backward on the matrix cores) */
 int dfsph_mlp_loss_grad(void* handle, const float* x, const float* t, uint32_t n, float* grad, double* loss,

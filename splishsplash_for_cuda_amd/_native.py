@@ -272,6 +272,7 @@ HIP_SIGNATURES = {
     "dfsph_mlp_set_params": (_I, [_P, _F32P, _F32P]),
     "dfsph_mlp_get_params": (_I, [_P, _F32P, _F32P]),
     "dfsph_mlp_forward": (_I, [_P, _F32P, _U32, _F32P, _I]),
+    "dfsph_mlp_forward_noalloc": (_I, [_P, _F32P, _U32, _F32P, _I]),
     "dfsph_mlp_loss_grad": (_I, [_P, _F32P, _F32P, _U32, _F32P, C.POINTER(_D), _I]),
     "dfsph_mlp_harvest": (_I, [_P, _U32, _U32]),
     "dfsph_mlp_dataset_size": (_I, [_P, _U32P]),
@@ -284,6 +285,7 @@ HIP_SIGNATURES = {
 
 MLP_NPARAM = 9282
 MLP_NNORM = 28
+MLP_NOALLOC_MAX = 65536
 MLP_NFEAT = 12
 
 KERNEL_NAMES = ["sort", "neighbors", "div_ws_init", "div_init", "div_jacobi", "div_loop_end", "xsph",
@@ -626,6 +628,15 @@ class Solver:
         y = np.zeros((x.shape[0], 2), np.float32)
         _check(self.lib.dfsph_mlp_forward(self.h, _ptr(x, C.c_float), x.shape[0], _ptr(y, C.c_float),
                                           1 if normalise else 0), "dfsph_mlp_forward")
+        return y
+
+    def mlp_forward_noalloc(self, x, normalise=True):
+        """dfsph_mlp_forward_noalloc: the same outputs, staged in the library's
+        static device array (no device allocation)"""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_NFEAT)
+        y = np.zeros((x.shape[0], 2), np.float32)
+        _check(self.lib.dfsph_mlp_forward_noalloc(self.h, _ptr(x, C.c_float), x.shape[0], _ptr(y, C.c_float),
+                                                  1 if normalise else 0), "dfsph_mlp_forward_noalloc")
         return y
 
     def mlp_loss_grad(self, x, t, normalise=False):

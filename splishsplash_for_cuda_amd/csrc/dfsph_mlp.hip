@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -553,6 +554,34 @@ int pack(MlpState* m, hipStream_t s)
 
 }  // namespace
 
+// Staging of dfsph_mlp_forward_noalloc: features then outputs of up to
+// DFSPH_MLP_NOALLOC_MAX samples, one copy per device for the life of the
+// process.  Host code uses the device address that hipGetSymbolAddress
+// returns for the current device, never the array's name.
+static __device__ __align__(256) float scratch[(size_t)DFSPH_MLP_NOALLOC_MAX * (dfsph::kIn + dfsph::kOut)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// the scratch address on the current device, looked up on its first use
+int scratch_address(float** out)
+{
+    int dev = 0;
+    MHIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return mfail(DFSPH_ERR_STATE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        MHIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return DFSPH_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int dfsph_mlp_set_params(void* handle, const float* params, const float* norm)
@@ -610,6 +639,29 @@ int dfsph_mlp_forward(void* handle, const float* x, uint32_t n, float* y, int no
     if (dx) (void)hipFree(dx);
     if (dy) (void)hipFree(dy);
     return rc;
+}
+
+int dfsph_mlp_forward_noalloc(void* handle, const float* x, uint32_t n, float* y, int normalise)
+{
+    if (n > DFSPH_MLP_NOALLOC_MAX)
+        return mfail(DFSPH_ERR_ARG, "dfsph_mlp_forward_noalloc: n exceeds DFSPH_MLP_NOALLOC_MAX (" +
+                                        std::to_string(DFSPH_MLP_NOALLOC_MAX) + ")");
+    if (n && (!x || !y)) return mfail(DFSPH_ERR_ARG, "null input or output");
+    const dfsph::MlpView v = dfsph::handle_view(handle);
+    MlpState* m = *v.mlp;            // never created here: that would allocate
+    if (!m || !m->loaded) return mfail(DFSPH_ERR_NO_MODEL, "no model parameters set");
+    if (n == 0) return DFSPH_OK;
+    float* base = nullptr;
+    if (int rc = scratch_address(&base)) return rc;
+    float* dx = base;
+    float* dy = base + (size_t)DFSPH_MLP_NOALLOC_MAX * dfsph::kIn;
+    MHIP(hipMemcpyAsync(dx, x, sizeof(float) * n * dfsph::kIn, hipMemcpyHostToDevice, v.s));
+    dfsph::k_mlp_infer<<<dfsph::infer_grid(n), dfsph::kInferBlock, 0, v.s>>>(m->model, dx, n, dy, normalise,
+                                                                              dfsph::kOut, 1);
+    MHIP(hipGetLastError());
+    MHIP(hipMemcpyAsync(y, dy, sizeof(float) * n * dfsph::kOut, hipMemcpyDeviceToHost, v.s));
+    MHIP(hipStreamSynchronize(v.s));
+    return DFSPH_OK;
 }
 
 int dfsph_mlp_loss_grad(void* handle, const float* x, const float* t, uint32_t n, float* grad, double* loss,
