@@ -135,6 +135,17 @@ const char* dfsph_last_error(void);
 int dfsph_create(const dfsph_params* p, void** handle);
 /* DFSPHCData::~DFSPHCData (DFSPH_c_arrays_structure.cpp:946-973) */
 int dfsph_destroy(void* handle);
+/* the parameters the handle was created with (the DFSPHCData members,
+ * DFSPH_c_arrays_structure.h:297-420, read back) */
+int dfsph_get_params(void* handle, dfsph_params* out);
+/* The HIP stream the handle queues its work on (a hipStream_t; NULL: the
+ * handle's own non-blocking stream, made at create).  The reference queues
+ * on the default stream; a caller's stream orders the solver against the
+ * caller's own kernels and copies.  Setting it waits for the work already
+ * queued; the caller keeps the stream alive while the handle uses it and
+ * keeps it at the default priority. */
+int dfsph_set_stream(void* handle, void* stream);
+int dfsph_get_stream(void* handle, void** stream);
 
 /* Static boundary set: allocate_UnifiedParticleSet_cuda + load_UnifiedParticleSet_cuda
  * (DFSPH_memory_management_cuda.cu:502-564, :566-650), the one-time boundary cell table
@@ -309,6 +320,7 @@ int dfsph_profile_read(void* handle, uint32_t kernel, uint32_t* launches, double
 #define DFSPH_TAG_ACTIVE 10000000u
 #define DFSPH_TAG_ACTIVE_NEIGHBORS 20000000u
 #define DFSPH_TAG_UNTAGGED 40000000u
+#define DFSPH_TAG_AIR 50000000u
 #define DFSPH_TAG_REMOVAL 310000000u
 /* mode 0: whole fluid.  mode 1: velocity / position updates on the particles
  * tagged ACTIVE, density / factor / source / neighbor lists also on those
@@ -329,6 +341,19 @@ int dfsph_set_restriction(void* handle, int mode, uint32_t count_active, uint32_
 int dfsph_tag_neighbors(void* handle, uint32_t tag_in, uint32_t tag_out, int untagged_only, uint32_t* changed);
 /* count_particle_with_tag_kernel (RestFluidLoader.cu:2238-2252) */
 int dfsph_count_tagged(void* handle, uint32_t tag, uint32_t* count);
+/* One density-extraction pass of the rest-fluid loader
+ * (RestFLuidLoader::tagDataToSurface, RestFluidLoader.cu:2700-3130, selection
+ * particle_selection_rule_1_kernel, :773-890), on the lists of the last
+ * unrestricted search: the densities are computed (boundary included).
+ * Particles tagged DFSPH_TAG_AIR (the reference's air particles above the
+ * fluid surface) only add density and stay.  Any other particle is a
+ * candidate and is removed (order kept) when the boundary alone gives it more
+ * than boundary_limit, or when its density exceeds limit and no listed
+ * candidate above limit is denser (of equal densities the lower index goes),
+ * so of two crowded neighbors one goes per pass.  removed: the count;
+ * avg_density: the mean density of the kept candidates, before the removal. */
+int dfsph_remove_density_peaks(void* handle, float limit, float boundary_limit, uint32_t* removed,
+                               double* avg_density);
 /* Stable reorder: ACTIVE, then ACTIVE_NEIGHBORS, then the rest (the prefix
  * order of restriction mode 2; the reference radix-sorts by tag,
  * DynamicWindow.cu:2990-2998); returns both counts. */

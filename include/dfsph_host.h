@@ -68,6 +68,20 @@ int dfsph_cuda_set_open_boundaries(void* solver, const dfsph_surface* boundary, 
 int dfsph_cuda_set_dynamic_window(void* solver, const dfsph_surface* simulation, const dfsph_surface* fluid,
                                   const float* background, uint32_t n, int tracked_body, int wall_body,
                                   double trigger, double max_displacement, double band);
+/* RestFLuidLoaderInterface::initializeFluidToSurface + stabilizeFluid
+ * (RestFluidLoader.h:31-309, RestFluidLoader.cu:127-207): the solver's fluid
+ * becomes the n background points (a fluid at rest, e.g. a lattice of spacing
+ * 2 r 0.8^(1/3), larger than the volume) inside `fluid`, less those the density
+ * extraction passes remove, the background points outside `fluid` within one
+ * kernel radius serving as air particles (dfsph_remove_density_peaks at a limit stepping
+ * from density_start down to density_end by step_density; 0: 1900, 1000 and
+ * 50), settled by stab_iters damped solver iterations of 3 ms (0: 10) and left
+ * at rest (velocities and warm starts zero, the time and rigid bodies
+ * restored).  loaded: the particle count; passes: extraction passes run;
+ * avg_density: the mean density after the last pass. */
+int dfsph_cuda_load_rest_fluid(void* solver, const dfsph_surface* fluid, const float* background, uint32_t n,
+                               float density_start, float density_end, float step_density, uint32_t stab_iters,
+                               uint32_t* loaded, uint32_t* passes, double* avg_density);
 /* the window's total displacement (3 doubles), its moves so far, and the
  * particles loaded and removed by the last move */
 int dfsph_cuda_dynamic_window_state(void* solver, double* displacement, uint32_t* moves, uint32_t* loaded,

@@ -1179,6 +1179,54 @@ uint32_t or_outflow(const or_state* s, const or_surface* interior, const or_surf
     return removed;
 }
 
+/* One density-extraction pass of the rest-fluid loader
+ * (RestFLuidLoader::tagDataToSurface, RestFluidLoader.cu:2700-3130, its
+ * selection particle_selection_rule_1_kernel, :773-890): densities on the
+ * current lists (boundary included).  air (n flags, may be NULL) marks the
+ * air particles, which only add density and stay.  For a candidate (any
+ * other), keep[i] = 0 when the boundary alone gives it more than
+ * boundary_limit, or when rho_i > limit and no listed candidate above limit
+ * is denser (of equal densities the lower index goes).  Returns the count
+ * removed; *avg: the mean density of the kept candidates; rho_out (n, may be
+ * NULL): the densities.  The state keeps its particles. */
+uint32_t or_rest_extract(or_state* s, double limit, double boundary_limit, const uint8_t* air, uint8_t* keep,
+                         double* avg, double* rho_out)
+{
+    const uint32_t n = s->n;
+    for (uint32_t i = 0; i < n; ++i) density_and_factor(s, i);
+    uint32_t removed = 0, kept = 0;
+    double sum = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (rho_out) rho_out[i] = s->rho[i];
+        if (air && air[i]) { keep[i] = 1; continue; }
+        const uint32_t* row = ROW(i);
+        const double* xi = XI(i);
+        double rb = 0.0, d[3];
+        for (uint32_t k = 0; k < s->nbc[i]; ++k) {
+            const uint32_t j = row[s->nf[i] + k];
+            sub3(xi, XB(j), d);
+            rb += s->psi[j] * kW(s, d);
+        }
+        const double ri = s->rho[i];
+        int leave = rb > boundary_limit;
+        if (!leave && ri > limit) {
+            int densest = 1;
+            for (uint32_t k = 0; k < s->nf[i]; ++k) {
+                const uint32_t j = row[k];
+                if (air && air[j]) continue;
+                const double rj = s->rho[j];
+                if (rj > limit && (rj > ri || (rj == ri && j < i))) densest = 0;
+            }
+            leave = densest;
+        }
+        keep[i] = (uint8_t)!leave;
+        removed += (uint32_t)leave;
+        if (!leave) { sum += ri; kept++; }
+    }
+    if (avg) *avg = kept ? sum / (double)kept : 0.0;
+    return removed;
+}
+
 /* ------------------------------------------------------------------ */
 /* Features and training targets of the learned warm start (no reference */
 /* counterpart; inserted where the reference forms its warm starts,     */

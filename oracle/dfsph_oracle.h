@@ -157,6 +157,13 @@ uint32_t or_inflow(const or_state* s, const or_surface* boundary, uint32_t nc, c
                    double max_density, double mass, uint8_t* accept, double* vel, double* dmin_out,
                    double* dens_out);
 uint32_t or_outflow(const or_state* s, const or_surface* interior, const or_surface* surface, uint8_t* keep);
+/* one density-extraction pass of the rest-fluid loader (RestFluidLoader.cu:2700-3130,
+ * :773-890) on the lists of the last or_neighbor_search: keep[i] = 0 for a
+ * candidate (air[i] == 0 or air NULL) the boundary alone makes denser than
+ * boundary_limit, or one denser than limit with no denser listed candidate
+ * above limit; returns the count */
+uint32_t or_rest_extract(or_state* s, double limit, double boundary_limit, const uint8_t* air, uint8_t* keep,
+                         double* avg, double* rho_out);
 
 /* kernel evaluation (A3): analytic or table per flag */
 double or_kernel_W(double r, double hk, int table);

@@ -144,6 +144,8 @@ def lib():
                                        C.c_double, C.POINTER(C.c_uint8), _DP, _DP, _DP]),
             "or_outflow": (C.c_uint32, [C.c_void_p, C.POINTER(OrSurface), C.POINTER(OrSurface),
                                         C.POINTER(C.c_uint8)]),
+            "or_rest_extract": (C.c_uint32, [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_uint8),
+                                             C.POINTER(C.c_uint8), _DP, _DP]),
             "or_add_body": (C.c_int, [C.c_void_p, C.POINTER(OrBodyDesc), C.c_uint32, _DP, _DP]),
             "or_set_body": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(OrBodyDesc)]),
             "or_get_body": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(OrBodyDesc), _DP, _DP]),
@@ -346,6 +348,20 @@ class Oracle:
                          acc.ctypes.data_as(C.POINTER(C.c_uint8)), vel.ctypes.data_as(_DP), dmin.ctypes.data_as(_DP),
                          dens.ctypes.data_as(_DP))
         return acc.astype(bool), vel, dmin, dens
+
+    def rest_extract(self, limit, boundary_limit, air=None):
+        """one extraction pass of the rest-fluid loader on the current lists
+        (air: flags of the air particles, or None): (keep, mean density of the
+        kept candidates, densities)"""
+        n = self.n
+        keep = np.zeros(n, np.uint8)
+        avg = C.c_double()
+        rho = np.zeros(n)
+        a = None if air is None else np.ascontiguousarray(air, np.uint8)
+        self.L.or_rest_extract(self.s, float(limit), float(boundary_limit),
+                               None if a is None else a.ctypes.data_as(C.POINTER(C.c_uint8)),
+                               keep.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(avg), rho.ctypes.data_as(_DP))
+        return keep.astype(bool), avg.value, rho
 
     def outflow_keep(self, interior, surf):
         keep = np.zeros(self.n, np.uint8)

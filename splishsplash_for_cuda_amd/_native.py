@@ -190,6 +190,7 @@ class DFSPHSurface(C.Structure):
 
 SURFACE_PLANE, SURFACE_CUBOID, SURFACE_CYLINDER, SURFACE_SPHERE, SURFACE_STAR = 0, 1, 2, 4, 5
 TAG_ACTIVE, TAG_ACTIVE_NEIGHBORS, TAG_UNTAGGED, TAG_REMOVAL = 10000000, 20000000, 40000000, 310000000
+TAG_AIR = 50000000
 
 
 def surface(kind, reversed=False, o=(0, 0, 0), n=(0, 1, 0), center=(0, 0, 0), half=(0, 0, 0), radius=0.0,
@@ -222,6 +223,8 @@ _I = C.c_int
 HIP_SIGNATURES = {
     "dfsph_set_restriction": (_I, [_P, _I, _U32, _U32, C.c_int64]),
     "dfsph_tag_neighbors": (_I, [_P, _U32, _U32, _I, _U32P]),
+    "dfsph_remove_density_peaks": (_I, [_P, C.c_float, C.c_float, _U32P, C.POINTER(C.c_double)]),
+    "dfsph_get_params": (_I, [_P, C.POINTER(DFSPHParams)]),
     "dfsph_count_tagged": (_I, [_P, _U32, _U32P]),
     "dfsph_order_active_first": (_I, [_P, _U32P, _U32P]),
     "dfsph_remove_tagged": (_I, [_P, _U32, _U32P]),
@@ -238,6 +241,8 @@ HIP_SIGNATURES = {
     "dfsph_last_error": (C.c_char_p, []),
     "dfsph_create": (_I, [C.POINTER(DFSPHParams), C.POINTER(C.c_void_p)]),
     "dfsph_destroy": (_I, [_P]),
+    "dfsph_set_stream": (_I, [_P, _P]),
+    "dfsph_get_stream": (_I, [_P, C.POINTER(C.c_void_p)]),
     "dfsph_upload_boundary": (_I, [_P, _F32P, _F32P, _U32]),
     "dfsph_upload_fluid": (_I, [_P, _F32P, _F32P, _F32P, _U32P, _U32]),
     "dfsph_neighbor_search": (_I, [_P, _I]),
@@ -303,6 +308,8 @@ HOST_SIGNATURES = {
                                            _I, C.c_double, C.c_double, C.c_double]),
     "dfsph_cuda_dynamic_window_state": (_I, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint32),
                                              C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "dfsph_cuda_load_rest_fluid": (_I, [_P, C.POINTER(DFSPHSurface), _F32P, _U32, C.c_float, C.c_float, C.c_float,
+                                        _U32, _U32P, _U32P, C.POINTER(C.c_double)]),
     "dfsph_cuda_handle": (C.c_void_p, [_P]),
     "dfsph_cuda_save": (_I, [_P, C.c_char_p]),
     "dfsph_cuda_load": (_I, [_P, C.c_char_p]),
@@ -418,6 +425,16 @@ class Solver:
         except Exception:
             pass
 
+    def set_stream(self, stream):
+        """dfsph_set_stream: queue the solver's work on a HIP stream (an int handle such as
+        torch.cuda.Stream().cuda_stream; 0 or None: the handle's own stream)"""
+        _check(self.lib.dfsph_set_stream(self.h, C.c_void_p(int(stream or 0))), "dfsph_set_stream")
+
+    def get_stream(self) -> int:
+        p = C.c_void_p()
+        _check(self.lib.dfsph_get_stream(self.h, C.byref(p)), "dfsph_get_stream")
+        return int(p.value or 0)
+
     # ---- data in ----
     def upload_boundary(self, xyz, psi=None):
         x = _f32(xyz).reshape(-1)
@@ -529,6 +546,18 @@ class Solver:
         _check(self.lib.dfsph_tag_neighbors(self.h, tag_in, tag_out, 1 if untagged_only else 0, C.byref(c)),
                "dfsph_tag_neighbors")
         return c.value
+
+    def remove_density_peaks(self, limit, boundary_limit):
+        """dfsph_remove_density_peaks: one extraction pass; returns (removed, mean density of the kept)"""
+        c, a = C.c_uint32(), C.c_double()
+        _check(self.lib.dfsph_remove_density_peaks(self.h, float(limit), float(boundary_limit), C.byref(c),
+                                                   C.byref(a)), "dfsph_remove_density_peaks")
+        return c.value, a.value
+
+    def get_params(self) -> "DFSPHParams":
+        p = DFSPHParams()
+        _check(self.lib.dfsph_get_params(self.h, C.byref(p)), "dfsph_get_params")
+        return p
 
     def count_tagged(self, tag):
         c = C.c_uint32()
@@ -869,6 +898,19 @@ class HostSolver:
         if rc != DFSPH_OK:
             raise DFSPHError(rc, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
         return np.array(d[:]), m.value, l.value, r.value
+
+    def load_rest_fluid(self, fluid, background_xyz, density_start=0.0, density_end=0.0, step_density=0.0,
+                        stab_iters=0):
+        """dfsph_cuda_load_rest_fluid: the fluid becomes the background inside `fluid`, extracted and
+        settled at rest; returns (particles, extraction passes, mean density after the last pass)"""
+        b = _f32(background_xyz).reshape(-1)
+        n, k, a = C.c_uint32(), C.c_uint32(), C.c_double()
+        rc = self.lib.dfsph_cuda_load_rest_fluid(self.s, C.byref(fluid), _ptr(b, C.c_float), b.size // 3,
+                                                 float(density_start), float(density_end), float(step_density),
+                                                 int(stab_iters), C.byref(n), C.byref(k), C.byref(a))
+        if rc != DFSPH_OK:
+            raise DFSPHError(rc, self.lib.dfsph_cuda_last_error().decode(errors="replace"))
+        return n.value, k.value, a.value
 
     def set_divergence(self, enable: bool):
         self.lib.dfsph_cuda_set_divergence(self.s, 1 if enable else 0)

@@ -20,7 +20,7 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["dfsph_kernels.hip", "dfsph_capi.hip", "dfsph_mlp.hip", "dfsph_window.hip", "dfsph_bodies.hip"]
-HOST_SOURCES = ["DFSPH_CUDA.cpp", "SceneLoader.cpp", "DynamicWindow.cpp"]
+HOST_SOURCES = ["DFSPH_CUDA.cpp", "SceneLoader.cpp", "DynamicWindow.cpp", "RestFluidLoader.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled on purpose: hipcc may emit them from

@@ -1,6 +1,7 @@
 // DFSPH_CUDA.cpp -- C++ host classes over the C ABI of libdfsph_hip.so.
 // Built with g++ (no HIP header): every GPU action goes through dfsph_hip.h.
 #include "DFSPH_CUDA.h"
+#include "RestFluidLoader.h"
 
 #include <algorithm>
 #include <cmath>
@@ -575,6 +576,29 @@ int dfsph_cuda_dynamic_window_state(void* s, double* displacement, uint32_t* mov
         if (moves) *moves = st.moves;
         if (loaded) *loaded = st.loaded;
         if (removed) *removed = st.removed;
+    });
+}
+
+int dfsph_cuda_load_rest_fluid(void* s, const dfsph_surface* fluid, const float* background, uint32_t n,
+                               float density_start, float density_end, float step_density, uint32_t stab_iters,
+                               uint32_t* loaded, uint32_t* passes, double* avg_density)
+{
+    return guarded([&] {
+        if (!fluid || (n && !background))
+            throw SPH::DFSPHError(DFSPH_ERR_ARG, "the fluid surface and the background buffer are required");
+        SPH::RestFluidParams p;
+        p.fluid = *fluid;
+        if (n) p.background.assign(background, background + 3 * (size_t)n);
+        if (density_start > 0.0f) p.density_start = density_start;
+        if (density_end > 0.0f) p.density_end = density_end;
+        if (step_density > 0.0f) p.step_density = step_density;
+        if (stab_iters) p.stab_iters = stab_iters;
+        SPH::RestFluidLoader loader(S(s)->data().handle(), p);
+        loader.initializeFluidToSurface();
+        loader.stabilizeFluid();
+        if (loaded) *loaded = loader.state().loaded;
+        if (passes) *passes = loader.state().passes;
+        if (avg_density) *avg_density = loader.state().avg_density;
     });
 }
 

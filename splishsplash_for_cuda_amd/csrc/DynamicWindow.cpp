@@ -33,11 +33,11 @@ DynamicWindow::DynamicWindow(void* handle, const DynamicWindowParams& p) : m_h(h
         throw DFSPHError(DFSPH_ERR_ARG, "dynamic window: the region is a cuboid, a cylinder or a sphere");
     if (p.simulation.reversed) throw DFSPHError(DFSPH_ERR_ARG, "dynamic window: the region cannot be reversed");
     if (p.background.size() % 3) throw DFSPHError(DFSPH_ERR_ARG, "dynamic window: background is xyz triples");
-    dfsph_info info;
-    wcheck(dfsph_get_info(m_h, &info), "dfsph_get_info");
-    // the kernel support is 4 r (the cell edge is 1.001 of it)
-    m_hk = info.cell_size / 1.001;
-    m_r = m_hk / 4.0;
+    dfsph_params sp;
+    wcheck(dfsph_get_params(m_h, &sp), "dfsph_get_params");
+    m_r = sp.particle_radius;
+    m_hk = 4.0 * m_r;
+    m_rho0 = sp.density0;
     if (m_p.trigger <= 0.0) m_p.trigger = 2.5 * m_hk;                   // DFSPH_CUDA.cpp:4017
     if (m_p.max_displacement <= 0.0) m_p.max_displacement = 3.5 * m_hk; // :4021
     if (m_p.band <= 0.0) m_p.band = m_p.max_displacement + 1.5 * m_hk;  // S_bufferInterior, DynamicWindow.cu:305-307

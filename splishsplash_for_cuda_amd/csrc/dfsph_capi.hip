@@ -50,7 +50,8 @@ constexpr double kCellMargin = 1.001;   // cell edge = 1.001 h_k (conservative b
 
 struct Handle {
     dfsph_params p{};
-    hipStream_t s = nullptr;
+    hipStream_t s = nullptr;          // the stream all work is queued on
+    hipStream_t own_s = nullptr;      // the handle's own stream (s unless the caller set one)
     SimConst c{};
     bool grid_ready = false;
     uint32_t n = 0, nb = 0, cap = 0, tmp_cap = 0;
@@ -118,6 +119,7 @@ struct Handle {
     DevTime* tm = nullptr;
     double* partial = nullptr;
     float* partial_f = nullptr;
+    double* rest_sum = nullptr;       // dfsph_remove_density_peaks: density sum of the kept particles
     uint32_t partial_n = 0;
     DevCtl* hctl = nullptr;           // pinned
     DevTime* htm = nullptr;           // pinned
@@ -253,7 +255,7 @@ Handle::~Handle()
     dfree(key); dfree(slot); dfree(perm); dfree(hist); dfree(cstart); dfree(keep); dfree(rank);
     dfree(keep8); dfree(scan_tmp); dfree(bpos); dfree(bcstart); dfree(tile_rank); dfree(nbr_part);
     dfree(dbody); dfree(x0); dfree(sbody); dfree(braw); dfree(bvraw); dfree(bvel); dfree(sforce);
-    dfree(ctl); dfree(tm); dfree(partial); dfree(partial_f);
+    dfree(ctl); dfree(tm); dfree(partial); dfree(partial_f); dfree(rest_sum);
     if (hctl) (void)hipHostFree(hctl);
     if (htm) (void)hipHostFree(htm);
     if (hrec) (void)hipHostFree(hrec);
@@ -263,7 +265,7 @@ Handle::~Handle()
         (void)hipEventDestroy(r.b);
     }
     for (auto e : prof_pool) (void)hipEventDestroy(e);
-    if (s) (void)hipStreamDestroy(s);
+    if (own_s) (void)hipStreamDestroy(own_s);
 }
 
 template <class T>
@@ -934,7 +936,8 @@ int dfsph_create(const dfsph_params* p, void** out)
     if (h->p.check_interval == 0) h->p.check_interval = 4;
     int rc = DFSPH_OK;
     do {
-        if (hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking) != hipSuccess) { rc = fail(DFSPH_ERR_HIP, "stream"); break; }
+        if (hipStreamCreateWithFlags(&h->own_s, hipStreamNonBlocking) != hipSuccess) { rc = fail(DFSPH_ERR_HIP, "stream"); break; }
+        h->s = h->own_s;
         if ((rc = dalloc(h->ctl, 1)) != DFSPH_OK) break;
         if ((rc = dalloc(h->tm, 1)) != DFSPH_OK) break;
         if (hipHostMalloc((void**)&h->hctl, sizeof(DevCtl)) != hipSuccess ||
@@ -983,6 +986,30 @@ int dfsph_create(const dfsph_params* p, void** out)
 int dfsph_destroy(void* hv)
 {
     delete H(hv);
+    return DFSPH_OK;
+}
+
+int dfsph_get_params(void* hv, dfsph_params* out)
+{
+    if (!out) return fail(DFSPH_ERR_ARG, "null params");
+    *out = H(hv)->p;
+    return DFSPH_OK;
+}
+
+int dfsph_set_stream(void* hv, void* stream)
+{
+    Handle* h = H(hv);
+    CHK(flush_vel(h));
+    // the work queued so far completes before work on the new stream starts
+    CHK(sync(h));
+    h->s = stream ? static_cast<hipStream_t>(stream) : h->own_s;
+    return DFSPH_OK;
+}
+
+int dfsph_get_stream(void* hv, void** stream)
+{
+    if (!stream) return fail(DFSPH_ERR_ARG, "null stream");
+    *stream = H(hv)->s;
     return DFSPH_OK;
 }
 
@@ -1553,6 +1580,34 @@ int dfsph_tag_neighbors(void* hv, uint32_t tag_in, uint32_t tag_out, int untagge
     HIPCHK(hipMemcpyAsync(&h->hctl->pad, &h->ctl->pad, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s));
     CHK(sync(h));
     if (changed) *changed = h->hctl->pad;
+    return DFSPH_OK;
+}
+
+int dfsph_remove_density_peaks(void* hv, float limit, float boundary_limit, uint32_t* removed, double* avg_density)
+{
+    Handle* h = H(hv);
+    CHK(flush_vel(h));
+    if (!h->lists_valid) return fail(DFSPH_ERR_STATE, "neighbor lists are stale: call dfsph_neighbor_search");
+    if (h->rmode != 0) return fail(DFSPH_ERR_STATE, "dfsph_remove_density_peaks runs without a restriction");
+    const uint32_t n = h->n;
+    if (n == 0) {
+        if (removed) *removed = 0;
+        if (avg_density) *avg_density = 0.0;
+        return DFSPH_OK;
+    }
+    if (!h->rest_sum) CHK(dalloc(h->rest_sum, 2));
+    CHK(density_only(h));
+    dfsph::launch_fill_u32(h->s, reinterpret_cast<uint32_t*>(h->rest_sum), 0u, 4);   // 0.0, 0.0
+    dfsph::launch_fill_u32(h->s, h->keep + n, 0u, 1);              // the compaction scan's sentinel
+    dfsph::launch_rest_extract(h->s, h->c, n, lists(h), h->pos[h->pc], h->bpos, h->rho, h->tag[h->pc], limit,
+                               boundary_limit, h->keep, h->rest_sum, dfsph::grid_blocks_xcd(n));
+    HIPCHK(hipGetLastError());
+    double sum[2] = {0.0, 0.0};
+    HIPCHK(hipMemcpyAsync(sum, h->rest_sum, sizeof(sum), hipMemcpyDeviceToHost, h->s));
+    uint32_t rm = 0;
+    CHK(compact(h, &rm));                 // synchronises the stream
+    if (removed) *removed = rm;
+    if (avg_density) *avg_density = sum[1] > 0.0 ? sum[0] / sum[1] : 0.0;
     return DFSPH_OK;
 }
 

@@ -198,6 +198,7 @@ struct FluidPtrs {
 constexpr uint32_t kTagActive = 10000000u;
 constexpr uint32_t kTagActiveNeighbors = 20000000u;
 constexpr uint32_t kTagUntagged = 40000000u;
+constexpr uint32_t kTagAir = 50000000u;
 constexpr uint32_t kTagRemoval = 310000000u;
 
 // A BufferFluidSurface (BufferFluidSurface.cu:421-470) by value: the C ABI's

@@ -111,6 +111,9 @@ void launch_count_tag(hipStream_t s, const uint32_t* tag, uint32_t n, uint32_t t
 void launch_tag_neighbors(hipStream_t s, const SimConst& c, uint32_t n, const NbrLists& L, const uint32_t* src,
                           uint32_t* dst, uint32_t tag_in, uint32_t tag_out, int untagged_only, uint32_t grid);
 void launch_count_changed(hipStream_t s, const uint32_t* a, const uint32_t* b, uint32_t n, uint32_t* changed);
+void launch_rest_extract(hipStream_t s, const SimConst& c, uint32_t n, const NbrLists& L, const float4* pos,
+                         const float4* bpos, const float* rho, const uint32_t* tag, float limit, float boundary_limit,
+                         uint32_t* keep, double* sum, uint32_t grid);
 void launch_tag_classes(hipStream_t s, const uint32_t* tag, uint32_t n, uint32_t* f0, uint32_t* f1);
 void launch_partition3(hipStream_t s, uint32_t n, const uint32_t* f0, const uint32_t* r0, const uint32_t* f1,
                        const uint32_t* r1, const FluidPtrs& in, const FluidPtrs& out);

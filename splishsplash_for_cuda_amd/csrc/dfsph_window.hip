@@ -266,6 +266,68 @@ __global__ __launch_bounds__(kBlock) void k_tag_neighbors(SimConst c, uint32_t n
     }
 }
 
+// One extraction pass of the rest-fluid loader (RestFLuidLoader::tagDataToSurface,
+// RestFluidLoader.cu:2700-3130, its selection particle_selection_rule_1_kernel,
+// :773-890).  Particles tagged air (kTagAir) only add density: they stay and
+// are not counted.  A candidate (any other tag) leaves when the boundary
+// alone gives it a density above boundary_limit (the reference's 0.7 rho0),
+// or when its density rho[i] (of the last density computation, boundary and
+// air included) exceeds limit and no listed candidate above the limit is
+// denser; of equal densities the lower index leaves.  So of two crowded
+// neighbors one leaves per pass.  keep[i] = 0 / 1; sum[0] += rho and sum[1]
+// += 1 over the kept candidates.  keep[n] is set by the caller.
+__global__ __launch_bounds__(kBlock) void k_rest_extract(SimConst c, uint32_t n, NbrLists L,
+                                                         const float4* __restrict__ pos,
+                                                         const float4* __restrict__ bpos,
+                                                         const float* __restrict__ rho,
+                                                         const uint32_t* __restrict__ tag, float limit,
+                                                         float boundary_limit, uint32_t* __restrict__ keep,
+                                                         double* __restrict__ sum)
+{
+    const uint32_t b = xcd_block();
+    const uint32_t i = b * kBlock + threadIdx.x;
+    double kept_rho = 0.0, kept = 0.0;
+    if (i < n && tag[i] == kTagAir) {
+        keep[i] = 1u;
+    } else if (i < n) {
+        const uint32_t cn = L.cnt[i];
+        const uint32_t nf = cn & 0xffffu, nbn = cn >> 16;
+        const float4 pi = pos[i];
+        float rb = 0.0f;
+        for_each_nbr32(L.nbrb, c.cap, i, nbn, 0u, [&](uint32_t j, bool valid) {
+            const float4 pj = bpos[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            rb += valid ? pj.w * kernel_W(c, dist2(dx, dy, dz)) : 0.0f;
+        });
+        const float ri = rho[i];
+        bool leave = rb > boundary_limit;
+        if (!leave && ri > limit) {
+            bool densest = true;
+            auto look = [&](uint32_t j, bool valid) {
+                if (!valid || j == i || tag[j] == kTagAir) return;
+                const float rj = rho[j];
+                if (rj > limit && (rj > ri || (rj == ri && j < i))) densest = false;
+            };
+            const uint32_t sn = L.stage_n[b];
+            if (sn != kUnstaged) {
+                const uint32_t* g = L.stg + (size_t)b * kStageCap;
+                for_each_nbr16(L.nbr16, c.cap, i, nf, [&](uint32_t e, bool v) { look(g[e >> 4], v); });
+            } else {
+                for_each_nbr32(L.nbrf, c.cap, i, nf, i, look);
+            }
+            leave = densest;
+        }
+        keep[i] = leave ? 0u : 1u;
+        kept_rho = leave ? 0.0 : (double)ri;
+        kept = leave ? 0.0 : 1.0;
+    }
+    const double ws = wave_sum(kept_rho), wk = wave_sum(kept);
+    if ((threadIdx.x & 63) == 0 && wk != 0.0) {
+        atomicAdd(sum, ws);
+        atomicAdd(sum + 1, wk);
+    }
+}
+
 // changed[0] += number of i with a[i] != b[i]
 __global__ void k_count_changed(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b, uint32_t n,
                                 uint32_t* changed)
@@ -376,6 +438,13 @@ void launch_tag_neighbors(hipStream_t s, const SimConst& c, uint32_t n, const Nb
                           uint32_t* dst, uint32_t tag_in, uint32_t tag_out, int untagged_only, uint32_t grid)
 {
     if (n) k_tag_neighbors<<<grid, kBlock, 0, s>>>(c, n, L, src, dst, tag_in, tag_out, untagged_only);
+}
+
+void launch_rest_extract(hipStream_t s, const SimConst& c, uint32_t n, const NbrLists& L, const float4* pos,
+                         const float4* bpos, const float* rho, const uint32_t* tag, float limit, float boundary_limit,
+                         uint32_t* keep, double* sum, uint32_t grid)
+{
+    if (n) k_rest_extract<<<grid, kBlock, 0, s>>>(c, n, L, pos, bpos, rho, tag, limit, boundary_limit, keep, sum);
 }
 
 void launch_count_changed(hipStream_t s, const uint32_t* a, const uint32_t* b, uint32_t n, uint32_t* changed)

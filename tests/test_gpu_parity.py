@@ -382,6 +382,32 @@ def test_runs_are_bit_reproducible():
         assert np.array_equal(out[0][k], out[1][k]), k
 
 
+def test_caller_stream_gives_the_same_run():
+    """dfsph_set_stream: a run whose steps move to a caller's stream after
+    two steps (and back to the handle's own for the last) ends bit for bit
+    like a run on the handle's own stream."""
+    import torch
+    sc = scene.config1()
+    ref = _solver(sc, dt=0.002)
+    for _ in range(6):
+        ref.step()
+    s = _solver(sc, dt=0.002)
+    own = s.get_stream()
+    st = torch.cuda.Stream()
+    s.step()
+    s.step()
+    s.set_stream(st.cuda_stream)
+    assert s.get_stream() == st.cuda_stream
+    for _ in range(3):
+        s.step()
+    s.set_stream(0)
+    assert s.get_stream() == own
+    s.step()
+    a, b = ref.state_by_id(), s.state_by_id()
+    for k in ("id", "pos", "vel", "kappa", "kappaV"):
+        assert np.array_equal(a[k], b[k]), k
+
+
 @pytest.mark.parametrize("case", ["reference", "learned", "body"])
 def test_checkpoint_resume_is_bit_exact(tmp_path, case):
     """handleSimulationSave / handleSimulationLoad: a run resumed from a

@@ -805,3 +805,109 @@ def test_warm_start_targets_are_size_invariant(oracle_mod, which):
         out.append(o.warm_start_targets()[:, 0 if which == "divergence" else 1])
     assert np.abs(out[0]).max() > 0
     np.testing.assert_allclose(out[1], out[0], rtol=1e-9, atol=1e-12 * np.abs(out[0]).max())
+
+
+# ------------------------------------------------- rest-fluid extraction pass
+
+def _brute_density(x, m, xb, psi, hk):
+    """density by enumeration of every pair (no neighbor lists): m W(0) + sum m W + sum psi W"""
+    rho = np.zeros(len(x))
+    rb = np.zeros(len(x))
+    for i in range(len(x)):
+        rho[i] = m * _W(0.0, hk)
+        for j in range(len(x)):
+            if j != i:
+                rho[i] += m * _W(np.linalg.norm(x[i] - x[j]) / hk, hk)
+        for j in range(len(xb)):
+            rb[i] += psi[j] * _W(np.linalg.norm(x[i] - xb[j]) / hk, hk)
+    return rho + rb, rb
+
+
+def _extract_case(om, x, xb, limit, blimit, r=0.025):
+    o = om.Oracle(particle_radius=r)
+    o.set_boundary(xb)
+    psi = o.boundary_psi(len(xb))
+    o.set_fluid(x)
+    m = float(o.get(om.F_MASS)[0])
+    o.neighbor_search()
+    keep, avg, rho = o.rest_extract(limit, blimit)
+    want, rb = _brute_density(x, m, xb, psi, 4 * r)
+    return keep, avg, rho, want, rb
+
+
+def test_rest_extract_by_enumeration(oracle_mod):
+    """One extraction pass against densities enumerated pair by pair:
+    - a lattice block with one extra particle 0.4 r from an interior one: at
+      a limit between the bulk and the crowded pair, exactly the denser of the
+      two goes (the pair's own densities decide, not the lattice's);
+    - a particle 0.3 r from a wall particle goes at any limit (the boundary
+      alone gives it more than 0.7 rho0), one 2 r from the wall stays;
+    - the mean is over the kept particles."""
+    om = oracle_mod
+    r = 0.025
+    d = 2 * r
+    g = np.arange(5) * d
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) + 1.0
+    a = 62                                                  # the center of the 5^3 block
+    extra = lat[a] + np.array([0.4 * r, 0.0, 0.0])
+    wall = np.stack(np.meshgrid(np.arange(-8, 9) * d, np.arange(-8, 9) * d, indexing="ij"), -1).reshape(-1, 2)
+    xb = np.column_stack([wall[:, 0] + 1.1, np.zeros(len(wall)), wall[:, 1] + 1.1])    # y = 0 plane
+    near = np.array([1.1 + 0.3 * r, 0.0, 1.1])              # 0.3 r above wall particle (1.1, 0, 1.1)
+    far = np.array([1.1 + d, 2 * r, 1.1 + d])
+    x = np.vstack([lat, extra, near, far])
+    # densities first, with a limit nothing passes
+    keep, avg, rho, want, rb = _extract_case(om, x, xb, 1e9, 1e9, r)
+    assert keep.all()
+    assert np.allclose(rho, want, rtol=1e-12, atol=1e-9)
+    assert avg == pytest.approx(want.mean(), rel=1e-12)
+    i_extra, i_near, i_far = len(lat), len(lat) + 1, len(lat) + 2
+    assert rb[i_near] > 700.0 and rb[i_far] < 700.0
+    pair = (a, i_extra)
+    bulk = np.delete(want, [a, i_extra, i_near])
+    limit = 0.5 * (max(bulk.max(), want[i_far]) + min(want[a], want[i_extra]))
+    assert min(want[a], want[i_extra]) > limit > bulk.max()
+    keep, avg, _, _, _ = _extract_case(om, x, xb, limit, 700.0, r)
+    gone = set(np.where(~keep)[0])
+    denser = pair[int(np.argmax([want[a], want[i_extra]]))]
+    assert gone == {denser, i_near}
+    assert avg == pytest.approx(want[keep].mean(), rel=1e-12)
+
+
+def test_rest_extract_removes_separate_peaks_together(oracle_mod):
+    """Two crowded pairs farther apart than the support: one particle of each
+    goes in the same pass; of a pair with equal densities (mirror images of
+    each other about a lattice plane) the lower index goes."""
+    om = oracle_mod
+    r = 0.025
+    x = np.array([[0.0, 0.0, 0.0], [0.5 * r, 0.0, 0.0],
+                  [1.0, 0.0, 0.0], [1.0 + 0.6 * r, 0.0, 0.0]])
+    keep, _, rho, want, _ = _extract_case(om, x, _far_boundary(), 0.5 * want_min(x, r), 1e9, r)
+    assert rho[0] == rho[1] and rho[2] == rho[3]
+    assert list(np.where(~keep)[0]) == [0, 2]
+
+
+def want_min(x, r):
+    """the smallest density of x (pairs in isolation, by enumeration)"""
+    m = 0.8 * (2 * r) ** 3 * 1000.0
+    rho, _ = _brute_density(x, m, np.zeros((0, 3)), np.zeros(0), 4 * r)
+    return rho.min()
+
+
+def test_rest_extract_air_particles_add_density_only(oracle_mod):
+    """An air particle 0.3 r from a candidate: the pair is equally dense; the
+    candidate goes (air does not compete), the air particle stays, and the
+    mean counts the kept candidates only (here the isolated one)."""
+    om = oracle_mod
+    r = 0.025
+    x = np.array([[0.0, 0.0, 0.0], [0.3 * r, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    air = np.array([0, 1, 0], np.uint8)
+    o = om.Oracle(particle_radius=r)
+    o.set_boundary(_far_boundary())
+    o.set_fluid(x)
+    o.neighbor_search()
+    rho_all = o.rest_extract(1e9, 1e9)[2]
+    limit = 0.5 * (rho_all[0] + rho_all[2])
+    assert rho_all[0] == rho_all[1] and rho_all[0] > limit > rho_all[2]
+    keep, avg, _ = o.rest_extract(limit, 1e9, air=air)
+    assert list(keep) == [False, True, True]
+    assert avg == rho_all[2]
