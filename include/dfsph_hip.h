@@ -367,17 +367,28 @@ int dfsph_remove_tagged(void* handle, uint32_t tag, uint32_t* removed);
 #define DFSPH_SURFACE_PLANE 0      /* inside: (p - o) . n >= 0                     */
 #define DFSPH_SURFACE_CUBOID 1     /* |p - center| < half per axis                 */
 #define DFSPH_SURFACE_CYLINDER 2   /* axis y: |dy| <= half[1], radial dist < radius */
+#define DFSPH_SURFACE_MESH 3       /* closed triangle mesh (dfsph_mesh_upload), moved by center */
 #define DFSPH_SURFACE_SPHERE 4     /* |p - center| < radius                         */
 #define DFSPH_SURFACE_STAR 5       /* prism along y, linear-sided star in x-z       */
 typedef struct dfsph_surface {
     int32_t type;                  /* DFSPH_SURFACE_*                               */
     int32_t reversed;              /* exterior becomes interior                     */
     float o[3], n[3];              /* plane                                         */
-    float center[3], half[3];      /* cuboid; cylinder / star use half[1]           */
+    float center[3], half[3];      /* cuboid; cylinder / star use half[1]; mesh: offset */
     float radius, radius_internal; /* cylinder, sphere; star outer / inner radius   */
     uint32_t points;               /* star                                          */
     float direction[3];            /* star: unit direction of the first point (x-z) */
+    uint32_t mesh;                 /* mesh: the id dfsph_mesh_upload returned        */
 } dfsph_surface;
+/* MeshPerso (BufferFluidSurface.cu:51-410): a closed triangle mesh for
+ * DFSPH_SURFACE_MESH surfaces, kept on the device until the handle is
+ * destroyed.  vertices: 3 nv floats; faces: 3 nf vertex indices.  A point is
+ * inside when a ray from it in one fixed direction crosses the mesh an odd
+ * number of times (MeshPerso::isInside, :224-270; Moeller-Trumbore
+ * ray-triangle tests); points outside the mesh's bounding box are outside.
+ * The mesh must be closed.  mesh: the id, 1 and up. */
+int dfsph_mesh_upload(void* handle, const float* vertices, uint32_t nv, const uint32_t* faces, uint32_t nf,
+                      uint32_t* mesh);
 /* BufferFluidSurface::isinside on the device for n points (inside[i] = 0 / 1) */
 int dfsph_surface_inside(void* handle, const dfsph_surface* s, const float* xyz, uint32_t n, uint8_t* inside);
 /* OpenBoundariesSimple::init (OpenBoundariesSimple.cu:117-357): the surfaces and

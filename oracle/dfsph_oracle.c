@@ -1050,12 +1050,56 @@ int or_step(or_state* s, or_stats* st)
 /* BufferFluidSurface::isinside (BufferFluidSurface.cu:775-880).  margin: the
  * distance of the deciding quantity from its threshold (points with a tiny
  * margin can flip between precisions). */
+/* MeshPerso::isInside (BufferFluidSurface.cu:224-270): parity of the
+ * crossings of the ray q + t u, t > 0, with the triangles, each found by the
+ * Moeller-Trumbore test (Moeller and Trumbore 1997: Cramer's rule on
+ * [-u, e1, e2] for the barycentric coordinates a, b and t).  The ray
+ * direction is the library's fixed one.  *margin: the smallest of |t| over
+ * the crossings and of the distances, within a triangle's plane, from the
+ * ray's hit point to the triangle's edges (small values mean FP32 and FP64
+ * may count differently). */
+static int mesh_inside(const or_surface* S, const double* q, double* margin)
+{
+    static const double u[3] = {0.31190428, 0.75361035, 0.57860795};
+    uint32_t crossings = 0;
+    double mg = 1e300;
+    for (uint32_t t = 0; t < S->ntri; ++t) {
+        const double* v0 = S->tri + 9 * (size_t)t;
+        const double* v1 = v0 + 3;
+        const double* v2 = v0 + 6;
+        double e1[3], e2[3], s[3], pv[3], w[3];
+        sub3(v1, v0, e1); sub3(v2, v0, e2); sub3(q, v0, s);
+        pv[0] = u[1] * e2[2] - u[2] * e2[1]; pv[1] = u[2] * e2[0] - u[0] * e2[2]; pv[2] = u[0] * e2[1] - u[1] * e2[0];
+        const double det = dot3(e1, pv);
+        if (fabs(det) < 1e-12) continue;
+        const double inv = 1.0 / det;
+        const double a = inv * dot3(s, pv);
+        w[0] = s[1] * e1[2] - s[2] * e1[1]; w[1] = s[2] * e1[0] - s[0] * e1[2]; w[2] = s[0] * e1[1] - s[1] * e1[0];
+        const double b = inv * dot3(u, w);
+        const double tt = inv * dot3(e2, w);
+        const double c = 1.0 - a - b;
+        const double size = sqrt(dot3(e1, e1));
+        const double edge = fmin(fabs(a), fmin(fabs(b), fabs(c))) * size;
+        const int in_tri = a >= 0.0 && b >= 0.0 && c >= 0.0;
+        if (tt > -1e-3 * size) mg = fmin(mg, edge);
+        if (in_tri) mg = fmin(mg, fabs(tt));
+        if (in_tri && tt > 0.0) crossings++;
+    }
+    if (margin) *margin = mg;
+    return (int)(crossings & 1u);
+}
+
 int or_surface_inside(const or_surface* S, const double* p, double* margin)
 {
     int r = 0;
     double mg = 1e300;
     const double vx = p[0] - S->center[0], vy = p[1] - S->center[1], vz = p[2] - S->center[2];
     switch (S->type) {
+    case 3: {
+        const double q[3] = {vx, vy, vz};
+        r = mesh_inside(S, q, &mg);
+        break;
+    }
     case 0: {
         const double d = (p[0] - S->o[0]) * S->n[0] + (p[1] - S->o[1]) * S->n[1] + (p[2] - S->o[2]) * S->n[2];
         r = !(d < 0.0); mg = fabs(d); break;

@@ -146,11 +146,13 @@ int or_body_psi(const or_state* s, uint32_t id, double* psi);
 
 /* A19: BufferFluidSurface and the open boundary */
 typedef struct or_surface {
-    int type, reversed;       /* 0 plane, 1 cuboid, 2 cylinder, 4 sphere, 5 star */
-    double o[3], n[3], center[3], half[3];
+    int type, reversed;       /* 0 plane, 1 cuboid, 2 cylinder, 3 mesh, 4 sphere, 5 star */
+    double o[3], n[3], center[3], half[3];   /* mesh: center is the offset of the vertices */
     double radius, radius_internal;
     uint32_t points;
     double dir[3];
+    const double* tri;        /* mesh: 9 doubles per triangle (its three vertices) */
+    uint32_t ntri;
 } or_surface;
 int or_surface_inside(const or_surface* S, const double* p, double* margin);
 uint32_t or_inflow(const or_state* s, const or_surface* boundary, uint32_t nc, const double* cand, double min_dist,

@@ -39,15 +39,24 @@ class OrParams(C.Structure):
 class OrSurface(C.Structure):
     _fields_ = [("type", C.c_int), ("reversed", C.c_int), ("o", C.c_double * 3), ("n", C.c_double * 3),
                 ("center", C.c_double * 3), ("half", C.c_double * 3), ("radius", C.c_double),
-                ("radius_internal", C.c_double), ("points", C.c_uint32), ("dir", C.c_double * 3)]
+                ("radius_internal", C.c_double), ("points", C.c_uint32), ("dir", C.c_double * 3),
+                ("tri", C.POINTER(C.c_double)), ("ntri", C.c_uint32)]
 
 
-SURFACE_TYPES = {"plane": 0, "cuboid": 1, "cylinder": 2, "sphere": 4, "star": 5}
+SURFACE_TYPES = {"plane": 0, "cuboid": 1, "cylinder": 2, "mesh": 3, "sphere": 4, "star": 5}
 
 
 def surface(kind, reversed=False, o=(0, 0, 0), n=(0, 1, 0), center=(0, 0, 0), half=(0, 0, 0), radius=0.0,
-            radius_internal=0.0, points=0, direction=(0, 0, 1)):
+            radius_internal=0.0, points=0, direction=(0, 0, 1), vertices=None, faces=None):
+    """an or_surface; 'mesh' takes vertices (nv, 3) and faces (nf, 3), moved by center"""
     s = OrSurface()
+    if kind == "mesh":
+        v = np.asarray(vertices, np.float64).reshape(-1, 3)
+        f = np.asarray(faces, np.int64).reshape(-1, 3)
+        tri = np.ascontiguousarray(v[f].reshape(-1))
+        s._tri = tri                       # kept alive with the structure
+        s.tri = tri.ctypes.data_as(C.POINTER(C.c_double))
+        s.ntri = len(f)
     s.type = SURFACE_TYPES[kind]
     s.reversed = 1 if reversed else 0
     s.o = (C.c_double * 3)(*o)

@@ -173,7 +173,7 @@ def rigid_body(type=BODY_DYNAMIC, mass=0.0, inertia=(0, 0, 0), position=(0, 0, 0
 
 
 class DFSPHSurface(C.Structure):
-    """dfsph_surface: a BufferFluidSurface (plane, cuboid, cylinder, sphere, star)."""
+    """dfsph_surface: a BufferFluidSurface (plane, cuboid, cylinder, mesh, sphere, star)."""
     _fields_ = [
         ("type", C.c_int32),
         ("reversed", C.c_int32),
@@ -185,21 +185,23 @@ class DFSPHSurface(C.Structure):
         ("radius_internal", C.c_float),
         ("points", C.c_uint32),
         ("direction", C.c_float * 3),
+        ("mesh", C.c_uint32),
     ]
 
 
-SURFACE_PLANE, SURFACE_CUBOID, SURFACE_CYLINDER, SURFACE_SPHERE, SURFACE_STAR = 0, 1, 2, 4, 5
+SURFACE_PLANE, SURFACE_CUBOID, SURFACE_CYLINDER, SURFACE_MESH, SURFACE_SPHERE, SURFACE_STAR = 0, 1, 2, 3, 4, 5
 TAG_ACTIVE, TAG_ACTIVE_NEIGHBORS, TAG_UNTAGGED, TAG_REMOVAL = 10000000, 20000000, 40000000, 310000000
 TAG_AIR = 50000000
 
 
 def surface(kind, reversed=False, o=(0, 0, 0), n=(0, 1, 0), center=(0, 0, 0), half=(0, 0, 0), radius=0.0,
-            radius_internal=0.0, points=0, direction=(0, 0, 1)):
-    """A DFSPHSurface; kind is 'plane', 'cuboid', 'cylinder', 'sphere' or 'star'
-    (cylinder / star: half height in half[1])."""
+            radius_internal=0.0, points=0, direction=(0, 0, 1), mesh=0):
+    """A DFSPHSurface; kind is 'plane', 'cuboid', 'cylinder', 'mesh', 'sphere' or 'star'
+    (cylinder / star: half height in half[1]; mesh: the id of Solver.mesh_upload, moved by center)."""
     s = DFSPHSurface()
-    s.type = {"plane": SURFACE_PLANE, "cuboid": SURFACE_CUBOID, "cylinder": SURFACE_CYLINDER,
+    s.type = {"plane": SURFACE_PLANE, "cuboid": SURFACE_CUBOID, "cylinder": SURFACE_CYLINDER, "mesh": SURFACE_MESH,
               "sphere": SURFACE_SPHERE, "star": SURFACE_STAR}[kind]
+    s.mesh = mesh
     s.reversed = 1 if reversed else 0
     s.o = (C.c_float * 3)(*o)
     s.n = (C.c_float * 3)(*n)
@@ -229,6 +231,7 @@ HIP_SIGNATURES = {
     "dfsph_order_active_first": (_I, [_P, _U32P, _U32P]),
     "dfsph_remove_tagged": (_I, [_P, _U32, _U32P]),
     "dfsph_surface_inside": (_I, [_P, C.POINTER(DFSPHSurface), _F32P, _U32, _U8P]),
+    "dfsph_mesh_upload": (_I, [_P, _F32P, _U32, _U32P, _U32, _U32P]),
     "dfsph_open_boundary_init": (_I, [_P, C.POINTER(DFSPHSurface), C.POINTER(DFSPHSurface),
                                       C.POINTER(DFSPHSurface), _F32P, _U32, _U32P]),
     "dfsph_open_boundary_apply": (_I, [_P, C.POINTER(C.c_double), C.c_float, C.c_float, _I, _I, _U32P, _U32P]),
@@ -573,6 +576,15 @@ class Solver:
         c = C.c_uint32()
         _check(self.lib.dfsph_remove_tagged(self.h, tag, C.byref(c)), "dfsph_remove_tagged")
         return c.value
+
+    def mesh_upload(self, vertices, faces) -> int:
+        """dfsph_mesh_upload: a closed triangle mesh for 'mesh' surfaces; returns its id"""
+        v = _f32(vertices).reshape(-1)
+        f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1)
+        m = C.c_uint32()
+        _check(self.lib.dfsph_mesh_upload(self.h, _ptr(v, C.c_float), v.size // 3, _ptr(f, C.c_uint32), f.size // 3,
+                                          C.byref(m)), "dfsph_mesh_upload")
+        return m.value
 
     def surface_inside(self, surf, xyz):
         x = _f32(xyz).reshape(-1)

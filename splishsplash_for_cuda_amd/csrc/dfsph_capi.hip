@@ -120,6 +120,12 @@ struct Handle {
     double* partial = nullptr;
     float* partial_f = nullptr;
     double* rest_sum = nullptr;       // dfsph_remove_density_peaks: density sum of the kept particles
+    struct MeshRec {                  // dfsph_mesh_upload: triangles (3 float4 each) and bounding box
+        float4* tri = nullptr;
+        uint32_t ntri = 0;
+        float bmin[3], bmax[3];
+    };
+    std::vector<MeshRec> meshes;
     uint32_t partial_n = 0;
     DevCtl* hctl = nullptr;           // pinned
     DevTime* htm = nullptr;           // pinned
@@ -256,6 +262,7 @@ Handle::~Handle()
     dfree(keep8); dfree(scan_tmp); dfree(bpos); dfree(bcstart); dfree(tile_rank); dfree(nbr_part);
     dfree(dbody); dfree(x0); dfree(sbody); dfree(braw); dfree(bvraw); dfree(bvel); dfree(sforce);
     dfree(ctl); dfree(tm); dfree(partial); dfree(partial_f); dfree(rest_sum);
+    for (auto& m : meshes) dfree(m.tri);
     if (hctl) (void)hipHostFree(hctl);
     if (htm) (void)hipHostFree(htm);
     if (hrec) (void)hipHostFree(hrec);
@@ -1649,7 +1656,8 @@ int dfsph_remove_tagged(void* hv, uint32_t tag, uint32_t* removed)
 }
 
 // ---- open boundary ----
-static dfsph::Surface to_surface(const dfsph_surface& a)
+// (a surface checked by check_surface; a mesh id refers to an uploaded mesh)
+static dfsph::Surface to_surface(const Handle* h, const dfsph_surface& a)
 {
     dfsph::Surface S{};
     S.type = a.type;
@@ -1660,24 +1668,63 @@ static dfsph::Surface to_surface(const dfsph_surface& a)
     S.radius = a.radius;
     S.radius_internal = a.radius_internal;
     S.points = a.points;
+    if (a.type == DFSPH_SURFACE_MESH) {
+        const Handle::MeshRec& m = h->meshes[a.mesh - 1];
+        S.tri = m.tri;
+        S.ntri = m.ntri;
+        for (int k = 0; k < 3; ++k) { S.bmin[k] = m.bmin[k]; S.bmax[k] = m.bmax[k]; }
+    }
     return S;
 }
 
-static int check_surface(const dfsph_surface* a)
+static int check_surface(const Handle* h, const dfsph_surface* a)
 {
     if (!a) return fail(DFSPH_ERR_ARG, "null surface");
     const int t = a->type;
     if (t != DFSPH_SURFACE_PLANE && t != DFSPH_SURFACE_CUBOID && t != DFSPH_SURFACE_CYLINDER &&
-        t != DFSPH_SURFACE_SPHERE && t != DFSPH_SURFACE_STAR)
-        return fail(DFSPH_ERR_ARG, "unknown surface type (meshes are not supported)");
+        t != DFSPH_SURFACE_SPHERE && t != DFSPH_SURFACE_STAR && t != DFSPH_SURFACE_MESH)
+        return fail(DFSPH_ERR_ARG, "unknown surface type");
     if (t == DFSPH_SURFACE_STAR && a->points == 0) return fail(DFSPH_ERR_ARG, "star surface needs points > 0");
+    if (t == DFSPH_SURFACE_MESH && (a->mesh == 0 || a->mesh > h->meshes.size()))
+        return fail(DFSPH_ERR_ARG, "mesh surface: no mesh with this id (dfsph_mesh_upload)");
+    return DFSPH_OK;
+}
+
+int dfsph_mesh_upload(void* hv, const float* vertices, uint32_t nv, const uint32_t* faces, uint32_t nf,
+                      uint32_t* mesh)
+{
+    Handle* h = H(hv);
+    if (!vertices || !faces || !mesh || nv < 3 || nf < 4)
+        return fail(DFSPH_ERR_ARG, "a closed mesh needs vertices and at least 4 faces");
+    Handle::MeshRec m;
+    std::vector<float4> T(3 * (size_t)nf);
+    for (int k = 0; k < 3; ++k) { m.bmin[k] = vertices[k]; m.bmax[k] = vertices[k]; }
+    for (uint32_t v = 0; v < nv; ++v)
+        for (int k = 0; k < 3; ++k) {
+            m.bmin[k] = std::min(m.bmin[k], vertices[3 * v + k]);
+            m.bmax[k] = std::max(m.bmax[k], vertices[3 * v + k]);
+        }
+    for (uint32_t f = 0; f < nf; ++f)
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t v = faces[3 * f + c];
+            if (v >= nv) return fail(DFSPH_ERR_ARG, "face index beyond the vertices");
+            T[3 * (size_t)f + c] = make_float4(vertices[3 * v], vertices[3 * v + 1], vertices[3 * v + 2], 0.0f);
+        }
+    CHK(dalloc(m.tri, T.size()));
+    m.ntri = nf;
+    if (hipMemcpy(m.tri, T.data(), sizeof(float4) * T.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        dfree(m.tri);
+        return fail(DFSPH_ERR_HIP, "hipMemcpy");
+    }
+    h->meshes.push_back(m);
+    *mesh = (uint32_t)h->meshes.size();
     return DFSPH_OK;
 }
 
 int dfsph_surface_inside(void* hv, const dfsph_surface* s, const float* xyz, uint32_t n, uint8_t* inside)
 {
     Handle* h = H(hv);
-    CHK(check_surface(s));
+    CHK(check_surface(h, s));
     if (n == 0) return DFSPH_OK;
     if (!xyz || !inside) return fail(DFSPH_ERR_ARG, "null points");
     std::vector<float4> P(n);
@@ -1691,7 +1738,7 @@ int dfsph_surface_inside(void* hv, const dfsph_surface* s, const float* xyz, uin
         if (hipMemcpyAsync(d, P.data(), sizeof(float4) * n, hipMemcpyHostToDevice, h->s) != hipSuccess)
             rc = fail(DFSPH_ERR_HIP, "hipMemcpy");
         if (rc == DFSPH_OK) {
-            dfsph::launch_surface_keep(h->s, d, n, to_surface(*s), 1, k);
+            dfsph::launch_surface_keep(h->s, d, n, to_surface(h, *s), 1, k);
             if (hipMemcpyAsync(K.data(), k, 4ull * n, hipMemcpyDeviceToHost, h->s) != hipSuccess)
                 rc = fail(DFSPH_ERR_HIP, "hipMemcpy");
         }
@@ -1708,13 +1755,13 @@ int dfsph_open_boundary_init(void* hv, const dfsph_surface* boundary, const dfsp
                              const dfsph_surface* fluid_surface, const float* inflow_xyz, uint32_t n, uint32_t* kept)
 {
     Handle* h = H(hv);
-    CHK(check_surface(boundary));
-    CHK(check_surface(interior));
-    CHK(check_surface(fluid_surface));
+    CHK(check_surface(h, boundary));
+    CHK(check_surface(h, interior));
+    CHK(check_surface(h, fluid_surface));
     if (n && !inflow_xyz) return fail(DFSPH_ERR_ARG, "null inflow positions");
-    h->ob_boundary = to_surface(*boundary);
-    h->ob_interior = to_surface(*interior);
-    h->ob_surface = to_surface(*fluid_surface);
+    h->ob_boundary = to_surface(h, *boundary);
+    h->ob_interior = to_surface(h, *interior);
+    h->ob_surface = to_surface(h, *fluid_surface);
     for (int a = 0; a < 3; ++a) h->ob_disp[a] = 0.0;
     h->ob_n = 0;
     dfree(h->ob_cand);

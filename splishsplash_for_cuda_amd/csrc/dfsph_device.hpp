@@ -203,15 +203,23 @@ constexpr uint32_t kTagRemoval = 310000000u;
 
 // A BufferFluidSurface (BufferFluidSurface.cu:421-470) by value: the C ABI's
 // dfsph_surface with the same fields.
-enum SurfaceType : int { kSurfPlane = 0, kSurfCuboid = 1, kSurfCylinder = 2, kSurfSphere = 4, kSurfStar = 5 };
+// the direction of the inside test's ray for mesh surfaces: generic (no
+// symmetry axis of a box or lattice), unit length
+__device__ constexpr float kMeshRay[3] = {0.31190428f, 0.75361035f, 0.57860795f};
+enum SurfaceType : int {
+    kSurfPlane = 0, kSurfCuboid = 1, kSurfCylinder = 2, kSurfMesh = 3, kSurfSphere = 4, kSurfStar = 5
+};
 struct Surface {
     int type;
     int reversed;
     float o[3], n[3];              // plane: point and normal (inside: (p - o).n >= 0)
-    float center[3], half[3];      // cuboid half lengths; cylinder / star: half[1] = half height
+    float center[3], half[3];      // cuboid half lengths; cylinder / star: half[1] = half height; mesh: offset
     float radius, radius_internal; // cylinder, sphere; star: outer / inner radius
     uint32_t points;               // star
     float dir[3];                  // star: direction of the first point (unit, in x-z)
+    const float4* tri;             // mesh: 3 vertices per triangle (mesh coordinates)
+    uint32_t ntri;                 // mesh
+    float bmin[3], bmax[3];        // mesh: bounding box (mesh coordinates)
 };
 
 // Restriction of the solver to part of the fluid (DFSPHCData::restriction_mode,

@@ -18,11 +18,52 @@ __device__ __forceinline__ float norm3(float x, float y, float z)
     return __builtin_sqrtf(x * x + y * y + z * z);
 }
 
+// Whether the ray q + t u, t > 0, crosses the triangle (v0, v1, v2): the
+// Moeller-Trumbore test (Moeller and Trumbore, "Fast, minimum storage
+// ray-triangle intersection", 1997): barycentric coordinates (a, b) and the
+// ray parameter t from Cramer's rule on [-u, e1, e2].
+__device__ __forceinline__ bool ray_crosses(const float q[3], const float u[3], const float4& v0, const float4& v1,
+                                            const float4& v2)
+{
+    const float e1[3] = {v1.x - v0.x, v1.y - v0.y, v1.z - v0.z};
+    const float e2[3] = {v2.x - v0.x, v2.y - v0.y, v2.z - v0.z};
+    const float p[3] = {u[1] * e2[2] - u[2] * e2[1], u[2] * e2[0] - u[0] * e2[2], u[0] * e2[1] - u[1] * e2[0]};
+    const float det = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
+    if (fabsf(det) < 1e-12f) return false;                  // the ray runs along the triangle's plane
+    const float inv = 1.0f / det;
+    const float s[3] = {q[0] - v0.x, q[1] - v0.y, q[2] - v0.z};
+    const float a = inv * (s[0] * p[0] + s[1] * p[1] + s[2] * p[2]);
+    if (a < 0.0f || a > 1.0f) return false;
+    const float w[3] = {s[1] * e1[2] - s[2] * e1[1], s[2] * e1[0] - s[0] * e1[2], s[0] * e1[1] - s[1] * e1[0]};
+    const float b = inv * (u[0] * w[0] + u[1] * w[1] + u[2] * w[2]);
+    if (b < 0.0f || a + b > 1.0f) return false;
+    return inv * (e2[0] * w[0] + e2[1] * w[1] + e2[2] * w[2]) > 0.0f;
+}
+
+// MeshPerso::isInside (BufferFluidSurface.cu:224-270): parity of the
+// crossings of one ray of a fixed, generic direction (no mesh edge or vertex
+// lies along it for the meshes of this library's tests)
+__device__ bool mesh_inside(const Surface& S, float px, float py, float pz)
+{
+    const float q[3] = {px - S.center[0], py - S.center[1], pz - S.center[2]};
+    for (int a = 0; a < 3; ++a)
+        if (q[a] < S.bmin[a] || q[a] > S.bmax[a]) return false;
+    const float u[3] = {kMeshRay[0], kMeshRay[1], kMeshRay[2]};
+    uint32_t crossings = 0;
+    for (uint32_t t = 0; t < S.ntri; ++t)
+        crossings += ray_crosses(q, u, S.tri[3 * t], S.tri[3 * t + 1], S.tri[3 * t + 2]) ? 1u : 0u;
+    return (crossings & 1u) != 0u;
+}
+
 // BufferFluidSurface::isinside (BufferFluidSurface.cu:775-880)
 __device__ bool surface_inside(const Surface& S, float px, float py, float pz)
 {
     bool r = false;
     switch (S.type) {
+    case kSurfMesh: {
+        r = mesh_inside(S, px, py, pz);
+        break;
+    }
     case kSurfPlane: {
         const float d = (px - S.o[0]) * S.n[0] + (py - S.o[1]) * S.n[1] + (pz - S.o[2]) * S.n[2];
         r = !(d < 0.0f);
@@ -72,7 +113,7 @@ __device__ bool surface_inside(const Surface& S, float px, float py, float pz)
 }
 
 // BufferFluidSurface::getNormal (BufferFluidSurface.cu:730-772): towards the
-// centre for the radial shapes, zero for planes and cuboids; unit length.
+// centre for the radial shapes, zero for planes, cuboids and meshes; unit length.
 __device__ void surface_normal(const Surface& S, float px, float py, float pz, float* nx, float* ny, float* nz)
 {
     float x = 0.0f, y = 0.0f, z = 0.0f;

@@ -62,6 +62,62 @@ def test_surface_inside_matches_oracle(oracle_mod):
             assert 0.05 < want.mean() < 0.95
 
 
+def test_mesh_surface_inside_matches_oracle(oracle_mod):
+    """DFSPH_SURFACE_MESH (MeshPerso::isInside, BufferFluidSurface.cu:224-270)
+    on the device against the oracle's ray parity, for a box, an octahedron
+    and a non-convex star prism, moved by their centers and reversed: the
+    same answer away from the faces and from rays grazing an edge (oracle
+    margin > 1e-5)."""
+    import meshes
+    om = oracle_mod
+    rng = np.random.default_rng(11)
+    c = (0.25, -0.1, 0.4)
+    pts = (np.array(c) + rng.uniform(-0.7, 0.7, size=(20000, 3))).astype(np.float32)
+    s = _solver(scene.small_dam_break())
+    shapes = [meshes.cube_mesh((-0.4, -0.25, -0.3), (0.4, 0.25, 0.3)), meshes.octahedron(0.5),
+              meshes.star_prism(7, 0.55, 0.25, 0.3, direction=(0.6, 0.0, 0.8))]
+    for v, f in shapes:
+        mid = s.mesh_upload(v, f)
+        for rev in (False, True):
+            gs = S.surface("mesh", mesh=mid, center=c, reversed=rev)
+            os_ = om.surface("mesh", vertices=v, faces=f, center=c, reversed=rev)
+            got = s.surface_inside(gs, pts)
+            want, margin = om.surface_inside(os_, pts.astype(np.float64))
+            ok = margin > 1e-5
+            assert ok.mean() > 0.99
+            assert np.array_equal(got[ok], want[ok]), (len(f), rev)
+            assert 0.02 < want.mean() < 0.98
+    with pytest.raises(S.DFSPHError):
+        s.surface_inside(S.surface("mesh", mesh=99), pts[:4])
+
+
+def test_open_boundary_with_a_mesh_interior_equals_the_cuboid(oracle_mod):
+    """OpenBoundariesSimple with its interior given as a closed box mesh
+    instead of the cuboid (half 0.45 m across the 0.6 m basin, so part of
+    the inflow ring lies outside it): the same inflow and outflow over a few
+    steps (identical states, particles added)."""
+    import meshes
+    states = []
+    for kind in ("cuboid", "mesh"):
+        sc, cand, R = _basin()
+        s = _solver(sc, dt=0.002)
+        if kind == "cuboid":
+            interior = S.surface("cuboid", center=(0.0, 0.0, 0.0), half=(0.45, 10.0, 0.45))
+        else:
+            v, f = meshes.cube_mesh((-0.45, -10.0, -0.45), (0.45, 10.0, 0.45))
+            interior = S.surface("mesh", mesh=s.mesh_upload(v, f))
+        boundary = S.surface("cylinder", center=(0, 0, 0), half=(0, 10.0, 0), radius=R)
+        fl = S.surface("plane", o=(0, 0.5, 0), n=(0, -1, 0))
+        s.open_boundary_init(boundary, interior, fl, cand)
+        for _ in range(4):
+            s.open_boundary_apply(1.75 * sc.particle_radius, 700.0)
+            s.step()
+        states.append(s.state_by_id())
+        assert len(states[-1]["id"]) > sc.n_fluid
+    for k in ("id", "pos", "vel"):
+        assert np.array_equal(states[0][k], states[1][k]), k
+
+
 def _basin():
     """A cylindrical basin (r = 1.5 m boundary), fluid up to y = 0.5: the open
     boundary of OpenBoundariesSimple's cylinder configuration at a small size."""

@@ -9,6 +9,8 @@ import os
 import numpy as np
 import pytest
 
+import meshes
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -911,3 +913,45 @@ def test_rest_extract_air_particles_add_density_only(oracle_mod):
     keep, avg, _ = o.rest_extract(limit, 1e9, air=air)
     assert list(keep) == [False, True, True]
     assert avg == rho_all[2]
+
+
+# ------------------------------------------------------------ mesh surfaces
+
+def test_mesh_surface_inside_closed_forms(oracle_mod):
+    """MeshPerso::isInside by ray parity against shapes with a closed-form
+    inside: a box mesh against the cuboid, an octahedron against
+    |x| + |y| + |z| < R, each moved by its center and reversed, for random
+    points away from the faces (margin > 1e-9); the face orientation does not
+    matter (a parity count)."""
+    om = oracle_mod
+    rng = np.random.default_rng(3)
+    c = np.array([0.3, -0.2, 0.5])
+    v, f = meshes.cube_mesh((-0.4, -0.25, -0.3), (0.4, 0.25, 0.3))
+    pts = c + rng.uniform(-0.6, 0.6, (3000, 3))
+    for rev in (False, True):
+        for faces in (f, f[:, ::-1]):
+            m = om.surface("mesh", vertices=v, faces=faces, center=c, reversed=rev)
+            box = om.surface("cuboid", center=c, half=(0.4, 0.25, 0.3), reversed=rev)
+            a, mg = om.surface_inside(m, pts)
+            b, _ = om.surface_inside(box, pts)
+            sharp = mg > 1e-9
+            assert sharp.mean() > 0.99
+            assert np.array_equal(a[sharp], b[sharp])
+    v, f = meshes.octahedron(0.5)
+    m = om.surface("mesh", vertices=v, faces=f, center=c)
+    a, mg = om.surface_inside(m, pts)
+    want = np.abs(pts - c).sum(1) < 0.5
+    sharp = (mg > 1e-9) & (np.abs(np.abs(pts - c).sum(1) - 0.5) > 1e-9)
+    assert want[sharp].sum() > 100
+    assert np.array_equal(a[sharp], want[sharp])
+    # a non-convex shape: the prism over a 7-pointed star against the analytic star surface
+    kw = dict(radius=0.55, radius_internal=0.25, half_height=0.3, direction=(0.6, 0.0, 0.8))
+    v, f = meshes.star_prism(7, **kw)
+    m = om.surface("mesh", vertices=v, faces=f, center=c)
+    star = om.surface("star", center=c, half=(0.0, 0.3, 0.0), radius=0.55, radius_internal=0.25, points=7,
+                      direction=(0.6, 0.0, 0.8))
+    a, mg = om.surface_inside(m, pts)
+    b, mg2 = om.surface_inside(star, pts)
+    sharp = (mg > 1e-9) & (mg2 > 1e-9)
+    assert b[sharp].sum() > 100
+    assert np.array_equal(a[sharp], b[sharp])
