@@ -13,9 +13,12 @@ The solver is SPH::DFSPHCUDA (libdfsph_host.so) with
     far end neither reflects the waves nor drains,
   * every step's compaction of removed particles, and optionally the learned
     warm start.
-The dynamic window of the reference (DynamicWindow.cu, a simulated region that
-follows the box with fluid loaded from a background buffer) is not built: the
-whole basin is simulated.
+With window_tank() the simulated region is instead a window around the box
+(the reference's DynamicWindow, DynamicWindow.cu:150-3760, driven as
+DFSPHCUDA::applyDynamicWindow, DFSPH_CUDA.cpp:3988-4139): the region, walled
+at its two x faces by plates that move with it, follows the box, and fluid is
+loaded from the basin's rest lattice (the background buffer) along its
+leading edge while what it leaves behind is removed.
 """
 from __future__ import annotations
 
@@ -127,6 +130,95 @@ def make_solver(t: WaveTank, params: N.DFSPHParams, learned: bool = False) -> N.
         hs.view.add_rigid_body(N.rigid_body(**desc), pts)
     b, i, s = open_boundary_surfaces(t)
     hs.set_open_boundaries(b, i, s, t.inflow)
+    if learned:
+        from . import learned as L
+        L.attach(hs.view)
+    return hs
+
+
+@dataclass
+class WindowTank:
+    """the window configuration of a tank: the region (a cuboid about x = cx,
+    half length `half`), its x walls as a kinematic body (plates at cx +- plate,
+    body frame), the fluid inside, the background buffer with its rest
+    surface `level`, and the box (kinematic, moved by the caller)"""
+    tank: WaveTank
+    cx: float
+    half: float
+    plate: float
+    plates: np.ndarray
+    fluid: np.ndarray
+    background: np.ndarray
+    level: float
+    box_side: float
+
+
+def window_tank(t: WaveTank, window_length: float = 2.0, background: np.ndarray = None) -> WindowTank:
+    """The window of `window_length` about the box of tank t.  The background
+    buffer is a fluid at rest filling the basin (the reference loads one from
+    a file made by letting a fluid settle): `background`, e.g. the tank's
+    fluid after it has settled, or by default the tank's water lattice (at
+    0.8 rho0 it settles lower than it stands, so windows loading from it gain
+    fluid).  The plates stand at the window's ends, the region reaches 2 r
+    beyond them (so the fluid interior, 3 r inside the region, reaches 1 r
+    inside the plates), the initial fluid is the background more than 1.5 r
+    inside the plates, and the rest surface is the background's top."""
+    r = t.particle_radius
+    d = 2.0 * r
+    bg = _basin_lattice(t) if background is None else np.asarray(background, np.float64)
+    top = np.sort(bg[:, 1])[-max(1, len(bg) // 1000)]          # the top, less splash
+    level = float(top) + r
+    cx = float(t.box["position"][0])
+    plate = 0.5 * window_length
+    ys = np.linspace(-0.5 * t.height, 0.5 * t.height, int(round(t.height / d)) + 1)
+    zs = np.linspace(-0.5 * t.width, 0.5 * t.width, int(round(t.width / d)) + 1)
+    Y, Z = np.meshgrid(ys, zs, indexing="ij")
+    plates = np.concatenate([np.stack([np.full(Y.size, sx * plate), Y.ravel(), Z.ravel()], 1) for sx in (-1, 1)])
+    side = float(np.ptp(t.bodies[1][1][:, 0]))      # the box's edge
+    c = np.array([cx, level, 0.5 * t.width])
+    fl = bg[(np.abs(bg[:, 0] - cx) < plate - 1.5 * r) & (np.abs(bg - c).max(1) >= 0.5 * side + 1.5 * r)]
+    return WindowTank(tank=t, cx=cx, half=plate + d, plate=plate, plates=plates.astype(np.float32),
+                      fluid=fl.astype(np.float32), background=bg.astype(np.float32), level=level, box_side=side)
+
+
+def _basin_lattice(t: WaveTank) -> np.ndarray:
+    """the water lattice of build() over the whole basin (no jitter, no hole for the box)"""
+    r = t.particle_radius
+    d = 2.0 * r
+    px = 0.15
+    counts = (int(np.floor((t.length - d - (px + 1.5 * d)) / d + 1e-6)) + 1, int(round(t.depth / d)),
+              int(np.floor(t.width / d - 2.0 + 1e-6)) + 1)
+    x0 = t.length - d - (counts[0] - 1) * d
+    return fluid_block((x0, d, d), counts, d)
+
+
+def settle_basin(t: WaveTank, params: N.DFSPHParams, seconds: float = 1.5) -> np.ndarray:
+    """a background buffer for window_tank(): the basin's water lattice (no
+    piston, no box) let settle for `seconds`; its positions"""
+    hs = N.HostSolver(params, t.walls, _basin_lattice(t).astype(np.float32))
+    time = 0.0
+    while time < seconds:
+        time = hs.step().time
+    x = hs.view.download(N.FIELD_POS)
+    hs.close()
+    return x
+
+
+def make_window_solver(w: WindowTank, params: N.DFSPHParams, learned: bool = False) -> N.HostSolver:
+    """SPH::DFSPHCUDA over the window: the basin walls, the window's fluid, the
+    box (body 0, kinematic: the caller sets its place each step) and the
+    plates (body 1), the dynamic window tracking body 0 with body 1 as its
+    walls (no piston, no absorbing end: the window's walls confine the fluid)"""
+    t = w.tank
+    hs = N.HostSolver(params, t.walls, w.fluid)
+    h = 0.5 * w.box_side
+    cube = box_surface((-h, -h, -h), (h, h, h), 2.0 * t.particle_radius).astype(np.float32)
+    hs.view.add_rigid_body(N.rigid_body(type=N.BODY_KINEMATIC, position=(w.cx, w.level, 0.5 * t.width)), cube)
+    hs.view.add_rigid_body(N.rigid_body(type=N.BODY_KINEMATIC, position=(w.cx, 0.5 * t.height, 0.5 * t.width)),
+                           w.plates)
+    region = N.surface("cuboid", center=(w.cx, 0.5 * t.height, 0.5 * t.width), half=(w.half, t.height, t.width))
+    fluid_surf = N.surface("plane", o=(0.0, w.level, 0.0), n=(0.0, -1.0, 0.0))
+    hs.set_dynamic_window(region, fluid_surf, w.background, tracked_body=0, wall_body=1)
     if learned:
         from . import learned as L
         L.attach(hs.view)
