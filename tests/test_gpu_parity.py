@@ -386,24 +386,30 @@ def test_caller_stream_gives_the_same_run():
     """dfsph_set_stream: a run whose steps move to a caller's stream after
     two steps (and back to the handle's own for the last) ends bit for bit
     like a run on the handle's own stream."""
-    import torch
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
     sc = scene.config1()
     ref = _solver(sc, dt=0.002)
     for _ in range(6):
         ref.step()
     s = _solver(sc, dt=0.002)
     own = s.get_stream()
-    st = torch.cuda.Stream()
-    s.step()
-    s.step()
-    s.set_stream(st.cuda_stream)
-    assert s.get_stream() == st.cuda_stream
-    for _ in range(3):
+    st = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(st)) == 0           # default priority
+    try:
         s.step()
-    s.set_stream(0)
-    assert s.get_stream() == own
-    s.step()
-    a, b = ref.state_by_id(), s.state_by_id()
+        s.step()
+        s.set_stream(st.value)
+        assert s.get_stream() == st.value
+        for _ in range(3):
+            s.step()
+        s.set_stream(0)
+        assert s.get_stream() == own
+        s.step()
+        a, b = ref.state_by_id(), s.state_by_id()
+    finally:
+        s.close()
+        hip.hipStreamDestroy(st)
     for k in ("id", "pos", "vel", "kappa", "kappaV"):
         assert np.array_equal(a[k], b[k]), k
 
