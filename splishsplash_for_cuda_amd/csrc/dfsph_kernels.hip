@@ -900,8 +900,21 @@ __global__ __launch_bounds__(kBlock) void k_solve_finalize(SolveCtl* ctl, const 
 {
     __shared__ double lds4[kBlock / 64];
     if (!ctl->done) {
+        // each thread sums its strided share in order; the loads of 8 terms
+        // are issued together (adding the 0.0 of a missing term is exact, so
+        // the sum is the one-at-a-time sum, bit for bit)
+        constexpr uint32_t kBatch = 8;
         double acc = 0.0;
-        for (uint32_t k = threadIdx.x; k < nparts; k += blockDim.x) acc += partial[k];
+        for (uint32_t k0 = threadIdx.x; k0 < nparts; k0 += kBatch * kBlock) {
+            double v[kBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) {
+                const uint32_t k = k0 + u * kBlock;
+                v[u] = k < nparts ? partial[k] : 0.0;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kBatch; ++u) acc += v[u];
+        }
         const double tot = block_sum(acc, lds4);
         if (threadIdx.x == 0) {
             const double err = ctl->n_div > 0.0 ? tot / ctl->n_div : 0.0;
@@ -1219,8 +1232,18 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_xsph(SimConst c, con
 __global__ __launch_bounds__(kBlock) void k_cfl_finalize(DevCtl* ctl, const float* partial, uint32_t nparts)
 {
     __shared__ float lds4[kBlock / 64];
+    constexpr uint32_t kBatch = 8;
     float mm = 0.0f;
-    for (uint32_t k = threadIdx.x; k < nparts; k += blockDim.x) mm = fmaxf(mm, partial[k]);
+    for (uint32_t k0 = threadIdx.x; k0 < nparts; k0 += kBatch * kBlock) {
+        float v[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+            const uint32_t k = k0 + u * kBlock;
+            v[u] = k < nparts ? partial[k] : 0.0f;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) mm = fmaxf(mm, v[u]);
+    }
     mm = wave_max(mm);
     if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = mm;
     __syncthreads();
