@@ -74,7 +74,13 @@ uint32_t RestFluidLoader::initializeFluidToSurface()
         if (near) { xyz.insert(xyz.end(), x, x + 3); tag.push_back(DFSPH_TAG_AIR); }
     }
     const uint32_t n0 = (uint32_t)tag.size();
+    // the upload restarts the step history: the solver's time goes on as it was
+    dfsph_info info;
+    rcheck(dfsph_get_info(m_h, &info), "dfsph_get_info");
+    if (n0 > info.fluid_capacity && info.fluid_capacity > 0)
+        rcheck(dfsph_change_max_particle_number(m_h, n0 + n0 / 4), "dfsph_change_max_particle_number");
     rcheck(dfsph_upload_fluid(m_h, xyz.data(), nullptr, nullptr, nullptr, n0), "dfsph_upload_fluid");
+    rcheck(dfsph_set_time(m_h, info.h, info.h_past, info.time), "dfsph_set_time");
     if (n0) rcheck(dfsph_upload_field(m_h, DFSPH_FIELD_TAG, tag.data(), n0), "dfsph_upload_field");
     // extraction passes at a limit stepping down to density_end
     float limit = m_p.density_start;

@@ -781,6 +781,8 @@ int grow_capacity(Handle* h, uint32_t new_cap)
         return rc;
     }
     release();
+    // a capacity fixed at create follows the growth (a later upload keeps it)
+    if (h->p.fluid_capacity) h->p.fluid_capacity = new_cap;
     h->lists_valid = false;
     h->acc_valid = false;
     h->pred_valid = false;

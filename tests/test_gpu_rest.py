@@ -127,3 +127,27 @@ def test_rest_fluid_loader_fits_a_settled_fluid_to_a_new_box():
     assert dmin > r
     assert v1.max() < 1.0 and v1.max() < 1.5 * v_lat.max()
     hs.close()
+
+
+def test_rest_fluid_loader_edge_cases():
+    """An empty background leaves an empty fluid (no pass fails on zero
+    particles) and the solver's time as it was; a background entirely above
+    the fluid surface is all air and leaves nothing either; a bad step size
+    is refused."""
+    r, d, walls = _box()
+    p = S.default_params(particle_radius=r, dt=0.002, max_neighbors=128)
+    hs = S.HostSolver(p, walls, np.array([[0.3, 0.3, 0.3]], np.float32))
+    for _ in range(3):
+        hs.step()
+    t0 = hs.view.info().time
+    below = S.surface("plane", o=(0, 0.2, 0), n=(0, -1, 0))
+    n, passes, avg = hs.load_rest_fluid(below, np.zeros((0, 3), np.float32))
+    assert n == 0 and hs.view.info().n_fluid == 0
+    assert hs.view.info().time == t0 > 0.0
+    g = np.arange(0.25, 0.4, d)
+    above = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    n, passes, avg = hs.load_rest_fluid(below, above)
+    assert n == 0 and hs.view.info().n_fluid == 0
+    with pytest.raises(S.DFSPHError):
+        hs.load_rest_fluid(below, above, density_start=900.0, density_end=1000.0)
+    hs.close()

@@ -89,6 +89,13 @@ def test_mesh_surface_inside_matches_oracle(oracle_mod):
             assert 0.02 < want.mean() < 0.98
     with pytest.raises(S.DFSPHError):
         s.surface_inside(S.surface("mesh", mesh=99), pts[:4])
+    v, f = meshes.cube_mesh((0, 0, 0), (1, 1, 1))
+    with pytest.raises(S.DFSPHError):
+        s.mesh_upload(v, f[:3])                        # not closed: fewer than 4 faces
+    bad = f.copy()
+    bad[0, 0] = 8                                      # a vertex index past the 8 vertices
+    with pytest.raises(S.DFSPHError):
+        s.mesh_upload(v, bad)
 
 
 def test_open_boundary_with_a_mesh_interior_equals_the_cuboid(oracle_mod):
