@@ -557,3 +557,35 @@ def test_water_at_rest_stays_at_rest(ws):
     bound, bound_late = (1.0, 0.6) if lw else (1.0, 0.4)
     assert max(vmax) < bound and max(late) < bound_late and st.n_fluid == len(fl)
     s.close()
+
+
+def test_empty_and_single_particle_fluids():
+    """Edge cases of the step: a boundary with no fluid steps without error
+    (time advances, nothing appears); one particle far from the walls falls
+    freely, v = g t and y = y0 + g t^2 / 2 summed over the steps (no
+    neighbors: no pressure, no viscosity), every solve at zero error."""
+    sc = scene.small_dam_break()
+    s = S.Solver(S.default_params(particle_radius=sc.particle_radius, dt=0.002))
+    s.upload_boundary(sc.boundary)
+    s.upload_fluid(np.zeros((0, 3), np.float32))
+    for _ in range(3):
+        st = s.step()
+        assert st.n_fluid == 0
+    assert s.info().n_fluid == 0 and s.get_time()[2] == pytest.approx(0.006, rel=1e-6)
+    s.close()
+    lo, hi = sc.boundary.min(0), sc.boundary.max(0)
+    x0 = 0.5 * (lo + hi)
+    s = S.Solver(S.default_params(particle_radius=sc.particle_radius, dt=0.002))
+    s.upload_boundary(sc.boundary)
+    s.upload_fluid(x0[None, :].astype(np.float32))
+    y, v = float(x0[1]), 0.0
+    for _ in range(20):
+        st = s.step()
+        assert st.err == 0.0 and st.err_v == 0.0
+        v -= 9.81 * 0.002
+        y += 0.002 * v                       # symplectic Euler, as the step integrates
+    x = s.download(S.FIELD_POS)[0]
+    vel = s.download(S.FIELD_VEL)[0]
+    assert vel[1] == pytest.approx(v, rel=1e-5) and abs(vel[0]) < 1e-7 and abs(vel[2]) < 1e-7
+    assert x[1] == pytest.approx(y, abs=1e-5) and x[0] == pytest.approx(x0[0], abs=1e-6)
+    s.close()
