@@ -401,12 +401,15 @@ struct PairL {
 
 __device__ __forceinline__ PairL grad_listed(const SimConst& c, float r2)
 {
-    const float r2c = fmaxf(r2, 1.0e-24f);
-    const float ir = __builtin_amdgcn_rsqf(r2c);
-    const float r = r2c * ir;
+    // a coincident pair (r2 = 0) gives ir = inf and a NaN coefficient, which
+    // the caller drops with ok, so r2 needs no clamp here
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * ir;
     const float d = c.hk - r;
     // both pieces are computed and one is selected: a branch per pair (the
-    // select-to-branch transform) costs more than the other piece
+    // select-to-branch transform) costs more than the other piece.  The
+    // outer piece keeps the factored form gout (h - r)^2 / r: expanded, its
+    // terms cancel near the support's edge.
     const float gi = __builtin_fmaf(c.gin_a, r, c.gin_b), go = (c.gout * ir) * (d * d);
     PairL o;
     o.W = 0.0f;
@@ -417,14 +420,14 @@ __device__ __forceinline__ PairL grad_listed(const SimConst& c, float r2)
 
 __device__ __forceinline__ PairL pair_listed(const SimConst& c, float r2)
 {
-    const float r2c = fmaxf(r2, 1.0e-24f);
+    const float r2c = fmaxf(r2, 1.0e-24f);     // W of a coincident pair is W(0)
     const float ir = __builtin_amdgcn_rsqf(r2c);
     const float r = r2c * ir;
-    const float d = c.hk - r;
     const float q = r * c.inv_hk;
     const float f = 1.0f - q;
     const bool in = __builtin_unpredictable(r2 <= c.hk2_4);
     const float wi = c.kcoef * __builtin_fmaf(6.0f * q * q, q - 1.0f, 1.0f), wo = 2.0f * c.kcoef * f * f * f;
+    const float d = c.hk - r;
     const float gi = __builtin_fmaf(c.gin_a, r, c.gin_b), go = (c.gout * ir) * (d * d);
     PairL o;
     o.W = in ? wi : wo;

@@ -1079,16 +1079,18 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_source(SimConst 
     if (i < n && is_source(L.rs, i)) {
         const uint32_t cn = hd.cnt;
         const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
+        // with a uniform mass the fluid sum is taken without it and scaled once
         float dr = 0.0f;
         auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const PairL kp = grad_listed(c, dist2(dx, dy, dz));
-            const float mg = keep(valid && kp.ok, pj.w * kp.g);
+            const float mg = keep(valid && kp.ok, UM ? kp.g : pj.w * kp.g);
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         };
         fluid_nbrs<kStageMaxPV>(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return st.get(e, c.m_uniform); },
                    [&](uint32_t j) { return PV{pos[j], vel[j]}; },
                    [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
+        if (UM) dr *= c.m_uniform;
         for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, hd.b0, hd.b1,
             [&](uint32_t j) { return load_bnd<SOL>(bpos, L.sol, j); }, [&](const BN& bj, bool valid) {
             const float4& pj = bj.p;
@@ -1134,17 +1136,19 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(SimConst c, const DevTime* t,
     if (DIV) kappaV[i] = kv + ki;
     const uint32_t cn = L.cnt[i];
     const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
+    // with a uniform mass the fluid sum is taken without it and scaled once
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     auto fluid = [&](const float4& pj, float kj, bool valid) {
         const float ks = ki + kj;
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const PairL kp = grad_listed(c, dist2(dx, dy, dz));
-        const float s = keep(valid && kp.ok && fabsf(ks) > kEps, ks * pj.w * kp.g);
+        const float s = keep(valid && kp.ok && fabsf(ks) > kEps, UM ? ks * kp.g : ks * pj.w * kp.g);
         ax += s * dx; ay += s * dy; az += s * dz;
     };
     fluid_nbrs(L, c.cap, b, i, nf, sn, [&](uint32_t e) { return st.get(e, c.m_uniform); },
                [&](uint32_t j) { return PK{pos[j], kk[j]}; },
                [&](const PK& d, bool valid) { fluid(d.p, d.k, valid); });
+    if (UM) { ax *= c.m_uniform; ay *= c.m_uniform; az *= c.m_uniform; }
     if (fabsf(ki) > kEps) {
         const float mi = pi.w;
         for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, [&](uint32_t j) { return load_bnd<SOL>(bpos, L.sol, j); },
@@ -1379,16 +1383,18 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_density_source(SimCo
         const float hf = t->h_future;
         const uint32_t cn = hd.cnt;
         const uint32_t nf = cn & 0xffffu, nb = cn >> 16;
+        // with a uniform mass the fluid sum is taken without it and scaled once
         float dr = 0.0f;
         auto fluid = [&](const float4& pj, const float4& vj, bool valid) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const PairL kp = grad_listed(c, dist2(dx, dy, dz));
-            const float mg = keep(valid && kp.ok, pj.w * kp.g);
+            const float mg = keep(valid && kp.ok, UM ? kp.g : pj.w * kp.g);
             dr += mg * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
         };
         fluid_nbrs<kStageMaxPV>(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return st.get(e, c.m_uniform); },
                    [&](uint32_t j) { return PV{pos[j], vel[j]}; },
                    [&](const PV& d, bool valid) { fluid(d.p, d.v, valid); });
+        if (UM) dr *= c.m_uniform;
         (void)nf;
         for_each_nbr32(L.nbrb, c.cap, i, nb, 0u, hd.b0, hd.b1,
             [&](uint32_t j) { return load_bnd<SOL>(bpos, L.sol, j); }, [&](const BN& bj, bool valid) {

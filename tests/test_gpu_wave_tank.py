@@ -107,12 +107,14 @@ def test_full_wave_tank_stays_bounded():
     piston of 0.1 m at 1.5 s, the box at 500 kg/m^3, the absorbing end), 2000
     steps with each warm start: every step within the tolerances, the fluid
     count within 15 % of the start, and the speeds (sampled every 50 steps)
-    bounded: with the reference warm start the fastest particle (the splash
-    at the piston's edges) stays below 8 m/s (measured 6.5), and with the
-    learned warm start the 100th fastest particle stays within 1.25 x and the
-    fastest within 2 x of the reference run's (single splash particles differ
-    between two runs; a run that blows up, as an earlier form of the learned
-    warm start did at 27 m/s, exceeds both)."""
+    bounded: with the reference warm start the 100th fastest particle stays
+    below 6.5 m/s (measured 5.3 and 5.5 in two builds that differ only in
+    rounding) and the fastest, a single droplet of the splash at the
+    piston's edges, below 12 m/s (measured 6.5 and 8.3: single droplets
+    differ between runs); with the learned warm start the 100th fastest
+    particle stays within 1.25 x and the fastest within 2 x of the reference
+    run's.  A run that blows up, as an earlier form of the learned warm start
+    did at 27 m/s, exceeds these bounds."""
     ref = _full_tank_run(False, 2000)
     lrn = _full_tank_run(True, 2000)
     for name, (n0, n1, vmax, vtop, iv, time) in (("reference", ref), ("learned", lrn)):
@@ -121,7 +123,7 @@ def test_full_wave_tank_stays_bounded():
                                                                  vtop.max(), iv), flush=True)
         assert abs(n1 - n0) < 0.15 * n0, name
         assert np.isfinite(vmax).all(), name
-    assert ref[2].max() < 8.0
+    assert ref[3].max() < 6.5 and ref[2].max() < 12.0
     assert lrn[3].max() <= 1.25 * ref[3].max()
     assert lrn[2].max() <= 2.0 * ref[2].max()
 
