@@ -12,8 +12,10 @@ particle set.
 Legs, each on its own solver, the same steps:
   plain    the reference warm start: warm-up, then the timed steps (no
            profiling events inside the timed region), then the same number
-           of profiled steps (HIP events around every solver kernel) for the
-           roofline and the kernel shares;
+           of profiled steps for the roofline and the kernel shares: every
+           solver sweep carries a HIP event pair in its own dispatch
+           (hipExtLaunchKernel), so its duration is the kernel's execution;
+           the sort and the search are bracketed by event records;
   learned  the committed learned warm start, timed and profiled the same way;
            its ms per step, iterations per step, error maxima, the time of its
            feature sweep and inference, and the final density field against
@@ -309,7 +311,7 @@ def main():
             "min_iters": list(min_iters),
             "unstaged_blocks": int(leg["inf1"].unstaged_blocks),
             "timing": "timed steps without profiling events; roofline and kernel shares from %d further "
-                      "profiled steps (HIP events on the solver's stream)" % prof_steps,
+                      "profiled steps (HIP events in each sweep's dispatch, on the solver's stream)" % prof_steps,
         },
         "roofline": roof,
         "kernel_share": {k: round(v[1] / total_ms, 4) for k, v in prof.items() if v[0]},

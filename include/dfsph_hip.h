@@ -73,7 +73,7 @@ typedef struct dfsph_params {
                                         the reference warm start and skips the features and the
                                         network (at the floors no warm start saves an iteration) */
 
-/* list_mode: how sweep blocks (256 particles) read their fluid neighbors */
+/* list_mode: how sweep blocks (dfsph_info.sweep_block particles) read their fluid neighbors */
 #define DFSPH_LISTS_AUTO 0       /* LDS-staged set when it fits, else 16-bit set ids, else 32-bit */
 #define DFSPH_LISTS_NO_LDS 1     /* 16-bit set ids read through the set's global ids               */
 #define DFSPH_LISTS_GLOBAL 2     /* 32-bit global ids only                                          */
@@ -101,11 +101,12 @@ typedef struct dfsph_info {
     uint32_t max_neighbors_seen;   /* largest count of the last neighbor search */
     uint32_t overflow;             /* particles whose list was truncated        */
     uint32_t escaped;              /* outside the box after the last position update */
-    uint32_t unstaged_blocks;      /* 256-particle blocks whose neighbor set exceeded
+    uint32_t unstaged_blocks;      /* sweep blocks whose neighbor set exceeded
                                       the LDS stage (32-bit global lists)      */
     uint64_t neighbors_fluid_total;    /* stored fluid neighbors, all particles    */
     uint64_t neighbors_boundary_total; /* stored boundary neighbors, all particles */
     uint64_t neighbors_fluid_unstaged; /* of the fluid ones, in 32-bit global lists */
+    uint32_t sweep_block;          /* particles per sweep block (owner of a neighbor set) */
 } dfsph_info;
 
 /* fields for dfsph_download / dfsph_upload_field */
@@ -287,9 +288,12 @@ int dfsph_neighbor_stats(void* handle, uint32_t* max_seen, uint32_t* overflow, u
 /* Per-kernel HIP-event timing, the device-side counterpart of the phase timers
  * of DFSPHCUDA::step (DFSPH_CUDA.cpp:185-194, 480-495) and SegmentedTiming
  * (SPlisHSPlasH/Utilities/SegmentedTiming.h:22-84).  mask: one bit per
- * DFSPH_KERNEL_* id; an event pair is recorded around each launch of those
- * kernels on the handle's stream (0 turns timing off).  Velocity sweeps that
- * exit at once because their loop had already converged are not counted. */
+ * DFSPH_KERNEL_* id (0 turns timing off).  The single-kernel sweeps (ids 2-12)
+ * carry an event pair in their own dispatch (hipExtLaunchKernel), so their
+ * duration is the kernel's execution alone; the sort, the search (with its
+ * one-block finalize) and the MLP are bracketed by event records on the
+ * handle's stream.  Velocity sweeps that exit at once because their loop had
+ * already converged are not counted. */
 #define DFSPH_KERNEL_SORT 0          /* cell hash, scan, counting sort, reorder */
 #define DFSPH_KERNEL_NEIGHBORS 1     /* neighbor list build                     */
 #define DFSPH_KERNEL_DIV_WS_INIT 2   /* density + factor + source + warm start  */

@@ -122,6 +122,7 @@ class DFSPHInfo(C.Structure):
         ("neighbors_fluid_total", C.c_uint64),
         ("neighbors_boundary_total", C.c_uint64),
         ("neighbors_fluid_unstaged", C.c_uint64),
+        ("sweep_block", C.c_uint32),
     ]
 
 

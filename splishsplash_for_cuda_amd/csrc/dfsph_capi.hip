@@ -158,6 +158,7 @@ struct Handle {
     struct ProfRec {
         int kind;
         hipEvent_t a, b;
+        bool bound;                   // events carried by the sweep's own dispatch
     };
     uint32_t prof_mask = 0;
     std::vector<ProfRec> prof_pending;
@@ -168,7 +169,17 @@ struct Handle {
     ~Handle();
 };
 
-// Event pair around one launch of kernel `kind` when it is being timed;
+// Kernel ids whose timed region is one sweep kernel: their events travel in
+// that kernel's dispatch.  The others (sort, search + its finalize, the MLP)
+// span several kernels and are bracketed by event records on the stream.
+constexpr uint32_t kBoundKinds = (1u << DFSPH_KERNEL_DIV_WS_INIT) | (1u << DFSPH_KERNEL_DIV_INIT) |
+                                 (1u << DFSPH_KERNEL_DIV_JACOBI) | (1u << DFSPH_KERNEL_DIV_LOOP_END) |
+                                 (1u << DFSPH_KERNEL_XSPH) | (1u << DFSPH_KERNEL_PRESSURE_WS) |
+                                 (1u << DFSPH_KERNEL_DENS_INIT) | (1u << DFSPH_KERNEL_DENS_JACOBI) |
+                                 (1u << DFSPH_KERNEL_DENS_LOOP_END) | (1u << DFSPH_KERNEL_UPDATE_POS) |
+                                 (1u << DFSPH_KERNEL_FEATURES);
+
+// Event pair for one launch of kernel `kind` when it is being timed;
 // returns the record index or -1.
 int pbeg(Handle* h, int kind)
 {
@@ -184,14 +195,25 @@ int pbeg(Handle* h, int kind)
             return -1;
         }
     }
-    (void)hipEventRecord(e[0], h->s);
-    h->prof_pending.push_back({kind, e[0], e[1]});
+    const bool bound = (kBoundKinds >> kind) & 1u;
+    if (bound)
+        dfsph::arm_sweep_events(e[0], e[1]);
+    else
+        (void)hipEventRecord(e[0], h->s);
+    h->prof_pending.push_back({kind, e[0], e[1], bound});
     return (int)h->prof_pending.size() - 1;
 }
 
 void pend(Handle* h, int idx)
 {
-    if (idx >= 0) (void)hipEventRecord(h->prof_pending[idx].b, h->s);
+    if (idx < 0) return;
+    auto& r = h->prof_pending[idx];
+    if (!r.bound) {
+        (void)hipEventRecord(r.b, h->s);
+    } else if (dfsph::sweep_events_armed()) {
+        dfsph::disarm_sweep_events();   // nothing was launched (no particles): no record
+        r.kind = -1;
+    }
 }
 
 void pdrop(Handle* h, int idx)
@@ -1508,6 +1530,7 @@ int dfsph_get_info(void* hv, dfsph_info* info)
     info->neighbors_boundary_total = h->hctl->nbr_boundary;
     info->unstaged_blocks = h->hctl->unstaged_blocks;
     info->neighbors_fluid_unstaged = h->hctl->nbr_fluid_unstaged;
+    info->sweep_block = (uint32_t)dfsph::kBlock;
     return DFSPH_OK;
 }
 

@@ -16,8 +16,9 @@
 //   stg   uint32 [blocks][kStageCap]  global ids of each block's staged set
 //
 // Cells are ordered in 8x8x8-cell tiles (tiles x-fastest, then z, then y),
-// Morton order inside a tile, so 256 consecutive particles occupy a compact
-// region and the neighbor set of a block is about 4.5 times the block.
+// Morton order inside a tile, so a block of consecutive particles occupies a
+// compact region: the neighbor set of a 256-particle block is about 5.2 times
+// the block, that of a 512-particle block about 3.9 times (C2, at rest).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,7 +26,16 @@
 
 namespace dfsph {
 
-constexpr int kBlock = 256;        // 4 wavefronts of 64
+// Particles per block of every per-particle kernel; the sweep blocks own the
+// neighbor sets, so this also sets the LDS stages below.  512 (8 wavefronts):
+// a 512-particle set is about 3.9 times the block against 5.2 for 256, so the
+// stages hold more particles per wavefront and the position + velocity sweeps
+// run 16 wavefronts per CU instead of 12 (C4: 4.84 against 5.20 ms per step).
+#ifndef DFSPH_BLOCK
+#define DFSPH_BLOCK 512
+#endif
+constexpr int kBlock = DFSPH_BLOCK;
+static_assert(kBlock == 256 || kBlock == 512, "256 or 512 threads: 4 or 8 wavefronts of 64");
 constexpr int kNumXcd = 8;         // MI355X: 8 XCDs, blocks dispatched round robin
 constexpr float kEps = 1.0e-5f;    // m_eps (DFSPH_define_cuda.h:7)
 
@@ -136,20 +146,25 @@ struct Publish {
 };
 
 // ---- staged neighbor sets ----
-// A sweep block (256 consecutive particles) owns the set of fluid particles in
+// A sweep block (kBlock consecutive particles) owns the set of fluid particles in
 // the cells adjacent to its particles' cells, cell by cell.  Up to
 // kStageMax of them are loaded into LDS once per sweep (coalesced) and the
 // block's fluid lists are 16-bit indices into the set; sets up to kStageCap
 // keep the 16-bit lists and read the particles through the set's global ids;
 // larger ones (sparse splashes, more than kAdjMax / 27 own cells, the
 // unsorted search) use 32-bit global lists (stage_n = kUnstaged).
-// LDS-staged particles per block.  2048 holds nearly every block's set (C2:
-// mean 1150-1400, p99 1780): a set read through its global ids instead costs
-// about 1.5x per sweep, more than the occupancy the larger stage gives up
-// (position + k sweeps 40 KB: 4 blocks per CU; position + velocity sweeps
-// 64 KB: 2 blocks per CU; measured 1.86 against 2.00 ms per C2 step at 1600).
+// LDS-staged particles per block.  A set read through its global ids instead
+// costs about 1.5x per sweep, more than the occupancy a smaller stage would
+// win, so the cap holds nearly every set.  256-particle blocks: 2048 (C2 sets:
+// mean 1150-1400, p99 1780).  512-particle blocks: 3328, the most that keeps
+// three position + k blocks (16 B records, 52 KB) and two position + velocity
+// blocks (24 B, 78 KB) in a CU's 160 KB.
 #ifndef DFSPH_STAGE_MAX
+#if DFSPH_BLOCK == 512
+#define DFSPH_STAGE_MAX 3328
+#else
 #define DFSPH_STAGE_MAX 2048
+#endif
 #endif
 constexpr uint32_t kStageMax = DFSPH_STAGE_MAX;
 // The sweeps that stage position and velocity (32 B per particle) may stage
@@ -159,8 +174,23 @@ constexpr uint32_t kStageMax = DFSPH_STAGE_MAX;
 #define DFSPH_PV_STAGE DFSPH_STAGE_MAX
 #endif
 constexpr uint32_t kStageMaxPV = DFSPH_PV_STAGE;
+// The sweeps that stage seven floats per particle (position, velocity and one
+// scalar: the XSPH sweep and the divergence warm start, 28 B): 2880 keeps two
+// 512-particle blocks (79 KB) in a CU.
+#ifndef DFSPH_X_STAGE
+#if DFSPH_BLOCK == 512
+#define DFSPH_X_STAGE 2880
+#else
+#define DFSPH_X_STAGE DFSPH_PV_STAGE
+#endif
+#endif
+constexpr uint32_t kStageMaxX = DFSPH_X_STAGE;
 #ifndef DFSPH_PV_WAVES
+#if DFSPH_BLOCK == 512
+#define DFSPH_PV_WAVES 4          // two 8-wave blocks per CU: at most 128 VGPRs
+#else
 #define DFSPH_PV_WAVES 1
+#endif
 #endif
 constexpr uint32_t kStageCap = 4096;       // largest set with 16-bit lists
 constexpr uint32_t kUnstaged = 0xffffffffu;

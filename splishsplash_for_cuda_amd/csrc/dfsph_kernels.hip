@@ -1,9 +1,11 @@
 // dfsph_kernels.hip -- hand-written gfx950 kernels of the DFSPH time step.
 //
-// One thread per particle, particles in cell-sorted order, 256-thread
-// blocks with the XCD-aware block remap of dfsph_device.hpp.  Every kernel
+// One thread per particle, particles in cell-sorted order, kBlock-thread
+// blocks (512) with the XCD-aware block remap of dfsph_device.hpp.  Every kernel
 // cites the reference kernel(s) it replaces (paths relative to the
 // reference repository root).
+#include <hip/hip_ext.h>
+
 #include "dfsph_device.hpp"
 #include "dfsph_launch.hpp"
 
@@ -356,9 +358,9 @@ struct ListWriter32 {
 // bounding box of those cells, x fastest, then z, then y, so the set lists
 // them in (y, z, x) order and the three cells x-1..x+1 of any row (y, z) are
 // consecutive: one contiguous range of the set per row.
-constexpr uint32_t kNbrStage = 2048;       // set positions staged in LDS by the search
+constexpr uint32_t kNbrStage = kBlock == 512 ? 3584 : 2048;   // set positions staged in LDS by the search
 constexpr uint32_t kDense = 8192;          // bounding-box cells of a tabled set
-constexpr uint32_t kSetCells = 384;        // set cells of a tabled set (LDS: 4 blocks per CU)
+constexpr uint32_t kSetCells = kBlock == 512 ? 768 : 384;      // set cells of a tabled set
 constexpr uint32_t kNoRow = 0xffffffffu;
 constexpr int kIntMax = 0x7fffffff;
 
@@ -410,7 +412,8 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
     __shared__ uint32_t s_bs[kSetCells];       // boundary range of each set cell
     __shared__ uint32_t s_be[kSetCells];
     __shared__ uint32_t s_boff[kSetCells + 1]; // boundary set offsets of the set cells
-    __shared__ int s_w[6 * (kBlock / 64)];
+    constexpr int NW = kBlock / 64;
+    __shared__ int s_w[6 * NW];
     static_assert(kDense * sizeof(uint16_t) <= sizeof(s_raw), "dense table exceeds the stage");
     static_assert((kStageCap - 1) * 16 <= 0xffffu && kSetCells < 0xffffu, "16-bit byte offsets / cell ids");
     uint16_t* s_dense = reinterpret_cast<uint16_t*>(s_raw);
@@ -441,17 +444,17 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
         const int lo_y = wave_min_i(act ? cy : kIntMax), hi_y = wave_max_i(act ? cy : -1);
         const int lo_z = wave_min_i(act ? cz : kIntMax), hi_z = wave_max_i(act ? cz : -1);
         if ((threadIdx.x & 63) == 0) {
-            s_w[w] = lo_x; s_w[4 + w] = lo_y; s_w[8 + w] = lo_z;
-            s_w[12 + w] = hi_x; s_w[16 + w] = hi_y; s_w[20 + w] = hi_z;
+            s_w[w] = lo_x; s_w[NW + w] = lo_y; s_w[2 * NW + w] = lo_z;
+            s_w[3 * NW + w] = hi_x; s_w[4 * NW + w] = hi_y; s_w[5 * NW + w] = hi_z;
         }
         __syncthreads();
         int lo[3] = {kIntMax, kIntMax, kIntMax}, hi[3] = {-1, -1, -1};
 #pragma unroll
-        for (int k = 0; k < kBlock / 64; ++k) {
+        for (int k = 0; k < NW; ++k) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                lo[a] = min(lo[a], s_w[4 * a + k]);
-                hi[a] = max(hi[a], s_w[12 + 4 * a + k]);
+                lo[a] = min(lo[a], s_w[NW * a + k]);
+                hi[a] = max(hi[a], s_w[NW * (3 + a) + k]);
             }
         }
         __syncthreads();                        // s_w is reused by the scans
@@ -729,7 +732,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(SimConst c, const float4* 
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t a[5] = {0u, 0u, 0u, 0u, 0u};
-        for (int v = 0; v < kBlock / 64; ++v)
+        for (int v = 0; v < NW; ++v)
             for (int k = 0; k < 5; ++k) {
                 const uint32_t x = (uint32_t)s_w[5 * v + k];
                 a[k] = k == 0 ? max(a[0], x) : a[k] + x;
@@ -831,8 +834,8 @@ struct PV {
 };
 
 // LDS stage of (position, velocity) records.  UM (every fluid particle has the
-// mass c.m_uniform): 24 B per particle, (x y z vx) and (vy vz), so that three
-// 256-particle blocks fit a CU (48 KB each) instead of two; the mass is put
+// mass c.m_uniform): 24 B per particle, (x y z vx) and (vy vz), so that two
+// 512-particle blocks fit a CU (78 KB each) instead of one; the mass is put
 // back from the constant, the same value bit for bit.  Entries of the 16-bit
 // lists are byte offsets of 16-byte slots; the 8-byte slots are at half.
 template <bool UM>
@@ -857,13 +860,43 @@ struct PVStage<true> {
         return PV{make_float4(u.x, u.y, u.z, m), make_float4(u.w, w.x, w.y, 0.0f)};
     }
 };
+// LDS stage of (position, velocity, scalar) records.  UM: 28 B per particle,
+// (x y z vx), (vy vz) and the scalar, the mass put back from the constant;
+// otherwise 32 B, (x y z m) and (vx vy vz scalar).
+template <bool UM>
+struct PVSStage {
+    float4 p[kStageMaxX], v[kStageMaxX];
+    __device__ __forceinline__ void put(uint32_t k, const PV& d) { p[k] = d.p; v[k] = d.v; }
+    __device__ __forceinline__ PV get(uint32_t off, float) const { return PV{lds_f4(p, off), lds_f4(v, off)}; }
+};
+template <>
+struct PVSStage<true> {
+    float4 a[kStageMaxX];
+    float2 b[kStageMaxX];
+    float s[kStageMaxX];
+    __device__ __forceinline__ void put(uint32_t k, const PV& d)
+    {
+        a[k] = make_float4(d.p.x, d.p.y, d.p.z, d.v.x);
+        b[k] = make_float2(d.v.y, d.v.z);
+        s[k] = d.v.w;
+    }
+    __device__ __forceinline__ PV get(uint32_t off, float m) const
+    {
+        const float4 u = lds_f4(a, off);
+        const float2 w = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(b) + (off >> 1));
+        const float q = lds_f1(s, off);
+        return PV{make_float4(u.x, u.y, u.z, m), make_float4(u.w, w.x, w.y, q)};
+    }
+};
+
 struct PK {
     float4 p;                      // position + mass
     float k;                       // k_j of the Jacobi sweep / the warm start
 };
 
 // LDS stage of (position, k) records: 20 B per particle, or 16 B (x y z k)
-// with a uniform fluid mass (UM; five blocks per CU instead of four).
+// with a uniform fluid mass (UM; three 512-particle blocks per CU instead of
+// two).
 template <bool UM>
 struct PKStage {
     float4 p[kStageMax];
@@ -941,8 +974,8 @@ __global__ __launch_bounds__(kBlock) void k_solve_finalize(SolveCtl* ctl, const 
 // the last solve's kappaV increments.  Velocities are read from vel_i and
 // written to vel_o.  The applied k_i starts the particle's total of the step
 // in kvtot (when given).
-template <int WS, bool SOL>
-__global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_ws_init(SimConst c, const DevTime* t, uint32_t n,
+template <int WS, bool SOL, bool UM>
+__global__ __launch_bounds__(kBlock, UM ? DFSPH_PV_WAVES : 1) void k_div_ws_init(SimConst c, const DevTime* t, uint32_t n,
                                                         const float4* __restrict__ pos,
                                                         const float4* __restrict__ bpos, NbrLists L,
                                                         const float4* __restrict__ vel_i,
@@ -951,8 +984,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_ws_init(SimConst
                                                         float* __restrict__ rho, float* __restrict__ alpha,
                                                         float* __restrict__ radv, float* __restrict__ kvtot)
 {
-    __shared__ float4 sP[kStageMaxPV];
-    __shared__ float4 sV[kStageMaxPV];           // (v, warm-start k) of the staged set
+    __shared__ PVSStage<UM> st;                  // (x, v, warm-start k) of the staged set
     const float ratio = t->ratio, h = t->h, inv_h = t->inv_h;
     auto ws_of = [&](uint32_t j) -> float {
         if (WS == 2) return 0.0f;
@@ -968,10 +1000,10 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_ws_init(SimConst
     const float4 vi = vel_i[ic];
     const float4 pi = pos[ic];
     const float ki = ws_of(ic);
-    const uint32_t sn = stage_block<kStageMaxPV>(L, b, sn0, [&](uint32_t g) {
+    const uint32_t sn = stage_block<kStageMaxX>(L, b, sn0, [&](uint32_t g) {
         const float4 v = vel_i[g];
         return PV{pos[g], make_float4(v.x, v.y, v.z, ws_of(g))};
-    }, [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
+    }, [&](uint32_t k, const PV& d) { st.put(k, d); });
     if (i >= n) return;
     if (kvtot) kvtot[i] = ki;
     if (!is_source(L.rs, i)) { vel_o[i] = vi; return; }
@@ -996,7 +1028,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_ws_init(SimConst
             ax += s * px; ay += s * py; az += s * pz;
         }
     };
-    fluid_nbrs<kStageMaxPV>(L, stride, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+    fluid_nbrs<kStageMaxX>(L, stride, b, i, sn, hd, [&](uint32_t e) { return st.get(e, c.m_uniform); },
                [&](uint32_t j) {
                    const float4 v = vel_i[j];
                    return PV{pos[j], make_float4(v.x, v.y, v.z, ws_of(j))};
@@ -1052,7 +1084,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_ws_init(SimConst
 // reduction that replaces cub::DeviceReduce::Sum + cudaMemcpy (:774-781).
 // Also stores k_i = rho_adv_i * alpha_i for the next Jacobi sweep.
 template <bool INIT, bool SOL, bool UM>
-__global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_div_source(SimConst c, uint32_t n, const float4* __restrict__ pos,
+__global__ __launch_bounds__(kBlock, UM ? DFSPH_PV_WAVES : 1) void k_div_source(SimConst c, uint32_t n, const float4* __restrict__ pos,
                                                        const float4* __restrict__ bpos, NbrLists L,
                                                        const float4* __restrict__ vel,
                                                        const float* __restrict__ alpha,
@@ -1176,8 +1208,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_xsph(SimConst c, con
                                                  DevCtl* ctl, float* partial)
 {
     __shared__ float lds4[kBlock / 64];
-    __shared__ float4 sP[kStageMaxPV];
-    __shared__ float4 sV[kStageMaxPV];           // (v, m / rho) of the staged set
+    __shared__ PVSStage<true> st;                // (x, v, m / rho) of the staged set (the mass is not used)
     const uint32_t b = xcd_block();
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t ic = i < n ? i : 0u;         // own loads before staging, so they overlap it
@@ -1185,11 +1216,11 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_xsph(SimConst c, con
     const Head hd = list_head(L, sn0, c.cap, ic);
     const float4 pi = pos[ic];
     const float4 vi = vel[ic];
-    const uint32_t sn = stage_block<kStageMaxPV>(L, b, sn0, [&](uint32_t g) {
+    const uint32_t sn = stage_block<kStageMaxX>(L, b, sn0, [&](uint32_t g) {
         const float4 p = pos[g];
         const float4 v = vel[g];
         return PV{p, make_float4(v.x, v.y, v.z, p.w / rho[g])};
-    }, [&](uint32_t k, const PV& d) { sP[k] = d.p; sV[k] = d.v; });
+    }, [&](uint32_t k, const PV& d) { st.put(k, d); });
     float m2 = 0.0f;
     if (i < n && !is_compute(L.rs, i)) {
         // restricted out: its stored acceleration still enters the CFL bound
@@ -1208,7 +1239,7 @@ __global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_xsph(SimConst c, con
             const float s = keep(valid, coef * vj.w * w);
             ax -= s * (vi.x - vj.x); ay -= s * (vi.y - vj.y); az -= s * (vi.z - vj.z);
         };
-        fluid_nbrs<kStageMaxPV>(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return PV{lds_f4(sP, e), lds_f4(sV, e)}; },
+        fluid_nbrs<kStageMaxX>(L, c.cap, b, i, sn, hd, [&](uint32_t e) { return st.get(e, 0.0f); },
                    [&](uint32_t j) {
                        const float4 p = pos[j];
                        const float4 v = vel[j];
@@ -1353,7 +1384,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure_ws(SimConst c, const DevTim
 // Otherwise DFSPH_pressure_loop_end_kernel (:1264-1284) + FP64 error reduction
 // (replaces cub Sum + cudaMemcpy, :1321-1325).
 template <bool INIT, bool SOL, bool UM>
-__global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_density_source(SimConst c, const DevTime* t, uint32_t n,
+__global__ __launch_bounds__(kBlock, UM ? DFSPH_PV_WAVES : 1) void k_density_source(SimConst c, const DevTime* t, uint32_t n,
                                                            const float4* __restrict__ pos,
                                                            const float4* __restrict__ bpos, NbrLists L,
                                                            const float4* __restrict__ vel,
@@ -1488,7 +1519,7 @@ __global__ void k_finish_totals(const DevTime* t, uint32_t n, int pressure, cons
 //  6 boundary fraction n_b / n     7-9 mean neighbor offset / h_k
 // 10 mean neighbor distance / h_k 11 fluid share of the density
 template <bool SOL, bool UM>
-__global__ __launch_bounds__(kBlock, DFSPH_PV_WAVES) void k_features(SimConst c, const DevTime* t, uint32_t n,
+__global__ __launch_bounds__(kBlock, UM ? DFSPH_PV_WAVES : 1) void k_features(SimConst c, const DevTime* t, uint32_t n,
                                                      const float4* __restrict__ pos,
                                                      const float4* __restrict__ bpos, NbrLists L,
                                                      const float4* __restrict__ vel,
@@ -1593,6 +1624,29 @@ static inline uint32_t nblk_xcd(uint32_t n)
 }
 
 uint32_t grid_blocks_xcd(uint32_t n) { return nblk_xcd(n ? n : 1); }
+
+static thread_local hipEvent_t tl_ev_start = nullptr, tl_ev_stop = nullptr;
+
+void arm_sweep_events(hipEvent_t start, hipEvent_t stop)
+{
+    tl_ev_start = start;
+    tl_ev_stop = stop;
+}
+bool sweep_events_armed() { return tl_ev_start != nullptr; }
+void disarm_sweep_events() { tl_ev_start = tl_ev_stop = nullptr; }
+
+// A sweep launch of g blocks of kBlock threads; it takes the armed timing
+// events, if any, into its dispatch.
+template <class... P, class... A>
+static void sweep_launch(void (*k)(P...), uint32_t g, hipStream_t s, A... a)
+{
+    if (tl_ev_start) {
+        hipExtLaunchKernelGGL(k, dim3(g), dim3(kBlock), 0u, s, tl_ev_start, tl_ev_stop, 0u, a...);
+        disarm_sweep_events();
+    } else {
+        k<<<g, kBlock, 0, s>>>(a...);
+    }
+}
 
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint32_t n)
 {
@@ -1715,11 +1769,14 @@ void launch_div_ws_init(hipStream_t s, int ws, const SimConst& c, const DevTime*
     if (!n) return;
     const uint32_t g = nblk_xcd(n);
     const bool sol = L.sol.bvel != nullptr;
-#define DWS(W, S) k_div_ws_init<W, S><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel_i, vel_o, kws, lw, rho, alpha, radv, kvtot)
+    const bool um = c.m_uniform > 0.0f;
+#define DWU(W, S, U) sweep_launch(k_div_ws_init<W, S, U>, g, s, c, t, n, pos, bpos, L, vel_i, vel_o, kws, lw, rho, alpha, radv, kvtot)
+#define DWS(W, S) do { if (um) DWU(W, S, true); else DWU(W, S, false); } while (0)
     if (ws == 0) { if (sol) DWS(0, true); else DWS(0, false); }
     else if (ws == 1) { if (sol) DWS(1, true); else DWS(1, false); }
     else { if (sol) DWS(2, true); else DWS(2, false); }
 #undef DWS
+#undef DWU
 }
 
 void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, const float4* pos,
@@ -1730,7 +1787,7 @@ void launch_div_source(hipStream_t s, bool init, const SimConst& c, uint32_t n, 
     const uint32_t g = nblk_xcd(n ? n : 1);
     const bool sol = L.sol.bvel != nullptr;
     const bool um = c.m_uniform > 0.0f;
-#define DS(I, S, U, CI) k_div_source<I, S, U><<<g, kBlock, 0, s>>>(c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial, CI)
+#define DS(I, S, U, CI) sweep_launch(k_div_source<I, S, U>, g, s, c, n, pos, bpos, L, vel, alpha, radv, kk, kappaV, ctl, partial, CI)
 #define DSU(I, S, CI) do { if (um) DS(I, S, true, CI); else DS(I, S, false, CI); } while (0)
     if (init) { if (sol) DSU(true, true, ci); else DSU(true, false, ci); }
     else { if (sol) DSU(false, true, CtlInit{}); else DSU(false, false, CtlInit{}); }
@@ -1751,7 +1808,7 @@ void launch_jacobi(hipStream_t s, bool div, const SimConst& c, const DevTime* t,
     const uint32_t g = nblk_xcd(n);
     const bool sol = L.sol.bvel != nullptr;
     const bool um = c.m_uniform > 0.0f;
-#define JA(D, S, U) k_jacobi<D, S, U><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, kk, kappaV, vel, ctl)
+#define JA(D, S, U) sweep_launch(k_jacobi<D, S, U>, g, s, c, t, n, pos, bpos, L, kk, kappaV, vel, ctl)
 #define JAU(D, S) do { if (um) JA(D, S, true); else JA(D, S, false); } while (0)
     if (div) { if (sol) JAU(true, true); else JAU(true, false); }
     else { if (sol) JAU(false, true); else JAU(false, false); }
@@ -1764,7 +1821,7 @@ void launch_xsph(hipStream_t s, const SimConst& c, const DevTime* t, uint32_t n,
                  DevCtl* ctl, float* partial)
 {
     const uint32_t g = nblk_xcd(n ? n : 1);
-    k_xsph<<<g, kBlock, 0, s>>>(c, t, n, pos, L, vel, rho, acc, ctl, partial);
+    sweep_launch(k_xsph, g, s, c, t, n, pos, L, vel, rho, acc, ctl, partial);
     k_cfl_finalize<<<1, kBlock, 0, s>>>(ctl, partial, g);
 }
 
@@ -1782,7 +1839,7 @@ void launch_pressure_ws(hipStream_t s, int ws, bool apply_acc, const SimConst& c
     if (!n) return;
     const uint32_t g = nblk_xcd(n);
     const bool sol = L.sol.bvel != nullptr;
-#define PW(W, A, S) k_pressure_ws<W, A, S><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, acc, vel, kws, lw, rescale, ktot)
+#define PW(W, A, S) sweep_launch(k_pressure_ws<W, A, S>, g, s, c, t, n, pos, bpos, L, acc, vel, kws, lw, rescale, ktot)
     if (ws == 1) {
         if (apply_acc) { if (sol) PW(1, true, true); else PW(1, true, false); }
         else { if (sol) PW(1, false, true); else PW(1, false, false); }
@@ -1801,7 +1858,7 @@ void launch_density_source(hipStream_t s, bool init, const SimConst& c, const De
     const uint32_t g = nblk_xcd(n ? n : 1);
     const bool sol = L.sol.bvel != nullptr;
     const bool um = c.m_uniform > 0.0f;
-#define DN(I, S, U, CI) k_density_source<I, S, U><<<g, kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl, partial, CI)
+#define DN(I, S, U, CI) sweep_launch(k_density_source<I, S, U>, g, s, c, t, n, pos, bpos, L, vel, rho, alpha, radv, kk, kappa, ctl, partial, CI)
 #define DNU(I, S, CI) do { if (um) DN(I, S, true, CI); else DN(I, S, false, CI); } while (0)
     if (init) { if (sol) DNU(true, true, ci); else DNU(true, false, ci); }
     else { if (sol) DNU(false, true, CtlInit{}); else DNU(false, false, CtlInit{}); }
@@ -1813,8 +1870,8 @@ void launch_update_pos(hipStream_t s, const DevTime* t, uint32_t n, const float4
                        const float* bmin, const float* bmax, DevCtl* ctl, const Restrict& rs)
 {
     if (n)
-        k_update_pos<<<nblk(n), kBlock, 0, s>>>(t, n, vel, pos, bmin[0], bmin[1], bmin[2], bmax[0], bmax[1], bmax[2], ctl,
-                                                rs);
+        sweep_launch(k_update_pos, nblk(n), s, t, n, vel, pos, bmin[0], bmin[1], bmin[2], bmax[0], bmax[1], bmax[2], ctl,
+                     rs);
 }
 
 void launch_keep_inside(hipStream_t s, uint32_t n, const float4* pos, const float* bmin, const float* bmax,
@@ -1846,8 +1903,8 @@ void launch_features(hipStream_t s, const SimConst& c, const DevTime* t, uint32_
 {
     if (!n) return;
     const bool um = c.m_uniform > 0.0f;
-#define KF(S, U) k_features<S, U><<<nblk_xcd(n), kBlock, 0, s>>>(c, t, n, pos, bpos, L, vel, kap, kapv, ktot, kvtot, \
-                                                              diameter, feat, prev_p, prev_v)
+#define KF(S, U) sweep_launch(k_features<S, U>, nblk_xcd(n), s, c, t, n, pos, bpos, L, vel, kap, kapv, ktot, kvtot, \
+                                  diameter, feat, prev_p, prev_v)
     if (L.sol.bvel) { if (um) KF(true, true); else KF(true, false); }
     else { if (um) KF(false, true); else KF(false, false); }
 #undef KF

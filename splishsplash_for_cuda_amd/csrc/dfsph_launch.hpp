@@ -10,6 +10,16 @@
 namespace dfsph {
 
 uint32_t grid_blocks_xcd(uint32_t n);
+
+// Kernel timing bound to a dispatch: the next single-kernel sweep launch on
+// this thread (the source, Jacobi, warm-start, XSPH, feature and position
+// sweeps) carries the two events in its own dispatch (hipExtLaunchKernel), so
+// their interval is that kernel's execution alone, without the gaps that
+// separately recorded events would add around it.  The launch disarms them;
+// sweep_events_armed() tells whether no such launch took place.
+void arm_sweep_events(hipEvent_t start, hipEvent_t stop);
+bool sweep_events_armed();
+void disarm_sweep_events();
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, uint32_t n);
 void launch_fill_f32(hipStream_t s, float* p, float v, uint32_t n);
 size_t scan_temp_words(uint32_t n);

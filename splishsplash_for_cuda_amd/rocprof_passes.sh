@@ -8,8 +8,11 @@
 # same counter runs over the plain copy kernel of known size used to calibrate
 # FETCH_SIZE / WRITE_SIZE.  EXTRA_COUNTERS adds passes: a space-separated list
 # whose entries join the counters of one pass with '+' (within the per-block
-# limits of one pass).  Every step has its own time limit and the script
-# stops at the first failure.  Summaries: python splishsplash_for_cuda_amd/rocprof_summary.py OUTDIR
+# limits of one pass).  TRACE_ARGS (default: the same arguments) gives the
+# kernel-trace run its own bench.py arguments, e.g. the default bench command
+# while the counter passes run a shorter one.  Every step has its own time
+# limit and the script stops at the first failure.  Summaries:
+# python splishsplash_for_cuda_amd/rocprof_summary.py OUTDIR
 set -u
 OUT=${1:?output directory}
 shift
@@ -22,9 +25,10 @@ OUT=$(cd "$OUT" && pwd)
 export TMPDIR=/tmp
 cd "$ROOT"
 
-echo "== kernel trace: bench.py ${BENCH_ARGS[*]}"
-timeout -k 10 300 rocprofv3 --kernel-trace --stats --output-format csv -d "$OUT/trace" -o run -- \
-    python3 bench.py "${BENCH_ARGS[@]}" > "$OUT/trace.log" 2>&1 || { echo "trace failed"; tail -5 "$OUT/trace.log"; exit 1; }
+if [ -n "${TRACE_ARGS+x}" ]; then read -r -a T_ARGS <<< "$TRACE_ARGS"; else T_ARGS=("${BENCH_ARGS[@]}"); fi
+echo "== kernel trace: bench.py ${T_ARGS[*]}"
+timeout -k 10 400 rocprofv3 --kernel-trace --stats --output-format csv -d "$OUT/trace" -o run -- \
+    python3 bench.py "${T_ARGS[@]}" > "$OUT/trace.log" 2>&1 || { echo "trace failed"; tail -5 "$OUT/trace.log"; exit 1; }
 
 for m in FETCH_SIZE WRITE_SIZE; do
     echo "== copy calibration $m"

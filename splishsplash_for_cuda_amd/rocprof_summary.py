@@ -77,6 +77,10 @@ def main():
     ap.add_argument("--write-profiles", action="store_true")
     ap.add_argument("--command", default="bash splishsplash_for_cuda_amd/rocprof_passes.sh OUTDIR --steps 5 "
                                          "--warmup 5 --no-cpu-baseline")
+    ap.add_argument("--window", default=None,
+                    help="W,K,P: the bench run's warm-up, timed and profiled steps; the kernel trace is then "
+                         "also averaged over the plain leg's profiled steps (steps W+K+1 .. W+K+P), the "
+                         "launches bench.py times for its roofline")
     args = ap.parse_args()
     d = args.outdir
     lines = []
@@ -100,6 +104,28 @@ def main():
         lines.append("The velocity sweeps (`k_jacobi`) include at most one launch per solve that exits at once "
                      "(loop already converged); their minimum shows it.")
         lines.append("")
+
+    trace_path = os.path.join(d, "trace", "run_kernel_trace.csv")
+    if args.window and os.path.exists(trace_path):
+        w, k, pst = (int(x) for x in args.window.split(","))
+        with open(trace_path) as f:
+            rows = sorted(((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"])
+                           for r in csv.DictReader(f)), key=lambda r: r[0])
+        ends = [e for (_, e, nm) in rows if nm.startswith("dfsph::k_update_pos(")]
+        if len(ends) >= w + k + pst:
+            t0, t1 = ends[w + k - 1], ends[w + k + pst - 1]
+            per = defaultdict(list)
+            for (a0, a1, nm) in rows:
+                if t0 < a0 <= t1:
+                    per[nm].append((a1 - a0) / 1e3)
+            lines.append("## Profiled window of the plain leg (steps %d-%d, the launches bench.py times)"
+                         % (w + k + 1, w + k + pst))
+            lines.append("")
+            lines.append("| kernel | calls | avg us | min us |")
+            lines.append("|---|---|---|---|")
+            for nm, v in sorted(per.items(), key=lambda kv: -sum(kv[1]))[:16]:
+                lines.append("| %s | %d | %.1f | %.1f |" % (short(nm), len(v), sum(v) / len(v), min(v)))
+            lines.append("")
 
     # calibration factors
     factors = {}

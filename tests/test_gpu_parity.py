@@ -64,10 +64,12 @@ def test_neighbor_lists_bit_exact_every_list_form(oracle_mod, list_mode):
     pos = s.download(S.FIELD_POS).astype(np.float32)
     ids = s.download(S.FIELD_ID)
     inf = s.info()
+    blocks = (inf.n_fluid + inf.sweep_block - 1) // inf.sweep_block
+    assert inf.sweep_block in (256, 512)
     if list_mode == S.LISTS_AUTO:
-        assert inf.unstaged_blocks < (inf.n_fluid + 255) // 256
+        assert inf.unstaged_blocks < blocks
     else:
-        assert inf.unstaged_blocks == (inf.n_fluid + 255) // 256
+        assert inf.unstaged_blocks == blocks
     if list_mode == S.LISTS_GLOBAL:
         assert inf.neighbors_fluid_unstaged == inf.neighbors_fluid_total
     else:
