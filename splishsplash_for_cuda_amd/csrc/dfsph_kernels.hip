@@ -358,7 +358,10 @@ struct ListWriter32 {
 // bounding box of those cells, x fastest, then z, then y, so the set lists
 // them in (y, z, x) order and the three cells x-1..x+1 of any row (y, z) are
 // consecutive: one contiguous range of the set per row.
-constexpr uint32_t kNbrStage = kBlock == 512 ? 3584 : 2048;   // set positions staged in LDS by the search
+#ifndef DFSPH_NBR_STAGE
+#define DFSPH_NBR_STAGE (DFSPH_BLOCK == 512 ? 3584 : 2048)
+#endif
+constexpr uint32_t kNbrStage = DFSPH_NBR_STAGE;   // set positions staged in LDS by the search
 constexpr uint32_t kDense = 8192;          // bounding-box cells of a tabled set
 constexpr uint32_t kSetCells = kBlock == 512 ? 768 : 384;      // set cells of a tabled set
 constexpr uint32_t kNoRow = 0xffffffffu;

@@ -118,7 +118,7 @@ def main():
             for (a0, a1, nm) in rows:
                 if t0 < a0 <= t1:
                     per[nm].append((a1 - a0) / 1e3)
-            lines.append("## Profiled window of the plain leg (steps %d-%d, the launches bench.py times)"
+            lines.append("## Profiled window (steps %d-%d of the first leg, the launches bench.py times)"
                          % (w + k + 1, w + k + pst))
             lines.append("")
             lines.append("| kernel | calls | avg us | min us |")
