@@ -310,6 +310,10 @@ def main():
             "overflow": int(max(rec["overflow"])),
             "min_iters": list(min_iters),
             "unstaged_blocks": int(leg["inf1"].unstaged_blocks),
+            "sweep_block": int(leg["inf1"].sweep_block),
+            "neighbors_per_particle": {
+                "fluid": round(leg["inf1"].neighbors_fluid_total / max(leg["inf1"].n_fluid, 1), 2),
+                "boundary": round(leg["inf1"].neighbors_boundary_total / max(leg["inf1"].n_fluid, 1), 2)},
             "timing": "timed steps without profiling events; roofline and kernel shares from %d further "
                       "profiled steps (HIP events in each sweep's dispatch, on the solver's stream)" % prof_steps,
         },
