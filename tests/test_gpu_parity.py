@@ -384,6 +384,40 @@ def test_runs_are_bit_reproducible():
         assert np.array_equal(out[0][k], out[1][k]), k
 
 
+def test_profiled_run_is_the_same_run_and_times_every_phase():
+    """dfsph_profile_*: the sweeps launched with their timing events in the
+    dispatch (hipExtLaunchKernel) compute what the plain launches do, bit
+    for bit, and every phase of a step is timed once per launch."""
+    from splishsplash_for_cuda_amd._native import KERNEL_NAMES
+    sc = scene.config1()
+    ref = _solver(sc, dt=0.002)
+    for _ in range(5):
+        ref.step()
+    s = _solver(sc, dt=0.002)
+    s.profile_enable()
+    s.profile_reset()
+    its = [s.step() for _ in range(5)]
+    s.synchronize()
+    prof = s.profile()
+    a, b = ref.state_by_id(), s.state_by_id()
+    for k in ("id", "pos", "vel", "kappa", "kappaV"):
+        assert np.array_equal(a[k], b[k]), k
+    once = ("sort", "neighbors", "div_ws_init", "div_init", "xsph", "pressure_ws", "dens_init", "update_pos")
+    for name in once:
+        n, ms = prof[name]
+        assert n == 5, (name, n)
+        assert 0.0 < ms < 1e3, (name, ms)
+    # one velocity sweep and one loop-end sweep per Jacobi iteration (the
+    # velocity sweep queued ahead that exits at once is not counted)
+    nv = sum(st.iters_v for st in its)
+    nd = sum(st.iters for st in its)
+    assert prof["div_loop_end"][0] == nv and prof["dens_loop_end"][0] == nd
+    assert prof["div_jacobi"][0] == nv and prof["dens_jacobi"][0] == nd
+    assert all(prof[k][1] > 0.0 for k in ("div_loop_end", "dens_loop_end", "div_jacobi", "dens_jacobi"))
+    assert set(prof) == set(KERNEL_NAMES)
+    s.profile_enable(0)
+
+
 def test_caller_stream_gives_the_same_run():
     """dfsph_set_stream: a run whose steps move to a caller's stream after
     two steps (and back to the handle's own for the last) ends bit for bit
