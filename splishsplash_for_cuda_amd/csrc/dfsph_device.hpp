@@ -431,9 +431,10 @@ struct PairL {
 
 __device__ __forceinline__ PairL grad_listed(const SimConst& c, float r2)
 {
-    // a coincident pair (r2 = 0) gives ir = inf and a NaN coefficient, which
-    // the caller drops with ok, so r2 needs no clamp here
-    const float ir = __builtin_amdgcn_rsqf(r2);
+    // r2 is clamped at 1e-12 (r = 1 um): a coincident pair then has a finite
+    // coefficient and a zero offset, so it adds nothing to any sum (its
+    // gradient is the coefficient times the offset) and needs no select
+    const float ir = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-12f));
     const float r = r2 * ir;
     const float d = c.hk - r;
     // both pieces are computed and one is selected: a branch per pair (the
@@ -444,7 +445,7 @@ __device__ __forceinline__ PairL grad_listed(const SimConst& c, float r2)
     PairL o;
     o.W = 0.0f;
     o.g = __builtin_unpredictable(r2 <= c.hk2_4) ? gi : go;
-    o.ok = r2 > 1.0e-12f;
+    o.ok = true;
     return o;
 }
 
