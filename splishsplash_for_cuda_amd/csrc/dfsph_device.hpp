@@ -30,7 +30,7 @@ namespace dfsph {
 // neighbor sets, so this also sets the LDS stages below.  512 (8 wavefronts):
 // a 512-particle set is about 3.9 times the block against 5.2 for 256, so the
 // stages hold more particles per wavefront and the position + velocity sweeps
-// run 16 wavefronts per CU instead of 12 (C4: 4.84 against 5.20 ms per step).
+// run 16 wavefronts per CU instead of 12 (C4: 4.71 against 5.20 ms per step).
 #ifndef DFSPH_BLOCK
 #define DFSPH_BLOCK 512
 #endif

@@ -358,12 +358,21 @@ struct ListWriter32 {
 // bounding box of those cells, x fastest, then z, then y, so the set lists
 // them in (y, z, x) order and the three cells x-1..x+1 of any row (y, z) are
 // consecutive: one contiguous range of the set per row.
+// LDS of the search with 512-particle blocks: 2672 staged positions and 512
+// set cells take 52 KB, three blocks per CU.  The search is bound by the
+// latency of its per-block phases (set construction, staging, candidate
+// tests between barriers), so the third block gains more than the larger
+// stage of two blocks per CU would (C4: 0.62 against 0.76 ms per search);
+// larger sets search through global memory.
 #ifndef DFSPH_NBR_STAGE
-#define DFSPH_NBR_STAGE (DFSPH_BLOCK == 512 ? 3584 : 2048)
+#define DFSPH_NBR_STAGE (DFSPH_BLOCK == 512 ? 2672 : 2048)
 #endif
 constexpr uint32_t kNbrStage = DFSPH_NBR_STAGE;   // set positions staged in LDS by the search
 constexpr uint32_t kDense = 8192;          // bounding-box cells of a tabled set
-constexpr uint32_t kSetCells = kBlock == 512 ? 768 : 384;      // set cells of a tabled set
+#ifndef DFSPH_SET_CELLS
+#define DFSPH_SET_CELLS (DFSPH_BLOCK == 512 ? 512 : 384)
+#endif
+constexpr uint32_t kSetCells = DFSPH_SET_CELLS;   // set cells of a tabled set
 constexpr uint32_t kNoRow = 0xffffffffu;
 constexpr int kIntMax = 0x7fffffff;
 
